@@ -1,0 +1,339 @@
+"""alps_amd -- MI355X (gfx950) exact-diagonalization engine for the ALPS
+`sparsediag` hot path: fixed-Sz spin-1/2 Hamiltonians, matrix-free H.v and
+the ietl::lanczos recurrence, behind the C ABI of include/alps_ed.h.
+
+This module is a thin ctypes binding of that ABI (libalps_ed.so, built by
+alps_amd/build.py).  There is no fallback: if the library is missing or no
+GPU is present, the calls raise.
+"""
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libalps_ed.so")
+
+F32, F64 = 0, 1
+_DTYPES = {"f32": F32, "float32": F32, F32: F32, "f64": F64, "float64": F64, F64: F64}
+
+_lib = None
+
+
+class EDError(RuntimeError):
+    """Error status returned by the engine (see include/alps_ed.h)."""
+
+    def __init__(self, status, message):
+        super().__init__(f"alps_ed status {status}: {message}")
+        self.status = status
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"alps_amd: engine library not built: {LIB_PATH} (run python -m alps_amd.build)")
+        L = ctypes.CDLL(LIB_PATH)
+        P, i32, u64, f64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_double
+        sig = {
+            "alps_ed_version": ([], i32),
+            "alps_ed_last_error": ([], ctypes.c_char_p),
+            "alps_ed_device_info": ([ctypes.c_char_p, ctypes.c_size_t, P, P], i32),
+            "alps_ed_set_stream": ([P], i32),
+            "alps_ed_synchronize": ([], i32),
+            "alps_ed_operator_create": ([P, P], i32),
+            "alps_ed_operator_destroy": ([P], i32),
+            "alps_ed_operator_dimension": ([P, P], i32),
+            "alps_ed_operator_bytes": ([P, P], i32),
+            "alps_ed_operator_states": ([P, u64, u64, P], i32),
+            "alps_ed_operator_index": ([P, P, u64, P], i32),
+            "alps_ed_operator_elements": ([P, u64, u64, i32, P, P, P], i32),
+            "alps_ed_vector_create": ([u64, i32, P], i32),
+            "alps_ed_vector_destroy": ([P], i32),
+            "alps_ed_vector_size": ([P, P], i32),
+            "alps_ed_vector_dtype": ([P, P], i32),
+            "alps_ed_vector_data": ([P, P], i32),
+            "alps_ed_vector_upload": ([P, P, u64], i32),
+            "alps_ed_vector_download": ([P, P, u64], i32),
+            "alps_ed_vector_fill_random": ([P, u64, f64, f64], i32),
+            "alps_ed_vector_fill": ([P, f64], i32),
+            "alps_ed_vector_copy": ([P, P], i32),
+            "alps_ed_vector_scale": ([P, f64], i32),
+            "alps_ed_vector_axpby": ([P, f64, P, f64], i32),
+            "alps_ed_vector_dot": ([P, P, P], i32),
+            "alps_ed_vector_nrm2": ([P, P], i32),
+            "alps_ed_matvec": ([P, P, P], i32),
+            "alps_ed_lanczos_step_vectors": ([P, P, P, P, f64, P, P], i32),
+            "alps_ed_lanczos_create": ([P, i32, i32, P], i32),
+            "alps_ed_lanczos_destroy": ([P], i32),
+            "alps_ed_lanczos_start": ([P, u64, P], i32),
+            "alps_ed_lanczos_steps": ([P, i32], i32),
+            "alps_ed_lanczos_coefficients": ([P, P, P, i32, P], i32),
+            "alps_ed_lanczos_vectors": ([P, P, P], i32),
+            "alps_ed_tmatrix_analyze": ([i32, P, P, P, P, P, P], i32),
+            "alps_ed_sparsediag": ([P, P, P], i32),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status != 0:
+        raise EDError(status, lib().alps_ed_last_error().decode(errors="replace"))
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class _Model(ctypes.Structure):
+    _fields_ = [("n_sites", ctypes.c_int), ("n_up", ctypes.c_int), ("n_bonds", ctypes.c_int),
+                ("bond_site1", ctypes.c_void_p), ("bond_site2", ctypes.c_void_p),
+                ("jz", ctypes.c_void_p), ("jxy", ctypes.c_void_p), ("k", ctypes.c_void_p),
+                ("h", ctypes.c_void_p), ("d", ctypes.c_void_p)]
+
+
+class _SolveParams(ctypes.Structure):
+    _fields_ = [("max_iter", ctypes.c_int), ("nev", ctypes.c_int), ("fixed_iterations", ctypes.c_int),
+                ("seed", ctypes.c_uint64), ("dtype", ctypes.c_int)]
+
+
+class _SolveResult(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_uint64), ("n_eigenvalues", ctypes.c_int),
+                ("eigenvalues", ctypes.c_double * 64), ("iterations", ctypes.c_int),
+                ("error_code", ctypes.c_int), ("seconds", ctypes.c_double)]
+
+
+@dataclass
+class SpinModel:
+    """Spin-1/2 model: bonds (site1, site2) in edge order with per-bond Jz,
+    Jxy, K and per-site h, D (the "spin" HAMILTONIAN of the ALPS model
+    library, lib/xml/models.xml.in:342-373), in the sector with n_up up spins."""
+    n_sites: int
+    bond_site1: np.ndarray
+    bond_site2: np.ndarray
+    jz: np.ndarray
+    jxy: np.ndarray
+    n_up: int = -1
+    k: np.ndarray = None
+    h: np.ndarray = None
+    d: np.ndarray = None
+
+    def __post_init__(self):
+        nb = len(self.bond_site1)
+        self.bond_site1 = np.ascontiguousarray(self.bond_site1, dtype=np.int32)
+        self.bond_site2 = np.ascontiguousarray(self.bond_site2, dtype=np.int32)
+        self.jz = np.ascontiguousarray(np.broadcast_to(self.jz, (nb,)), dtype=np.float64)
+        self.jxy = np.ascontiguousarray(np.broadcast_to(self.jxy, (nb,)), dtype=np.float64)
+        z = 0.0
+        self.k = np.ascontiguousarray(np.broadcast_to(z if self.k is None else self.k, (nb,)), dtype=np.float64)
+        self.h = np.ascontiguousarray(np.broadcast_to(z if self.h is None else self.h, (self.n_sites,)), dtype=np.float64)
+        self.d = np.ascontiguousarray(np.broadcast_to(z if self.d is None else self.d, (self.n_sites,)), dtype=np.float64)
+        if self.n_up < 0:
+            self.n_up = self.n_sites // 2
+
+    def cstruct(self):
+        m = _Model()
+        m.n_sites, m.n_up, m.n_bonds = self.n_sites, self.n_up, len(self.bond_site1)
+        m.bond_site1, m.bond_site2 = _p(self.bond_site1), _p(self.bond_site2)
+        m.jz, m.jxy, m.k, m.h, m.d = _p(self.jz), _p(self.jxy), _p(self.k), _p(self.h), _p(self.d)
+        return m
+
+    def with_sector(self, n_up):
+        return SpinModel(self.n_sites, self.bond_site1, self.bond_site2, self.jz, self.jxy, n_up,
+                         self.k, self.h, self.d)
+
+
+def device_info():
+    name = ctypes.create_string_buffer(256)
+    mem = ctypes.c_uint64()
+    ncu = ctypes.c_int()
+    _check(lib().alps_ed_device_info(name, 256, ctypes.byref(mem), ctypes.byref(ncu)))
+    return dict(name=name.value.decode(), hbm_bytes=mem.value, n_cu=ncu.value)
+
+
+def synchronize():
+    _check(lib().alps_ed_synchronize())
+
+
+class Operator:
+    """Matrix-free device Hamiltonian of one sector (alps_ed_operator)."""
+
+    def __init__(self, model: SpinModel):
+        self.model = model
+        self._m = model.cstruct()
+        h = ctypes.c_void_p()
+        _check(lib().alps_ed_operator_create(ctypes.byref(self._m), ctypes.byref(h)))
+        self.handle = h
+        d = ctypes.c_uint64()
+        _check(lib().alps_ed_operator_dimension(self.handle, ctypes.byref(d)))
+        self.dim = d.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().alps_ed_operator_destroy(self.handle)
+            self.handle = None
+
+    __del__ = close
+
+    def states(self, first=0, count=None):
+        count = self.dim - first if count is None else count
+        out = np.zeros(max(count, 1), dtype=np.uint32)
+        _check(lib().alps_ed_operator_states(self.handle, first, count, _p(out)))
+        return out[:count]
+
+    def index(self, states):
+        s = np.ascontiguousarray(states, dtype=np.uint32)
+        out = np.zeros(max(len(s), 1), dtype=np.uint64)
+        _check(lib().alps_ed_operator_index(self.handle, _p(s), len(s), _p(out)))
+        return out[: len(s)]
+
+    def elements(self, row0=0, nrows=None):
+        """Explicit rows as CSR (rowptr, cols, vals) in reference order."""
+        nrows = self.dim - row0 if nrows is None else nrows
+        mp = len(self.model.bond_site1) + 1
+        cols = np.zeros(max(nrows * mp, 1), dtype=np.uint32)
+        vals = np.zeros(max(nrows * mp, 1), dtype=np.float64)
+        cnt = np.zeros(max(nrows, 1), dtype=np.int32)
+        _check(lib().alps_ed_operator_elements(self.handle, row0, nrows, mp, _p(cols), _p(vals), _p(cnt)))
+        cnt = cnt[:nrows]
+        rowptr = np.zeros(nrows + 1, dtype=np.uint64)
+        rowptr[1:] = np.cumsum(cnt)
+        sel = (np.arange(mp)[None, :] < cnt[:, None]).reshape(-1)
+        return rowptr, cols[: nrows * mp][sel], vals[: nrows * mp][sel]
+
+    def matvec(self, x: "Vector", y: "Vector"):
+        _check(lib().alps_ed_matvec(self.handle, x.handle, y.handle))
+        return y
+
+
+class Vector:
+    """Device vector (alps_ed_vector) stored in FP32 or FP64."""
+
+    def __init__(self, n, dtype="f32"):
+        self.n = int(n)
+        self.dtype = _DTYPES[dtype]
+        h = ctypes.c_void_p()
+        _check(lib().alps_ed_vector_create(self.n, self.dtype, ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().alps_ed_vector_destroy(self.handle)
+            self.handle = None
+
+    __del__ = close
+
+    @classmethod
+    def from_numpy(cls, a, dtype="f64"):
+        v = cls(len(a), dtype)
+        v.upload(a)
+        return v
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        _check(lib().alps_ed_vector_upload(self.handle, _p(a), len(a)))
+
+    def numpy(self):
+        out = np.zeros(max(self.n, 1))
+        _check(lib().alps_ed_vector_download(self.handle, _p(out), self.n))
+        return out[: self.n]
+
+    def fill_random(self, seed, lo=-1.0, hi=1.0):
+        _check(lib().alps_ed_vector_fill_random(self.handle, seed, lo, hi))
+
+    def fill(self, value):
+        _check(lib().alps_ed_vector_fill(self.handle, value))
+
+    def copy_from(self, other):
+        _check(lib().alps_ed_vector_copy(self.handle, other.handle))
+
+    def scale(self, a):
+        _check(lib().alps_ed_vector_scale(self.handle, a))
+
+    def axpby(self, a, x, b):
+        _check(lib().alps_ed_vector_axpby(self.handle, a, x.handle, b))
+
+    def dot(self, other):
+        r = ctypes.c_double()
+        _check(lib().alps_ed_vector_dot(self.handle, other.handle, ctypes.byref(r)))
+        return r.value
+
+    def nrm2(self):
+        r = ctypes.c_double()
+        _check(lib().alps_ed_vector_nrm2(self.handle, ctypes.byref(r)))
+        return r.value
+
+    def data_ptr(self):
+        p = ctypes.c_void_p()
+        _check(lib().alps_ed_vector_data(self.handle, ctypes.byref(p)))
+        return p.value
+
+
+def lanczos_step_vectors(op, v_prev, v_cur, w, beta_prev):
+    a, b = ctypes.c_double(), ctypes.c_double()
+    _check(lib().alps_ed_lanczos_step_vectors(op.handle, v_prev.handle, v_cur.handle, w.handle,
+                                              beta_prev, ctypes.byref(a), ctypes.byref(b)))
+    return a.value, b.value
+
+
+class Lanczos:
+    """Device-resident Lanczos recurrence (alps_ed_lanczos)."""
+
+    def __init__(self, op: Operator, dtype="f32", max_steps=1000):
+        self.op = op
+        self.max_steps = max_steps
+        h = ctypes.c_void_p()
+        _check(lib().alps_ed_lanczos_create(op.handle, _DTYPES[dtype], max_steps, ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().alps_ed_lanczos_destroy(self.handle)
+            self.handle = None
+
+    __del__ = close
+
+    def start(self, seed=0, start: Vector = None):
+        _check(lib().alps_ed_lanczos_start(self.handle, seed, None if start is None else start.handle))
+
+    def steps(self, n):
+        _check(lib().alps_ed_lanczos_steps(self.handle, n))
+
+    def coefficients(self):
+        a = np.zeros(self.max_steps + 1)
+        b = np.zeros(self.max_steps + 1)
+        n = ctypes.c_int()
+        _check(lib().alps_ed_lanczos_coefficients(self.handle, _p(a), _p(b), self.max_steps, ctypes.byref(n)))
+        return a[: n.value].copy(), b[: n.value].copy()
+
+    def vectors(self, dtype="f64"):
+        vp, vc = Vector(self.op.dim, dtype), Vector(self.op.dim, dtype)
+        _check(lib().alps_ed_lanczos_vectors(self.handle, vp.handle, vc.handle))
+        return vp, vc
+
+
+def tmatrix_analyze(alpha, beta):
+    a = np.ascontiguousarray(alpha, dtype=np.float64)
+    b = np.ascontiguousarray(beta, dtype=np.float64)
+    n = len(a)
+    ev, err = np.zeros(n), np.zeros(n)
+    mult = np.zeros(n, dtype=np.int32)
+    k = ctypes.c_int()
+    _check(lib().alps_ed_tmatrix_analyze(n, _p(a), _p(b), _p(ev), _p(err), _p(mult), ctypes.byref(k)))
+    return dict(eig=ev[: k.value], err=err[: k.value], mult=mult[: k.value])
+
+
+def sparsediag(model: SpinModel, max_iter=0, nev=1, seed=0, dtype="f64", fixed=False):
+    """One sector of SparseDiagMatrix::do_subspace on the device."""
+    m = model.cstruct()
+    p = _SolveParams(max_iter, nev, int(fixed), seed, _DTYPES[dtype])
+    r = _SolveResult()
+    _check(lib().alps_ed_sparsediag(ctypes.byref(m), ctypes.byref(p), ctypes.byref(r)))
+    return dict(dim=r.dim, ev=np.array(r.eigenvalues[: r.n_eigenvalues]), iterations=r.iterations,
+                error_code=r.error_code, seconds=r.seconds)

@@ -1,0 +1,220 @@
+// ed_layout.hpp -- host-side description of one fixed-Sz sector of a spin-1/2
+// Hamiltonian, in the form the gfx950 kernels consume.
+//
+// Basis order.  A state is a bit string of N bits, bit (N-1-s) set iff site s
+// is up.  The reference orders the fixed-Sz basis lexicographically by the
+// per-site index vector with site 0 most significant and index 0 = Sz=-1/2
+// (src/alps/model/basisstates.h:176-278, sitebasisstates.h:75-104), which is
+// ascending integer order of these bit strings among those with k set bits.
+// The index of a state is therefore its colex rank
+//     rank(s) = sum_{m=1..k} C(p_m, m),   p_1 < ... < p_k the set bits,
+// the combinatorial number system.  It replaces the reference's binary search
+// (basisstates.h:68-74) by table lookups.
+//
+// Split ranking.  The N bits are cut into a low part (bits 0..B-1) and a high
+// part (bits B..N-1).  The first kl = popcount(lo) set bits are the low ones,
+// so rank(s) = lo_rank[lo] + hi_base[hi] where
+//     lo_rank[lo] = colex rank of lo among B-bit words with popcount kl,
+//     hi_base[hi] = sum over set bits of hi at global position p, index m
+//                   counted from kl+1, of C(p, m).
+// All states sharing `hi` occupy one contiguous row segment of length
+// C(B, k - popcount(hi)) starting at hi_base[hi]; within a segment the rows
+// are the low words of that popcount in ascending order (lo_state).
+//
+// Matrix elements.  Spin-1/2 model of lib/xml/models.xml.in:342-373 with the
+// local matrices of test/model/example3.output: per site -h*Sz + D*Sz^2, per
+// bond Jz*Sz*Sz + Jxy*exchange_xy + K*biquadratic, i.e. a diagonal e_par
+// (parallel spins), e_anti (antiparallel) and an off-diagonal t for the flip
+// of an antiparallel pair.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace alps_ed {
+
+constexpr int kMaxSites = 32;
+
+inline uint64_t binom(int n, int k) {
+  if (k < 0 || n < 0 || k > n) return 0;
+  uint64_t r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (uint64_t)(n - k + i) / (uint64_t)i;
+  return r;
+}
+
+struct BondTerm {
+  int s1, s2;           // sites, edge order of the graph
+  double e_par;         // 0.25*Jz + 0.0625*K
+  double e_anti;        // -0.25*Jz + 0.3125*K
+  double t;             // 0.5*Jxy - 0.25*K
+};
+
+struct SiteTerm {
+  double e_dn;          // -h*(-1/2) + D/4
+  double e_up;          // -h*(+1/2) + D/4
+};
+
+struct ModelSpec {
+  int n_sites = 0;
+  int n_up = 0;
+  std::vector<BondTerm> bonds;
+  std::vector<SiteTerm> sites;
+};
+
+// Local matrix elements from couplings, written as the reference's two-term
+// sums (each product is exact, and a two-term sum is order independent).
+inline BondTerm make_bond(int s1, int s2, double jz, double jxy, double k) {
+  BondTerm b;
+  b.s1 = s1;
+  b.s2 = s2;
+  b.e_par = (0.25 * jz) + (0.0625 * k);
+  b.e_anti = (-0.25 * jz) + (0.3125 * k);
+  b.t = (0.5 * jxy) + (-0.25 * k);
+  return b;
+}
+
+inline SiteTerm make_site(double h, double d) {
+  SiteTerm s;
+  s.e_dn = (-h * -0.5) + (d * 0.25);
+  s.e_up = (-h * 0.5) + (d * 0.25);
+  return s;
+}
+
+// Flip bond in bit coordinates.
+struct FlipBond {
+  uint32_t mask;   // both bits (positions in the word the kernel tests)
+  uint32_t mask2;  // second word's bit for cross bonds (0 otherwise)
+  double t;
+};
+
+// Cross bond: one bit in the high word, one in the low word.
+struct CrossBond {
+  uint32_t hbit;   // bit in the high word (hi-local coordinates)
+  uint32_t lbit;   // bit in the low word
+  double e_par, e_anti, t;
+};
+
+struct SectorLayout {
+  int N = 0, k = 0, B = 0, NH = 0;
+  uint64_t D = 0;
+  std::vector<uint32_t> lo_rank;       // [2^B]
+  std::vector<uint32_t> lo_state;      // [2^B] grouped by popcount, ascending
+  std::vector<uint32_t> lo_class_off;  // [B+2]
+  std::vector<uint32_t> hi_base;       // [2^NH]; 0xffffffff when no state has this hi
+  std::vector<uint32_t> seg_hi;        // valid hi words, ascending
+  std::vector<uint32_t> seg_start;     // first row of each segment
+  std::vector<uint32_t> tiles;         // (hi, r0) pairs, 64-row tiles
+  std::vector<double> diag_lo;         // [2^B] site + low-low bond diagonal
+  std::vector<double> diag_hi;         // [2^NH] site + high-high bond diagonal
+  std::vector<FlipBond> ll, hh;        // low-low and high-high flip bonds
+  std::vector<CrossBond> xb;           // cross bonds (all, for the diagonal)
+  // reference-order description for the element enumeration
+  std::vector<double> site_dn, site_up;          // per site
+  std::vector<uint32_t> bond_p1, bond_p2;        // bit positions per bond
+  std::vector<double> bond_par, bond_anti, bond_t;
+};
+
+inline int choose_low_bits(int N) {
+  // Low words of at least 12 bits keep row segments long (C(12,6) = 924
+  // rows) for small N; at large N the split is near the middle so that both
+  // tables stay small (2^15 entries at N = 30).
+  int B = (N + 1) / 2;
+  if (B < 12) B = N < 12 ? N : 12;
+  return B;
+}
+
+inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1) {
+  SectorLayout L;
+  const int N = m.n_sites;
+  if (N < 1 || N > kMaxSites) throw std::invalid_argument("number of sites must be in [1, 32]");
+  if ((int)m.sites.size() != N) throw std::invalid_argument("site term count != number of sites");
+  for (const BondTerm& b : m.bonds)
+    if (b.s1 < 0 || b.s1 >= N || b.s2 < 0 || b.s2 >= N || b.s1 == b.s2)
+      throw std::invalid_argument("bond with invalid sites");
+  L.N = N;
+  L.k = m.n_up;
+  L.D = (m.n_up < 0 || m.n_up > N) ? 0 : binom(N, m.n_up);
+  if (L.D >= 0xffffffffULL) throw std::invalid_argument("sector dimension exceeds 2^32-1");
+  const int B = B_override > 0 && B_override <= N ? B_override : choose_low_bits(N);
+  L.B = B;
+  L.NH = N - B;
+  const uint32_t nlo = 1u << B;
+  const uint32_t nhi = 1u << L.NH;
+  // low tables
+  L.lo_rank.assign(nlo, 0);
+  L.lo_class_off.assign(B + 2, 0);
+  for (uint32_t v = 0; v < nlo; ++v) {
+    uint64_t r = 0;
+    int m_idx = 0;
+    for (int p = 0; p < B; ++p)
+      if (v >> p & 1u) r += binom(p, ++m_idx);
+    L.lo_rank[v] = (uint32_t)r;
+    L.lo_class_off[__builtin_popcount(v) + 1]++;
+  }
+  for (int c = 0; c <= B; ++c) L.lo_class_off[c + 1] += L.lo_class_off[c];
+  L.lo_state.assign(nlo, 0);
+  for (uint32_t v = 0; v < nlo; ++v) {
+    int c = __builtin_popcount(v);
+    L.lo_state[L.lo_class_off[c] + L.lo_rank[v]] = v;
+  }
+  // high tables and segments
+  L.hi_base.assign(nhi, 0xffffffffu);
+  for (uint32_t h = 0; h < nhi; ++h) {
+    int kl = L.k - __builtin_popcount(h);
+    if (kl < 0 || kl > B || L.D == 0) continue;
+    uint64_t r = 0;
+    int m_idx = kl;
+    for (int q = 0; q < L.NH; ++q)
+      if (h >> q & 1u) r += binom(B + q, ++m_idx);
+    L.hi_base[h] = (uint32_t)r;
+    L.seg_hi.push_back(h);
+    L.seg_start.push_back((uint32_t)r);
+    uint64_t len = binom(B, kl);
+    for (uint64_t r0 = 0; r0 < len; r0 += 64) {
+      L.tiles.push_back(h);
+      L.tiles.push_back((uint32_t)r0);
+    }
+  }
+  // terms in bit coordinates
+  auto pos = [N](int s) { return (uint32_t)(N - 1 - s); };
+  L.diag_lo.assign(nlo, 0.0);
+  L.diag_hi.assign(nhi, 0.0);
+  for (int s = 0; s < N; ++s) {
+    uint32_t p = pos(s);
+    L.site_dn.push_back(m.sites[s].e_dn);
+    L.site_up.push_back(m.sites[s].e_up);
+    if ((int)p < B) {
+      for (uint32_t v = 0; v < nlo; ++v) L.diag_lo[v] += (v >> p & 1u) ? m.sites[s].e_up : m.sites[s].e_dn;
+    } else {
+      uint32_t q = p - B;
+      for (uint32_t h = 0; h < nhi; ++h) L.diag_hi[h] += (h >> q & 1u) ? m.sites[s].e_up : m.sites[s].e_dn;
+    }
+  }
+  for (const BondTerm& b : m.bonds) {
+    uint32_t p1 = pos(b.s1), p2 = pos(b.s2);
+    L.bond_p1.push_back(p1);
+    L.bond_p2.push_back(p2);
+    L.bond_par.push_back(b.e_par);
+    L.bond_anti.push_back(b.e_anti);
+    L.bond_t.push_back(b.t);
+    bool lo1 = (int)p1 < B, lo2 = (int)p2 < B;
+    if (lo1 && lo2) {
+      for (uint32_t v = 0; v < nlo; ++v)
+        L.diag_lo[v] += ((v >> p1 ^ v >> p2) & 1u) ? b.e_anti : b.e_par;
+      if (b.t != 0.0) L.ll.push_back({(1u << p1) | (1u << p2), 0u, b.t});
+    } else if (!lo1 && !lo2) {
+      uint32_t q1 = p1 - B, q2 = p2 - B;
+      for (uint32_t h = 0; h < nhi; ++h)
+        L.diag_hi[h] += ((h >> q1 ^ h >> q2) & 1u) ? b.e_anti : b.e_par;
+      if (b.t != 0.0) L.hh.push_back({(1u << q1) | (1u << q2), 0u, b.t});
+    } else {
+      uint32_t ph = lo1 ? p2 : p1, pl = lo1 ? p1 : p2;
+      L.xb.push_back({1u << (ph - B), 1u << pl, b.e_par, b.e_anti, b.t});
+    }
+  }
+  return L;
+}
+
+}  // namespace alps_ed

@@ -1,0 +1,196 @@
+/* alps_ed.h -- C ABI of the MI355X exact-diagonalization engine (gfx950).
+ *
+ * This is the device boundary of the ALPS `sparsediag` hot path: Hamiltonian
+ * construction over the fixed-Sz spin-1/2 basis, the sparse H.v product and
+ * the ietl::lanczos recurrence.  In the reference the boundary does not
+ * exist as an ABI: sparsediag instantiates the header-only
+ *     ietl::lanczos<ublas::mapped_vector_of_mapped_vector<double>,
+ *                   ietl::vectorspace<ublas::vector<double>>>
+ * (applications/diag/sparsediag/sparsediag.h:26,96,104-113) and the work
+ * happens in uBLAS prod / inner_prod / norm_2 (src/ietl/interface/ublas.h:27-63)
+ * and in hamiltonian_matrix::build (src/alps/model/hamiltonian_matrix.hpp:423-450).
+ * Each entry point below names the reference interface it replaces.  The C++
+ * ietl vector space over these calls is include/alps_ed/ietl_gpu.hpp.
+ *
+ * Conventions
+ *   - Every function returns ALPS_ED_OK (0) or a negative status; the text of
+ *     the last error of the calling thread is alps_ed_last_error().
+ *   - Handles are opaque; all sizes are element counts.
+ *   - Vectors live in device memory (HBM) of the current HIP device and are
+ *     stored as FP32 or FP64 (alps_ed_dtype).  Scalars crossing the ABI are
+ *     FP64; dot products and norms accumulate in FP64 whatever the storage.
+ *   - All work is issued on the stream set by alps_ed_set_stream (default:
+ *     the null stream).  Functions that return host values synchronize it.
+ *   - Results are deterministic: reductions use a fixed tree, never float
+ *     atomics, so repeated runs give bitwise identical alpha/beta.
+ */
+#ifndef ALPS_ED_H
+#define ALPS_ED_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ALPS_ED_OK 0
+#define ALPS_ED_ERR_ARGUMENT -1   /* invalid argument (std::invalid_argument in C++) */
+#define ALPS_ED_ERR_DEVICE -2     /* HIP runtime error */
+#define ALPS_ED_ERR_MEMORY -3     /* device allocation failed or over budget */
+#define ALPS_ED_ERR_SOLVER -4     /* tridiagonal solve failed (tmatrix.h:144-145) */
+#define ALPS_ED_ERR_STATE -5      /* call out of order (lanczos.h:362-363) */
+
+typedef enum { ALPS_ED_F32 = 0, ALPS_ED_F64 = 1 } alps_ed_dtype;
+
+typedef struct alps_ed_operator_s* alps_ed_operator;
+typedef struct alps_ed_vector_s* alps_ed_vector;
+typedef struct alps_ed_lanczos_s* alps_ed_lanczos;
+typedef struct alps_ed_batch_s* alps_ed_batch;
+
+/* Library ------------------------------------------------------------------ */
+
+int alps_ed_version(void);                 /* 10000*major + 100*minor + patch */
+const char* alps_ed_last_error(void);
+int alps_ed_device_info(char* name, size_t name_len, uint64_t* hbm_bytes, int* n_cu);
+/* Stream for all subsequent work of this thread (a hipStream_t; NULL = null stream). */
+int alps_ed_set_stream(void* hip_stream);
+int alps_ed_synchronize(void);
+
+/* Model -------------------------------------------------------------------- */
+
+/* A spin-1/2 model with conserved Sz on n_sites <= 32 sites, restricted to
+ * the sector with n_up sites up (Sz_total = n_up - n_sites/2).  Bonds are
+ * given in the graph's edge order (latticegraph.h:98-136) with per-bond
+ * couplings; the local terms are those of the "spin" HAMILTONIAN
+ * (lib/xml/models.xml.in:342-373):
+ *   site s:  -h[s]*Sz + d[s]*Sz^2
+ *   bond b:  jz[b]*Sz*Sz + jxy[b]*(S+S- + S-S+)/2 + k[b]*biquadratic
+ * Null coupling arrays mean zero. */
+typedef struct {
+  int n_sites;
+  int n_up;
+  int n_bonds;
+  const int* bond_site1;
+  const int* bond_site2;
+  const double* jz;
+  const double* jxy;
+  const double* k;
+  const double* h;
+  const double* d;
+} alps_ed_model;
+
+/* Operator (replaces hamiltonian_matrix<M>::build and its matrix_,
+ * src/alps/model/hamiltonian_matrix.hpp:67-231,388-450; the operator is
+ * matrix-free: nothing of size nnz is stored). ---------------------------- */
+
+int alps_ed_operator_create(const alps_ed_model* model, alps_ed_operator* op);
+int alps_ed_operator_destroy(alps_ed_operator op);
+/* hamiltonian_matrix::dimension (hamiltonian_matrix.hpp:92). */
+int alps_ed_operator_dimension(alps_ed_operator op, uint64_t* dim);
+/* Device bytes held by the operator's tables. */
+int alps_ed_operator_bytes(alps_ed_operator op, uint64_t* bytes);
+/* Basis states [first, first+count) as bit strings, bit (N-1-s) set iff site
+ * s is up; the order is basis_states::build's (basisstates.h:176-278). */
+int alps_ed_operator_states(alps_ed_operator op, uint64_t first, uint64_t count, uint32_t* states_out);
+/* basis_states::index (basisstates.h:68-74): index of each state, or the
+ * dimension when the state is not in the sector. */
+int alps_ed_operator_index(alps_ed_operator op, const uint32_t* states, uint64_t count, uint64_t* index_out);
+/* Explicit matrix rows [row0, row0+nrows): the elements H(i,j) that
+ * hamiltonian_matrix::build accumulates (sites first, then bonds in edge
+ * order, hamiltonian_matrix.hpp:388-397), columns ascending, the explicit
+ * diagonal always present (:44-58).  Row r occupies cols/vals
+ * [r*max_per_row, r*max_per_row + counts[r]).  Needs max_per_row >=
+ * n_bonds + 1. */
+int alps_ed_operator_elements(alps_ed_operator op, uint64_t row0, uint64_t nrows, int max_per_row,
+                              uint32_t* cols, double* vals, int* counts);
+
+/* Vectors (replace ietl::vectorspace<ublas::vector<double>>, new_vector,
+ * vec_dimension, generate, clear, copy, dot, two_norm and the uBLAS vector
+ * expressions; src/ietl/vectorspace.h:23-46,101-109,
+ * src/ietl/interface/ublas.h:27-55, lanczos.h:383-402). ------------------- */
+
+int alps_ed_vector_create(uint64_t n, alps_ed_dtype dtype, alps_ed_vector* v);
+int alps_ed_vector_destroy(alps_ed_vector v);
+int alps_ed_vector_size(alps_ed_vector v, uint64_t* n);
+int alps_ed_vector_dtype(alps_ed_vector v, alps_ed_dtype* dtype);
+/* Raw device pointer of the storage (float* or double*). */
+int alps_ed_vector_data(alps_ed_vector v, void** device_ptr);
+int alps_ed_vector_upload(alps_ed_vector v, const double* host, uint64_t n);
+int alps_ed_vector_download(alps_ed_vector v, double* host, uint64_t n);
+/* Counter-based start vector: v[i] = lo + (hi-lo)*u_i with u_i from the
+ * splitmix64 output of seed + (i+1)*0x9e3779b97f4a7c15, uniform on [0,1). */
+int alps_ed_vector_fill_random(alps_ed_vector v, uint64_t seed, double lo, double hi);
+int alps_ed_vector_fill(alps_ed_vector v, double value);
+int alps_ed_vector_copy(alps_ed_vector dst, alps_ed_vector src);
+int alps_ed_vector_scale(alps_ed_vector v, double a);                       /* v *= a */
+int alps_ed_vector_axpby(alps_ed_vector y, double a, alps_ed_vector x, double b); /* y = a*x + b*y */
+int alps_ed_vector_dot(alps_ed_vector x, alps_ed_vector y, double* result);
+int alps_ed_vector_nrm2(alps_ed_vector x, double* result);
+
+/* ietl::mult(m, x, y) -> y = prod(m, x) (interface/ublas.h:57-63). */
+int alps_ed_matvec(alps_ed_operator op, alps_ed_vector x, alps_ed_vector y);
+
+/* One ietl::lanczos::make_step on caller vectors (lanczos.h:392-406), fused:
+ * w = H v_cur; alpha = <v_cur,w>; w -= alpha v_cur + beta_prev v_prev;
+ * beta = |w|; w /= beta.  v_prev and v_cur are not modified.  Used by the
+ * device vector space of include/alps_ed/ietl_gpu.hpp. */
+int alps_ed_lanczos_step_vectors(alps_ed_operator op, alps_ed_vector v_prev, alps_ed_vector v_cur,
+                                 alps_ed_vector w, double beta_prev, double* alpha, double* beta);
+
+/* Lanczos recurrence with the Krylov state resident on the device
+ * (ietl::lanczos::make_first_step / make_step / generate_tmatrix,
+ * src/ietl/lanczos.h:341-406).  Storage dtype applies to the three Lanczos
+ * vectors; alpha, beta and the normalisations are FP64 on the device. ----- */
+
+int alps_ed_lanczos_create(alps_ed_operator op, alps_ed_dtype dtype, int max_steps, alps_ed_lanczos* lz);
+int alps_ed_lanczos_destroy(alps_ed_lanczos lz);
+/* make_first_step with the start vector given by fill_random(seed, -1, 1),
+ * or by `start` when it is not NULL (must have the operator's dimension). */
+int alps_ed_lanczos_start(alps_ed_lanczos lz, uint64_t seed, alps_ed_vector start);
+/* nsteps more make_step calls, enqueued without host synchronization. */
+int alps_ed_lanczos_steps(alps_ed_lanczos lz, int nsteps);
+/* Copies alpha[0..n), beta[0..n) with n = steps done so far (synchronizes). */
+int alps_ed_lanczos_coefficients(alps_ed_lanczos lz, double* alpha, double* beta, int max_n, int* n);
+/* The current Lanczos vector v_n (normalized) and its predecessor, copied
+ * into caller vectors of the same dimension. */
+int alps_ed_lanczos_vectors(alps_ed_lanczos lz, alps_ed_vector v_prev, alps_ed_vector v_cur);
+
+/* T-matrix analysis (ietl::Tmatrix::compute, src/ietl/tmatrix.h:121-228):
+ * distinct non-ghost eigenvalues of the n x n tridiagonal with diagonal
+ * alpha and off-diagonal beta[0..n-2], their error bounds |beta[n-1]*z_n|
+ * and multiplicities.  Host computation, no device work. */
+int alps_ed_tmatrix_analyze(int n, const double* alpha, const double* beta, double* eigenvalues,
+                            double* errors, int* multiplicities, int* n_out);
+
+/* SparseDiagMatrix<double>::do_subspace (sparsediag.h:93-149) for one
+ * sector: builds the operator, runs the Lanczos recurrence on the device
+ * with the lanczos_iteration_nlowest stopping rule (iteration.h:107-134;
+ * rtol = atol = 100*eps) or a fixed iteration count, and returns the lowest
+ * `nev` distinct non-ghost eigenvalues.  max_iter <= 0 selects the
+ * reference default min(10*D, 1000); D == 1 returns H(0,0). */
+typedef struct {
+  int max_iter;
+  int nev;
+  int fixed_iterations;   /* fixed_lanczos_iteration semantics when non-zero */
+  uint64_t seed;
+  alps_ed_dtype dtype;    /* storage of the Lanczos vectors */
+} alps_ed_solve_params;
+
+typedef struct {
+  uint64_t dim;
+  int n_eigenvalues;
+  double eigenvalues[64];
+  int iterations;
+  int error_code;         /* 1 when max_iter was reached (iteration.h:80) */
+  double seconds;         /* device wall time of the solve */
+} alps_ed_solve_result;
+
+int alps_ed_sparsediag(const alps_ed_model* model, const alps_ed_solve_params* params,
+                       alps_ed_solve_result* result);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ALPS_ED_H */

@@ -1,0 +1,218 @@
+"""GPU parity: the gfx950 engine (through its C ABI) against the CPU oracle.
+
+Bars: bit-exact for basis states, indices and matrix elements; FP64 H.v to
+1e-13 relative; FP32-storage H.v to 2e-6 relative; eigenvalues to 1e-10
+relative (the reference's own ED comparisons use
+max(1e-11, 1e-11|ref|), lib/pyalps/apptest.py:388)."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import alps_amd as A
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def to_product(m: O.Model, n_up) -> A.SpinModel:
+    return A.SpinModel(m.n_sites, m.bond_s, m.bond_t, m.jz, m.jxy, n_up, k=m.k, h=m.h, d=m.d)
+
+
+def random_xxz(L, periodic, seed, field=True):
+    rng = np.random.default_rng(seed)
+    nb = L if periodic else L - 1
+    s = np.arange(nb)
+    J = rng.uniform(0.5, 1.5, nb)
+    delta = rng.uniform(0, 2)
+    return O.Model(L, s, (s + 1) % L, delta * J, J, h=rng.uniform(0, 0.5) if field else 0.0)
+
+
+def general_model(seed):
+    rng = np.random.default_rng(seed)
+    N, s, t, _ = O.lattice("square lattice", 4, 3)
+    nb = s.size
+    return O.Model(N, s, t, rng.uniform(-1, 1, nb), rng.uniform(-1, 1, nb), k=rng.uniform(-0.5, 0.5, nb),
+                   h=rng.uniform(-0.5, 0.5, N), d=rng.uniform(-0.5, 0.5, N))
+
+
+CASES = {
+    "heis_chain12": (lambda: O.lattice_model("chain lattice", 12), [0, 3, 6, 9, 12]),
+    "heis_chain16": (lambda: O.lattice_model("chain lattice", 16), [8, 5]),
+    "xxz_open13": (lambda: random_xxz(13, False, 1), [6, 4]),
+    "xxz_ring14": (lambda: random_xxz(14, True, 2), [7, 3]),
+    "j1j2_4x4": (lambda: O.lattice_model("frustrated square lattice", 4, 4, by_type={1: (0.5, 0.5)}), [8, 7]),
+    "ladder7": (lambda: O.lattice_model("ladder", 7), [7]),
+    "general_4x3": (lambda: general_model(5), [6, 5, 0, 1]),
+    "ring2": (lambda: O.lattice_model("chain lattice", 2), [1]),
+}
+
+
+def case_params():
+    for name, (mk, sectors) in CASES.items():
+        for k in sectors:
+            yield pytest.param(name, k, id=f"{name}-k{k}")
+
+
+@pytest.mark.parametrize("name,k", list(case_params()))
+def test_states_and_elements_bit_exact(name, k):
+    say("case", name, k)
+    om = CASES[name][0]()
+    op = A.Operator(to_product(om, k))
+    want = O.basis(om.n_sites, k)
+    assert op.dim == len(want)
+    assert np.array_equal(op.states(), want)
+    assert np.array_equal(op.index(want), np.arange(len(want), dtype=np.uint64))
+    rp, col, val = op.elements()
+    orp, ocol, oval = O.build_csr(om, k)
+    assert np.array_equal(rp, orp)
+    assert np.array_equal(col, ocol)
+    assert np.array_equal(val.view(np.int64), oval.view(np.int64)), "matrix elements differ bitwise"
+
+
+@pytest.mark.parametrize("name,k", list(case_params()))
+def test_matvec_vs_oracle(name, k):
+    say("case", name, k)
+    om = CASES[name][0]()
+    op = A.Operator(to_product(om, k))
+    rng = np.random.default_rng(k + 17)
+    x = rng.uniform(-1, 1, op.dim)
+    y_ref = O.spmv(om, k, x)
+    nrm = max(np.linalg.norm(y_ref), 1e-300)
+    y64 = op.matvec(A.Vector.from_numpy(x, "f64"), A.Vector(op.dim, "f64")).numpy()
+    assert np.linalg.norm(y64 - y_ref) <= 1e-13 * nrm
+    x32 = A.Vector.from_numpy(x, "f32")
+    y32 = op.matvec(x32, A.Vector(op.dim, "f32")).numpy()
+    y_ref32 = O.spmv(om, k, x32.numpy())  # same FP32-rounded input
+    assert np.linalg.norm(y32 - y_ref32) <= 2e-6 * max(np.linalg.norm(y_ref32), 1e-300)
+
+
+def test_index_rejects_states_outside_the_sector():
+    op = A.Operator(to_product(O.lattice_model("chain lattice", 10), 5))
+    bad = np.array([0, 0b1111, 0b11111 << 5 | 1, 1 << 10 | 0b1111], dtype=np.uint32)
+    assert np.all(op.index(bad) == op.dim)
+
+
+def test_full_size_l24_rows_and_symmetry():
+    """C2 shape (L=24, Sz=0, D=2,704,156): sampled rows of the FP64 product
+    against oracle rows built by binary-search lookup; basis round trip;
+    <x,Hy> = <Hx,y>."""
+    L, k = 24, 12
+    om = O.lattice_model("chain lattice", L)
+    op = A.Operator(to_product(om, k))
+    assert op.dim == math.comb(24, 12) == 2704156
+    rng = np.random.default_rng(24)
+    x = rng.uniform(-1, 1, op.dim)
+    X = A.Vector.from_numpy(x, "f64")
+    Y = op.matvec(X, A.Vector(op.dim, "f64"))
+    y = Y.numpy()
+    say("matvec done")
+    for r0 in (0, 1234567, op.dim - 3000):
+        rp, col, val = O.build_csr(om, k, r0, r0 + 3000)
+        yr = np.array([val[rp[i]:rp[i + 1]] @ x[col[rp[i]:rp[i + 1]]] for i in range(3000)])
+        np.testing.assert_allclose(y[r0:r0 + 3000], yr, rtol=1e-13, atol=1e-13)
+        st = op.states(r0, 3000)
+        assert np.array_equal(op.index(st), np.arange(r0, r0 + 3000, dtype=np.uint64))
+        assert np.all(np.diff(st.astype(np.int64)) > 0)
+        say("rows", r0, "ok")
+    z = rng.uniform(-1, 1, op.dim)
+    Z = A.Vector.from_numpy(z, "f64")
+    HZ = op.matvec(Z, A.Vector(op.dim, "f64"))
+    lhs, rhs = X.dot(HZ), Z.dot(Y)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_device_lanczos_coefficients_vs_oracle():
+    om = O.lattice_model("chain lattice", 14)
+    op = A.Operator(to_product(om, 7))
+    ref = O.sparsediag(om, 7, max_iter=40, fixed=True, seed=4)
+    lz = A.Lanczos(op, "f64", max_steps=40)
+    lz.start(seed=4)
+    lz.steps(39)
+    a, b = lz.coefficients()
+    assert len(a) == 40
+    np.testing.assert_allclose(a[:25], ref["alpha"][:25], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(b[:25], ref["beta"][:25], rtol=1e-9, atol=1e-9)
+
+
+def test_fused_step_on_caller_vectors_matches_generic_sequence():
+    om = random_xxz(12, True, 8)
+    op = A.Operator(to_product(om, 6))
+    n = op.dim
+    rng = np.random.default_rng(3)
+    v0 = rng.uniform(-1, 1, n)
+    v0 /= np.linalg.norm(v0)
+    v1 = rng.uniform(-1, 1, n)
+    v1 -= (v1 @ v0) * v0
+    v1 /= np.linalg.norm(v1)
+    b0 = 0.37
+    w = O.spmv(om, 6, v1)
+    a_ref = v1 @ w
+    w = w - a_ref * v1 - b0 * v0
+    b_ref = np.linalg.norm(w)
+    P, C, W = A.Vector.from_numpy(v0), A.Vector.from_numpy(v1), A.Vector(n, "f64")
+    a, b = A.lanczos_step_vectors(op, P, C, W, b0)
+    assert abs(a - a_ref) < 1e-12 and abs(b - b_ref) < 1e-12
+    np.testing.assert_allclose(W.numpy(), w / b_ref, atol=1e-12)
+    np.testing.assert_array_equal(C.numpy(), v1)  # inputs untouched
+
+
+def test_recurrence_is_deterministic():
+    op = A.Operator(to_product(O.lattice_model("chain lattice", 20), 10))
+    runs = []
+    for _ in range(2):
+        lz = A.Lanczos(op, "f32", max_steps=30)
+        lz.start(seed=11)
+        lz.steps(29)
+        runs.append(lz.coefficients())
+    assert np.array_equal(runs[0][0].view(np.int64), runs[1][0].view(np.int64))
+    assert np.array_equal(runs[0][1].view(np.int64), runs[1][1].view(np.int64))
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_sparsediag_l16_fixed200(dtype):
+    """L=16 Heisenberg ring, Sz=0 (D=12,870), 200 Lanczos iterations,
+    lowest eigenvalue and the next distinct one."""
+    om = O.lattice_model("chain lattice", 16)
+    ref = O.sparsediag(om, 8, max_iter=200, nev=2, fixed=True, seed=1)
+    got = A.sparsediag(to_product(om, 8), max_iter=200, nev=2, fixed=True, seed=1, dtype=dtype)
+    say(dtype, got["ev"], ref["ev"], got["iterations"])
+    assert got["dim"] == 12870 and got["iterations"] == 200
+    np.testing.assert_allclose(got["ev"], ref["ev"], rtol=1e-10, atol=0)
+
+
+@pytest.mark.parametrize("name,k", [("heis_chain16", 8), ("j1j2_4x4", 8), ("xxz_ring14", 7), ("general_4x3", 6)])
+def test_sparsediag_nlowest_fp64_vs_oracle(name, k):
+    om = CASES[name][0]()
+    ref = O.sparsediag(om, k, nev=3, seed=2)
+    got = A.sparsediag(to_product(om, k), nev=3, seed=2, dtype="f64")
+    say(name, got["ev"], ref["ev"], got["iterations"], ref["iterations"])
+    np.testing.assert_allclose(got["ev"], ref["ev"], rtol=1e-10, atol=1e-12)
+
+
+def test_sparsediag_edge_dimensions():
+    m = to_product(O.lattice_model("chain lattice", 6, h=0.3), 0)
+    r = A.sparsediag(m)
+    assert r["dim"] == 1 and r["iterations"] == 0 and r["ev"][0] == 1.5 + 6 * 0.15
+    r = A.sparsediag(m.with_sector(7))
+    assert r["dim"] == 0 and len(r["ev"]) == 0
+
+
+def test_sparsediag_l24_fp32_vs_fp64():
+    """C2: L=24 Heisenberg ring, Sz=0 (D=2,704,156), lowest 4 distinct
+    eigenvalues; FP32-stored Lanczos vectors against the FP64 run."""
+    m = to_product(O.lattice_model("chain lattice", 24), 12)
+    r64 = A.sparsediag(m, nev=4, seed=1, dtype="f64")
+    r32 = A.sparsediag(m, nev=4, seed=1, dtype="f32")
+    say("f64", r64["ev"].tolist(), r64["iterations"], r64["seconds"])
+    say("f32", r32["ev"].tolist(), r32["iterations"], r32["seconds"])
+    assert r64["error_code"] == 0
+    rel = np.abs(r32["ev"] - r64["ev"]) / np.abs(r64["ev"])
+    say("rel err", rel.tolist())
+    assert rel[0] <= 1e-10

@@ -1,0 +1,348 @@
+"""Pins of the CPU oracle (oracle/ed_oracle.hpp) against the reference's own
+golden files, LAPACK dstev fixtures, closed forms and independent
+constructions.  Each test names the oracle function it pins and the
+mistake it would catch."""
+import json
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+# ---------------------------------------------------------------------------
+# basis order (build_basis, state_bits) -- basisstates.h:176-278
+# ---------------------------------------------------------------------------
+def _parse_states(path):
+    rows = []
+    for line in open(path):
+        line = line.strip()
+        if line.startswith("["):
+            sz = re.findall(r"\|1/2 (-?1/2) >", line)
+            rows.append([1 if s == "1/2" else 0 for s in sz])
+    return rows
+
+
+def _bits(row):
+    n = len(row)
+    return sum(1 << (n - 1 - s) for s, up in enumerate(row) if up)
+
+
+def test_basis_order_fixed_sz_example12():
+    # test/model/example12: square lattice L=2 (4 sites), Sz_total=0
+    rows = _parse_states(os.path.join(GOLDEN, "model_example12.output"))
+    assert len(rows) == 6
+    assert [_bits(r) for r in rows] == list(O.basis(4, 2))
+
+
+def test_basis_order_full_example10():
+    # test/model/example10: no constraint, all 16 states; restricted to each
+    # popcount the order must be the oracle's fixed-Sz order
+    rows = _parse_states(os.path.join(GOLDEN, "model_example10.output"))
+    bits = [_bits(r) for r in rows]
+    assert bits == sorted(bits) == list(range(16))
+    for k in range(5):
+        assert [b for b in bits if bin(b).count("1") == k] == list(O.basis(4, k))
+
+
+def test_basis_rank_is_colex():
+    # the index of a state is sum_m C(p_m, m) over its set bits (ascending)
+    for N, k in [(6, 3), (8, 2), (9, 5), (10, 5)]:
+        states = O.basis(N, k)
+        assert len(states) == math.comb(N, k)
+        for i, s in enumerate(states):
+            p = [b for b in range(N) if s >> b & 1]
+            assert sum(math.comb(pm, m + 1) for m, pm in enumerate(p)) == i
+
+
+# ---------------------------------------------------------------------------
+# local matrix elements (site_element, bond_element) -- example3 / example4
+# ---------------------------------------------------------------------------
+def _eval_block(text, name, env):
+    m = re.search(name + r" =\s*\n(.*?);", text, re.S)
+    expr = m.group(1).replace("{", "[").replace("}", "]")
+    return np.array(eval(expr, {"__builtins__": {}}, env), dtype=float)
+
+
+def _two_site_model(env):
+    return O.Model(2, [0], [1], env["Jz0"], env["Jxy0"], k=env["K0"], h=env["h0"], d=env["D0"])
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_two_site_elements_example3_example4(seed):
+    rng = np.random.default_rng(seed)
+    env = {n: float(rng.uniform(-2, 2)) for n in ["Jz0", "Jxy0", "K0", "h0", "D0"]}
+    env["Gamma0"] = 0.0  # Sz-conserving model
+    t3 = open(os.path.join(GOLDEN, "model_example3.output")).read()
+    t4 = open(os.path.join(GOLDEN, "model_example4.output")).read()
+    site = _eval_block(t3, "HSpinSite", env)          # [is][js]
+    bond = _eval_block(t3, "HSpinBond", env)          # [i1][i2][j1][j2]
+    full4 = _eval_block(t4, "HSpinHalf", env)         # two sites, one bond
+    # example3's local matrices assembled for a dimer must equal example4
+    asm = np.zeros((4, 4))
+    for i1 in range(2):
+        for i2 in range(2):
+            for j1 in range(2):
+                for j2 in range(2):
+                    v = bond[i1, i2, j1, j2]
+                    if i2 == j2:
+                        v += site[i1, j1]
+                    if i1 == j1:
+                        v += site[i2, j2]
+                    asm[2 * i1 + i2, 2 * j1 + j2] = v
+    np.testing.assert_allclose(asm, full4, rtol=1e-12, atol=1e-12)
+    # the oracle's sector matrices are the diagonal blocks of HSpinHalf
+    m = _two_site_model(env)
+    for k, idx in [(0, [0]), (1, [1, 2]), (2, [3])]:
+        A = O.dense(m, k)
+        np.testing.assert_allclose(A, full4[np.ix_(idx, idx)], rtol=1e-13, atol=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# lattice graphs (named_lattice) -- test/lattice/example6
+# ---------------------------------------------------------------------------
+def test_lattice_bonds_example6():
+    want = [tuple(map(int, m)) for m in re.findall(r"between site (\d+) and site (\d+) has type (\d+)",
+                                                    open(os.path.join(GOLDEN, "lattice_example6.output")).read())]
+    N, s, t, ty = O.lattice("anisotropic square lattice", 10, 4)
+    assert N == 40
+    assert list(zip(s.tolist(), t.tolist(), ty.tolist())) == want
+
+
+def test_lattice_shapes():
+    # bond counts and boundary handling of the one-vertex unit cells
+    assert O.lattice("chain lattice", 8)[1].size == 8
+    assert O.lattice("open chain lattice", 8)[1].size == 7
+    assert O.lattice("chain lattice", 1)[1].size == 0          # source == target dropped
+    N, s, t, ty = O.lattice("frustrated square lattice", 6, 5)
+    assert N == 30 and s.size == 120 and (ty == 0).sum() == 60 and (ty == 1).sum() == 60
+    # site index x*W + y; the first cell's four edges in unit-cell order
+    assert list(zip(s[:4], t[:4], ty[:4])) == [(0, 1, 0), (0, 5, 0), (0, 6, 1), (0, 9, 1)]
+    N, s, t, ty = O.lattice("ladder", 6)
+    assert N == 12 and s.size == 12 + 6                          # W defaults to 2, rungs open
+    N, s, t, ty = O.lattice("open square lattice", 3, 4)
+    assert N == 12 and s.size == 2 * 12 - 3 - 4
+
+
+# ---------------------------------------------------------------------------
+# tridiagonal eigensolver (tridiag_eig) -- LAPACK dstev fixtures
+# ---------------------------------------------------------------------------
+def test_tridiag_against_dstev_fixtures():
+    cases = json.load(open(os.path.join(GOLDEN, "dstev_cases.json")))
+    for c in cases:
+        d, e = np.array(c["d"]), np.array(c["e"])
+        w_ref = np.array(c["eigenvalues"])
+        w, z = O.tridiag(d, e, vectors=True)
+        scale = max(1.0, np.abs(w_ref).max())
+        np.testing.assert_allclose(w, w_ref, atol=1e-13 * scale, rtol=0, err_msg=c["label"])
+        # eigenvectors are orthonormal and satisfy T z = w z
+        T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+        np.testing.assert_allclose(z.T @ z, np.eye(len(d)), atol=1e-12)
+        np.testing.assert_allclose(T @ z, z * w, atol=1e-12 * scale)
+        gaps = np.diff(w_ref)
+        if len(gaps) == 0 or gaps.min() > 1e-8:  # last row unique up to sign
+            np.testing.assert_allclose(np.abs(z[-1]), c["zlast_abs"], atol=1e-10, err_msg=c["label"])
+
+
+def test_tridiag_toeplitz_closed_form():
+    n = 30
+    w = O.tridiag(np.full(n, 0.25), np.full(n - 1, -0.75))
+    k = np.arange(1, n + 1)
+    np.testing.assert_allclose(w, np.sort(0.25 - 1.5 * np.cos(k * np.pi / (n + 1))), atol=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# matrix build (build_matrix) -- independent constructions
+# ---------------------------------------------------------------------------
+_SZ = np.diag([-0.5, 0.5])      # local index 0 = Sz -1/2
+_SP = np.array([[0, 0], [1, 0.]])  # S+ : |0> -> |1>
+_SM = _SP.T
+
+
+def _site_op(op, s, N):
+    mats = [np.eye(2)] * N
+    mats[s] = op
+    out = mats[0]
+    for m in mats[1:]:
+        out = np.kron(out, m)
+    return out
+
+
+def kron_hamiltonian(m: O.Model):
+    """Dense H as a sum of Kronecker products of local operators (site 0 the
+    most significant factor), then restricted per sector."""
+    N = m.n_sites
+    H = np.zeros((2 ** N, 2 ** N))
+    for s in range(N):
+        H += -m.h[s] * _site_op(_SZ, s, N) + m.d[s] * _site_op(_SZ @ _SZ, s, N)
+    for b in range(m.n_bonds):
+        i, j = m.bond_s[b], m.bond_t[b]
+        SzSz = _site_op(_SZ, i, N) @ _site_op(_SZ, j, N)
+        ex = 0.5 * (_site_op(_SP, i, N) @ _site_op(_SM, j, N) + _site_op(_SM, i, N) @ _site_op(_SP, j, N))
+        SS = SzSz + ex
+        H += m.jz[b] * SzSz + m.jxy[b] * ex + m.k[b] * (SS @ SS)
+    return H
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_matrix_matches_kronecker_construction(seed):
+    rng = np.random.default_rng(seed)
+    N = int(rng.integers(4, 9))
+    nb = int(rng.integers(N, 2 * N + 1))
+    s = rng.integers(0, N, nb)
+    t = (s + rng.integers(1, N, nb)) % N      # distinct ends, repeats allowed
+    m = O.Model(N, s, t, rng.uniform(-1.5, 1.5, nb), rng.uniform(-1.5, 1.5, nb), k=rng.uniform(-1, 1, nb),
+                h=rng.uniform(-1, 1, N), d=rng.uniform(-1, 1, N))
+    H = kron_hamiltonian(m)
+    for k in range(N + 1):
+        idx = O.basis(N, k).astype(np.int64)
+        np.testing.assert_allclose(O.dense(m, k), H[np.ix_(idx, idx)], atol=1e-13, rtol=0)
+
+
+def test_matrix_ising_limit_bruteforce():
+    # Jxy = K = 0: H is diagonal with E(s) = sum_b Jz/4 (+-1) - sum_s h Sz
+    rng = np.random.default_rng(7)
+    N, s, t, _ = O.lattice("open square lattice", 3, 3)
+    jz = rng.uniform(-1, 1, s.size)
+    h = rng.uniform(-0.5, 0.5, N)
+    m = O.Model(N, s, t, jz, 0.0, h=h)
+    for k in (3, 4, 5):
+        st = O.basis(N, k)
+        up = ((st[:, None] >> (N - 1 - np.arange(N))[None, :]) & 1).astype(float)
+        sz = up - 0.5
+        E = (jz[None, :] * sz[:, s] * sz[:, t]).sum(1) - (h[None, :] * sz).sum(1)
+        A = O.dense(m, k)
+        np.testing.assert_allclose(np.diag(A), E, atol=1e-14)
+        assert np.count_nonzero(A - np.diag(np.diag(A))) == 0
+
+
+def _free_fermion_e0(L, J, M, periodic, h=0.0):
+    T = np.zeros((L, L))
+    for i in range(L - 1):
+        T[i, i + 1] = T[i + 1, i] = J[i] / 2
+    if periodic and L > 2:
+        sgn = (-1) ** (M - 1)  # Jordan-Wigner boundary sign
+        T[L - 1, 0] = T[0, L - 1] = sgn * J[L - 1] / 2
+    eps = np.linalg.eigvalsh(T)
+    return eps[:M].sum() - h * (M - L / 2)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_xx_chain_free_fermions(periodic):
+    rng = np.random.default_rng(11 + periodic)
+    for L in (6, 8, 10, 11):
+        J = rng.uniform(0.5, 1.5, L)
+        h = float(rng.uniform(0, 0.5))
+        nb = L if periodic else L - 1
+        s = np.arange(nb)
+        t = (s + 1) % L
+        m = O.Model(L, s, t, 0.0, J[:nb], h=h)
+        for M in range(0, L + 1):
+            e0 = np.linalg.eigvalsh(O.dense(m, M))[0]
+            assert abs(e0 - _free_fermion_e0(L, J, M, periodic, h)) < 1e-11, (L, M)
+
+
+# ---------------------------------------------------------------------------
+# Lanczos + Tmatrix (Lanczos, Tmatrix, Iteration, do_subspace)
+# ---------------------------------------------------------------------------
+def _distinct(w, tol=1e-9):
+    out = [w[0]]
+    for x in w[1:]:
+        if x - out[-1] > tol:
+            out.append(x)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("name,L,W,k", [("chain lattice", 12, 0, 6), ("chain lattice", 11, 0, 5),
+                                        ("square lattice", 3, 4, 6), ("frustrated square lattice", 4, 3, 6),
+                                        ("ladder", 6, 0, 6)])
+def test_lanczos_nlowest_matches_dense(name, L, W, k):
+    m = O.lattice_model(name, L, W, by_type={1: (0.5, 0.5)} if "frustrated" in name else None)
+    w = _distinct(np.linalg.eigvalsh(O.dense(m, k)))
+    r = O.sparsediag(m, k, nev=4, seed=3)
+    assert r["error_code"] == 0
+    np.testing.assert_allclose(r["ev"], w[:4], rtol=1e-10, atol=1e-10)
+
+
+def test_lanczos_ring_of_four():
+    # Heisenberg ring L=4, J=1: ground state -2J (singlet), then -1 (triplet)
+    r = O.sparsediag(O.lattice_model("chain lattice", 4), 2, nev=2)
+    np.testing.assert_allclose(r["ev"], [-2.0, -1.0], atol=1e-14)
+
+
+def test_lanczos_three_term_relation():
+    # alpha/beta from the recurrence reproduce H's spectrum extremes and the
+    # first step's alpha is the Rayleigh quotient of the start vector
+    m = O.lattice_model("chain lattice", 10)
+    A = O.dense(m, 5)
+    r = O.sparsediag(m, 5, max_iter=40, fixed=True, seed=5)
+    D = A.shape[0]
+    v0 = np.array([O.start_value(5, i) for i in range(D)])
+    v0 /= np.linalg.norm(v0)
+    assert abs(r["alpha"][0] - v0 @ A @ v0) < 1e-12
+    w1 = A @ v0 - r["alpha"][0] * v0
+    assert abs(r["beta"][0] - np.linalg.norm(w1)) < 1e-12
+    assert r["iterations"] == 40 and len(r["alpha"]) == 40 and r["error_code"] == 1
+
+
+def test_start_vector_splitmix64_known_answer():
+    # first output of splitmix64 from state 0 is 0xe220a8397b1dcdaf
+    z = 0xE220A8397B1DCDAF
+    u = (z >> 11) * 2.0 ** -53
+    assert O.start_value(0, 0, 0.0, 1.0) == u
+    assert O.start_value(0, 0, -1.0, 1.0) == -1.0 + 2.0 * u
+
+
+def _sym2(a, b, c):
+    # eigenpairs of [[a, b], [b, c]]
+    w, v = np.linalg.eigh(np.array([[a, b], [b, c]]))
+    return w, v
+
+
+def test_tmatrix_two_by_two_closed_form():
+    a = [0.3, -1.1]
+    b = [0.7, 0.25]
+    w, v = _sym2(a[0], b[0], a[1])
+    t = O.tmatrix(a, b)
+    np.testing.assert_allclose(t["eig"], w, atol=1e-15)
+    np.testing.assert_allclose(t["err"], np.abs(b[1] * v[1, :]), atol=1e-15)
+    assert list(t["mult"]) == [1, 1]
+
+
+def test_tmatrix_ghost_scan_bound():
+    # T = [a0] (+) 2x2 block, coupled by a negligible beta0: the lowest
+    # eigenvalue of T minus its first row/column coincides with the block's
+    # lower eigenvalue, which is flagged a ghost; the upper one is outside
+    # the scan (j < n-2) and survives.
+    a = [-3.0, 0.5, 1.5]
+    b = [1e-30, 0.4, 0.3]
+    wb, vb = _sym2(a[1], b[1], a[2])
+    t = O.tmatrix(a, b)
+    np.testing.assert_allclose(t["eig"], [a[0], wb[1]], atol=1e-15)
+    np.testing.assert_allclose(t["err"], [0.0, abs(b[2] * vb[1, 1])], atol=1e-15)
+
+
+def test_tmatrix_signed_threshold():
+    # all-negative spectrum: thold = 5*multol (signed max), so two values
+    # 1e-10 apart stay distinct; |lambda| * eps^(2/3) would merge them
+    a = [-5.0, -5.0 + 1e-10]
+    b = [1e-30, 0.5]
+    t = O.tmatrix(a, b)
+    assert len(t["eig"]) == 2 and list(t["mult"]) == [1, 1]
+    multol = 0.5 * 2 * np.finfo(float).eps * (10000 + 2)
+    assert abs(t["multol"] - multol) < 1e-20
+    assert abs(t["thold"] - 5 * multol) < 1e-20
+
+
+def test_sparsediag_edge_dimensions():
+    m = O.lattice_model("chain lattice", 6, h=0.3)
+    r = O.sparsediag(m, 0)           # D = 1: H(0,0), no Lanczos
+    assert r["dim"] == 1 and r["iterations"] == 0
+    assert r["ev"][0] == 1.5 + 6 * 0.15   # 6 parallel bonds Jz/4, 6 sites -h*(-1/2)
+    r = O.sparsediag(m, 7)           # D = 0: nothing
+    assert r["dim"] == 0 and len(r["ev"]) == 0
