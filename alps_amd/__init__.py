@@ -65,7 +65,8 @@ def lib():
             "alps_ed_vector_nrm2": ([P, P], i32),
             "alps_ed_matvec": ([P, P, P], i32),
             "alps_ed_lanczos_step_vectors": ([P, P, P, P, f64, P, P], i32),
-            "alps_ed_lanczos_create": ([P, i32, i32, P], i32),
+            "alps_ed_lanczos_create": ([P, i32, i32, i32, P], i32),
+            "alps_ed_lanczos_ritz": ([P, i32, i32, P, P], i32),
             "alps_ed_lanczos_destroy": ([P], i32),
             "alps_ed_lanczos_start": ([P, u64, P], i32),
             "alps_ed_lanczos_steps": ([P, i32], i32),
@@ -285,11 +286,11 @@ def lanczos_step_vectors(op, v_prev, v_cur, w, beta_prev):
 class Lanczos:
     """Device-resident Lanczos recurrence (alps_ed_lanczos)."""
 
-    def __init__(self, op: Operator, dtype="f32", max_steps=1000):
+    def __init__(self, op: Operator, dtype="f32", max_steps=1000, keep_basis=False):
         self.op = op
         self.max_steps = max_steps
         h = ctypes.c_void_p()
-        _check(lib().alps_ed_lanczos_create(op.handle, _DTYPES[dtype], max_steps, ctypes.byref(h)))
+        _check(lib().alps_ed_lanczos_create(op.handle, _DTYPES[dtype], max_steps, int(keep_basis), ctypes.byref(h)))
         self.handle = h
 
     def close(self):
@@ -311,6 +312,15 @@ class Lanczos:
         n = ctypes.c_int()
         _check(lib().alps_ed_lanczos_coefficients(self.handle, _p(a), _p(b), self.max_steps, ctypes.byref(n)))
         return a[: n.value].copy(), b[: n.value].copy()
+
+    def ritz(self, coeffs):
+        """Ritz vectors sum_j coeffs[i, j] v_j over the kept basis."""
+        c = np.ascontiguousarray(np.atleast_2d(coeffs), dtype=np.float64)
+        nvec, m = c.shape
+        outs = [Vector(self.op.dim, "f64") for _ in range(nvec)]
+        arr = (ctypes.c_void_p * max(nvec, 1))(*[o.handle for o in outs])
+        _check(lib().alps_ed_lanczos_ritz(self.handle, nvec, m, _p(c), arr))
+        return outs
 
     def vectors(self, dtype="f64"):
         vp, vc = Vector(self.op.dim, dtype), Vector(self.op.dim, dtype)

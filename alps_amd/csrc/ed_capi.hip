@@ -105,36 +105,27 @@ void upload(DevBuf& b, const std::vector<T>& v) {
   if (!v.empty()) hip_check(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
 }
 
-// Scratch for grid reductions: partial sums, two counters, result slots.
+// Scratch for reductions: two partial-sum arrays (one per producer kind,
+// so a consumer never reads an array that is being rewritten) and result
+// slots.  Partial counts follow the producing kernel's grid.
 struct Reducer {
-  DevBuf partials, counters, results;
-  size_t n_partials = 0;
+  DevBuf pa, pb, results;
+  size_t cap = 0;
   void ensure(size_t nblocks) {
-    if (nblocks <= n_partials) return;
-    partials.alloc(nblocks * sizeof(double));
-    n_partials = nblocks;
-    if (!counters.p) {
-      counters.alloc(4 * sizeof(unsigned));
-      hip_check(hipMemset(counters.p, 0, 4 * sizeof(unsigned)), "hipMemset");
-      hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
-      results.alloc(8 * sizeof(double));
-    }
+    if (nblocks <= cap) return;
+    pa.alloc(nblocks * sizeof(double));
+    pb.alloc(nblocks * sizeof(double));
+    cap = nblocks;
+    if (!results.p) results.alloc(16 * sizeof(double));
   }
-  GridReduce make(int counter, double* out, double* out2, int mode) {
-    GridReduce r;
-    r.partials = partials.as<double>();
-    r.counter = counters.as<unsigned>() + counter;
-    r.out = out;
-    r.out2 = out2;
-    r.mode = mode;
-    return r;
-  }
+  double* a() { return pa.as<double>(); }
+  double* b() { return pb.as<double>(); }
   double* result(int i) { return results.as<double>() + i; }
 };
 
 Reducer& thread_reducer() {
   thread_local Reducer r;
-  r.ensure(4096);
+  r.ensure(2048);
   return r;
 }
 
@@ -147,8 +138,9 @@ Reducer& thread_reducer() {
 struct alps_ed_operator_s {
   SectorLayout layout;
   DevLayout dev{};
-  DevBuf lo_rank, lo_state, lo_class_off, hi_base, seg_hi, seg_start, tiles, diag_lo, diag_hi, ll, hh,
-      xb, site_dn, site_up, bond_p1, bond_p2, bond_par, bond_anti, bond_t;
+  DevBuf lo_rank, lo_state, lo_class_off, hi_base, seg_hi, seg_start, proc_hi, diag_lo_cls, diag_hi, nbp16,
+      nbp_off, xn16, xn_off, ll_t, hh_mask, hh_t, xb_hbit, xb_par, xb_anti, xb_t, site_dn, site_up, bond_p1,
+      bond_p2, bond_par, bond_anti, bond_t;
   uint64_t bytes = 0;
 };
 
@@ -163,10 +155,23 @@ struct alps_ed_lanczos_s {
   alps_ed_dtype dtype = ALPS_ED_F32;
   int max_steps = 0;
   int steps = 0;  // alpha entries produced
+  bool keep_basis = false;
+  // Lanczos vector u_j (v_j = scale[j] * u_j): ping-pong buffers, or, when the
+  // basis is kept, chunks of kChunk consecutive vectors allocated on demand.
+  static constexpr int kChunk = 32;
   DevBuf u[2], w;
+  std::vector<std::unique_ptr<DevBuf>> chunks;
   DevBuf alpha, beta, scale;  // FP64 on device
   Reducer red;
   hipStream_t stream = nullptr;
+
+  size_t vec_bytes() const { return op->layout.D * (dtype == ALPS_ED_F32 ? 4 : 8); }
+  void* slot(int j) {
+    if (!keep_basis) return u[j & 1].p;
+    const size_t c = (size_t)j / kChunk;
+    while (chunks.size() <= c) chunks.emplace_back(new DevBuf(vec_bytes() * kChunk));
+    return static_cast<char*>(chunks[c]->p) + (size_t)(j % kChunk) * vec_bytes();
+  }
 };
 
 namespace {
@@ -192,61 +197,56 @@ ModelSpec model_spec(const alps_ed_model* m) {
 
 void operator_upload(alps_ed_operator op) {
   SectorLayout& L = op->layout;
-  upload(op->lo_rank, L.lo_rank);
-  upload(op->lo_state, L.lo_state);
-  upload(op->lo_class_off, L.lo_class_off);
-  upload(op->hi_base, L.hi_base);
-  upload(op->seg_hi, L.seg_hi);
-  upload(op->seg_start, L.seg_start);
-  upload(op->tiles, L.tiles);
-  upload(op->diag_lo, L.diag_lo);
-  upload(op->diag_hi, L.diag_hi);
-  upload(op->ll, L.ll);
-  upload(op->hh, L.hh);
-  upload(op->xb, L.xb);
-  upload(op->site_dn, L.site_dn);
-  upload(op->site_up, L.site_up);
-  upload(op->bond_p1, L.bond_p1);
-  upload(op->bond_p2, L.bond_p2);
-  upload(op->bond_par, L.bond_par);
-  upload(op->bond_anti, L.bond_anti);
-  upload(op->bond_t, L.bond_t);
   DevLayout& d = op->dev;
+  d = DevLayout{};
   d.N = L.N;
   d.k = L.k;
   d.B = L.B;
   d.NH = L.NH;
   d.D = (uint32_t)L.D;
   d.n_seg = (uint32_t)L.seg_hi.size();
-  d.n_tiles = (uint32_t)(L.tiles.size() / 2);
-  d.n_ll = (int)L.ll.size();
-  d.n_hh = (int)L.hh.size();
-  d.n_xb = (int)L.xb.size();
+  d.max_seg = L.max_seg;
+  d.n_ll_p = L.n_ll_p;
+  d.n_hh_p = L.n_hh_p;
+  d.n_xb_p = L.n_xb_p;
   d.n_bonds = (int)L.bond_p1.size();
-  d.lo_rank = op->lo_rank.as<uint32_t>();
-  d.lo_state = op->lo_state.as<uint32_t>();
-  d.lo_class_off = op->lo_class_off.as<uint32_t>();
-  d.hi_base = op->hi_base.as<uint32_t>();
-  d.seg_hi = op->seg_hi.as<uint32_t>();
-  d.seg_start = op->seg_start.as<uint32_t>();
-  d.tiles = op->tiles.as<uint32_t>();
-  d.diag_lo = op->diag_lo.as<double>();
-  d.diag_hi = op->diag_hi.as<double>();
-  d.ll = op->ll.as<FlipBond>();
-  d.hh = op->hh.as<FlipBond>();
-  d.xb = op->xb.as<CrossBond>();
-  d.site_dn = op->site_dn.as<double>();
-  d.site_up = op->site_up.as<double>();
-  d.bond_p1 = op->bond_p1.as<uint32_t>();
-  d.bond_p2 = op->bond_p2.as<uint32_t>();
-  d.bond_par = op->bond_par.as<double>();
-  d.bond_anti = op->bond_anti.as<double>();
-  d.bond_t = op->bond_t.as<double>();
+#define ALPS_ED_UP(field, T) \
+  upload(op->field, L.field); \
+  d.field = op->field.as<T>();
+  ALPS_ED_UP(lo_rank, uint32_t)
+  ALPS_ED_UP(lo_state, uint32_t)
+  ALPS_ED_UP(lo_class_off, uint32_t)
+  ALPS_ED_UP(hi_base, uint32_t)
+  ALPS_ED_UP(seg_hi, uint32_t)
+  ALPS_ED_UP(seg_start, uint32_t)
+  ALPS_ED_UP(proc_hi, uint32_t)
+  ALPS_ED_UP(diag_lo_cls, double)
+  ALPS_ED_UP(diag_hi, double)
+  ALPS_ED_UP(nbp16, uint16_t)
+  ALPS_ED_UP(nbp_off, uint32_t)
+  ALPS_ED_UP(xn16, uint16_t)
+  ALPS_ED_UP(xn_off, uint32_t)
+  ALPS_ED_UP(ll_t, double)
+  ALPS_ED_UP(hh_mask, uint32_t)
+  ALPS_ED_UP(hh_t, double)
+  ALPS_ED_UP(xb_hbit, uint32_t)
+  ALPS_ED_UP(xb_par, double)
+  ALPS_ED_UP(xb_anti, double)
+  ALPS_ED_UP(xb_t, double)
+  ALPS_ED_UP(site_dn, double)
+  ALPS_ED_UP(site_up, double)
+  ALPS_ED_UP(bond_p1, uint32_t)
+  ALPS_ED_UP(bond_p2, uint32_t)
+  ALPS_ED_UP(bond_par, double)
+  ALPS_ED_UP(bond_anti, double)
+  ALPS_ED_UP(bond_t, double)
+#undef ALPS_ED_UP
   op->bytes = 0;
-  for (DevBuf* b : {&op->lo_rank, &op->lo_state, &op->lo_class_off, &op->hi_base, &op->seg_hi,
-                    &op->seg_start, &op->tiles, &op->diag_lo, &op->diag_hi, &op->ll, &op->hh, &op->xb,
-                    &op->site_dn, &op->site_up, &op->bond_p1, &op->bond_p2, &op->bond_par,
-                    &op->bond_anti, &op->bond_t})
+  for (DevBuf* b : {&op->lo_rank, &op->lo_state, &op->lo_class_off, &op->hi_base, &op->seg_hi, &op->seg_start,
+                    &op->proc_hi, &op->diag_lo_cls, &op->diag_hi, &op->nbp16, &op->nbp_off, &op->xn16, &op->xn_off,
+                    &op->ll_t, &op->hh_mask, &op->hh_t, &op->xb_hbit, &op->xb_par, &op->xb_anti, &op->xb_t,
+                    &op->site_dn, &op->site_up, &op->bond_p1, &op->bond_p2, &op->bond_par, &op->bond_anti,
+                    &op->bond_t})
     op->bytes += b->bytes;
 }
 
@@ -264,28 +264,39 @@ void with_dtype(alps_ed_dtype t, F&& f) {
 template <class V>
 double dot_sync(uint64_t n, const V* x, const V* y) {
   Reducer& r = thread_reducer();
-  hip_check(launch_dot<V>(n, x, y, r.make(0, r.result(0), nullptr, 0), g_stream), "dot");
+  hip_check(launch_dot<V>(n, x, y, r.a(), g_stream), "dot");
+  hip_check(launch_finish(Partials{r.a(), stream_blocks(n)}, nullptr, r.result(0), nullptr, kSum, g_stream), "finish");
   double h = 0.0;
   hip_check(hipMemcpyAsync(&h, r.result(0), sizeof(double), hipMemcpyDeviceToHost, g_stream), "copy");
   hip_check(hipStreamSynchronize(g_stream), "sync");
   return h;
 }
 
-// One Lanczos step of the device-resident recurrence (see alps_ed_lanczos_*).
+// One Lanczos step of the device-resident recurrence (see alps_ed_lanczos_*):
+// w = scale[j] H u_j with alpha_j; r = w - alpha_j v_j - beta_{j-1} v_{j-1}
+// stored as u_{j+1} with scale[j+1] = 1/|r| = 1/beta_j.
 template <class V>
 void lanczos_step_dev(alps_ed_lanczos lz, int j) {
   const DevLayout& L = lz->op->dev;
-  V* cur = lz->u[j & 1].as<V>();
-  V* prev = lz->u[(j + 1) & 1].as<V>();
+  V* cur = static_cast<V*>(lz->slot(j));
+  V* prev = j > 0 ? static_cast<V*>(lz->slot(j - 1)) : cur;
+  V* next = static_cast<V*>(lz->slot(j + 1));
   V* w = lz->w.as<V>();
   double* alpha = lz->alpha.as<double>();
   double* beta = lz->beta.as<double>();
   double* scale = lz->scale.as<double>();
-  hip_check(launch_hv<V>(L, cur, w, scale + j, lz->red.make(0, alpha + j, nullptr, 0), lz->stream), "hv");
-  hip_check(launch_update<V>(L.D, w, cur, prev, prev, alpha + j, scale + j, j > 0 ? beta + j - 1 : nullptr,
-                             j > 0 ? scale + j - 1 : nullptr,
-                             lz->red.make(1, beta + j, scale + j + 1, 1), lz->stream),
+  // w = H u_j; alpha_j = scale_j^2 <u_j, w>
+  hip_check(launch_hv<V>(L, cur, w, lz->red.a(), lz->stream), "hv");
+  hip_check(launch_finish(Partials{lz->red.a(), hv_blocks(L)}, scale + j, alpha + j, nullptr, kAlpha, lz->stream),
+            "finish");
+  // u_{j+1} = scale_j w - alpha_j v_j - beta_{j-1} v_{j-1}
+  hip_check(launch_update<V>(L.D, w, cur, prev, next, alpha + j, scale + j, j > 0 ? beta + j - 1 : nullptr,
+                             j > 0 ? scale + j - 1 : nullptr, lz->red.b(), lz->stream),
             "update");
+  // beta_j = |u_{j+1}|, scale_{j+1} = 1/beta_j
+  hip_check(launch_finish(Partials{lz->red.b(), stream_blocks(L.D)}, nullptr, beta + j, scale + j + 1, kNorm,
+                          lz->stream),
+            "finish");
 }
 
 }  // namespace
@@ -523,7 +534,7 @@ int alps_ed_vector_scale(alps_ed_vector v, double a) {
     check_vec(v);
     with_dtype(v->dtype, [&](auto z) {
       using V = decltype(z);
-      hip_check(launch_axpby<V>(v->n, v->buf.as<V>(), 0.0, nullptr, a, nullptr, nullptr, false, g_stream), "scale");
+      hip_check(launch_axpby<V>(v->n, v->buf.as<V>(), 0.0, nullptr, a, g_stream), "scale");
     });
   });
 }
@@ -535,7 +546,7 @@ int alps_ed_vector_axpby(alps_ed_vector y, double a, alps_ed_vector x, double b)
     require(x->n == y->n && x->dtype == y->dtype, "size or dtype mismatch");
     with_dtype(y->dtype, [&](auto z) {
       using V = decltype(z);
-      hip_check(launch_axpby<V>(y->n, y->buf.as<V>(), a, x->buf.as<V>(), b, nullptr, nullptr, false, g_stream), "axpby");
+      hip_check(launch_axpby<V>(y->n, y->buf.as<V>(), a, x->buf.as<V>(), b, g_stream), "axpby");
     });
   });
 }
@@ -572,10 +583,9 @@ int alps_ed_matvec(alps_ed_operator op, alps_ed_vector x, alps_ed_vector y) {
     require(x->n == op->layout.D && y->n == op->layout.D, "vector size != operator dimension");
     require(x->dtype == y->dtype, "dtype mismatch");
     require(x != y, "in-place product is not supported");
-    GridReduce none{};
     with_dtype(x->dtype, [&](auto z) {
       using V = decltype(z);
-      hip_check(launch_hv<V>(op->dev, x->buf.as<V>(), y->buf.as<V>(), nullptr, none, g_stream), "hv");
+      hip_check(launch_hv<V>(op->dev, x->buf.as<V>(), y->buf.as<V>(), nullptr, g_stream), "hv");
     });
   });
 }
@@ -600,11 +610,12 @@ int alps_ed_lanczos_step_vectors(alps_ed_operator op, alps_ed_vector v_prev, alp
     with_dtype(w->dtype, [&](auto z) {
       using V = decltype(z);
       V* wv = w->buf.as<V>();
-      hip_check(launch_hv<V>(op->dev, v_cur->buf.as<V>(), wv, nullptr, r.make(0, res, nullptr, 0), g_stream), "hv");
-      hip_check(launch_update<V>(n, wv, v_cur->buf.as<V>(), v_prev->buf.as<V>(), wv, res, nullptr, res + 2,
-                                 nullptr, r.make(1, res + 1, res + 3, 1), g_stream),
+      hip_check(launch_hv<V>(op->dev, v_cur->buf.as<V>(), wv, r.a(), g_stream), "hv");
+      hip_check(launch_finish(Partials{r.a(), hv_blocks(op->dev)}, nullptr, res, nullptr, kAlpha, g_stream), "finish");
+      hip_check(launch_update<V>(n, wv, v_cur->buf.as<V>(), v_prev->buf.as<V>(), wv, res, nullptr, res + 2, nullptr,
+                                 r.b(), g_stream),
                 "update");
-      hip_check(launch_axpby<V>(n, wv, 0.0, wv, 0.0, res + 3, nullptr, false, g_stream), "scale");
+      hip_check(launch_normalize<V>(n, wv, Partials{r.b(), stream_blocks(n)}, res + 1, g_stream), "normalize");
     });
     double h[2];
     hip_check(hipMemcpyAsync(h, res, 2 * sizeof(double), hipMemcpyDeviceToHost, g_stream), "copy");
@@ -614,10 +625,12 @@ int alps_ed_lanczos_step_vectors(alps_ed_operator op, alps_ed_vector v_prev, alp
   });
 }
 
+
 // ---------------------------------------------------------------------------
 // device-resident Lanczos
 // ---------------------------------------------------------------------------
-int alps_ed_lanczos_create(alps_ed_operator op, alps_ed_dtype dtype, int max_steps, alps_ed_lanczos* out) {
+int alps_ed_lanczos_create(alps_ed_operator op, alps_ed_dtype dtype, int max_steps, int keep_basis,
+                           alps_ed_lanczos* out) {
   return guarded([&] {
     require(op && out, "null argument");
     require(dtype == ALPS_ED_F32 || dtype == ALPS_ED_F64, "unknown dtype");
@@ -627,11 +640,13 @@ int alps_ed_lanczos_create(alps_ed_operator op, alps_ed_dtype dtype, int max_ste
     lz->op = op;
     lz->dtype = dtype;
     lz->max_steps = max_steps;
+    lz->keep_basis = keep_basis != 0;
     const uint64_t n = op->layout.D;
-    const size_t es = dtype_size(dtype);
-    lz->u[0].alloc(n * es);
-    lz->u[1].alloc(n * es);
-    lz->w.alloc(n * es);
+    if (!lz->keep_basis) {
+      lz->u[0].alloc(lz->vec_bytes());
+      lz->u[1].alloc(lz->vec_bytes());
+    }
+    lz->w.alloc(lz->vec_bytes());
     lz->alpha.alloc((max_steps + 2) * sizeof(double));
     lz->beta.alloc((max_steps + 2) * sizeof(double));
     lz->scale.alloc((max_steps + 3) * sizeof(double));
@@ -653,7 +668,7 @@ int alps_ed_lanczos_start(alps_ed_lanczos lz, uint64_t seed, alps_ed_vector star
     lz->stream = g_stream;
     with_dtype(lz->dtype, [&](auto z) {
       using V = decltype(z);
-      V* u0 = lz->u[0].as<V>();
+      V* u0 = static_cast<V*>(lz->slot(0));
       if (start) {
         require(start->n == n, "start vector size != operator dimension");
         if (start->dtype == lz->dtype)
@@ -665,11 +680,11 @@ int alps_ed_lanczos_start(alps_ed_lanczos lz, uint64_t seed, alps_ed_vector star
       } else {
         hip_check(launch_fill_random<V>(n, u0, seed, -1.0, 1.0, lz->stream), "fill_random");
       }
-      double* scale = lz->scale.as<double>();
-      // scale[0] = 1/|u0|; the norm goes to a scratch slot of beta's tail
-      hip_check(launch_dot<V>(n, u0, u0, lz->red.make(1, lz->beta.as<double>() + lz->max_steps + 1, scale, 1),
-                              lz->stream),
-                "norm");
+      // scale[0] = 1/|u0|; |u0| goes to a spare slot of beta
+      hip_check(launch_dot<V>(n, u0, u0, lz->red.b(), lz->stream), "norm");
+      hip_check(launch_finish(Partials{lz->red.b(), stream_blocks(n)}, nullptr, lz->beta.as<double>() + lz->max_steps + 1,
+                              lz->scale.as<double>(), kNorm, lz->stream),
+                "finish");
       lanczos_step_dev<V>(lz, 0);
     });
     lz->steps = 1;
@@ -681,6 +696,7 @@ int alps_ed_lanczos_steps(alps_ed_lanczos lz, int nsteps) {
     require(lz != nullptr, "null argument");
     if (lz->steps == 0) throw StateError("lanczos_steps called before lanczos_start");
     require(nsteps >= 0 && lz->steps + nsteps <= lz->max_steps, "steps beyond max_steps");
+    if (nsteps == 0) return;
     with_dtype(lz->dtype, [&](auto z) {
       using V = decltype(z);
       for (int s = 0; s < nsteps; ++s) lanczos_step_dev<V>(lz, lz->steps + s);
@@ -708,7 +724,7 @@ int alps_ed_lanczos_vectors(alps_ed_lanczos lz, alps_ed_vector v_prev, alps_ed_v
     require(lz != nullptr, "null argument");
     if (lz->steps == 0) throw StateError("no Lanczos vectors before lanczos_start");
     const uint64_t n = lz->op->layout.D;
-    const int j = lz->steps;  // v_j is in u[j&1] with scale[j]
+    const int j = lz->steps;  // v_j = scale[j] u_j
     alps_ed_vector outs[2] = {v_prev, v_cur};
     for (int q = 0; q < 2; ++q) {
       alps_ed_vector o = outs[q];
@@ -718,7 +734,7 @@ int alps_ed_lanczos_vectors(alps_ed_lanczos lz, alps_ed_vector v_prev, alps_ed_v
       const double* sc = lz->scale.as<double>() + idx;
       with_dtype(lz->dtype, [&](auto z) {
         using V = decltype(z);
-        const V* src = lz->u[idx & 1].as<V>();
+        const V* src = static_cast<const V*>(lz->slot(idx));
         if (o->dtype == ALPS_ED_F32)
           hip_check(launch_convert<float, V>(n, o->buf.as<float>(), src, sc, lz->stream), "convert");
         else
@@ -726,6 +742,39 @@ int alps_ed_lanczos_vectors(alps_ed_lanczos lz, alps_ed_vector v_prev, alps_ed_v
       });
     }
     hip_check(hipStreamSynchronize(lz->stream), "sync");
+  });
+}
+
+int alps_ed_lanczos_ritz(alps_ed_lanczos lz, int nvec, int m, const double* coeffs, alps_ed_vector* outs) {
+  return guarded([&] {
+    require(lz && coeffs && outs, "null argument");
+    if (!lz->keep_basis) throw StateError("Ritz vectors need a Lanczos object created with keep_basis");
+    require(m >= 1 && m <= lz->steps, "m must be in [1, steps]");
+    require(nvec >= 0, "nvec must be >= 0");
+    const uint64_t n = lz->op->layout.D;
+    for (int i = 0; i < nvec; ++i) {
+      check_vec(outs[i]);
+      require(outs[i]->n == n && outs[i]->dtype == ALPS_ED_F64, "Ritz outputs must be FP64 of the dimension");
+    }
+    std::vector<void*> slots(m);
+    for (int j = 0; j < m; ++j) slots[j] = lz->slot(j);
+    DevBuf dslots(m * sizeof(void*)), dcoef((size_t)kMaxRitz * m * sizeof(double)), douts(kMaxRitz * sizeof(void*));
+    hip_check(hipMemcpyAsync(dslots.p, slots.data(), m * sizeof(void*), hipMemcpyHostToDevice, lz->stream), "copy");
+    for (int i0 = 0; i0 < nvec; i0 += kMaxRitz) {
+      const int nv = std::min(kMaxRitz, nvec - i0);
+      std::vector<void*> op(kMaxRitz, nullptr);
+      for (int i = 0; i < nv; ++i) op[i] = outs[i0 + i]->buf.p;
+      hip_check(hipMemcpyAsync(dcoef.p, coeffs + (size_t)i0 * m, (size_t)nv * m * sizeof(double),
+                               hipMemcpyHostToDevice, lz->stream), "copy");
+      hip_check(hipMemcpyAsync(douts.p, op.data(), kMaxRitz * sizeof(void*), hipMemcpyHostToDevice, lz->stream), "copy");
+      with_dtype(lz->dtype, [&](auto z) {
+        using V = decltype(z);
+        hip_check(launch_ritz<V>(n, dslots.as<const V*>(), lz->scale.as<double>(), m, nv, dcoef.as<double>(),
+                                 douts.as<double*>(), lz->stream),
+                  "ritz");
+      });
+      hip_check(hipStreamSynchronize(lz->stream), "sync");  // buffers are reused by the next pass
+    }
   });
 }
 
@@ -757,9 +806,18 @@ int alps_ed_tmatrix_analyze(int n, const double* alpha, const double* beta, doub
 
 namespace {
 
-// lanczos_iteration_nlowest::converged on the prefix T[0..m) (iteration.h:116-130).
-bool nlowest_converged(const std::vector<double>& a, const std::vector<double>& b, int m, int nev,
-                       std::vector<double>* ev_out) {
+// The lowest distinct non-ghost eigenvalues of T[0..m) with their error
+// bounds and eigenvector positions.  With FP32-stored Lanczos vectors the
+// recurrence produces near-copies of converged eigenvalues that differ by
+// O(eps_f32 |T|), above the FP64 merge threshold; they are merged here at
+// 64 eps_f32 |lambda| (a merged group has multiplicity > 1, error 0, as in
+// tmatrix.h:164-183).
+struct TValues {
+  std::vector<double> ev, err;
+  std::vector<int> index;
+};
+
+TValues analyze_prefix(const std::vector<double>& a, const std::vector<double>& b, int m, bool fp32) {
   double amax = a[0], bmax = b[0];
   for (int i = 1; i < m; ++i) {
     amax = std::max(amax, a[i]);
@@ -767,18 +825,76 @@ bool nlowest_converged(const std::vector<double>& a, const std::vector<double>& 
   }
   ietl::tmatrix_analysis an;
   an.run(a.data(), b.data(), m, amax, bmax, true);
-  if (ev_out) *ev_out = an.eigval_noghost;
+  TValues t;
+  const double dup = 64.0 * std::numeric_limits<float>::epsilon();
+  for (size_t i = 0; i < an.eigval_noghost.size(); ++i) {
+    const double lam = an.eigval_noghost[i];
+    if (fp32 && !t.ev.empty() && lam - t.ev.back() <= dup * std::max(1.0, std::fabs(t.ev.back()))) {
+      t.err.back() = 0.0;
+      continue;
+    }
+    t.ev.push_back(lam);
+    t.err.push_back(an.err_noghost[i]);
+    t.index.push_back(an.index_noghost[i]);
+  }
+  return t;
+}
+
+// lanczos_iteration_nlowest::converged (iteration.h:116-130), rtol = atol = 100 eps.
+bool nlowest_converged(const TValues& t, int nev) {
   const double tol = 100. * std::numeric_limits<double>::epsilon();
-  if ((int)an.eigval_noghost.size() < nev) return false;
+  if ((int)t.ev.size() < nev) return false;
   for (int i = 0; i < nev; ++i)
-    if (an.err_noghost[i] > std::max(tol, tol * std::fabs(an.eigval_noghost[i]))) return false;
+    if (t.err[i] > std::max(tol, tol * std::fabs(t.ev[i]))) return false;
   return true;
 }
 
-std::vector<double> tmatrix_eigenvalues(const std::vector<double>& a, const std::vector<double>& b, int m) {
-  std::vector<double> ev;
-  nlowest_converged(a, b, m, 1 << 30, &ev);
-  return ev;
+void check_status(int st) {
+  if (st != ALPS_ED_OK) throw DeviceError(g_last_error);
+}
+
+// Rayleigh quotients (FP64 accumulation) of the Ritz vectors of T[0..m)
+// eigenvalues with the given positions, from the stored FP32 basis.
+std::vector<double> refine_ritz(alps_ed_lanczos lz, const std::vector<double>& a, const std::vector<double>& b,
+                                int m, const std::vector<int>& index) {
+  std::vector<double> d(a.begin(), a.begin() + m), z((size_t)m * m);
+  if (ietl::tridiag::eig_full(m, d.data(), b.data(), z.data()) > 0)
+    throw std::runtime_error("LAPACK error, stev function failed.");
+  const uint64_t n = lz->op->layout.D;
+  std::vector<double> rq;
+  for (size_t i0 = 0; i0 < index.size(); i0 += kMaxRitz) {
+    const int nv = (int)std::min<size_t>(kMaxRitz, index.size() - i0);
+    std::vector<double> coef((size_t)nv * m);
+    for (int i = 0; i < nv; ++i)
+      for (int j = 0; j < m; ++j) coef[(size_t)i * m + j] = z[j + (size_t)index[i0 + i] * m];
+    std::vector<alps_ed_vector> ys(nv, nullptr);
+    std::vector<std::unique_ptr<alps_ed_vector_s>> hold;
+    for (int i = 0; i < nv; ++i) {
+      check_status(alps_ed_vector_create(n, ALPS_ED_F64, &ys[i]));
+      hold.emplace_back(ys[i]);
+    }
+    check_status(alps_ed_lanczos_ritz(lz, nv, m, coef.data(), ys.data()));
+    alps_ed_vector hy = nullptr;
+    check_status(alps_ed_vector_create(n, ALPS_ED_F64, &hy));
+    std::unique_ptr<alps_ed_vector_s> hold_hy(hy);
+    for (int i = 0; i < nv; ++i) {
+      double* res = lz->red.result(0);
+      hip_check(launch_hv<double>(lz->op->dev, ys[i]->buf.as<double>(), hy->buf.as<double>(), lz->red.a(),
+                                  lz->stream),
+                "hv");
+      hip_check(launch_finish(Partials{lz->red.a(), hv_blocks(lz->op->dev)}, nullptr, res, nullptr, kSum, lz->stream),
+                "finish");
+      hip_check(launch_dot<double>(n, ys[i]->buf.as<double>(), ys[i]->buf.as<double>(), lz->red.b(), lz->stream),
+                "dot");
+      hip_check(launch_finish(Partials{lz->red.b(), stream_blocks(n)}, nullptr, res + 1, nullptr, kSum, lz->stream),
+                "finish");
+      double h[2];
+      hip_check(hipMemcpyAsync(h, res, 2 * sizeof(double), hipMemcpyDeviceToHost, lz->stream), "copy");
+      hip_check(hipStreamSynchronize(lz->stream), "sync");
+      rq.push_back(h[0] / h[1]);
+    }
+  }
+  return rq;
 }
 
 }  // namespace
@@ -796,43 +912,43 @@ extern "C" int alps_ed_sparsediag(const alps_ed_model* model, const alps_ed_solv
     if (D == 0) return;  // sparsediag.h:101-102
     operator_upload(op.get());
     if (D == 1) {  // sparsediag.h:120-124: the single matrix element
-      double v = 0.0;
-      uint32_t c = 0;
-      int cnt = 0;
       std::vector<uint32_t> cols(op->layout.bond_p1.size() + 1);
       std::vector<double> vals(cols.size());
-      if (alps_ed_operator_elements(op.get(), 0, 1, (int)cols.size(), cols.data(), vals.data(), &cnt) != ALPS_ED_OK)
-        throw DeviceError(g_last_error);
+      int cnt = 0;
+      check_status(alps_ed_operator_elements(op.get(), 0, 1, (int)cols.size(), cols.data(), vals.data(), &cnt));
+      double v = 0.0;
       for (int q = 0; q < cnt; ++q)
-        if (cols[q] == c) v = vals[q];
+        if (cols[q] == 0) v = vals[q];
       result->n_eigenvalues = 1;
       result->eigenvalues[0] = v;
-      result->iterations = 0;
+      result->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       return;
     }
+    const bool fp32 = params->dtype == ALPS_ED_F32;
     const int max_iter = params->max_iter > 0 ? params->max_iter : (int)std::min<uint64_t>(10 * D, 1000);
     const int nev = params->nev > 0 ? params->nev : 1;
     require(nev <= 64, "nev must be <= 64");
     alps_ed_lanczos lzh = nullptr;
-    if (alps_ed_lanczos_create(op.get(), params->dtype, max_iter, &lzh) != ALPS_ED_OK) throw DeviceError(g_last_error);
+    check_status(alps_ed_lanczos_create(op.get(), params->dtype, max_iter, fp32 ? 1 : 0, &lzh));
     std::unique_ptr<alps_ed_lanczos_s> lz(lzh);
-    if (alps_ed_lanczos_start(lz.get(), params->seed, nullptr) != ALPS_ED_OK) throw DeviceError(g_last_error);
+    check_status(alps_ed_lanczos_start(lz.get(), params->seed, nullptr));
     std::vector<double> a(max_iter + 1), b(max_iter + 1);
-    int n = 1, checked = 1, final_n = -1;
-    int error_code = 0;
+    int n = 1, checked = 1, final_n = -1, error_code = 0;
+    // The stopping rule is tested on chunk boundaries; when it holds, the
+    // first prefix length at which it holds is found by scanning forward, so
+    // the T matrix is the one the per-step test of the reference would stop at.
     while (final_n < 0) {
-      int target = params->fixed_iterations ? max_iter : std::min(max_iter, n + std::max(1, std::min(n / 8, 64)));
+      const int target =
+          params->fixed_iterations ? max_iter : std::min(max_iter, n + std::max(1, std::min(n / 8, 64)));
       if (target > n) {
-        if (alps_ed_lanczos_steps(lz.get(), target - n) != ALPS_ED_OK) throw DeviceError(g_last_error);
+        check_status(alps_ed_lanczos_steps(lz.get(), target - n));
         n = target;
       }
       int got = 0;
-      if (alps_ed_lanczos_coefficients(lz.get(), a.data(), b.data(), n, &got) != ALPS_ED_OK)
-        throw DeviceError(g_last_error);
-      if (!params->fixed_iterations && n >= 2 && nlowest_converged(a, b, n, nev, nullptr)) {
-        // first converged prefix in (checked, n], as the per-step test would find
+      check_status(alps_ed_lanczos_coefficients(lz.get(), a.data(), b.data(), n, &got));
+      if (!params->fixed_iterations && n >= 2 && nlowest_converged(analyze_prefix(a, b, n, fp32), nev)) {
         for (int m = std::max(2, checked + 1); m <= n; ++m)
-          if (m == n || nlowest_converged(a, b, m, nev, nullptr)) {
+          if (m == n || nlowest_converged(analyze_prefix(a, b, m, fp32), nev)) {
             final_n = m;
             break;
           }
@@ -842,10 +958,31 @@ extern "C" int alps_ed_sparsediag(const alps_ed_model* model, const alps_ed_solv
       }
       checked = n;
     }
-    std::vector<double> ev = tmatrix_eigenvalues(a, b, final_n);
-    const int k = std::min<int>(nev, (int)ev.size());
-    result->n_eigenvalues = k;
-    for (int i = 0; i < k; ++i) result->eigenvalues[i] = ev[i];
+    TValues t = analyze_prefix(a, b, final_n, fp32);
+    std::vector<double> ev;
+    if (!fp32) {
+      ev.assign(t.ev.begin(), t.ev.begin() + std::min<size_t>(nev, t.ev.size()));
+    } else {
+      // Rayleigh quotients of the Ritz vectors; values that agree to 1e-9
+      // relative are the same level reached twice and are reported once.
+      size_t next = 0;
+      std::vector<double> rq;
+      while ((int)ev.size() < nev && next < t.ev.size()) {
+        const size_t cnt = std::min<size_t>(t.ev.size() - next, (size_t)(nev - (int)ev.size()));
+        std::vector<int> idx(t.index.begin() + next, t.index.begin() + next + cnt);
+        std::vector<double> r = refine_ritz(lz.get(), a, b, final_n, idx);
+        next += cnt;
+        for (double v : r) {
+          bool dup = false;
+          for (double e : ev)
+            if (std::fabs(v - e) <= 1e-9 * std::max(1.0, std::fabs(e))) dup = true;
+          if (!dup) ev.push_back(v);
+        }
+        std::sort(ev.begin(), ev.end());
+      }
+    }
+    result->n_eigenvalues = (int)ev.size();
+    for (size_t i = 0; i < ev.size(); ++i) result->eigenvalues[i] = ev[i];
     result->iterations = final_n;
     result->error_code = error_code;
     result->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

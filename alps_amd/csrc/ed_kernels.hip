@@ -2,18 +2,23 @@
 //
 // H.v is matrix-free on the bit-encoded fixed-Sz basis (the product the
 // reference computes as uBLAS prod over an explicit map-of-maps matrix,
-// src/ietl/interface/ublas.h:57-63).  One wavefront owns a 64-row tile of a
-// row segment that shares one high word `h`; lanes run over consecutive low
-// words.  Consequences on CDNA4:
-//   - h is wave-uniform (readfirstlane), so everything that depends on it --
-//     the high-word diagonal, the high-high flip bonds and their target
-//     segments -- is scalar work, and the x loads of high-high flips are
-//     64-lane contiguous runs (fully coalesced);
-//   - low-low flips stay inside the segment (a window of at most C(B,B/2)
-//     rows), cross flips land in one neighbouring segment;
-//   - the diagonal comes from two table lookups plus the cross bonds.
-// Accumulation is FP64 whatever the vector storage; reductions are
-// deterministic (fixed tree + last-block finish, no float atomics).
+// src/ietl/interface/ublas.h:57-63).  The rows of one sector split into
+// segments that share the high word h of the state (see ed_layout.hpp); one
+// workgroup owns one segment:
+//   - the segment's slice of x is staged in LDS, so every low-low spin flip
+//     is an LDS gather; its target row comes from a per-class table read
+//     with coalesced 16-bit loads;
+//   - h is uniform, so the high-high flips are scalar decisions whose x
+//     reads are contiguous runs of a neighbouring segment (coalesced);
+//   - a cross flip (one bit in h, one in the low word) reads one element of
+//     a neighbouring segment, its rank coming from a per-class table;
+//   - the diagonal is two table values plus the cross bonds.
+// Every global load is a wave-uniform base plus a per-lane offset, so the
+// address costs no vector ALU work.  Bond lists are padded with null bonds to
+// the unrolled chunk sizes, so no load depends on a runtime bond count.
+// Accumulation is FP64 whatever the vector storage.  Reductions are
+// deterministic and atomic-free: producers write one partial per block,
+// consumers add the partials in index order.
 #include <hip/hip_runtime.h>
 
 #include "ed_kernels.hpp"
@@ -28,155 +33,198 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Block sum, valid in thread 0.
-__device__ __forceinline__ double block_sum(double v) {
-  __shared__ double red_smem[kWavesPerBlock];
+// Block sum, returned to every thread.
+__device__ __forceinline__ double block_sum_all(double v) {
+  __shared__ double red_smem[kBlock / 64];
   v = wave_sum(v);
   const int wave = threadIdx.x >> 6;
+  __syncthreads();  // red_smem may still be read by a previous call
   if ((threadIdx.x & 63) == 0) red_smem[wave] = v;
   __syncthreads();
   double s = 0.0;
-  if (threadIdx.x == 0) {
 #pragma unroll
-    for (int w = 0; w < kWavesPerBlock; ++w) s += red_smem[w];
-  }
-  __syncthreads();
+  for (int w = 0; w < kBlock / 64; ++w) s += red_smem[w];
   return s;
 }
 
-__device__ __forceinline__ double load_coherent(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ double sum_partials(const Partials& P) {
+  double v = 0.0;
+  for (uint32_t i = threadIdx.x; i < P.n; i += kBlock) v += P.p[i];
+  return block_sum_all(v);
 }
 
-// Grid reduction finish: called by every thread of every block with the
-// thread's partial value.
-__device__ void grid_reduce(double v, const GridReduce& r) {
-  __shared__ bool is_last;
-  const double bs = block_sum(v);
-  if (threadIdx.x == 0) {
-    r.partials[blockIdx.x] = bs;
-    __threadfence();
-    const unsigned done = atomicAdd(r.counter, 1u);
-    is_last = (done == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  double s = 0.0;
-  for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x) s += load_coherent(r.partials + i);
-  s = block_sum(s);
-  if (threadIdx.x == 0) {
-    if (r.mode == 0) {
-      r.out[0] = s;
-    } else {
-      const double nrm = sqrt(s);
-      r.out[0] = nrm;
-      if (r.out2) r.out2[0] = 1.0 / nrm;
-    }
-    *r.counter = 0u;
-  }
+__device__ __forceinline__ void store_partial(double* out, double v) {
+  const double s = block_sum_all(v);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
-template <class V>
-__device__ __forceinline__ double ld(const V* p, uint32_t i) {
-  return (double)p[i];
+
+// Uniform reads of the layout tables.  The tables are never written while a
+// kernel runs, so they are read through the constant address space, which
+// lets the compiler use scalar loads for wave-uniform addresses.
+template <class T>
+__device__ __forceinline__ T ldc(const T* p) {
+  return *(const __attribute__((address_space(4))) T*)(p);
 }
 
 // ---------------------------------------------------------------------------
 // H.v
 // ---------------------------------------------------------------------------
 template <class V, bool DOT>
-__global__ __launch_bounds__(kBlock) void hv_kernel(DevLayout L, const V* __restrict__ x,
-                                                     V* __restrict__ y, const double* xscale,
-                                                     GridReduce red) {
-  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t tile = blockIdx.x * kWavesPerBlock + wave;
-  const double s = xscale ? *xscale : 1.0;
+__global__ __launch_bounds__(kBlock) void hv_kernel(DevLayout L, const V* __restrict__ x, V* __restrict__ y,
+                                                     double* __restrict__ dot_out) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  V* xs = reinterpret_cast<V*>(smem);  // segment slice; entry len is the zero slot
+  const uint32_t tid = threadIdx.x;
+  const uint32_t h = ldc(L.proc_hi + blockIdx.x);
+  const int kl = L.k - __builtin_popcount(h);
+  const uint32_t cls0 = ldc(L.lo_class_off + kl);
+  const uint32_t len = ldc(L.lo_class_off + kl + 1) - cls0;
+  const uint32_t base = ldc(L.hi_base + h);
+  const double dh = ldc(L.diag_hi + h);
+  const V* xb = x + base;
+  for (uint32_t r = tid; r < len; r += kBlock) xs[r] = xb[r];
+  if (tid == 0) xs[len] = (V)0;
+  const uint16_t* nbp = L.nbp16 + ldc(L.nbp_off + kl);
+  const uint16_t* xnp = L.xn16 + ldc(L.xn_off + kl);
+  const double* dlp = L.diag_lo_cls + cls0;
+  __syncthreads();
+
   double dotp = 0.0;
-  if (tile < L.n_tiles) {
-    const uint32_t h = __builtin_amdgcn_readfirstlane(L.tiles[2 * tile]);
-    const uint32_t r0 = __builtin_amdgcn_readfirstlane(L.tiles[2 * tile + 1]);
-    const int kl = L.k - __builtin_popcount(h);
-    const uint32_t cls0 = L.lo_class_off[kl];
-    const uint32_t len = L.lo_class_off[kl + 1] - cls0;
-    const uint32_t base = L.hi_base[h];
-    const double dh = L.diag_hi[h];
-    const uint32_t r = r0 + lane;
-    if (r < len) {
-      const uint32_t lo = L.lo_state[cls0 + r];
-      const uint32_t row = base + r;
-      const double xi = ld(x, row);
-      double d = dh + L.diag_lo[lo];
-      double off = 0.0;
-      // cross bonds: one bit in h (uniform), one in lo
-      for (int b = 0; b < L.n_xb; ++b) {
-        const CrossBond cb = L.xb[b];
-        const bool hb = (h & cb.hbit) != 0u;
-        const bool lb = (lo & cb.lbit) != 0u;
-        if (hb == lb) {
-          d += cb.e_par;
-        } else {
-          d += cb.e_anti;
-          const uint32_t j = L.hi_base[h ^ cb.hbit] + L.lo_rank[lo ^ cb.lbit];
-          off += cb.t * ld(x, j);
-        }
-      }
-      // low-low flips: target in the same segment
-      for (int b = 0; b < L.n_ll; ++b) {
-        const uint32_t m = L.ll[b].mask;
-        const uint32_t bits = lo & m;
-        if (bits != 0u && bits != m) off += L.ll[b].t * ld(x, base + L.lo_rank[lo ^ m]);
-      }
-      // high-high flips: wave-uniform test, coalesced 64-row target run
-      for (int b = 0; b < L.n_hh; ++b) {
-        const uint32_t m = L.hh[b].mask;
+  for (uint32_t r0 = 0; r0 < len; r0 += kBlock) {
+    const uint32_t r = r0 + tid;
+    const bool ok = r < len;
+    const uint32_t rc = ok ? r : 0u;  // clamped row: every read stays in bounds
+    double d = dh + dlp[rc];
+    double off = 0.0;
+    // low-low flips: target rows from the class table, values from LDS
+    for (int b0 = 0; b0 < L.n_ll_p; b0 += kChunkLL) {
+      uint32_t j[kChunkLL];
+#pragma unroll
+      for (int c = 0; c < kChunkLL; ++c) j[c] = (nbp + (b0 + c) * len)[rc];
+      V v[kChunkLL];
+#pragma unroll
+      for (int c = 0; c < kChunkLL; ++c) v[c] = xs[j[c]];
+#pragma unroll
+      for (int c = 0; c < kChunkLL; ++c) off += ldc(L.ll_t + b0 + c) * (double)v[c];
+    }
+    // high-high flips: uniform decision, contiguous target run
+    for (int b0 = 0; b0 < L.n_hh_p; b0 += kChunkHH) {
+      const V* xt[kChunkHH];
+      double tt[kChunkHH];
+#pragma unroll
+      for (int c = 0; c < kChunkHH; ++c) {
+        const uint32_t m = ldc(L.hh_mask + b0 + c);
         const uint32_t bits = h & m;
-        if (bits != 0u && bits != m) off += L.hh[b].t * ld(x, L.hi_base[h ^ m] + r);
+        const bool act = bits != 0u && bits != m;
+        xt[c] = act ? x + ldc(L.hi_base + (h ^ m)) : xb;
+        tt[c] = act ? ldc(L.hh_t + b0 + c) : 0.0;
       }
-      const double yv = s * (d * xi + off);
-      const V ys = (V)yv;
-      y[row] = ys;
-      if (DOT) dotp = (s * xi) * (double)ys;
+      V v[kChunkHH];
+#pragma unroll
+      for (int c = 0; c < kChunkHH; ++c) v[c] = xt[c][rc];
+#pragma unroll
+      for (int c = 0; c < kChunkHH; ++c) off += tt[c] * (double)v[c];
+    }
+    // cross bonds: diagonal always, flip when the high and low bits differ.
+    // If the neighbouring high word has no segment no row can flip (its low
+    // word would need a set bit more or fewer than it has), and the read
+    // falls back to this segment.
+    for (int b0 = 0; b0 < L.n_xb_p; b0 += kChunkX) {
+      uint32_t e[kChunkX];
+#pragma unroll
+      for (int c = 0; c < kChunkX; ++c) e[c] = (xnp + (b0 + c) * len)[rc];
+      V v[kChunkX];
+      double tt[kChunkX];
+#pragma unroll
+      for (int c = 0; c < kChunkX; ++c) {
+        const uint32_t hbit = ldc(L.xb_hbit + b0 + c);
+        const bool hb = (h & hbit) != 0u;
+        const uint32_t nb = ldc(L.hi_base + (h ^ hbit));
+        const V* xn = x + (nb == 0xffffffffu ? base : nb);
+        const bool flip = ((e[c] >> 15) != 0u) != hb;
+        d += flip ? ldc(L.xb_anti + b0 + c) : ldc(L.xb_par + b0 + c);
+        v[c] = xn[flip ? (e[c] & 0x7fffu) : 0u];
+        tt[c] = flip ? ldc(L.xb_t + b0 + c) : 0.0;
+      }
+#pragma unroll
+      for (int c = 0; c < kChunkX; ++c) off += tt[c] * (double)v[c];
+    }
+    const double xi = (double)xs[rc];
+    const double yv = d * xi + off;
+    if (ok) {
+      y[base + rc] = (V)yv;
+      if (DOT) dotp += xi * yv;  // FP64 value before the storage rounding
     }
   }
-  if (DOT) grid_reduce(dotp, red);
+  if (DOT) {
+    const double bs = block_sum_all(dotp);
+    if (tid == 0) dot_out[blockIdx.x] = bs;
+  }
 }
 
 // ---------------------------------------------------------------------------
-// Lanczos update + norm
+// Lanczos update, reductions, vector kernels
 // ---------------------------------------------------------------------------
-// `out` may alias w or u_prev: every element is read before it is written by
-// the same thread.
+// `out` may alias w or u_prev: every element is read before the same thread
+// writes it.
 template <class V>
-__global__ __launch_bounds__(kBlock) void update_kernel(uint64_t n, const V* w,
-                                                        const V* __restrict__ u_cur,
-                                                        const V* u_prev, V* out, const double* alpha,
-                                                        const double* s_cur, const double* beta_prev,
-                                                        const double* s_prev, GridReduce red) {
-  const double ca = (*alpha) * (s_cur ? *s_cur : 1.0);
+__global__ __launch_bounds__(kBlock) void update_kernel(uint64_t n, const V* w, const V* __restrict__ u_cur,
+                                                        const V* u_prev, V* out, const double* __restrict__ alpha,
+                                                        const double* __restrict__ s_cur,
+                                                        const double* __restrict__ beta_prev,
+                                                        const double* __restrict__ s_prev,
+                                                        double* __restrict__ norm_out) {
+  const double sc = s_cur ? *s_cur : 1.0;
+  const double ca = (*alpha) * sc;
   const double cb = beta_prev ? (*beta_prev) * (s_prev ? *s_prev : 1.0) : 0.0;
   double acc = 0.0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    double r = (double)w[i] - ca * (double)u_cur[i];
+    double r = sc * (double)w[i] - ca * (double)u_cur[i];
     if (beta_prev) r -= cb * (double)u_prev[i];
     const V rs = (V)r;
     out[i] = rs;
     const double rd = (double)rs;
     acc += rd * rd;
   }
-  grid_reduce(acc, red);
+  store_partial(norm_out, acc);
 }
 
 template <class V>
 __global__ __launch_bounds__(kBlock) void dot_kernel(uint64_t n, const V* __restrict__ x,
-                                                     const V* __restrict__ y, GridReduce red) {
+                                                     const V* __restrict__ y, double* partials) {
   double acc = 0.0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
     acc += (double)x[i] * (double)y[i];
-  grid_reduce(acc, red);
+  store_partial(partials, acc);
+}
+
+__global__ __launch_bounds__(kBlock) void finish_kernel(Partials in, const double* scale, double* out, double* out2,
+                                                        int mode) {
+  const double s = sum_partials(in);
+  if (threadIdx.x == 0) {
+    if (mode == kNorm) {
+      out[0] = sqrt(s);
+      if (out2) out2[0] = 1.0 / sqrt(s);
+    } else if (mode == kAlpha) {
+      const double sc = scale ? *scale : 1.0;
+      out[0] = sc * sc * s;
+    } else {
+      out[0] = s;
+    }
+  }
+}
+
+template <class V>
+__global__ __launch_bounds__(kBlock) void normalize_kernel(uint64_t n, V* v, Partials norm_in, double* b_out) {
+  const double ss = sum_partials(norm_in);
+  const double inv = 1.0 / sqrt(ss);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && b_out) *b_out = sqrt(ss);
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    v[i] = (V)(inv * (double)v[i]);
 }
 
 __device__ __forceinline__ uint64_t splitmix64_mix(uint64_t z) {
@@ -186,8 +234,8 @@ __device__ __forceinline__ uint64_t splitmix64_mix(uint64_t z) {
 }
 
 template <class V>
-__global__ __launch_bounds__(kBlock) void fill_random_kernel(uint64_t n, V* v, uint64_t seed,
-                                                             double lo, double hi) {
+__global__ __launch_bounds__(kBlock) void fill_random_kernel(uint64_t n, V* v, uint64_t seed, double lo,
+                                                             double hi) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const uint64_t z = splitmix64_mix(seed + (i + 1) * 0x9e3779b97f4a7c15ULL);
@@ -203,29 +251,46 @@ __global__ __launch_bounds__(kBlock) void fill_kernel(uint64_t n, V* v, double v
 }
 
 template <class V>
-__global__ __launch_bounds__(kBlock) void axpby_kernel(uint64_t n, V* __restrict__ y, double a,
-                                                       const V* __restrict__ x, double b,
-                                                       const double* ap, const double* bp,
-                                                       bool a_recip) {
-  double av = ap ? *ap : a;
-  if (a_recip) av = 1.0 / av;
-  const double bv = bp ? *bp : b;
+__global__ __launch_bounds__(kBlock) void axpby_kernel(uint64_t n, V* y, double a, const V* x, double b) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const double xv = x ? (double)x[i] : 0.0;
-    const double yv = bv == 0.0 ? 0.0 : bv * (double)y[i];
-    y[i] = (V)(av * xv + yv);
+    const double yv = b == 0.0 ? 0.0 : b * (double)y[i];
+    y[i] = (V)(a * xv + yv);
   }
 }
 
 template <class VD, class VS>
 __global__ __launch_bounds__(kBlock) void convert_kernel(uint64_t n, VD* __restrict__ dst,
-                                                         const VS* __restrict__ src,
-                                                         const double* scale) {
+                                                         const VS* __restrict__ src, const double* scale) {
   const double s = scale ? *scale : 1.0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
     dst[i] = (VD)(s * (double)src[i]);
+}
+
+// Ritz vectors over a stored Lanczos basis:
+//   out_i[r] = sum_{j<m} coef[i*m + j] * scale[j] * u_j[r],  i < nvec <= 8.
+template <class V>
+__global__ __launch_bounds__(kBlock) void ritz_kernel(uint64_t n, const V* const* __restrict__ slots,
+                                                      const double* __restrict__ scale, int m, int nvec,
+                                                      const double* __restrict__ coef,
+                                                      double* const* __restrict__ outs) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+    double acc[kMaxRitz];
+#pragma unroll
+    for (int i = 0; i < kMaxRitz; ++i) acc[i] = 0.0;
+    for (int j = 0; j < m; ++j) {
+      const double u = scale[j] * (double)slots[j][r];
+#pragma unroll
+      for (int i = 0; i < kMaxRitz; ++i)
+        if (i < nvec) acc[i] += coef[i * m + j] * u;
+    }
+#pragma unroll
+    for (int i = 0; i < kMaxRitz; ++i)
+      if (i < nvec) outs[i][r] = acc[i];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -260,8 +325,8 @@ __global__ __launch_bounds__(kBlock) void unrank_kernel(DevLayout L, uint64_t fi
   if (t < count) out[t] = unrank(L, (uint32_t)(first + t));
 }
 
-__global__ __launch_bounds__(kBlock) void rank_kernel(DevLayout L, const uint32_t* states,
-                                                      uint64_t count, uint64_t* out) {
+__global__ __launch_bounds__(kBlock) void rank_kernel(DevLayout L, const uint32_t* states, uint64_t count,
+                                                      uint64_t* out) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < count) out[t] = rank_of(L, states[t]);
 }
@@ -286,6 +351,7 @@ __global__ __launch_bounds__(64) void elements_kernel(DevLayout L, uint64_t row0
   }
   int n = 1;
   c[0] = row;
+  v[0] = 0.0;
   for (int b = 0; b < L.n_bonds; ++b) {
     const uint32_t p1 = L.bond_p1[b], p2 = L.bond_p2[b];
     const bool par = ((s >> p1) & 1u) == ((s >> p2) & 1u);
@@ -294,7 +360,7 @@ __global__ __launch_bounds__(64) void elements_kernel(DevLayout L, uint64_t row0
     const double tb = L.bond_t[b];
     if (!par && tb != 0.0) {
       const uint32_t j = rank_of(L, s ^ ((1u << p1) | (1u << p2)));
-      // insert keeping columns ascending; equal columns accumulate in bond order
+      // keep columns ascending; equal columns accumulate in bond order
       int pos = 0;
       while (pos < n && c[pos] < j) ++pos;
       if (pos < n && c[pos] == j) {
@@ -310,7 +376,6 @@ __global__ __launch_bounds__(64) void elements_kernel(DevLayout L, uint64_t row0
       }
     }
   }
-  // the diagonal slot was inserted first with a placeholder: fill it now
   for (int q = 0; q < n; ++q)
     if (c[q] == row) v[q] = d;
   counts[t] = n;
@@ -322,37 +387,46 @@ __global__ __launch_bounds__(64) void elements_kernel(DevLayout L, uint64_t row0
 // launchers
 // ---------------------------------------------------------------------------
 template <class V>
-hipError_t launch_hv(const DevLayout& L, const V* x, V* y, const double* xscale, GridReduce red,
-                     hipStream_t st) {
-  if (L.n_tiles == 0) return hipSuccess;
-  const unsigned nb = hv_blocks(L);
-  if (red.out)
-    hipLaunchKernelGGL((hv_kernel<V, true>), dim3(nb), dim3(kBlock), 0, st, L, x, y, xscale, red);
+hipError_t launch_hv(const DevLayout& L, const V* x, V* y, double* dot_out, hipStream_t st) {
+  if (L.n_seg == 0) return hipSuccess;
+  const size_t lds = (size_t)(L.max_seg + 1) * sizeof(V);
+  if (dot_out)
+    hipLaunchKernelGGL((hv_kernel<V, true>), dim3(hv_blocks(L)), dim3(kBlock), lds, st, L, x, y, dot_out);
   else
-    hipLaunchKernelGGL((hv_kernel<V, false>), dim3(nb), dim3(kBlock), 0, st, L, x, y, xscale, red);
+    hipLaunchKernelGGL((hv_kernel<V, false>), dim3(hv_blocks(L)), dim3(kBlock), lds, st, L, x, y, dot_out);
   return hipGetLastError();
 }
 
 template <class V>
-hipError_t launch_update(uint64_t n, const V* w, const V* u_cur, const V* u_prev, V* out,
-                         const double* alpha, const double* s_cur, const double* beta_prev,
-                         const double* s_prev, GridReduce red, hipStream_t st) {
-  hipLaunchKernelGGL((update_kernel<V>), dim3(stream_blocks(n)), dim3(kBlock), 0, st, n, w, u_cur,
-                     u_prev, out, alpha, s_cur, beta_prev, s_prev, red);
+hipError_t launch_update(uint64_t n, const V* w, const V* u_cur, const V* u_prev, V* out, const double* alpha,
+                         const double* s_cur, const double* beta_prev, const double* s_prev, double* norm_out,
+                         hipStream_t st) {
+  hipLaunchKernelGGL((update_kernel<V>), dim3(stream_blocks(n)), dim3(kBlock), 0, st, n, w, u_cur, u_prev, out,
+                     alpha, s_cur, beta_prev, s_prev, norm_out);
   return hipGetLastError();
 }
 
 template <class V>
-hipError_t launch_dot(uint64_t n, const V* x, const V* y, GridReduce red, hipStream_t st) {
-  hipLaunchKernelGGL((dot_kernel<V>), dim3(stream_blocks(n)), dim3(kBlock), 0, st, n, x, y, red);
+hipError_t launch_dot(uint64_t n, const V* x, const V* y, double* partials, hipStream_t st) {
+  hipLaunchKernelGGL((dot_kernel<V>), dim3(stream_blocks(n)), dim3(kBlock), 0, st, n, x, y, partials);
+  return hipGetLastError();
+}
+
+hipError_t launch_finish(Partials in, const double* scale, double* out, double* out2, int mode, hipStream_t st) {
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(kBlock), 0, st, in, scale, out, out2, mode);
+  return hipGetLastError();
+}
+
+template <class V>
+hipError_t launch_normalize(uint64_t n, V* v, Partials norm_in, double* b_out, hipStream_t st) {
+  hipLaunchKernelGGL((normalize_kernel<V>), dim3(stream_blocks(n)), dim3(kBlock), 0, st, n, v, norm_in, b_out);
   return hipGetLastError();
 }
 
 template <class V>
 hipError_t launch_fill_random(uint64_t n, V* v, uint64_t seed, double lo, double hi, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL((fill_random_kernel<V>), dim3(stream_blocks(n)), dim3(kBlock), 0, st, n, v, seed,
-                     lo, hi);
+  hipLaunchKernelGGL((fill_random_kernel<V>), dim3(stream_blocks(n)), dim3(kBlock), 0, st, n, v, seed, lo, hi);
   return hipGetLastError();
 }
 
@@ -364,24 +438,29 @@ hipError_t launch_fill(uint64_t n, V* v, double value, hipStream_t st) {
 }
 
 template <class V>
-hipError_t launch_axpby(uint64_t n, V* y, double a, const V* x, double b, const double* ap,
-                        const double* bp, bool a_recip, hipStream_t st) {
+hipError_t launch_axpby(uint64_t n, V* y, double a, const V* x, double b, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL((axpby_kernel<V>), dim3(stream_blocks(n)), dim3(kBlock), 0, st, n, y, a, x, b,
-                     ap, bp, a_recip);
+  hipLaunchKernelGGL((axpby_kernel<V>), dim3(stream_blocks(n)), dim3(kBlock), 0, st, n, y, a, x, b);
   return hipGetLastError();
 }
 
 template <class VD, class VS>
 hipError_t launch_convert(uint64_t n, VD* dst, const VS* src, const double* scale, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL((convert_kernel<VD, VS>), dim3(stream_blocks(n)), dim3(kBlock), 0, st, n, dst,
-                     src, scale);
+  hipLaunchKernelGGL((convert_kernel<VD, VS>), dim3(stream_blocks(n)), dim3(kBlock), 0, st, n, dst, src, scale);
   return hipGetLastError();
 }
 
-hipError_t launch_unrank(const DevLayout& L, uint64_t first, uint64_t count, uint32_t* out,
-                         hipStream_t st) {
+template <class V>
+hipError_t launch_ritz(uint64_t n, const V* const* slots, const double* scale, int m, int nvec,
+                       const double* coef, double* const* outs, hipStream_t st) {
+  if (n == 0 || nvec == 0) return hipSuccess;
+  hipLaunchKernelGGL((ritz_kernel<V>), dim3(stream_blocks(n)), dim3(kBlock), 0, st, n, slots, scale, m, nvec,
+                     coef, outs);
+  return hipGetLastError();
+}
+
+hipError_t launch_unrank(const DevLayout& L, uint64_t first, uint64_t count, uint32_t* out, hipStream_t st) {
   if (count == 0) return hipSuccess;
   const unsigned nb = (unsigned)((count + kBlock - 1) / kBlock);
   hipLaunchKernelGGL(unrank_kernel, dim3(nb), dim3(kBlock), 0, st, L, first, count, out);
@@ -400,22 +479,22 @@ hipError_t launch_elements(const DevLayout& L, uint64_t row0, uint64_t nrows, in
                            uint32_t* cols, double* vals, int* counts, hipStream_t st) {
   if (nrows == 0) return hipSuccess;
   const unsigned nb = (unsigned)((nrows + 63) / 64);
-  hipLaunchKernelGGL(elements_kernel, dim3(nb), dim3(64), 0, st, L, row0, nrows, max_per_row, cols,
-                     vals, counts);
+  hipLaunchKernelGGL(elements_kernel, dim3(nb), dim3(64), 0, st, L, row0, nrows, max_per_row, cols, vals,
+                     counts);
   return hipGetLastError();
 }
 
-#define ALPS_ED_INSTANTIATE(V)                                                                     \
-  template hipError_t launch_hv<V>(const DevLayout&, const V*, V*, const double*, GridReduce,      \
-                                   hipStream_t);                                                    \
-  template hipError_t launch_update<V>(uint64_t, const V*, const V*, const V*, V*, const double*, \
-                                       const double*, const double*, const double*, GridReduce,    \
-                                       hipStream_t);                                                \
-  template hipError_t launch_dot<V>(uint64_t, const V*, const V*, GridReduce, hipStream_t);        \
-  template hipError_t launch_fill_random<V>(uint64_t, V*, uint64_t, double, double, hipStream_t);  \
-  template hipError_t launch_fill<V>(uint64_t, V*, double, hipStream_t);                           \
-  template hipError_t launch_axpby<V>(uint64_t, V*, double, const V*, double, const double*,       \
-                                      const double*, bool, hipStream_t);
+#define ALPS_ED_INSTANTIATE(V)                                                                              \
+  template hipError_t launch_hv<V>(const DevLayout&, const V*, V*, double*, hipStream_t);                   \
+  template hipError_t launch_update<V>(uint64_t, const V*, const V*, const V*, V*, const double*,           \
+                                       const double*, const double*, const double*, double*, hipStream_t); \
+  template hipError_t launch_dot<V>(uint64_t, const V*, const V*, double*, hipStream_t);                    \
+  template hipError_t launch_normalize<V>(uint64_t, V*, Partials, double*, hipStream_t);                    \
+  template hipError_t launch_fill_random<V>(uint64_t, V*, uint64_t, double, double, hipStream_t);           \
+  template hipError_t launch_fill<V>(uint64_t, V*, double, hipStream_t);                                    \
+  template hipError_t launch_axpby<V>(uint64_t, V*, double, const V*, double, hipStream_t);                 \
+  template hipError_t launch_ritz<V>(uint64_t, const V* const*, const double*, int, int, const double*,    \
+                                     double* const*, hipStream_t);
 
 ALPS_ED_INSTANTIATE(float)
 ALPS_ED_INSTANTIATE(double)

@@ -13,20 +13,29 @@ namespace alps_ed {
 struct DevLayout {
   int N, k, B, NH;
   uint32_t D;
-  uint32_t n_seg, n_tiles;
-  int n_ll, n_hh, n_xb, n_bonds;
+  uint32_t n_seg, max_seg;
+  int n_ll_p, n_hh_p, n_xb_p, n_bonds;
   const uint32_t* lo_rank;
   const uint32_t* lo_state;
   const uint32_t* lo_class_off;
   const uint32_t* hi_base;
-  const uint32_t* seg_hi;
+  const uint32_t* seg_hi;     // ascending (unrank)
   const uint32_t* seg_start;
-  const uint32_t* tiles;
-  const double* diag_lo;
+  const uint32_t* proc_hi;    // H.v processing order
+  const double* diag_lo_cls;  // low-word diagonal in lo_state order
   const double* diag_hi;
-  const FlipBond* ll;
-  const FlipBond* hh;
-  const CrossBond* xb;
+  const uint16_t* nbp16;
+  const uint32_t* nbp_off;
+  const uint16_t* xn16;
+  const uint32_t* xn_off;
+  const double* ll_t;
+  const uint32_t* hh_mask;
+  const double* hh_t;
+  const uint32_t* xb_hbit;
+  const double* xb_par;
+  const double* xb_anti;
+  const double* xb_t;
+  // reference-order description for the element enumeration
   const double* site_dn;
   const double* site_up;
   const uint32_t* bond_p1;
@@ -36,62 +45,69 @@ struct DevLayout {
   const double* bond_t;
 };
 
-// A deterministic grid reduction: every block stores its partial sum, the
-// last block to finish (counted with a device-scope atomic) adds the
-// partials in index order and writes the result.
-//   mode 0: out[0] = sum
-//   mode 1: out[0] = sqrt(sum), out2[0] = 1/sqrt(sum)
-struct GridReduce {
-  double* partials;   // >= gridDim.x entries
-  unsigned* counter;  // zero before the launch; reset to zero by the last block
-  double* out;
-  double* out2;
-  int mode;
+// Reductions without atomics: the producing kernel stores one partial sum
+// per block; a consuming kernel adds the partials in index order (every
+// block of the consumer computes the same value, so results are
+// deterministic and need no grid synchronization).
+struct Partials {
+  const double* p;  // nullptr when absent
+  uint32_t n;
 };
 
 constexpr int kBlock = 256;
-constexpr int kWavesPerBlock = kBlock / 64;
+constexpr int kMaxRitz = 8;     // Ritz vectors per pass of launch_ritz
 
 // Number of blocks of the streaming kernels for a vector of length n.
 inline unsigned stream_blocks(uint64_t n) {
   uint64_t b = (n + (uint64_t)kBlock * 4 - 1) / ((uint64_t)kBlock * 4);
   if (b < 1) b = 1;
-  if (b > 2048) b = 2048;
+  if (b > 1024) b = 1024;
   return (unsigned)b;
 }
 
-inline unsigned hv_blocks(const DevLayout& L) {
-  unsigned b = (L.n_tiles + kWavesPerBlock - 1) / kWavesPerBlock;
-  return b ? b : 1;
-}
+// Blocks of the H.v kernel: one per segment.
+inline unsigned hv_blocks(const DevLayout& L) { return L.n_seg ? L.n_seg : 1; }
 
-// y = s * H x (s = *xscale when xscale != nullptr, else 1).  With red.out
-// set, also reduces alpha = sum_i (s x_i) * y_i (y as stored).
+// y = H x; with dot_out set, per-block partials of sum_i x_i (H x)_i (FP64,
+// before y is rounded to the storage type).
 template <class V>
-hipError_t launch_hv(const DevLayout& L, const V* x, V* y, const double* xscale, GridReduce red,
-                     hipStream_t st);
+hipError_t launch_hv(const DevLayout& L, const V* x, V* y, double* dot_out, hipStream_t st);
 
-// Lanczos update: r = w - (alpha*s_cur) u_cur - (beta_prev*s_prev) u_prev,
-// stored into `out` (which may alias w or u_prev); reduces |r| into red
-// (mode 1).  Scalars are read from device memory; beta_prev == nullptr drops
-// the last term (first step); null s_cur / s_prev mean 1.
-template <class V>
-hipError_t launch_update(uint64_t n, const V* w, const V* u_cur, const V* u_prev, V* out,
-                         const double* alpha, const double* s_cur, const double* beta_prev,
-                         const double* s_prev, GridReduce red, hipStream_t st);
+// Lanczos scalars between the kernels of one step (one block):
+//   mode kAlpha: alpha = scale^2 * sum(in)            -> out[0]
+//   mode kBeta:  beta = sqrt(sum(in)), scale = 1/beta -> out[0], out2[0]
+enum { kSum = 0, kNorm = 1, kAlpha = 2 };
+hipError_t launch_finish(Partials in, const double* scale, double* out, double* out2, int mode, hipStream_t st);
 
+// Lanczos update: r = s_cur*w - alpha*s_cur*u_cur - beta_prev*s_prev*u_prev
+// stored into `out` (may alias w or u_prev); per-block partials of |r|^2
+// into norm_out.  Scalars are read from device memory; beta_prev == nullptr
+// drops the last term; null s_cur / s_prev mean 1.
 template <class V>
-hipError_t launch_dot(uint64_t n, const V* x, const V* y, GridReduce red, hipStream_t st);
+hipError_t launch_update(uint64_t n, const V* w, const V* u_cur, const V* u_prev, V* out, const double* alpha,
+                         const double* s_cur, const double* beta_prev, const double* s_prev, double* norm_out,
+                         hipStream_t st);
+
+// per-block partials of sum x_i y_i
+template <class V>
+hipError_t launch_dot(uint64_t n, const V* x, const V* y, double* partials, hipStream_t st);
+// v *= 1/sqrt(sum(norm_in)); block 0 stores sqrt(sum) to b_out
+template <class V>
+hipError_t launch_normalize(uint64_t n, V* v, Partials norm_in, double* b_out, hipStream_t st);
+
 template <class V>
 hipError_t launch_fill_random(uint64_t n, V* v, uint64_t seed, double lo, double hi, hipStream_t st);
 template <class V>
 hipError_t launch_fill(uint64_t n, V* v, double value, hipStream_t st);
-// y = a*x + b*y; a and b from device memory when ap/bp are set.
+// y = a*x + b*y
 template <class V>
-hipError_t launch_axpby(uint64_t n, V* y, double a, const V* x, double b, const double* ap,
-                        const double* bp, bool a_recip, hipStream_t st);
+hipError_t launch_axpby(uint64_t n, V* y, double a, const V* x, double b, hipStream_t st);
 template <class VD, class VS>
 hipError_t launch_convert(uint64_t n, VD* dst, const VS* src, const double* scale, hipStream_t st);
+// out_i = sum_{j<m} coef[i*m+j] * scale[j] * slots[j], i < nvec <= kMaxRitz
+template <class V>
+hipError_t launch_ritz(uint64_t n, const V* const* slots, const double* scale, int m, int nvec,
+                       const double* coef, double* const* outs, hipStream_t st);
 
 hipError_t launch_unrank(const DevLayout& L, uint64_t first, uint64_t count, uint32_t* out, hipStream_t st);
 hipError_t launch_rank(const DevLayout& L, const uint32_t* states, uint64_t count, uint64_t* out,

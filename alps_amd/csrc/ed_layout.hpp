@@ -28,6 +28,7 @@
 // of an antiparallel pair.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -105,7 +106,23 @@ struct SectorLayout {
   std::vector<uint32_t> hi_base;       // [2^NH]; 0xffffffff when no state has this hi
   std::vector<uint32_t> seg_hi;        // valid hi words, ascending
   std::vector<uint32_t> seg_start;     // first row of each segment
-  std::vector<uint32_t> tiles;         // (hi, r0) pairs, 64-row tiles
+  std::vector<double> diag_lo_cls;     // diag_lo in lo_state order (coalesced)
+  std::vector<uint32_t> proc_hi;       // hi words in H.v processing order (by class)
+  uint32_t max_seg = 0;                // longest segment
+  // H.v tables.  Bond lists are padded with null bonds (coefficient 0,
+  // never active) to multiples of the kernel's chunk sizes.
+  //   nbp16[nbp_off[c] + b*C(B,c) + r]: row reached from row r of a class-c
+  //     segment by low-low bond b, or C(B,c) (the zero slot) if not allowed;
+  //   xn16[xn_off[c] + b*C(B,c) + r]: rank of lo ^ lbit_b in its class, with
+  //     bit 15 = the low bit of cross bond b in row r.
+  int n_ll_p = 0, n_hh_p = 0, n_xb_p = 0;
+  std::vector<uint16_t> nbp16, xn16;
+  std::vector<uint32_t> nbp_off, xn_off;  // [B+2]
+  std::vector<double> ll_t;               // [n_ll_p]
+  std::vector<uint32_t> hh_mask;          // [n_hh_p], 0 = null
+  std::vector<double> hh_t;
+  std::vector<uint32_t> xb_hbit;          // [n_xb_p], 0 = null
+  std::vector<double> xb_par, xb_anti, xb_t;
   std::vector<double> diag_lo;         // [2^B] site + low-low bond diagonal
   std::vector<double> diag_hi;         // [2^NH] site + high-high bond diagonal
   std::vector<FlipBond> ll, hh;        // low-low and high-high flip bonds
@@ -116,13 +133,18 @@ struct SectorLayout {
   std::vector<double> bond_par, bond_anti, bond_t;
 };
 
+// Chunk sizes of the H.v flip loops (bond lists are padded to multiples).
+constexpr int kChunkLL = 4;
+constexpr int kChunkHH = 4;
+constexpr int kChunkX = 2;
+
+inline int round_up(int n, int m) { return (n + m - 1) / m * m; }
+
 inline int choose_low_bits(int N) {
-  // Low words of at least 12 bits keep row segments long (C(12,6) = 924
-  // rows) for small N; at large N the split is near the middle so that both
-  // tables stay small (2^15 entries at N = 30).
-  int B = (N + 1) / 2;
-  if (B < 12) B = N < 12 ? N : 12;
-  return B;
+  // A 12-bit low word gives row segments of up to C(12,6) = 924 rows: long
+  // enough for coalesced streaming, short enough that a segment of x and the
+  // 16-bit rank table (8 KiB) sit in LDS.  Smaller systems use all bits.
+  return N < 12 ? N : 12;
 }
 
 inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1) {
@@ -137,7 +159,7 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1) {
   L.k = m.n_up;
   L.D = (m.n_up < 0 || m.n_up > N) ? 0 : binom(N, m.n_up);
   if (L.D >= 0xffffffffULL) throw std::invalid_argument("sector dimension exceeds 2^32-1");
-  const int B = B_override > 0 && B_override <= N ? B_override : choose_low_bits(N);
+  const int B = B_override > 0 && B_override <= N && B_override <= 16 ? B_override : choose_low_bits(N);
   L.B = B;
   L.NH = N - B;
   const uint32_t nlo = 1u << B;
@@ -171,12 +193,14 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1) {
     L.hi_base[h] = (uint32_t)r;
     L.seg_hi.push_back(h);
     L.seg_start.push_back((uint32_t)r);
-    uint64_t len = binom(B, kl);
-    for (uint64_t r0 = 0; r0 < len; r0 += 64) {
-      L.tiles.push_back(h);
-      L.tiles.push_back((uint32_t)r0);
-    }
   }
+  // segment processing order of the H.v kernel
+  L.proc_hi = L.seg_hi;
+  for (uint32_t h : L.seg_hi) L.max_seg = std::max<uint32_t>(L.max_seg, (uint32_t)binom(B, L.k - __builtin_popcount(h)));
+  // longest segments first, so that short ones fill the tail of the launch
+  std::stable_sort(L.proc_hi.begin(), L.proc_hi.end(), [&](uint32_t a, uint32_t b) {
+    return binom(B, L.k - __builtin_popcount(a)) > binom(B, L.k - __builtin_popcount(b));
+  });
   // terms in bit coordinates
   auto pos = [N](int s) { return (uint32_t)(N - 1 - s); };
   L.diag_lo.assign(nlo, 0.0);
@@ -213,6 +237,57 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1) {
       uint32_t ph = lo1 ? p2 : p1, pl = lo1 ? p1 : p2;
       L.xb.push_back({1u << (ph - B), 1u << pl, b.e_par, b.e_anti, b.t});
     }
+  }
+  L.diag_lo_cls.resize(nlo);
+  for (uint32_t i = 0; i < nlo; ++i) L.diag_lo_cls[i] = L.diag_lo[L.lo_state[i]];
+  // padded H.v tables
+  L.n_ll_p = round_up((int)L.ll.size(), kChunkLL);
+  L.n_hh_p = round_up((int)L.hh.size(), kChunkHH);
+  L.n_xb_p = round_up((int)L.xb.size(), kChunkX);
+  L.nbp_off.assign(B + 2, 0);
+  L.xn_off.assign(B + 2, 0);
+  for (int c = 0; c <= B; ++c) {
+    L.nbp_off[c + 1] = L.nbp_off[c] + (uint32_t)(binom(B, c) * L.n_ll_p);
+    L.xn_off[c + 1] = L.xn_off[c] + (uint32_t)(binom(B, c) * L.n_xb_p);
+  }
+  L.nbp16.assign(L.nbp_off[B + 1], 0);
+  L.xn16.assign(L.xn_off[B + 1], 0);
+  for (int c = 0; c <= B; ++c) {
+    const uint32_t len = (uint32_t)binom(B, c);
+    for (int b = 0; b < L.n_ll_p; ++b)
+      for (uint32_t r = 0; r < len; ++r) {
+        uint32_t v = len;
+        if (b < (int)L.ll.size()) {
+          const uint32_t lo = L.lo_state[L.lo_class_off[c] + r];
+          const uint32_t m = L.ll[b].mask, bits = lo & m;
+          if (bits != 0u && bits != m) v = L.lo_rank[lo ^ m];
+        }
+        L.nbp16[L.nbp_off[c] + b * len + r] = (uint16_t)v;
+      }
+    for (int b = 0; b < (int)L.xb.size(); ++b)
+      for (uint32_t r = 0; r < len; ++r) {
+        const uint32_t lo = L.lo_state[L.lo_class_off[c] + r];
+        const uint32_t lb = (lo & L.xb[b].lbit) != 0u;
+        L.xn16[L.xn_off[c] + b * len + r] = (uint16_t)(L.lo_rank[lo ^ L.xb[b].lbit] | (lb << 15));
+      }
+  }
+  L.ll_t.assign(L.n_ll_p, 0.0);
+  for (size_t b = 0; b < L.ll.size(); ++b) L.ll_t[b] = L.ll[b].t;
+  L.hh_mask.assign(L.n_hh_p, 0u);
+  L.hh_t.assign(L.n_hh_p, 0.0);
+  for (size_t b = 0; b < L.hh.size(); ++b) {
+    L.hh_mask[b] = L.hh[b].mask;
+    L.hh_t[b] = L.hh[b].t;
+  }
+  L.xb_hbit.assign(L.n_xb_p, 0u);
+  L.xb_par.assign(L.n_xb_p, 0.0);
+  L.xb_anti.assign(L.n_xb_p, 0.0);
+  L.xb_t.assign(L.n_xb_p, 0.0);
+  for (size_t b = 0; b < L.xb.size(); ++b) {
+    L.xb_hbit[b] = L.xb[b].hbit;
+    L.xb_par[b] = L.xb[b].e_par;
+    L.xb_anti[b] = L.xb[b].e_anti;
+    L.xb_t[b] = L.xb[b].t;
   }
   return L;
 }

@@ -143,7 +143,10 @@ int alps_ed_lanczos_step_vectors(alps_ed_operator op, alps_ed_vector v_prev, alp
  * src/ietl/lanczos.h:341-406).  Storage dtype applies to the three Lanczos
  * vectors; alpha, beta and the normalisations are FP64 on the device. ----- */
 
-int alps_ed_lanczos_create(alps_ed_operator op, alps_ed_dtype dtype, int max_steps, alps_ed_lanczos* lz);
+/* keep_basis != 0 keeps every Lanczos vector in HBM (max_steps+1 vectors,
+ * allocated in chunks as the recurrence advances) for alps_ed_lanczos_ritz. */
+int alps_ed_lanczos_create(alps_ed_operator op, alps_ed_dtype dtype, int max_steps, int keep_basis,
+                           alps_ed_lanczos* lz);
 int alps_ed_lanczos_destroy(alps_ed_lanczos lz);
 /* make_first_step with the start vector given by fill_random(seed, -1, 1),
  * or by `start` when it is not NULL (must have the operator's dimension). */
@@ -155,6 +158,12 @@ int alps_ed_lanczos_coefficients(alps_ed_lanczos lz, double* alpha, double* beta
 /* The current Lanczos vector v_n (normalized) and its predecessor, copied
  * into caller vectors of the same dimension. */
 int alps_ed_lanczos_vectors(alps_ed_lanczos lz, alps_ed_vector v_prev, alps_ed_vector v_cur);
+
+/* Ritz vectors from the kept basis (the basis transformation of
+ * lanczos::eigenvectors, src/ietl/lanczos.h:242-281, without the replay):
+ * outs[i] = sum_{j<m} coeffs[i*m + j] v_j for i < nvec, v_j the normalized
+ * Lanczos vectors, m <= steps; outs are FP64 vectors of the dimension. */
+int alps_ed_lanczos_ritz(alps_ed_lanczos lz, int nvec, int m, const double* coeffs, alps_ed_vector* outs);
 
 /* T-matrix analysis (ietl::Tmatrix::compute, src/ietl/tmatrix.h:121-228):
  * distinct non-ghost eigenvalues of the n x n tridiagonal with diagonal
@@ -168,7 +177,11 @@ int alps_ed_tmatrix_analyze(int n, const double* alpha, const double* beta, doub
  * with the lanczos_iteration_nlowest stopping rule (iteration.h:107-134;
  * rtol = atol = 100*eps) or a fixed iteration count, and returns the lowest
  * `nev` distinct non-ghost eigenvalues.  max_iter <= 0 selects the
- * reference default min(10*D, 1000); D == 1 returns H(0,0). */
+ * reference default min(10*D, 1000); D == 1 returns H(0,0).
+ * With FP64 storage the eigenvalues are those of T, as in the reference.
+ * With FP32 storage the basis is kept in HBM, near-copies of converged
+ * eigenvalues (within 64 eps_f32 |lambda|) are merged, and each reported
+ * eigenvalue is the FP64-accumulated Rayleigh quotient of its Ritz vector. */
 typedef struct {
   int max_iter;
   int nev;

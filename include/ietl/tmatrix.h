@@ -31,6 +31,9 @@ namespace ietl {
 struct tmatrix_analysis {
   std::vector<double> eigval_distinct, eigval_noghost, err, err_noghost;
   std::vector<int> mult, mult_noghost;
+  // position of each distinct value's first member among the sorted
+  // eigenvalues of T (selects its eigenvector, e.g. for Ritz vectors)
+  std::vector<int> index_distinct, index_noghost;
   double thold = 0.0, multol = 0.0;
 
   void run(const double* alpha, const double* beta, int n, double alpha_max, double beta_max,
@@ -41,6 +44,8 @@ struct tmatrix_analysis {
     err_noghost.clear();
     mult.clear();
     mult_noghost.clear();
+    index_distinct.clear();
+    index_noghost.clear();
     if (n <= 0) return;
     std::vector<double> eval(alpha, alpha + n);
     std::vector<double> zlast(do_err ? n : 0);
@@ -55,6 +60,7 @@ struct tmatrix_analysis {
     // merge eigenvalues closer than thold to the first of their group
     double group_first = eval[0];
     eigval_distinct.push_back(eval[0]);
+    index_distinct.push_back(0);
     int count = 1;
     auto close_group = [&](int last_index) {
       mult.push_back(count);
@@ -64,6 +70,7 @@ struct tmatrix_analysis {
       if (eval[i] - group_first > thold) {
         close_group(i - 1);
         eigval_distinct.push_back(eval[i]);
+        index_distinct.push_back(i);
         group_first = eval[i];
         count = 1;
       } else {
@@ -95,6 +102,7 @@ struct tmatrix_analysis {
       if (mult[i] == 0) continue;
       eigval_noghost.push_back(eigval_distinct[i]);
       mult_noghost.push_back(mult[i]);
+      index_noghost.push_back(index_distinct[i]);
       if (do_err) err_noghost.push_back(err[i]);
     }
   }

@@ -12,6 +12,7 @@
 #define ALPS_ED_IETL_TRIDIAG_H
 
 #include <cmath>
+#include <cstddef>
 #include <limits>
 #include <utility>
 #include <vector>
@@ -88,6 +89,69 @@ inline int eig(int n, double* d, const double* e_in, double* zlast) {
   }
   if (zlast)
     for (int i = 0; i < n; ++i) zlast[i] = z[i];
+  return 0;
+}
+
+// Eigenvalues (ascending, into d) and all eigenvectors (column-major n x n
+// into z) of the tridiagonal; same iteration, rotations applied to every row.
+inline int eig_full(int n, double* d, const double* e_in, double* z) {
+  if (n <= 0) return 0;
+  std::vector<double> e(n, 0.0);
+  for (int i = 0; i + 1 < n; ++i) e[i] = e_in[i];
+  for (int i = 0; i < n * n; ++i) z[i] = 0.0;
+  for (int i = 0; i < n; ++i) z[i + (std::size_t)i * n] = 1.0;
+  const double eps = std::numeric_limits<double>::epsilon();
+  for (int l = 0; l < n; ++l) {
+    for (int iter = 0;; ++iter) {
+      int m = l;
+      while (m < n - 1 && std::fabs(e[m]) > eps * (std::fabs(d[m]) + std::fabs(d[m + 1]))) ++m;
+      if (m == l) break;
+      if (iter >= 60) return l + 1;
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = std::hypot(g, 1.0);
+      g = d[m] - d[l] + e[l] / (g + std::copysign(r, g));
+      double s = 1.0, c = 1.0, p = 0.0;
+      bool split = false;
+      for (int i = m - 1; i >= l; --i) {
+        const double f = s * e[i];
+        const double b = c * e[i];
+        r = std::hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) {
+          d[i + 1] -= p;
+          e[m] = 0.0;
+          split = true;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        p = s * r;
+        d[i + 1] = g + p;
+        g = c * r - b;
+        double* zi = z + (std::size_t)i * n;
+        double* zi1 = z + (std::size_t)(i + 1) * n;
+        for (int k = 0; k < n; ++k) {
+          const double t = zi1[k];
+          zi1[k] = s * zi[k] + c * t;
+          zi[k] = c * zi[k] - s * t;
+        }
+      }
+      if (split) continue;
+      d[l] -= p;
+      e[l] = g;
+      e[m] = 0.0;
+    }
+  }
+  for (int i = 1; i < n; ++i) {  // insertion sort carrying the columns
+    int j = i;
+    while (j > 0 && d[j - 1] > d[j]) {
+      std::swap(d[j - 1], d[j]);
+      for (int k = 0; k < n; ++k) std::swap(z[k + (std::size_t)(j - 1) * n], z[k + (std::size_t)j * n]);
+      --j;
+    }
+  }
   return 0;
 }
 

@@ -216,3 +216,23 @@ def test_sparsediag_l24_fp32_vs_fp64():
     rel = np.abs(r32["ev"] - r64["ev"]) / np.abs(r64["ev"])
     say("rel err", rel.tolist())
     assert rel[0] <= 1e-10
+
+
+def test_ritz_vectors_from_kept_basis():
+    """Ritz vectors over the kept basis (lanczos::eigenvectors' basis
+    transformation) are eigenvectors of the oracle matrix once converged."""
+    om = O.lattice_model("chain lattice", 12)
+    op = A.Operator(to_product(om, 6))
+    lz = A.Lanczos(op, "f64", max_steps=60, keep_basis=True)
+    lz.start(seed=3)
+    lz.steps(59)
+    a, b = lz.coefficients()
+    T = np.diag(a) + np.diag(b[:-1], 1) + np.diag(b[:-1], -1)
+    w, Z = np.linalg.eigh(T)
+    ys = lz.ritz(Z[:, :1].T)
+    y = ys[0].numpy()
+    H = O.dense(om, 6)
+    e0 = np.linalg.eigvalsh(H)[0]
+    assert abs(w[0] - e0) < 1e-10
+    assert abs(np.linalg.norm(y) - 1.0) < 1e-8
+    assert np.linalg.norm(H @ y - e0 * y) < 1e-7
