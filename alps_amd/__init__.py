@@ -7,6 +7,7 @@ alps_amd/build.py).  There is no fallback: if the library is missing or no
 GPU is present, the calls raise.
 """
 import ctypes
+import json
 import os
 from dataclasses import dataclass
 
@@ -74,6 +75,8 @@ def lib():
             "alps_ed_lanczos_vectors": ([P, P, P], i32),
             "alps_ed_tmatrix_analyze": ([i32, P, P, P, P, P, P], i32),
             "alps_ed_sparsediag": ([P, P, P], i32),
+            "alps_ed_run_parameter_file": ([ctypes.c_char_p, P, ctypes.c_size_t, P], i32),
+            "alps_ed_describe_parameter_file": ([ctypes.c_char_p, P, ctypes.c_size_t, P], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -347,3 +350,25 @@ def sparsediag(model: SpinModel, max_iter=0, nev=1, seed=0, dtype="f64", fixed=F
     _check(lib().alps_ed_sparsediag(ctypes.byref(m), ctypes.byref(p), ctypes.byref(r)))
     return dict(dim=r.dim, ev=np.array(r.eigenvalues[: r.n_eigenvalues]), iterations=r.iterations,
                 error_code=r.error_code, seconds=r.seconds)
+
+
+def _json_call(fn, text):
+    need = ctypes.c_size_t()
+    buf = ctypes.create_string_buffer(1 << 16)
+    _check(fn(text.encode(), buf, len(buf), ctypes.byref(need)))
+    if need.value > len(buf):
+        buf = ctypes.create_string_buffer(need.value)
+        _check(fn(text.encode(), buf, len(buf), ctypes.byref(need)))
+    return json.loads(buf.value.decode())
+
+
+def describe_parameter_file(text):
+    """Tasks of a parameter file with their lattice, couplings and sectors
+    resolved on the host (no device needed)."""
+    return _json_call(lib().alps_ed_describe_parameter_file, text)["tasks"]
+
+
+def run_parameter_file(text):
+    """The sparsediag task for every task of a parameter file: per-sector
+    eigenvalues, ground-state energy and gap."""
+    return _json_call(lib().alps_ed_run_parameter_file, text)["tasks"]

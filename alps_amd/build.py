@@ -12,6 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libalps_ed.so")
+CLI = os.path.join(OUT_DIR, "alps_ed_sparsediag")
 
 SOURCES = ["ed_kernels.hip", "ed_capi.hip"]
 
@@ -46,6 +47,13 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    # command-line driver for parameter files, linked against the library
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(CSRC, "ed_sparsediag_main.cpp"), "-o", CLI,
+           "-L" + OUT_DIR, "-lalps_ed", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
     return LIB
 
 

@@ -16,6 +16,7 @@
 
 #include "alps_ed.h"
 #include "ed_kernels.hpp"
+#include "ed_sparsediag.hpp"
 
 using namespace alps_ed;
 
@@ -986,5 +987,136 @@ extern "C" int alps_ed_sparsediag(const alps_ed_model* model, const alps_ed_solv
     result->iterations = final_n;
     result->error_code = error_code;
     result->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
+namespace {
+
+std::string json_number(double v) {
+  if (std::isnan(v) || std::isinf(v)) return "null";
+  char b[40];
+  std::snprintf(b, sizeof b, "%.17g", v);
+  return b;
+}
+
+std::string json_string(const std::string& v) {
+  std::string o = "\"";
+  for (char c : v) {
+    if (c == '"' || c == '\\') o += '\\';
+    if ((unsigned char)c < 0x20) continue;
+    o += c;
+  }
+  return o + "\"";
+}
+
+std::string json_list(const std::vector<double>& v) {
+  std::string o = "[";
+  for (size_t i = 0; i < v.size(); ++i) o += (i ? ", " : "") + json_number(v[i]);
+  return o + "]";
+}
+
+std::string json_params(const Parameters& p) {
+  std::string js = "{";
+  bool first = true;
+  for (const auto& kv : p) {
+    js += (first ? "" : ", ") + json_string(kv.first) + ": " + json_string(kv.second);
+    first = false;
+  }
+  return js + "}";
+}
+
+void emit(const std::string& js, char* out, size_t out_len, size_t* needed) {
+  if (needed) *needed = js.size() + 1;
+  if (out && out_len) {
+    const size_t n = std::min(out_len - 1, js.size());
+    std::memcpy(out, js.data(), n);
+    out[n] = 0;
+  }
+}
+
+}  // namespace
+
+extern "C" int alps_ed_describe_parameter_file(const char* text, char* out, size_t out_len, size_t* needed) {
+  return guarded([&] {
+    require(text != nullptr, "parameter text is null");
+    const std::vector<Parameters> tasks = parse_parameter_file(text);
+    std::string js = "{\"tasks\": [";
+    for (size_t ti = 0; ti < tasks.size(); ++ti) {
+      const Parameters& p = tasks[ti];
+      js += ti ? ", {" : "{";
+      js += "\"parameters\": " + json_params(p);
+      if (p.count("LATTICE")) {
+        const TaskModel m = task_model(p);
+        std::vector<double> s1(m.s1.begin(), m.s1.end()), s2(m.s2.begin(), m.s2.end()), up;
+        const TaskSectors ts = task_sectors(p, m.n_sites);
+        for (int u : ts.n_up) up.push_back(u);
+        js += ", \"n_sites\": " + std::to_string(m.n_sites) + ", \"bond_s\": " + json_list(s1) +
+              ", \"bond_t\": " + json_list(s2) + ", \"jz\": " + json_list(m.jz) + ", \"jxy\": " + json_list(m.jxy) +
+              ", \"k\": " + json_list(m.k) + ", \"h\": " + json_list(m.h) + ", \"d\": " + json_list(m.d) +
+              ", \"n_up\": " + json_list(up) + ", \"whole_space\": " + (ts.whole_space ? "true" : "false");
+      }
+      js += "}";
+    }
+    js += "]}";
+    emit(js, out, out_len, needed);
+  });
+}
+
+extern "C" int alps_ed_run_parameter_file(const char* text, char* out, size_t out_len, size_t* needed) {
+  return guarded([&] {
+    require(text != nullptr, "parameter text is null");
+    const std::vector<Parameters> tasks = parse_parameter_file(text);
+    std::string js = "{\"tasks\": [";
+    for (size_t ti = 0; ti < tasks.size(); ++ti) {
+      const Parameters& p = tasks[ti];
+      const TaskModel m = task_model(p);
+      Evaluator ev(p);
+      alps_ed_solve_params sp{};
+      sp.max_iter = ev.defined("MAX_ITERATIONS") ? (int)std::lround(ev.param("MAX_ITERATIONS")) : 0;
+      sp.nev = ev.defined("NUMBER_EIGENVALUES") ? (int)std::lround(ev.param("NUMBER_EIGENVALUES")) : 1;
+      sp.fixed_iterations = 0;
+      sp.seed = ev.defined("ED_SEED") ? (uint64_t)std::llround(ev.param("ED_SEED")) : 0u;
+      sp.dtype = (p.count("ED_PRECISION") && p.at("ED_PRECISION") == "f32") ? ALPS_ED_F32 : ALPS_ED_F64;
+      js += ti ? ", {" : "{";
+      js += "\"parameters\": " + json_params(p) + ", \"n_sites\": " + std::to_string(m.n_sites) + ", \"sectors\": [";
+      const TaskSectors ts = task_sectors(p, m.n_sites);
+      std::vector<std::vector<double>> evs;
+      std::vector<std::string> entries;
+      uint64_t dim_sum = 0;
+      int it_sum = 0, err_max = 0;
+      double sec_sum = 0.0;
+      for (int n_up : ts.n_up) {
+        alps_ed_model am{m.n_sites, n_up, (int)m.s1.size(), m.s1.data(), m.s2.data(), m.jz.data(),
+                         m.jxy.data(),  m.k.data(), m.h.data(), m.d.data()};
+        alps_ed_solve_result r{};
+        check_status(alps_ed_sparsediag(&am, &sp, &r));
+        if (r.dim == 0) continue;  // empty subspaces are skipped (diag.h:115)
+        std::vector<double> e(r.eigenvalues, r.eigenvalues + r.n_eigenvalues);
+        evs.push_back(e);
+        dim_sum += r.dim;
+        it_sum += r.iterations;
+        err_max = std::max(err_max, r.error_code);
+        sec_sum += r.seconds;
+        entries.push_back("\"Sz_total\": " + json_number(n_up - 0.5 * m.n_sites) + ", \"n_up\": " +
+                          std::to_string(n_up) + ", \"dim\": " + std::to_string(r.dim) + ", \"iterations\": " +
+                          std::to_string(r.iterations) + ", \"error_code\": " + std::to_string(r.error_code) +
+                          ", \"seconds\": " + json_number(r.seconds) + ", \"eigenvalues\": " + json_list(e));
+      }
+      if (ts.whole_space) {
+        // one subspace without quantum numbers: the merged spectrum
+        const std::vector<double> e = merge_spectra(evs, sp.nev, 1e-9);
+        evs.assign(1, e);
+        entries.assign(1, "\"Sz_total\": null, \"n_up\": null, \"dim\": " + std::to_string(dim_sum) +
+                              ", \"iterations\": " + std::to_string(it_sum) + ", \"error_code\": " +
+                              std::to_string(err_max) + ", \"seconds\": " + json_number(sec_sum) +
+                              ", \"eigenvalues\": " + json_list(e));
+      }
+      for (size_t i = 0; i < entries.size(); ++i) js += (i ? ", {" : "{") + entries[i] + "}";
+      double g = 0, gap = 0;
+      ground_and_gap(evs, g, gap);
+      js += "], \"ground_state_energy\": " + json_number(g) + ", \"energy_gap\": " + json_number(gap) + "}";
+    }
+    js += "]}";
+    emit(js, out, out_len, needed);
   });
 }

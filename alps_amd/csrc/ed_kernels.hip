@@ -90,70 +90,110 @@ __global__ __launch_bounds__(kBlock) void hv_kernel(DevLayout L, const V* __rest
   const double* dlp = L.diag_lo_cls + cls0;
   __syncthreads();
 
+  // Each thread carries up to kRows rows (r = tid + q*kBlock) through every
+  // bond chunk, so the chunk's scalar data is loaded once.  Row slots past
+  // the segment are skipped with a uniform test.
+  const uint32_t nq = (len + kBlock - 1) / kBlock;
+  uint32_t rc[kRows];
+  bool ok[kRows];
+  double d[kRows], off[kRows];
+#pragma unroll
+  for (int q = 0; q < kRows; ++q) {
+    const uint32_t r = tid + q * kBlock;
+    ok[q] = r < len;
+    rc[q] = ok[q] ? r : 0u;  // clamped row: every read stays in bounds
+    d[q] = dh;
+    off[q] = 0.0;
+    if (q < (int)nq) d[q] += dlp[rc[q]];
+  }
+  // low-low flips: target rows from the class table, values from LDS
+  for (int b0 = 0; b0 < L.n_ll_p; b0 += kChunkLL) {
+    double t[kChunkLL];
+#pragma unroll
+    for (int c = 0; c < kChunkLL; ++c) t[c] = ldc(L.ll_t + b0 + c);
+#pragma unroll
+    for (int q = 0; q < kRows; ++q) {
+      if (q < (int)nq) {
+        uint32_t j[kChunkLL];
+#pragma unroll
+        for (int c = 0; c < kChunkLL; ++c) j[c] = (nbp + (b0 + c) * len)[rc[q]];
+        V v[kChunkLL];
+#pragma unroll
+        for (int c = 0; c < kChunkLL; ++c) v[c] = xs[j[c]];
+#pragma unroll
+        for (int c = 0; c < kChunkLL; ++c) off[q] += t[c] * (double)v[c];
+      }
+    }
+  }
+  // high-high flips: uniform decision, contiguous target run
+  for (int b0 = 0; b0 < L.n_hh_p; b0 += kChunkHH) {
+    const V* xt[kChunkHH];
+    double tt[kChunkHH];
+#pragma unroll
+    for (int c = 0; c < kChunkHH; ++c) {
+      const uint32_t m = ldc(L.hh_mask + b0 + c);
+      const uint32_t bits = h & m;
+      const bool act = bits != 0u && bits != m;
+      xt[c] = act ? x + ldc(L.hi_base + (h ^ m)) : xb;
+      tt[c] = act ? ldc(L.hh_t + b0 + c) : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < kRows; ++q) {
+      if (q < (int)nq) {
+        V v[kChunkHH];
+#pragma unroll
+        for (int c = 0; c < kChunkHH; ++c) v[c] = xt[c][rc[q]];
+#pragma unroll
+        for (int c = 0; c < kChunkHH; ++c) off[q] += tt[c] * (double)v[c];
+      }
+    }
+  }
+  // cross bonds: diagonal always, flip when the high and low bits differ.
+  // If the neighbouring high word has no segment no row can flip (its low
+  // word would need a set bit more or fewer than it has), and the read
+  // falls back to this segment.
+  for (int b0 = 0; b0 < L.n_xb_p; b0 += kChunkX) {
+    const V* xn[kChunkX];
+    bool hb[kChunkX];
+    double ea[kChunkX], ep[kChunkX], tx[kChunkX];
+#pragma unroll
+    for (int c = 0; c < kChunkX; ++c) {
+      const uint32_t hbit = ldc(L.xb_hbit + b0 + c);
+      hb[c] = (h & hbit) != 0u;
+      const uint32_t nb = ldc(L.hi_base + (h ^ hbit));
+      xn[c] = x + (nb == 0xffffffffu ? base : nb);
+      ea[c] = ldc(L.xb_anti + b0 + c);
+      ep[c] = ldc(L.xb_par + b0 + c);
+      tx[c] = ldc(L.xb_t + b0 + c);
+    }
+#pragma unroll
+    for (int q = 0; q < kRows; ++q) {
+      if (q < (int)nq) {
+        uint32_t e[kChunkX];
+#pragma unroll
+        for (int c = 0; c < kChunkX; ++c) e[c] = (xnp + (b0 + c) * len)[rc[q]];
+        V v[kChunkX];
+        bool f[kChunkX];
+#pragma unroll
+        for (int c = 0; c < kChunkX; ++c) {
+          f[c] = ((e[c] >> 15) != 0u) != hb[c];
+          v[c] = xn[c][f[c] ? (e[c] & 0x7fffu) : 0u];
+        }
+#pragma unroll
+        for (int c = 0; c < kChunkX; ++c) {
+          d[q] += f[c] ? ea[c] : ep[c];
+          off[q] += (f[c] ? tx[c] : 0.0) * (double)v[c];
+        }
+      }
+    }
+  }
   double dotp = 0.0;
-  for (uint32_t r0 = 0; r0 < len; r0 += kBlock) {
-    const uint32_t r = r0 + tid;
-    const bool ok = r < len;
-    const uint32_t rc = ok ? r : 0u;  // clamped row: every read stays in bounds
-    double d = dh + dlp[rc];
-    double off = 0.0;
-    // low-low flips: target rows from the class table, values from LDS
-    for (int b0 = 0; b0 < L.n_ll_p; b0 += kChunkLL) {
-      uint32_t j[kChunkLL];
 #pragma unroll
-      for (int c = 0; c < kChunkLL; ++c) j[c] = (nbp + (b0 + c) * len)[rc];
-      V v[kChunkLL];
-#pragma unroll
-      for (int c = 0; c < kChunkLL; ++c) v[c] = xs[j[c]];
-#pragma unroll
-      for (int c = 0; c < kChunkLL; ++c) off += ldc(L.ll_t + b0 + c) * (double)v[c];
-    }
-    // high-high flips: uniform decision, contiguous target run
-    for (int b0 = 0; b0 < L.n_hh_p; b0 += kChunkHH) {
-      const V* xt[kChunkHH];
-      double tt[kChunkHH];
-#pragma unroll
-      for (int c = 0; c < kChunkHH; ++c) {
-        const uint32_t m = ldc(L.hh_mask + b0 + c);
-        const uint32_t bits = h & m;
-        const bool act = bits != 0u && bits != m;
-        xt[c] = act ? x + ldc(L.hi_base + (h ^ m)) : xb;
-        tt[c] = act ? ldc(L.hh_t + b0 + c) : 0.0;
-      }
-      V v[kChunkHH];
-#pragma unroll
-      for (int c = 0; c < kChunkHH; ++c) v[c] = xt[c][rc];
-#pragma unroll
-      for (int c = 0; c < kChunkHH; ++c) off += tt[c] * (double)v[c];
-    }
-    // cross bonds: diagonal always, flip when the high and low bits differ.
-    // If the neighbouring high word has no segment no row can flip (its low
-    // word would need a set bit more or fewer than it has), and the read
-    // falls back to this segment.
-    for (int b0 = 0; b0 < L.n_xb_p; b0 += kChunkX) {
-      uint32_t e[kChunkX];
-#pragma unroll
-      for (int c = 0; c < kChunkX; ++c) e[c] = (xnp + (b0 + c) * len)[rc];
-      V v[kChunkX];
-      double tt[kChunkX];
-#pragma unroll
-      for (int c = 0; c < kChunkX; ++c) {
-        const uint32_t hbit = ldc(L.xb_hbit + b0 + c);
-        const bool hb = (h & hbit) != 0u;
-        const uint32_t nb = ldc(L.hi_base + (h ^ hbit));
-        const V* xn = x + (nb == 0xffffffffu ? base : nb);
-        const bool flip = ((e[c] >> 15) != 0u) != hb;
-        d += flip ? ldc(L.xb_anti + b0 + c) : ldc(L.xb_par + b0 + c);
-        v[c] = xn[flip ? (e[c] & 0x7fffu) : 0u];
-        tt[c] = flip ? ldc(L.xb_t + b0 + c) : 0.0;
-      }
-#pragma unroll
-      for (int c = 0; c < kChunkX; ++c) off += tt[c] * (double)v[c];
-    }
-    const double xi = (double)xs[rc];
-    const double yv = d * xi + off;
-    if (ok) {
-      y[base + rc] = (V)yv;
+  for (int q = 0; q < kRows; ++q) {
+    if (ok[q]) {
+      const double xi = (double)xs[rc[q]];
+      const double yv = d[q] * xi + off[q];
+      y[base + rc[q]] = (V)yv;
       if (DOT) dotp += xi * yv;  // FP64 value before the storage rounding
     }
   }

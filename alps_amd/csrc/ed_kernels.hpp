@@ -56,6 +56,8 @@ struct Partials {
 
 constexpr int kBlock = 256;
 constexpr int kMaxRitz = 8;     // Ritz vectors per pass of launch_ritz
+// H.v rows per thread: a segment of up to kRows*kBlock rows is one workgroup
+constexpr int kRows = 4;
 
 // Number of blocks of the streaming kernels for a vector of length n.
 inline unsigned stream_blocks(uint64_t n) {

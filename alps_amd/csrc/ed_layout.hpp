@@ -159,7 +159,9 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1) {
   L.k = m.n_up;
   L.D = (m.n_up < 0 || m.n_up > N) ? 0 : binom(N, m.n_up);
   if (L.D >= 0xffffffffULL) throw std::invalid_argument("sector dimension exceeds 2^32-1");
-  const int B = B_override > 0 && B_override <= N && B_override <= 16 ? B_override : choose_low_bits(N);
+  // segments longer than 1024 rows (kRows * kBlock of the H.v kernel) are
+  // not supported, which bounds the low word to 12 bits
+  const int B = B_override > 0 && B_override <= N && B_override <= 12 ? B_override : choose_low_bits(N);
   L.B = B;
   L.NH = N - B;
   const uint32_t nlo = 1u << B;

@@ -130,6 +130,20 @@ def main():
     ms_per_hv = ms / args.steps
     gbs = bytes_per_hv / (ms_per_hv * 1e-3) / 1e9
 
+    # plain device copy of the same FP32 vector (4*D read + 4*D written):
+    # the practical streaming rate, and the reference kernel of known size
+    # for scaling the HBM byte counters of a profile
+    nc = max(1, min(args.steps, 50))
+    for _ in range(3):
+        y.copy_from(x)
+    ev.sync()
+    e4, e5 = ev.event(), ev.event()
+    ev.record(e4)
+    for _ in range(nc):
+        y.copy_from(x)
+    ev.record(e5)
+    copy_gbs = bytes_per_hv / (ev.elapsed_ms(e4, e5) / nc * 1e-3) / 1e9
+
     # device-resident Lanczos step (H.v with alpha + update with beta)
     nl = min(args.steps, 400)
     lz = A.Lanczos(op, "f32", max_steps=nl + 2)
@@ -160,6 +174,7 @@ def main():
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "hv_kernel<float,false>", "bytes_per_launch": bytes_per_hv},
         "timed_wall_s": round(wall, 4),
+        "copy_gbs": round(copy_gbs, 2),
         "lanczos_ms_per_step": round(ms_lanczos, 5),
         "device": info["name"],
     }

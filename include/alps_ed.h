@@ -202,6 +202,26 @@ typedef struct {
 int alps_ed_sparsediag(const alps_ed_model* model, const alps_ed_solve_params* params,
                        alps_ed_solve_result* result);
 
+/* The sparsediag task for the spin-1/2 path: runs every task of a parameter
+ * file (the grammar of src/alps/parameter/parameterlist.h:20-36; LATTICE,
+ * L, W, H, MODEL="spin", local_S=1/2, J/Jz/Jxy/J'/J1/Jz#/Jxy#/K#/h#/D#,
+ * CONSERVED_QUANTUMNUMBERS, Sz_total, MAX_ITERATIONS, NUMBER_EIGENVALUES,
+ * plus ED_PRECISION = "f64" | "f32" and ED_SEED) as
+ * DiagMatrix::dostep does (applications/diag/diag.h:85-175): one Lanczos
+ * solve per Sz sector, then the ground-state energy and gap of
+ * SparseDiagMatrix::write_xml_body (sparsediag.h:59-90).  The results are
+ * written to `out` as a JSON document (at most out_len bytes including the
+ * terminating zero; *needed receives the full length). */
+int alps_ed_run_parameter_file(const char* text, char* out, size_t out_len, size_t* needed);
+
+/* The same parameter file resolved on the host only, without solving: per
+ * task its parameters, the lattice's sites and bonds in edge order (the
+ * graph of make_graph_from_lattice, src/alps/lattice/latticegraph.h:35-171),
+ * the per-bond Jz, Jxy, K and per-site h, D after the spin model's defaults
+ * (lib/xml/models.xml.in:342-373), and the Sz sectors to solve
+ * (applications/diag/diag.h:142-175).  Needs no device. */
+int alps_ed_describe_parameter_file(const char* text, char* out, size_t out_len, size_t* needed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -128,6 +128,23 @@ int oracle_tmatrix(int n, const double* alpha, const double* beta, double* eig, 
   }
 }
 
+// write_xml_body's ground-state energy and gap over sector results; counts[i]
+// values of sector i follow each other in `values`.  The gap is DBL_MAX when
+// undefined, as the reference's sentinel.
+int oracle_ground_and_gap(int n_sectors, const int* counts, const double* values, double* ground,
+                          double* gap) {
+  std::vector<std::vector<double>> sec(n_sectors);
+  size_t o = 0;
+  for (int i = 0; i < n_sectors; ++i) {
+    sec[i].assign(values + o, values + o + counts[i]);
+    o += counts[i];
+  }
+  std::pair<double, double> r = edo::ground_and_gap(sec);
+  *ground = r.first;
+  *gap = r.second;
+  return 0;
+}
+
 double oracle_start_value(uint64_t seed, uint64_t i, double lo, double hi) {
   return edo::start_value(seed, i, lo, hi);
 }

@@ -1,0 +1,81 @@
+// Host-only check of include/ietl/{lanczos,tmatrix,iteration,vectorspace}.h:
+// reads a CSR matrix, runs ietl::lanczos with std::vector<double> vectors
+// and prints alpha/beta and the distinct non-ghost eigenvalues.
+//   usage: ietl_host_lanczos <csr file> <max_iter> <nev> <seed> <fixed 0|1>
+// CSR file: D nnz, then D+1 row pointers, nnz columns, nnz values (text).
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <vector>
+
+#include <ietl/vectorspace.h>
+
+struct csr_matrix {
+  std::vector<uint64_t> rowptr;
+  std::vector<uint32_t> col;
+  std::vector<double> val;
+};
+
+namespace ietl {
+// y = prod(m, x): per row, ascending columns (interface/ublas.h:57-63)
+inline void mult(const csr_matrix& m, const std::vector<double>& x, std::vector<double>& y) {
+  for (size_t i = 0; i + 1 < m.rowptr.size(); ++i) {
+    double t = 0.0;
+    for (uint64_t p = m.rowptr[i]; p < m.rowptr[i + 1]; ++p) t += m.val[p] * x[m.col[p]];
+    y[i] = t;
+  }
+}
+}  // namespace ietl
+
+// the matrix binding must be visible before the algorithm, as with the
+// reference's interface headers
+#include <ietl/lanczos.h>
+
+// splitmix64 counter generator, uniform on [-1, 1)
+struct start_gen {
+  uint64_t seed, i = 0;
+  double operator()() {
+    uint64_t z = seed + (++i) * 0x9e3779b97f4a7c15ULL;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    z ^= z >> 31;
+    return -1.0 + 2.0 * ((double)(z >> 11) * (1.0 / 9007199254740992.0));
+  }
+};
+
+int main(int argc, char** argv) {
+  if (argc < 6) return 2;
+  std::ifstream in(argv[1]);
+  size_t D, nnz;
+  in >> D >> nnz;
+  csr_matrix m;
+  m.rowptr.resize(D + 1);
+  m.col.resize(nnz);
+  m.val.resize(nnz);
+  for (auto& v : m.rowptr) in >> v;
+  for (auto& v : m.col) in >> v;
+  for (auto& v : m.val) in >> v;
+  const int max_iter = atoi(argv[2]), nev = atoi(argv[3]);
+  const uint64_t seed = strtoull(argv[4], nullptr, 10);
+  typedef ietl::vectorspace<std::vector<double> > vs_t;
+  vs_t vs(D);
+  ietl::lanczos<csr_matrix, vs_t> lz(m, vs);
+  int iterations = 0, err = 0;
+  if (atoi(argv[5])) {
+    ietl::fixed_lanczos_iteration<double> it(max_iter);
+    lz.calculate_eigenvalues(it, start_gen{seed});
+    iterations = it.iterations();
+    err = it.error_code();
+  } else {
+    ietl::lanczos_iteration_nlowest<double> it(max_iter, nev);
+    lz.calculate_eigenvalues(it, start_gen{seed});
+    iterations = it.iterations();
+    err = it.error_code();
+  }
+  printf("iterations %d error %d\n", iterations, err);
+  for (size_t i = 0; i < lz.diagonal().size(); ++i) printf("ab %.17g %.17g\n", lz.diagonal()[i], lz.subdiagonal()[i]);
+  const std::vector<double>& ev = lz.eigenvalues();
+  for (size_t i = 0; i < ev.size() && i < (size_t)nev; ++i) printf("ev %.17g\n", ev[i]);
+  return 0;
+}

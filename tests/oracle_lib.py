@@ -43,6 +43,8 @@ def lib():
         L.oracle_sparsediag.restype = i32
         L.oracle_bench_hv.argtypes = [i32, i32] + model + [i64, i32, P, P]
         L.oracle_bench_hv.restype = f64
+        L.oracle_ground_and_gap.argtypes = [i32, P, P, P, P]
+        L.oracle_ground_and_gap.restype = i32
         _lib = L
     return _lib
 
@@ -178,6 +180,15 @@ def tmatrix(alpha, beta):
 
 def start_value(seed, i, lo=-1.0, hi=1.0):
     return lib().oracle_start_value(seed, i, lo, hi)
+
+
+def ground_and_gap(sectors):
+    """Ground-state energy and gap over per-sector eigenvalue lists."""
+    counts = np.array([len(v) for v in sectors], dtype=np.int32)
+    vals = np.concatenate([np.asarray(v, dtype=np.float64) for v in sectors] + [np.zeros(0)])
+    g, d = ctypes.c_double(), ctypes.c_double()
+    assert lib().oracle_ground_and_gap(len(sectors), _p(counts), _p(vals), ctypes.byref(g), ctypes.byref(d)) == 0
+    return g.value, d.value
 
 
 def sparsediag(m: Model, n_up, max_iter=0, nev=1, seed=0, fixed=False, max_ab=4096):

@@ -5,6 +5,8 @@ Bars: bit-exact for basis states, indices and matrix elements; FP64 H.v to
 relative (the reference's own ED comparisons use
 max(1e-11, 1e-11|ref|), lib/pyalps/apptest.py:388)."""
 import math
+import os
+import subprocess
 import sys
 
 import numpy as np
@@ -236,3 +238,80 @@ def test_ritz_vectors_from_kept_basis():
     assert abs(w[0] - e0) < 1e-10
     assert abs(np.linalg.norm(y) - 1.0) < 1e-8
     assert np.linalg.norm(H @ y - e0 * y) < 1e-7
+
+
+IETL_GPU = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "_build", "ietl_gpu_lanczos")
+
+
+@pytest.mark.parametrize("dtype,rtol", [("f64", 1e-10), ("f32", 1e-6)])
+def test_ietl_lanczos_on_device_vectorspace(dtype, rtol):
+    """ietl::lanczos<alps_ed::hamiltonian, alps_ed::gpu_vectorspace> (the C++
+    drop-in, generic recurrence with per-step host T analysis).  FP32 storage
+    reports the T-matrix eigenvalues unrefined, so its bar is 1e-6."""
+    om = O.lattice_model("chain lattice", 14)
+    ref = O.sparsediag(om, 7, nev=3)
+    out = subprocess.run([IETL_GPU, "14", "7", dtype, "1000", "3", "5"], capture_output=True, text=True,
+                         timeout=120)
+    say(out.stdout)
+    assert out.returncode == 0, out.stdout + out.stderr
+    ev = [float(l.split()[1]) for l in out.stdout.split("\n") if l.startswith("ev ")]
+    np.testing.assert_allclose(ev[:1], ref["ev"][:1], rtol=rtol, atol=0)
+    if dtype == "f64":
+        np.testing.assert_allclose(ev, ref["ev"], rtol=rtol, atol=0)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_parameter_file_sectors_ground_and_gap(prec):
+    # a sparsediag parameter file with Sz conserved: every sector's spectrum,
+    # the ground-state energy and the gap against the oracle
+    text = ('LATTICE="open ladder"; L=5; J0=1; J1=0.7; h=0.1; NUMBER_EIGENVALUES=2; '
+            f'CONSERVED_QUANTUMNUMBERS="Sz"; ED_PRECISION="{prec}"; {{}}')
+    say("parameter file", prec)
+    (task,) = A.run_parameter_file(text)
+    om = O.lattice_model("open ladder", 5, 0, h=0.1, by_type={1: (0.7, 0.7)})
+    assert [s["n_up"] for s in task["sectors"]] == list(range(11))
+    ref = []
+    for s in task["sectors"]:
+        r = O.sparsediag(om, s["n_up"], nev=2)
+        ref.append(r["ev"])
+        assert s["dim"] == r["dim"] and s["error_code"] == 0
+        np.testing.assert_allclose(s["eigenvalues"], r["ev"], rtol=1e-10, atol=1e-12)
+    g, gap = O.ground_and_gap(ref)
+    assert task["ground_state_energy"] == pytest.approx(g, rel=1e-10)
+    assert task["energy_gap"] == pytest.approx(gap, rel=1e-8, abs=1e-10)
+
+
+def test_parameter_file_whole_space():
+    # no conserved quantum numbers: the lowest distinct eigenvalues of the
+    # whole 2^N space, from the dense spectra of all sectors together
+    text = 'LATTICE="chain lattice"; L=9; J=1; NUMBER_EIGENVALUES=3; {} Sz_total=0.5; {}'
+    say("parameter file, whole space")
+    whole, one = A.run_parameter_file(text)
+    om = O.lattice_model("chain lattice", 9)
+    spec = np.sort(np.concatenate([np.linalg.eigvalsh(O.dense(om, k)) for k in range(10)]))
+    distinct = [spec[0]]
+    for x in spec[1:]:
+        if x - distinct[-1] > 1e-9:
+            distinct.append(x)
+    (s,) = whole["sectors"]
+    assert s["dim"] == 2 ** 9 and s["Sz_total"] is None
+    np.testing.assert_allclose(s["eigenvalues"], distinct[:3], rtol=1e-10)
+    # odd ring: the ground doublet sits in Sz=+-1/2, so the gap between
+    # sectors is zero while the whole space reports the next distinct level
+    assert whole["energy_gap"] == pytest.approx(distinct[1] - distinct[0], rel=1e-9)
+    (s1,) = one["sectors"]
+    assert s1["n_up"] == 5 and s1["dim"] == math.comb(9, 5)
+    assert s1["eigenvalues"][0] == pytest.approx(distinct[0], rel=1e-10)
+
+
+def test_cli_runs_parameter_file(tmp_path):
+    import json
+
+    f = tmp_path / "parm"
+    f.write_text('LATTICE="chain lattice"\nL=8\nJ=1\nSz_total=0\nNUMBER_EIGENVALUES=2\n{}\n')
+    cli = os.path.join(os.path.dirname(A.LIB_PATH), "alps_ed_sparsediag")
+    say("cli", cli)
+    out = subprocess.run([cli, str(f)], check=True, capture_output=True, text=True, timeout=60).stdout
+    (t,) = json.loads(out)["tasks"]
+    r = O.sparsediag(O.lattice_model("chain lattice", 8), 4, nev=2)
+    np.testing.assert_allclose(t["sectors"][0]["eigenvalues"], r["ev"], rtol=1e-10)

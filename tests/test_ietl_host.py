@@ -1,0 +1,41 @@
+"""The product's ietl templates (include/ietl/*.h) instantiated with host
+vectors, against the oracle: same start vector, same recurrence, so alpha,
+beta and the reported eigenvalues agree to rounding."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "_build", "ietl_host_lanczos")
+
+
+def run(m, k, max_iter, nev, seed, fixed, tmp_path):
+    rp, col, val = O.build_csr(m, k)
+    f = tmp_path / "csr.txt"
+    with open(f, "w") as fh:
+        fh.write(f"{len(rp) - 1} {len(col)}\n")
+        fh.write(" ".join(map(str, rp.tolist())) + "\n")
+        fh.write(" ".join(map(str, col.tolist())) + "\n")
+        fh.write(" ".join(repr(float(v)) for v in val) + "\n")
+    out = subprocess.run([BIN, str(f), str(max_iter), str(nev), str(seed), str(int(fixed))],
+                         capture_output=True, text=True, check=True).stdout.split("\n")
+    ab = np.array([list(map(float, l.split()[1:])) for l in out if l.startswith("ab ")])
+    ev = np.array([float(l.split()[1]) for l in out if l.startswith("ev ")])
+    its = int(out[0].split()[1])
+    return ab, ev, its
+
+
+@pytest.mark.parametrize("name,L,k,fixed", [("chain lattice", 12, 6, False), ("chain lattice", 10, 4, True),
+                                            ("frustrated square lattice", 4, 8, False)])
+def test_host_ietl_lanczos_matches_oracle(name, L, k, fixed, tmp_path):
+    m = O.lattice_model(name, L, by_type={1: (0.5, 0.5)} if "frustrated" in name else None)
+    ref = O.sparsediag(m, k, max_iter=60 if fixed else 0, nev=3, seed=7, fixed=fixed)
+    ab, ev, its = run(m, k, 60 if fixed else min(10 * len(O.basis(m.n_sites, k)), 1000), 3, 7, fixed, tmp_path)
+    assert its == ref["iterations"]
+    assert len(ab) == len(ref["alpha"])
+    np.testing.assert_allclose(ab[:, 0], ref["alpha"], rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(ab[:, 1], ref["beta"], rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(ev, ref["ev"], rtol=1e-12, atol=1e-12)

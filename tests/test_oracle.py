@@ -275,6 +275,41 @@ def test_lanczos_ring_of_four():
     np.testing.assert_allclose(r["ev"], [-2.0, -1.0], atol=1e-14)
 
 
+def _gap_by_sorting(sectors):
+    # the gap read from the multiset of every sector's lowest value plus the
+    # second value of the sector holding the global minimum (its first
+    # occurrence): sorted, second minus first
+    firsts = [v[0] for v in sectors if len(v)]
+    g = min(firsts)
+    ground_sector = next(v for v in sectors if len(v) and v[0] == g)
+    pool = sorted(firsts + list(ground_sector[1:2]))
+    return g, (pool[1] - pool[0]) if len(pool) > 1 else np.finfo(float).max
+
+
+def test_ground_and_gap_ring_of_four():
+    # Heisenberg ring L=4: singlet -2 in Sz=0, triplet -1 in Sz=0,+-1,
+    # quintet +1 in Sz=+-2: ground -2, gap 1 (closed form)
+    m = O.lattice_model("chain lattice", 4)
+    sectors = [O.sparsediag(m, k, nev=2)["ev"] for k in range(5)]
+    g, gap = O.ground_and_gap(sectors)
+    assert abs(g + 2.0) < 1e-13 and abs(gap - 1.0) < 1e-13
+
+
+def test_ground_and_gap_cases():
+    rng = np.random.default_rng(11)
+    cases = [[[0.0, 1.0]], [[-1.0]], [[], [2.0, 3.0], []], [[1.0, 5.0], [1.0, 2.0]],   # degenerate: gap 0
+             [[-3.0, -2.5], [-2.9], [-2.0, -1.0]], [[-2.9], [-3.0, -2.5]], [[-3.0, -1.0], [-2.0]]]
+    for _ in range(200):
+        cases.append([sorted(rng.integers(-4, 4, rng.integers(0, 3)).astype(float).tolist())
+                      for _ in range(rng.integers(1, 5))])
+    for sec in cases:
+        if not any(len(v) for v in sec):
+            continue
+        g, gap = O.ground_and_gap(sec)
+        g2, gap2 = _gap_by_sorting(sec)
+        assert g == g2 and gap == gap2, sec
+
+
 def test_lanczos_three_term_relation():
     # alpha/beta from the recurrence reproduce H's spectrum extremes and the
     # first step's alpha is the Rayleigh quotient of the start vector
