@@ -925,12 +925,15 @@ extern "C" int alps_ed_sparsediag(const alps_ed_model* model, const alps_ed_solv
       result->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       return;
     }
-    const bool fp32 = params->dtype == ALPS_ED_F32;
+    // FP32 storage only pays, and only keeps the refined eigenvalues at
+    // ~1e-11, where the sector is much larger than the Krylov space the
+    // solve builds; small sectors are solved with FP64 vectors.
+    const bool fp32 = params->dtype == ALPS_ED_F32 && D >= kFp32MinDim;
     const int max_iter = params->max_iter > 0 ? params->max_iter : (int)std::min<uint64_t>(10 * D, 1000);
     const int nev = params->nev > 0 ? params->nev : 1;
     require(nev <= 64, "nev must be <= 64");
     alps_ed_lanczos lzh = nullptr;
-    check_status(alps_ed_lanczos_create(op.get(), params->dtype, max_iter, fp32 ? 1 : 0, &lzh));
+    check_status(alps_ed_lanczos_create(op.get(), fp32 ? ALPS_ED_F32 : ALPS_ED_F64, max_iter, fp32 ? 1 : 0, &lzh));
     std::unique_ptr<alps_ed_lanczos_s> lz(lzh);
     check_status(alps_ed_lanczos_start(lz.get(), params->seed, nullptr));
     std::vector<double> a(max_iter + 1), b(max_iter + 1);

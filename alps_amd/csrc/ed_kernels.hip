@@ -21,186 +21,23 @@
 // consumers add the partials in index order.
 #include <hip/hip_runtime.h>
 
+#include "ed_device.hpp"
 #include "ed_kernels.hpp"
 
 namespace alps_ed {
 
 namespace {
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-// Block sum, returned to every thread.
-__device__ __forceinline__ double block_sum_all(double v) {
-  __shared__ double red_smem[kBlock / 64];
-  v = wave_sum(v);
-  const int wave = threadIdx.x >> 6;
-  __syncthreads();  // red_smem may still be read by a previous call
-  if ((threadIdx.x & 63) == 0) red_smem[wave] = v;
-  __syncthreads();
-  double s = 0.0;
-#pragma unroll
-  for (int w = 0; w < kBlock / 64; ++w) s += red_smem[w];
-  return s;
-}
-
-__device__ __forceinline__ double sum_partials(const Partials& P) {
-  double v = 0.0;
-  for (uint32_t i = threadIdx.x; i < P.n; i += kBlock) v += P.p[i];
-  return block_sum_all(v);
-}
-
-__device__ __forceinline__ void store_partial(double* out, double v) {
-  const double s = block_sum_all(v);
-  if (threadIdx.x == 0) out[blockIdx.x] = s;
-}
-
-
-// Uniform reads of the layout tables.  The tables are never written while a
-// kernel runs, so they are read through the constant address space, which
-// lets the compiler use scalar loads for wave-uniform addresses.
-template <class T>
-__device__ __forceinline__ T ldc(const T* p) {
-  return *(const __attribute__((address_space(4))) T*)(p);
-}
+using namespace dev;
 
 // ---------------------------------------------------------------------------
-// H.v
+// H.v: one workgroup per segment, longest segments first
 // ---------------------------------------------------------------------------
 template <class V, bool DOT>
 __global__ __launch_bounds__(kBlock) void hv_kernel(DevLayout L, const V* __restrict__ x, V* __restrict__ y,
                                                      double* __restrict__ dot_out) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  V* xs = reinterpret_cast<V*>(smem);  // segment slice; entry len is the zero slot
-  const uint32_t tid = threadIdx.x;
   const uint32_t h = ldc(L.proc_hi + blockIdx.x);
-  const int kl = L.k - __builtin_popcount(h);
-  const uint32_t cls0 = ldc(L.lo_class_off + kl);
-  const uint32_t len = ldc(L.lo_class_off + kl + 1) - cls0;
-  const uint32_t base = ldc(L.hi_base + h);
-  const double dh = ldc(L.diag_hi + h);
-  const V* xb = x + base;
-  for (uint32_t r = tid; r < len; r += kBlock) xs[r] = xb[r];
-  if (tid == 0) xs[len] = (V)0;
-  const uint16_t* nbp = L.nbp16 + ldc(L.nbp_off + kl);
-  const uint16_t* xnp = L.xn16 + ldc(L.xn_off + kl);
-  const double* dlp = L.diag_lo_cls + cls0;
-  __syncthreads();
-
-  // Each thread carries up to kRows rows (r = tid + q*kBlock) through every
-  // bond chunk, so the chunk's scalar data is loaded once.  Row slots past
-  // the segment are skipped with a uniform test.
-  const uint32_t nq = (len + kBlock - 1) / kBlock;
-  uint32_t rc[kRows];
-  bool ok[kRows];
-  double d[kRows], off[kRows];
-#pragma unroll
-  for (int q = 0; q < kRows; ++q) {
-    const uint32_t r = tid + q * kBlock;
-    ok[q] = r < len;
-    rc[q] = ok[q] ? r : 0u;  // clamped row: every read stays in bounds
-    d[q] = dh;
-    off[q] = 0.0;
-    if (q < (int)nq) d[q] += dlp[rc[q]];
-  }
-  // low-low flips: target rows from the class table, values from LDS
-  for (int b0 = 0; b0 < L.n_ll_p; b0 += kChunkLL) {
-    double t[kChunkLL];
-#pragma unroll
-    for (int c = 0; c < kChunkLL; ++c) t[c] = ldc(L.ll_t + b0 + c);
-#pragma unroll
-    for (int q = 0; q < kRows; ++q) {
-      if (q < (int)nq) {
-        uint32_t j[kChunkLL];
-#pragma unroll
-        for (int c = 0; c < kChunkLL; ++c) j[c] = (nbp + (b0 + c) * len)[rc[q]];
-        V v[kChunkLL];
-#pragma unroll
-        for (int c = 0; c < kChunkLL; ++c) v[c] = xs[j[c]];
-#pragma unroll
-        for (int c = 0; c < kChunkLL; ++c) off[q] += t[c] * (double)v[c];
-      }
-    }
-  }
-  // high-high flips: uniform decision, contiguous target run
-  for (int b0 = 0; b0 < L.n_hh_p; b0 += kChunkHH) {
-    const V* xt[kChunkHH];
-    double tt[kChunkHH];
-#pragma unroll
-    for (int c = 0; c < kChunkHH; ++c) {
-      const uint32_t m = ldc(L.hh_mask + b0 + c);
-      const uint32_t bits = h & m;
-      const bool act = bits != 0u && bits != m;
-      xt[c] = act ? x + ldc(L.hi_base + (h ^ m)) : xb;
-      tt[c] = act ? ldc(L.hh_t + b0 + c) : 0.0;
-    }
-#pragma unroll
-    for (int q = 0; q < kRows; ++q) {
-      if (q < (int)nq) {
-        V v[kChunkHH];
-#pragma unroll
-        for (int c = 0; c < kChunkHH; ++c) v[c] = xt[c][rc[q]];
-#pragma unroll
-        for (int c = 0; c < kChunkHH; ++c) off[q] += tt[c] * (double)v[c];
-      }
-    }
-  }
-  // cross bonds: diagonal always, flip when the high and low bits differ.
-  // If the neighbouring high word has no segment no row can flip (its low
-  // word would need a set bit more or fewer than it has), and the read
-  // falls back to this segment.
-  for (int b0 = 0; b0 < L.n_xb_p; b0 += kChunkX) {
-    const V* xn[kChunkX];
-    bool hb[kChunkX];
-    double ea[kChunkX], ep[kChunkX], tx[kChunkX];
-#pragma unroll
-    for (int c = 0; c < kChunkX; ++c) {
-      const uint32_t hbit = ldc(L.xb_hbit + b0 + c);
-      hb[c] = (h & hbit) != 0u;
-      const uint32_t nb = ldc(L.hi_base + (h ^ hbit));
-      xn[c] = x + (nb == 0xffffffffu ? base : nb);
-      ea[c] = ldc(L.xb_anti + b0 + c);
-      ep[c] = ldc(L.xb_par + b0 + c);
-      tx[c] = ldc(L.xb_t + b0 + c);
-    }
-#pragma unroll
-    for (int q = 0; q < kRows; ++q) {
-      if (q < (int)nq) {
-        uint32_t e[kChunkX];
-#pragma unroll
-        for (int c = 0; c < kChunkX; ++c) e[c] = (xnp + (b0 + c) * len)[rc[q]];
-        V v[kChunkX];
-        bool f[kChunkX];
-#pragma unroll
-        for (int c = 0; c < kChunkX; ++c) {
-          f[c] = ((e[c] >> 15) != 0u) != hb[c];
-          v[c] = xn[c][f[c] ? (e[c] & 0x7fffu) : 0u];
-        }
-#pragma unroll
-        for (int c = 0; c < kChunkX; ++c) {
-          d[q] += f[c] ? ea[c] : ep[c];
-          off[q] += (f[c] ? tx[c] : 0.0) * (double)v[c];
-        }
-      }
-    }
-  }
-  double dotp = 0.0;
-#pragma unroll
-  for (int q = 0; q < kRows; ++q) {
-    if (ok[q]) {
-      const double xi = (double)xs[rc[q]];
-      const double yv = d[q] * xi + off[q];
-      y[base + rc[q]] = (V)yv;
-      if (DOT) dotp += xi * yv;  // FP64 value before the storage rounding
-    }
-  }
-  if (DOT) {
-    const double bs = block_sum_all(dotp);
-    if (tid == 0) dot_out[blockIdx.x] = bs;
-  }
+  hv_segment<V, DOT>(L, h, x, y, DOT ? dot_out + blockIdx.x : nullptr);
 }
 
 // ---------------------------------------------------------------------------

@@ -55,6 +55,9 @@ struct Partials {
 };
 
 constexpr int kBlock = 256;
+// Smallest sector whose Lanczos vectors a solve stores in FP32 when asked to
+// (smaller sectors use FP64 storage).
+constexpr uint64_t kFp32MinDim = 1000;
 constexpr int kMaxRitz = 8;     // Ritz vectors per pass of launch_ritz
 // H.v rows per thread: a segment of up to kRows*kBlock rows is one workgroup
 constexpr int kRows = 4;

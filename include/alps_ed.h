@@ -181,7 +181,10 @@ int alps_ed_tmatrix_analyze(int n, const double* alpha, const double* beta, doub
  * With FP64 storage the eigenvalues are those of T, as in the reference.
  * With FP32 storage the basis is kept in HBM, near-copies of converged
  * eigenvalues (within 64 eps_f32 |lambda|) are merged, and each reported
- * eigenvalue is the FP64-accumulated Rayleigh quotient of its Ritz vector. */
+ * eigenvalue is the FP64-accumulated Rayleigh quotient of its Ritz vector.
+ * Sectors with D < 1000 are stored in FP64 whatever `dtype` asks: there the
+ * Krylov space the solve builds is a large part of the sector and FP32
+ * storage would cost accuracy for no measurable saving. */
 typedef struct {
   int max_iter;
   int nev;
