@@ -76,6 +76,15 @@ def lib():
             "alps_ed_tmatrix_analyze": ([i32, P, P, P, P, P, P], i32),
             "alps_ed_sparsediag": ([P, P, P], i32),
             "alps_ed_run_parameter_file": ([ctypes.c_char_p, P, ctypes.c_size_t, P], i32),
+            "alps_ed_batch_create": ([P, P], i32),
+            "alps_ed_batch_destroy": ([P], i32),
+            "alps_ed_batch_submit": ([P, P, P, P], i32),
+            "alps_ed_batch_progress": ([P, P, P], i32),
+            "alps_ed_batch_run": ([P], i32),
+            "alps_ed_batch_completed": ([P, P, i32, P], i32),
+            "alps_ed_batch_result": ([P, u64, P, P], i32),
+            "alps_ed_batch_stats": ([P, P], i32),
+            "alps_ed_batch_footprint": ([P, P, i32, P], i32),
             "alps_ed_describe_parameter_file": ([ctypes.c_char_p, P, ctypes.c_size_t, P], i32),
         }
         for name, (args, res) in sig.items():
@@ -111,6 +120,22 @@ class _SolveResult(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_uint64), ("n_eigenvalues", ctypes.c_int),
                 ("eigenvalues", ctypes.c_double * 64), ("iterations", ctypes.c_int),
                 ("error_code", ctypes.c_int), ("seconds", ctypes.c_double)]
+
+
+class _BatchConfig(ctypes.Structure):
+    _fields_ = [("dtype", ctypes.c_int), ("hbm_bytes", ctypes.c_uint64), ("host_bytes", ctypes.c_uint64),
+                ("steps_per_round", ctypes.c_int), ("max_active", ctypes.c_int), ("host_threads", ctypes.c_int)]
+
+
+class _BatchStatus(ctypes.Structure):
+    _fields_ = [("n_active", ctypes.c_int), ("n_pending", ctypes.c_int), ("hbm_used", ctypes.c_uint64),
+                ("hbm_bytes", ctypes.c_uint64), ("rounds", ctypes.c_uint64), ("steps", ctypes.c_uint64),
+                ("solves_done", ctypes.c_uint64)]
+
+
+def _result_dict(r):
+    return dict(dim=r.dim, ev=np.array(r.eigenvalues[: r.n_eigenvalues]), iterations=r.iterations,
+                error_code=r.error_code, seconds=r.seconds)
 
 
 @dataclass
@@ -372,3 +397,67 @@ def run_parameter_file(text):
     """The sparsediag task for every task of a parameter file: per-sector
     eigenvalues, ground-state energy and gap."""
     return _json_call(lib().alps_ed_run_parameter_file, text)["tasks"]
+
+
+class Batch:
+    """Many solves advanced together (alps_ed_batch_*): submit requests,
+    call progress() (one round) or run() (until all are done), read results
+    by ticket."""
+
+    def __init__(self, dtype="f32", hbm_bytes=0, host_bytes=0, steps_per_round=0, max_active=0, host_threads=0):
+        cfg = _BatchConfig(_DTYPES[dtype], hbm_bytes, host_bytes, steps_per_round, max_active, host_threads)
+        h = ctypes.c_void_p()
+        _check(lib().alps_ed_batch_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().alps_ed_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, model: SpinModel, max_iter=0, nev=1, seed=0, fixed=False):
+        m = model.cstruct()
+        p = _SolveParams(max_iter, nev, int(fixed), seed, F64)
+        t = ctypes.c_uint64()
+        _check(lib().alps_ed_batch_submit(self.handle, ctypes.byref(m), ctypes.byref(p), ctypes.byref(t)))
+        return t.value
+
+    def progress(self):
+        a, p = ctypes.c_int(), ctypes.c_int()
+        _check(lib().alps_ed_batch_progress(self.handle, ctypes.byref(a), ctypes.byref(p)))
+        return a.value, p.value
+
+    def run(self):
+        _check(lib().alps_ed_batch_run(self.handle))
+
+    def completed(self, max_n=65536):
+        buf = (ctypes.c_uint64 * max_n)()
+        n = ctypes.c_int()
+        _check(lib().alps_ed_batch_completed(self.handle, buf, max_n, ctypes.byref(n)))
+        return list(buf[: n.value])
+
+    def result(self, ticket):
+        r = _SolveResult()
+        ready = ctypes.c_int()
+        _check(lib().alps_ed_batch_result(self.handle, ticket, ctypes.byref(r), ctypes.byref(ready)))
+        return _result_dict(r) if ready.value else None
+
+    def stats(self):
+        s = _BatchStatus()
+        _check(lib().alps_ed_batch_stats(self.handle, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in _BatchStatus._fields_}
+
+
+def batch_footprint(model: SpinModel, dtype="f32", max_iter=0, nev=1):
+    """HBM bytes a batched request would hold while admitted."""
+    m = model.cstruct()
+    p = _SolveParams(max_iter, nev, 0, 0, F64)
+    b = ctypes.c_uint64()
+    _check(lib().alps_ed_batch_footprint(ctypes.byref(m), ctypes.byref(p), _DTYPES[dtype], ctypes.byref(b)))
+    return b.value

@@ -1,0 +1,944 @@
+// ed_batch.hip -- the batched solver and request loop behind the
+// alps_ed_batch_* C ABI (include/alps_ed.h).
+//
+// A batch holds requests (one do_subspace each: a sector of a spin-1/2
+// Hamiltonian) in three stages:
+//   pending  -- submitted, waiting for HBM;
+//   active   -- admitted: its tables, Lanczos vectors, scalars and partials
+//               live in one region of the batch's HBM arena, sized when it
+//               is admitted (admission is by footprint, first come first
+//               served);
+//   done     -- result recorded, region freed.
+// alps_ed_batch_progress runs one round: admit what fits, start the new
+// problems, advance every running problem by `steps_per_round` Lanczos
+// steps (four batched launches per step, no host round trip), then test
+// each problem's stopping rule on the host (in parallel threads) from the
+// alpha/beta the kernels mirror into mapped host memory, refine and retire
+// the finished ones.  A problem's arithmetic never depends on the rest of
+// the batch, so its result is the same alone and in any mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <functional>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "alps_ed.h"
+#include "ed_batch.hpp"
+#include "ed_host.hpp"
+#include "ed_kernels.hpp"
+
+using namespace alps_ed;
+using namespace alps_ed::host;
+
+namespace {
+
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// First-fit sub-allocator of byte offsets in one region.
+class Arena {
+ public:
+  void init(size_t size) {
+    size_ = size;
+    used_ = 0;
+    free_.clear();
+    if (size) free_[0] = size;
+  }
+  bool alloc(size_t bytes, size_t* off) {
+    bytes = align_up(bytes);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < bytes) continue;
+      *off = it->first;
+      const size_t rest = it->second - bytes, next = it->first + bytes;
+      free_.erase(it);
+      if (rest) free_[next] = rest;
+      used_ += bytes;
+      return true;
+    }
+    return false;
+  }
+  void release(size_t off, size_t bytes) {
+    bytes = align_up(bytes);
+    used_ -= bytes;
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  size_t used() const { return used_; }
+  size_t size() const { return size_; }
+
+ private:
+  size_t size_ = 0, used_ = 0;
+  std::map<size_t, size_t> free_;
+};
+
+// Persistent worker threads for the per-problem host analysis.
+class Pool {
+ public:
+  explicit Pool(int n) {
+    for (int i = 0; i < n; ++i) th_.emplace_back([this] { loop(); });
+  }
+  ~Pool() {
+    {
+      std::lock_guard<std::mutex> g(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+  // fn(i) for i in [0, n), on the workers and the calling thread
+  void parallel_for(size_t n, const std::function<void(size_t)>& fn) {
+    if (n == 0) return;
+    if (th_.empty() || n == 1) {
+      for (size_t i = 0; i < n; ++i) fn(i);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> g(m_);
+      fn_ = &fn;
+      n_ = n;
+      next_ = 0;
+      busy_ = (int)th_.size();
+      ++gen_;
+    }
+    cv_.notify_all();
+    run_items();
+    std::unique_lock<std::mutex> lk(m_);
+    done_cv_.wait(lk, [this] { return busy_ == 0; });
+    fn_ = nullptr;
+  }
+
+ private:
+  void run_items() {
+    for (;;) {
+      const size_t i = next_.fetch_add(1);
+      if (i >= n_) break;
+      (*fn_)(i);
+    }
+  }
+  void loop() {
+    uint64_t seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+      }
+      run_items();
+      {
+        std::lock_guard<std::mutex> g(m_);
+        if (--busy_ == 0) done_cv_.notify_all();
+      }
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex m_;
+  std::condition_variable cv_, done_cv_;
+  const std::function<void(size_t)>* fn_ = nullptr;
+  size_t n_ = 0;
+  std::atomic<size_t> next_{0};
+  int busy_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
+
+enum State { kPending, kRunning, kRefining, kDone };
+
+struct Problem {
+  uint64_t ticket = 0;
+  ModelSpec spec;
+  SectorLayout layout;
+  uint64_t D = 0;
+  bool fp32 = false;  // storage lane
+  int max_iter = 0, nev = 1;
+  bool fixed = false;
+  uint64_t seed = 0;
+  State state = kPending;
+  // HBM region and mapped host region
+  size_t dev_off = 0, dev_bytes = 0, host_off = 0, host_bytes = 0;
+  BatchJob job{};
+  uint32_t n_chunks = 0, cap = 0;
+  int nvr = 0;                  // Ritz vectors per refinement pass
+  double* coef = nullptr;       // device [nvr * (max_iter+1)]
+  double* ritz = nullptr;       // device [nvr * D]
+  double* hy = nullptr;         // device [D]
+  double* ritz_part = nullptr;  // mapped [nvr * n_chunks]
+  double* hv_part = nullptr;    // mapped [n_seg]
+  double* coef_host = nullptr;  // mapped staging of coef
+  double* host_alpha = nullptr;
+  double* host_beta = nullptr;
+  // recurrence and stopping rule
+  int j = 0;           // alpha values produced
+  int scan_from = 2;   // smallest prefix length not yet decided
+  int final_n = -1;
+  int error_code = 0;
+  TValues tv;
+  std::vector<double> ev;   // reported eigenvalues
+  size_t next_cand = 0;     // next candidate of tv to refine
+  std::vector<int> cand;    // tv positions in the current refinement pass
+  double t_submit = 0, t_admit = 0, t_done = 0;
+  alps_ed_solve_result result{};
+};
+
+}  // namespace
+
+struct alps_ed_batch_s {
+  alps_ed_batch_config cfg{};
+  hipStream_t stream = nullptr;
+  void* dev_base = nullptr;
+  char* host_base = nullptr;   // mapped, pinned
+  char* host_dev = nullptr;    // device view of host_base
+  Arena arena, harena;
+  std::unique_ptr<Pool> pool;
+  std::deque<std::unique_ptr<Problem>> pending;
+  std::vector<std::unique_ptr<Problem>> active;
+  std::unordered_map<uint64_t, alps_ed_solve_result> results;
+  std::vector<uint64_t> completed;
+  uint64_t next_ticket = 1;
+  uint64_t rounds = 0, steps = 0, solves_done = 0;
+  // device work lists (rebuilt every round)
+  DevBuf jobs_d, items_d, list_d;
+  size_t jobs_cap = 0, items_cap = 0, list_cap = 0;
+  // pinned staging for the work lists
+  void* stage = nullptr;
+  size_t stage_cap = 0;
+
+  ~alps_ed_batch_s() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (dev_base) (void)hipFree(dev_base);
+    if (host_base) (void)hipHostFree(host_base);
+    if (stage) (void)hipHostFree(stage);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+// Bytes of the layout tables a problem uploads.
+size_t table_bytes(const SectorLayout& L) {
+  size_t s = 0;
+  auto add = [&](size_t n, size_t es) { s += align_up(std::max<size_t>(n, 1) * es); };
+  add(L.lo_rank.size(), 4);
+  add(L.lo_state.size(), 4);
+  add(L.lo_class_off.size(), 4);
+  add(L.hi_base.size(), 4);
+  add(L.seg_hi.size(), 4);
+  add(L.seg_start.size(), 4);
+  add(L.proc_hi.size(), 4);
+  add(L.diag_lo_cls.size(), 8);
+  add(L.diag_hi.size(), 8);
+  add(L.nbp16.size(), 2);
+  add(L.nbp_off.size(), 4);
+  add(L.xn16.size(), 2);
+  add(L.xn_off.size(), 4);
+  add(L.ll_t.size(), 8);
+  add(L.hh_mask.size(), 4);
+  add(L.hh_t.size(), 8);
+  add(L.xb_hbit.size(), 4);
+  add(L.xb_par.size(), 8);
+  add(L.xb_anti.size(), 8);
+  add(L.xb_t.size(), 8);
+  add(L.site_dn.size(), 8);
+  add(L.site_up.size(), 8);
+  add(L.bond_p1.size(), 4);
+  add(L.bond_p2.size(), 4);
+  add(L.bond_par.size(), 8);
+  add(L.bond_anti.size(), 8);
+  add(L.bond_t.size(), 8);
+  return s;
+}
+
+// Copies the tables into [base, ...) and returns the device layout.
+DevLayout place_tables(const SectorLayout& L, char* base, hipStream_t st) {
+  DevLayout d{};
+  d.N = L.N;
+  d.k = L.k;
+  d.B = L.B;
+  d.NH = L.NH;
+  d.D = (uint32_t)L.D;
+  d.n_seg = (uint32_t)L.seg_hi.size();
+  d.max_seg = L.max_seg;
+  d.n_ll_p = L.n_ll_p;
+  d.n_hh_p = L.n_hh_p;
+  d.n_xb_p = L.n_xb_p;
+  d.n_bonds = (int)L.bond_p1.size();
+  char* p = base;
+  auto put = [&](const auto& v) {
+    using T = typename std::decay_t<decltype(v)>::value_type;
+    T* dst = reinterpret_cast<T*>(p);
+    if (!v.empty())
+      hip_check(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st), "upload tables");
+    p += align_up(std::max<size_t>(v.size(), 1) * sizeof(T));
+    return static_cast<const T*>(dst);
+  };
+  d.lo_rank = put(L.lo_rank);
+  d.lo_state = put(L.lo_state);
+  d.lo_class_off = put(L.lo_class_off);
+  d.hi_base = put(L.hi_base);
+  d.seg_hi = put(L.seg_hi);
+  d.seg_start = put(L.seg_start);
+  d.proc_hi = put(L.proc_hi);
+  d.diag_lo_cls = put(L.diag_lo_cls);
+  d.diag_hi = put(L.diag_hi);
+  d.nbp16 = put(L.nbp16);
+  d.nbp_off = put(L.nbp_off);
+  d.xn16 = put(L.xn16);
+  d.xn_off = put(L.xn_off);
+  d.ll_t = put(L.ll_t);
+  d.hh_mask = put(L.hh_mask);
+  d.hh_t = put(L.hh_t);
+  d.xb_hbit = put(L.xb_hbit);
+  d.xb_par = put(L.xb_par);
+  d.xb_anti = put(L.xb_anti);
+  d.xb_t = put(L.xb_t);
+  d.site_dn = put(L.site_dn);
+  d.site_up = put(L.site_up);
+  d.bond_p1 = put(L.bond_p1);
+  d.bond_p2 = put(L.bond_p2);
+  d.bond_par = put(L.bond_par);
+  d.bond_anti = put(L.bond_anti);
+  d.bond_t = put(L.bond_t);
+  return d;
+}
+
+// H(0,0) of a one-state sector in the reference's summation order
+// (hamiltonian_matrix.hpp:44-58, 269-291, 308-345): explicit 0.0, sites in
+// order, then bonds in edge order.
+double single_state_energy(const ModelSpec& m) {
+  const bool up = m.n_up == m.n_sites;  // the only states are all-down / all-up
+  double v = 0.0;
+  for (const SiteTerm& s : m.sites) v += up ? s.e_up : s.e_dn;
+  for (const BondTerm& b : m.bonds) v += b.e_par;
+  return v;
+}
+
+struct Footprint {
+  size_t dev = 0, host = 0;
+};
+
+Footprint footprint(const Problem& p) {
+  Footprint f;
+  const size_t es = p.fp32 ? 4 : 8;
+  const uint64_t stride = (p.D + 63) & ~uint64_t(63);
+  f.dev += table_bytes(p.layout);
+  f.dev += align_up((size_t)p.cap * stride * es);
+  f.dev += 3 * align_up((size_t)(p.max_iter + 2) * 8);  // alpha, beta, scale
+  f.dev += align_up((size_t)std::max<uint32_t>(p.layout.seg_hi.size(), 1) * 8);
+  f.dev += align_up((size_t)p.n_chunks * 8);
+  if (p.fp32) {
+    f.dev += align_up((size_t)p.nvr * (p.max_iter + 1) * 8);  // coef
+    f.dev += align_up((size_t)p.nvr * p.D * 8);              // Ritz vectors
+    f.dev += align_up((size_t)p.D * 8);                      // H y
+    f.host += align_up((size_t)p.nvr * p.n_chunks * 8);
+    f.host += align_up((size_t)std::max<uint32_t>(p.layout.seg_hi.size(), 1) * 8);
+    f.host += align_up((size_t)p.nvr * (p.max_iter + 1) * 8);
+  }
+  f.host += 2 * align_up((size_t)(p.max_iter + 2) * 8);  // alpha, beta mirrors
+  return f;
+}
+
+template <class T>
+T* dev_ptr(alps_ed_batch b, size_t off) {
+  return reinterpret_cast<T*>(static_cast<char*>(b->dev_base) + off);
+}
+
+// Places an admitted problem in its regions and uploads its tables.
+void place(alps_ed_batch b, Problem& p) {
+  char* base = static_cast<char*>(b->dev_base) + p.dev_off;
+  BatchJob& J = p.job;
+  J = BatchJob{};
+  J.L = place_tables(p.layout, base, b->stream);
+  char* q = base + table_bytes(p.layout);
+  const size_t es = p.fp32 ? 4 : 8;
+  const uint64_t stride = (p.D + 63) & ~uint64_t(63);
+  J.vec = q;
+  J.stride = stride;
+  J.cap = p.cap;
+  J.n_chunks = p.n_chunks;
+  J.j0 = 0;
+  J.jmax = p.max_iter;
+  J.seed = p.seed;
+  q += align_up((size_t)p.cap * stride * es);
+  const size_t sc = align_up((size_t)(p.max_iter + 2) * 8);
+  J.alpha = reinterpret_cast<double*>(q);
+  J.beta = reinterpret_cast<double*>(q + sc);
+  J.scale = reinterpret_cast<double*>(q + 2 * sc);
+  q += 3 * sc;
+  J.part_hv = reinterpret_cast<double*>(q);
+  q += align_up((size_t)std::max<uint32_t>(p.layout.seg_hi.size(), 1) * 8);
+  J.part_up = reinterpret_cast<double*>(q);
+  q += align_up((size_t)p.n_chunks * 8);
+  char* h = b->host_base + p.host_off;
+  const size_t hoff = (size_t)(h - b->host_base);
+  p.host_alpha = reinterpret_cast<double*>(h);
+  p.host_beta = reinterpret_cast<double*>(h + sc);
+  J.host_alpha = reinterpret_cast<double*>(b->host_dev + hoff);
+  J.host_beta = reinterpret_cast<double*>(b->host_dev + hoff + sc);
+  size_t ho = 2 * sc;
+  if (p.fp32) {
+    p.coef = reinterpret_cast<double*>(q);
+    q += align_up((size_t)p.nvr * (p.max_iter + 1) * 8);
+    p.ritz = reinterpret_cast<double*>(q);
+    q += align_up((size_t)p.nvr * p.D * 8);
+    p.hy = reinterpret_cast<double*>(q);
+    q += align_up((size_t)p.D * 8);
+    p.ritz_part = reinterpret_cast<double*>(h + ho);
+    ho += align_up((size_t)p.nvr * p.n_chunks * 8);
+    p.hv_part = reinterpret_cast<double*>(h + ho);
+    ho += align_up((size_t)std::max<uint32_t>(p.layout.seg_hi.size(), 1) * 8);
+    p.coef_host = reinterpret_cast<double*>(h + ho);
+    ho += align_up((size_t)p.nvr * (p.max_iter + 1) * 8);
+  }
+}
+
+// device view of a mapped host pointer
+template <class T>
+T* mapped_dev(alps_ed_batch b, T* h) {
+  return reinterpret_cast<T*>(b->host_dev + (reinterpret_cast<char*>(h) - b->host_base));
+}
+
+void finish_problem(alps_ed_batch b, Problem& p) {
+  p.state = kDone;
+  p.t_done = now_s();
+  alps_ed_solve_result& r = p.result;
+  r.dim = p.D;
+  r.n_eigenvalues = (int)std::min<size_t>(p.ev.size(), 64);
+  for (int i = 0; i < r.n_eigenvalues; ++i) r.eigenvalues[i] = p.ev[i];
+  r.iterations = p.final_n < 0 ? 0 : p.final_n;
+  r.error_code = p.error_code;
+  r.seconds = p.t_done - p.t_admit;
+  b->results[p.ticket] = r;
+  b->completed.push_back(p.ticket);
+  ++b->solves_done;
+}
+
+// Stopping rule on the alpha/beta produced so far (see alps_ed_sparsediag):
+// the first prefix length at which lanczos_iteration_nlowest holds, with a
+// cheap test of the lowest value gating the full T-matrix analysis.
+void decide(Problem& p) {
+  const int n = p.j;
+  if (p.fixed) {
+    if (n >= p.max_iter) p.final_n = p.max_iter;
+    return;
+  }
+  std::vector<double> a(p.host_alpha, p.host_alpha + n), bb(p.host_beta, p.host_beta + n);
+  const double tol = 100. * std::numeric_limits<double>::epsilon();
+  for (int m = std::max(2, p.scan_from); m <= n; ++m) {
+    double lam = 0.0;
+    const double e = lowest_ritz_error(a.data(), bb.data(), m, &lam);
+    if (!(e <= 4.0 * std::max(tol, tol * std::fabs(lam)))) continue;
+    if (!nlowest_converged(analyze_prefix(a, bb, m, p.fp32), p.nev)) continue;
+    // the full test holds at m; step back while it holds earlier too
+    int first = m;
+    while (first - 1 >= std::max(2, p.scan_from) && nlowest_converged(analyze_prefix(a, bb, first - 1, p.fp32), p.nev))
+      --first;
+    p.final_n = first;
+    return;
+  }
+  p.scan_from = std::max(p.scan_from, n + 1);
+  if (n >= p.max_iter) {
+    p.final_n = n;
+    p.error_code = 1;  // iteration.h:80
+  }
+}
+
+// After the stopping rule: eigenvalues for FP64 lanes; for FP32 lanes the
+// candidates of the first refinement pass.
+void conclude(Problem& p) {
+  std::vector<double> a(p.host_alpha, p.host_alpha + p.final_n), bb(p.host_beta, p.host_beta + p.final_n);
+  p.tv = analyze_prefix(a, bb, p.final_n, p.fp32);
+  if (!p.fp32) {
+    p.ev.assign(p.tv.ev.begin(), p.tv.ev.begin() + std::min<size_t>(p.nev, p.tv.ev.size()));
+    p.state = kDone;
+    return;
+  }
+  p.state = kRefining;
+}
+
+// Ritz coefficients of the next refinement pass (host part).
+void prepare_refine(Problem& p) {
+  const int m = p.final_n;
+  std::vector<double> a(p.host_alpha, p.host_alpha + m), bb(p.host_beta, p.host_beta + m);
+  std::vector<double> d(a);
+  std::vector<double> evals(d);
+  if (ietl::tridiag::eig(m, evals.data(), bb.data(), nullptr) > 0)
+    throw std::runtime_error("LAPACK error, stev function failed.");
+  const size_t want = (size_t)std::max(0, p.nev - (int)p.ev.size());
+  const size_t cnt = std::min({p.tv.ev.size() - p.next_cand, want, (size_t)p.nvr});
+  p.cand.assign(p.tv.index.begin() + p.next_cand, p.tv.index.begin() + p.next_cand + cnt);
+  p.next_cand += cnt;
+  std::vector<double> z(m);
+  for (size_t i = 0; i < cnt; ++i) {
+    tridiag_eigenvector(a.data(), bb.data(), m, evals[p.cand[i]], z.data());
+    for (int j = 0; j < m; ++j) p.coef_host[i * m + j] = z[j];
+  }
+}
+
+void rebuild_lists(alps_ed_batch b, const std::vector<Problem*>& run, std::vector<BatchItem>& hv32,
+                   std::vector<BatchItem>& hv64, std::vector<BatchItem>& up32, std::vector<BatchItem>& up64,
+                   uint32_t& seg32, uint32_t& seg64) {
+  hv32.clear();
+  hv64.clear();
+  up32.clear();
+  up64.clear();
+  seg32 = seg64 = 0;
+  for (uint32_t i = 0; i < run.size(); ++i) {
+    const Problem& p = *run[i];
+    auto& hv = p.fp32 ? hv32 : hv64;
+    auto& up = p.fp32 ? up32 : up64;
+    for (uint32_t s = 0; s < p.layout.seg_hi.size(); ++s) hv.push_back({i, s});
+    for (uint32_t c = 0; c < p.n_chunks; ++c) up.push_back({i, c});
+    (p.fp32 ? seg32 : seg64) = std::max(p.fp32 ? seg32 : seg64, p.layout.max_seg);
+  }
+}
+
+void ensure_buf(DevBuf& buf, size_t& cap, size_t bytes) {
+  if (bytes <= cap) return;
+  buf.alloc(std::max<size_t>(bytes, 2 * cap));
+  cap = buf.bytes;
+}
+
+void ensure_stage(alps_ed_batch b, size_t bytes) {
+  if (bytes <= b->stage_cap) return;
+  if (b->stage) hip_check(hipHostFree(b->stage), "hipHostFree");
+  b->stage = nullptr;
+  const size_t cap = std::max<size_t>(bytes, 2 * b->stage_cap);
+  hip_check(hipHostMalloc(&b->stage, cap, hipHostMallocDefault), "hipHostMalloc");
+  b->stage_cap = cap;
+}
+
+// Copies jobs, work items and the job list of a launch group to the device.
+struct Group {
+  BatchJob* jobs = nullptr;
+  BatchItem* hv32 = nullptr;
+  BatchItem* hv64 = nullptr;
+  BatchItem* up32 = nullptr;
+  BatchItem* up64 = nullptr;
+  uint32_t* list = nullptr;
+  uint32_t n_hv32 = 0, n_hv64 = 0, n_up32 = 0, n_up64 = 0, n_list = 0;
+  uint32_t seg32 = 0, seg64 = 0;
+};
+
+Group upload_group(alps_ed_batch b, const std::vector<Problem*>& run) {
+  std::vector<BatchItem> hv32, hv64, up32, up64;
+  Group g;
+  rebuild_lists(b, run, hv32, hv64, up32, up64, g.seg32, g.seg64);
+  const size_t nj = run.size();
+  const size_t bj = align_up(nj * sizeof(BatchJob));
+  const size_t bi = align_up((hv32.size() + 1) * sizeof(BatchItem)) + align_up((hv64.size() + 1) * sizeof(BatchItem)) +
+                    align_up((up32.size() + 1) * sizeof(BatchItem)) + align_up((up64.size() + 1) * sizeof(BatchItem));
+  const size_t bl = align_up((nj + 1) * sizeof(uint32_t));
+  const size_t total = bj + bi + bl;
+  ensure_buf(b->jobs_d, b->jobs_cap, total);
+  ensure_stage(b, total);
+  char* s = static_cast<char*>(b->stage);
+  char* d = b->jobs_d.as<char>();
+  for (size_t i = 0; i < nj; ++i) reinterpret_cast<BatchJob*>(s)[i] = run[i]->job;
+  g.jobs = reinterpret_cast<BatchJob*>(d);
+  size_t o = bj;
+  auto put_items = [&](const std::vector<BatchItem>& v, BatchItem*& dst, uint32_t& n) {
+    if (!v.empty()) std::memcpy(s + o, v.data(), v.size() * sizeof(BatchItem));
+    dst = reinterpret_cast<BatchItem*>(d + o);
+    n = (uint32_t)v.size();
+    o += align_up((v.size() + 1) * sizeof(BatchItem));
+  };
+  put_items(hv32, g.hv32, g.n_hv32);
+  put_items(hv64, g.hv64, g.n_hv64);
+  put_items(up32, g.up32, g.n_up32);
+  put_items(up64, g.up64, g.n_up64);
+  uint32_t* lst = reinterpret_cast<uint32_t*>(s + o);
+  for (uint32_t i = 0; i < nj; ++i) lst[i] = i;
+  g.list = reinterpret_cast<uint32_t*>(d + o);
+  g.n_list = (uint32_t)nj;
+  hip_check(hipMemcpyAsync(d, s, total, hipMemcpyHostToDevice, b->stream), "upload work lists");
+  return g;
+}
+
+void admit(alps_ed_batch b, std::vector<Problem*>& started) {
+  while (!b->pending.empty()) {
+    if (b->cfg.max_active > 0 && (int)b->active.size() >= b->cfg.max_active) break;
+    Problem& p = *b->pending.front();
+    const Footprint f = footprint(p);
+    size_t doff = 0, hoff = 0;
+    if (f.dev > b->arena.size() || f.host > b->harena.size()) {
+      // can never fit: fail this request instead of blocking the queue
+      std::unique_ptr<Problem> q = std::move(b->pending.front());
+      b->pending.pop_front();
+      q->t_admit = now_s();
+      q->error_code = -1;
+      q->ev.clear();
+      finish_problem(b, *q);
+      continue;
+    }
+    if (!b->arena.alloc(f.dev, &doff)) break;
+    if (!b->harena.alloc(f.host, &hoff)) {
+      b->arena.release(doff, f.dev);
+      break;
+    }
+    std::unique_ptr<Problem> q = std::move(b->pending.front());
+    b->pending.pop_front();
+    q->dev_off = doff;
+    q->dev_bytes = f.dev;
+    q->host_off = hoff;
+    q->host_bytes = f.host;
+    q->t_admit = now_s();
+    place(b, *q);
+    q->state = kRunning;
+    started.push_back(q.get());
+    b->active.push_back(std::move(q));
+  }
+}
+
+void retire(alps_ed_batch b) {
+  std::vector<std::unique_ptr<Problem>> keep;
+  for (auto& p : b->active) {
+    if (p->state == kDone) {
+      finish_problem(b, *p);
+      b->arena.release(p->dev_off, p->dev_bytes);
+      b->harena.release(p->host_off, p->host_bytes);
+    } else {
+      keep.push_back(std::move(p));
+    }
+  }
+  b->active.swap(keep);
+}
+
+// Refinement pass for every problem in kRefining: Ritz vectors from the
+// kept basis, H y and the Rayleigh quotients (all FP64 accumulation).
+void refine(alps_ed_batch b) {
+  std::vector<Problem*> ref;
+  for (auto& p : b->active)
+    if (p->state == kRefining) ref.push_back(p.get());
+  if (ref.empty()) return;
+  b->pool->parallel_for(ref.size(), [&](size_t i) { prepare_refine(*ref[i]); });
+  // drop problems with no candidate left
+  std::vector<Problem*> work;
+  for (Problem* p : ref) {
+    if (p->cand.empty()) {
+      p->state = kDone;
+      continue;
+    }
+    work.push_back(p);
+  }
+  if (work.empty()) return;
+  // coefficient upload and Ritz vectors
+  std::vector<RitzJob> rj(work.size());
+  std::vector<BatchItem> ri;
+  for (uint32_t i = 0; i < work.size(); ++i) {
+    Problem& p = *work[i];
+    const int m = p.final_n;
+    hip_check(hipMemcpyAsync(p.coef, p.coef_host, p.cand.size() * m * sizeof(double), hipMemcpyHostToDevice,
+                             b->stream),
+              "upload Ritz coefficients");
+    RitzJob& r = rj[i];
+    r.vec = p.job.vec;
+    r.stride = p.job.stride;
+    r.D = (uint32_t)p.D;
+    r.n_chunks = p.n_chunks;
+    r.m = m;
+    r.nv = (int)p.cand.size();
+    r.scale = p.job.scale;
+    r.coef = p.coef;
+    r.out = p.ritz;
+    r.part = mapped_dev(b, p.ritz_part);
+    for (uint32_t c = 0; c < p.n_chunks; ++c) ri.push_back({i, c});
+  }
+  DevBuf rjd(align_up(rj.size() * sizeof(RitzJob)) + align_up(ri.size() * sizeof(BatchItem)));
+  hip_check(hipMemcpyAsync(rjd.p, rj.data(), rj.size() * sizeof(RitzJob), hipMemcpyHostToDevice, b->stream), "upload");
+  BatchItem* rid = reinterpret_cast<BatchItem*>(rjd.as<char>() + align_up(rj.size() * sizeof(RitzJob)));
+  hip_check(hipMemcpyAsync(rid, ri.data(), ri.size() * sizeof(BatchItem), hipMemcpyHostToDevice, b->stream), "upload");
+  hip_check(launch_batch_ritz<float>(rjd.as<RitzJob>(), rid, (uint32_t)ri.size(), b->stream), "batch ritz");
+  // H y for each candidate, one plain H.v group per candidate rank
+  size_t max_nv = 0;
+  for (Problem* p : work) max_nv = std::max(max_nv, p->cand.size());
+  std::vector<std::vector<double>> yhy(work.size());
+  for (size_t v = 0; v < max_nv; ++v) {
+    std::vector<PlainHvJob> hj;
+    std::vector<BatchItem> hi;
+    std::vector<Problem*> who;
+    uint32_t max_seg = 0;
+    for (Problem* p : work) {
+      if (v >= p->cand.size()) continue;
+      PlainHvJob j;
+      j.L = p->job.L;
+      j.x = p->ritz + v * p->D;
+      j.y = p->hy;
+      j.part = mapped_dev(b, p->hv_part);
+      const uint32_t ji = (uint32_t)hj.size();
+      hj.push_back(j);
+      who.push_back(p);
+      for (uint32_t s = 0; s < p->layout.seg_hi.size(); ++s) hi.push_back({ji, s});
+      max_seg = std::max(max_seg, p->layout.max_seg);
+    }
+    DevBuf hd(align_up(hj.size() * sizeof(PlainHvJob)) + align_up(hi.size() * sizeof(BatchItem)));
+    hip_check(hipMemcpyAsync(hd.p, hj.data(), hj.size() * sizeof(PlainHvJob), hipMemcpyHostToDevice, b->stream),
+              "upload");
+    BatchItem* hid = reinterpret_cast<BatchItem*>(hd.as<char>() + align_up(hj.size() * sizeof(PlainHvJob)));
+    hip_check(hipMemcpyAsync(hid, hi.data(), hi.size() * sizeof(BatchItem), hipMemcpyHostToDevice, b->stream),
+              "upload");
+    hip_check(launch_batch_plain_hv(hd.as<PlainHvJob>(), hid, (uint32_t)hi.size(), max_seg, b->stream), "plain hv");
+    hip_check(hipStreamSynchronize(b->stream), "sync");
+    for (size_t k = 0; k < who.size(); ++k) {
+      Problem& p = *who[k];
+      double s = 0.0;
+      for (size_t i = 0; i < p.layout.seg_hi.size(); ++i) s += p.hv_part[i];
+      const size_t wi = std::find(work.begin(), work.end(), &p) - work.begin();
+      yhy[wi].push_back(s);
+    }
+  }
+  for (size_t i = 0; i < work.size(); ++i) {
+    Problem& p = *work[i];
+    for (size_t v = 0; v < p.cand.size(); ++v) {
+      double yy = 0.0;
+      for (uint32_t c = 0; c < p.n_chunks; ++c) yy += p.ritz_part[v * p.n_chunks + c];
+      const double rq = yhy[i][v] / yy;
+      bool dup = false;
+      for (double e : p.ev)
+        if (std::fabs(rq - e) <= 1e-9 * std::max(1.0, std::fabs(e))) dup = true;
+      if (!dup) p.ev.push_back(rq);
+    }
+    std::sort(p.ev.begin(), p.ev.end());
+    p.cand.clear();
+    if ((int)p.ev.size() >= p.nev || p.next_cand >= p.tv.ev.size()) p.state = kDone;
+  }
+}
+
+void round(alps_ed_batch b) {
+  ++b->rounds;
+  std::vector<Problem*> started;
+  admit(b, started);
+  // D = 0 and D = 1 sectors need no Lanczos (sparsediag.h:101-102,120-124)
+  for (Problem* p : started) {
+    if (p->D == 0) {
+      p->state = kDone;
+    } else if (p->D == 1) {
+      p->ev.assign(1, single_state_energy(p->spec));
+      p->final_n = 0;
+      p->state = kDone;
+    }
+  }
+  std::vector<Problem*> run;
+  for (auto& p : b->active)
+    if (p->state == kRunning) run.push_back(p.get());
+  if (!run.empty()) {
+    for (Problem* p : run) p->job.j0 = p->j;
+    Group g = upload_group(b, run);
+    // start vectors of the problems admitted in this round
+    {
+      std::vector<uint32_t> idx;
+      for (uint32_t i = 0; i < run.size(); ++i)
+        if (run[i]->j == 0 && std::find(started.begin(), started.end(), run[i]) != started.end()) idx.push_back(i);
+      if (!idx.empty()) {
+        std::vector<BatchItem> s32, s64;
+        for (uint32_t i : idx)
+          for (uint32_t c = 0; c < run[i]->n_chunks; ++c) (run[i]->fp32 ? s32 : s64).push_back({i, c});
+        DevBuf sd(align_up((s32.size() + 1) * sizeof(BatchItem)) + align_up((s64.size() + 1) * sizeof(BatchItem)) +
+                  align_up((idx.size() + 1) * 4));
+        char* base = sd.as<char>();
+        BatchItem* d32 = reinterpret_cast<BatchItem*>(base);
+        BatchItem* d64 = reinterpret_cast<BatchItem*>(base + align_up((s32.size() + 1) * sizeof(BatchItem)));
+        uint32_t* dl = reinterpret_cast<uint32_t*>(base + align_up((s32.size() + 1) * sizeof(BatchItem)) +
+                                                   align_up((s64.size() + 1) * sizeof(BatchItem)));
+        if (!s32.empty())
+          hip_check(hipMemcpyAsync(d32, s32.data(), s32.size() * sizeof(BatchItem), hipMemcpyHostToDevice, b->stream),
+                    "upload");
+        if (!s64.empty())
+          hip_check(hipMemcpyAsync(d64, s64.data(), s64.size() * sizeof(BatchItem), hipMemcpyHostToDevice, b->stream),
+                    "upload");
+        hip_check(hipMemcpyAsync(dl, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, b->stream), "upload");
+        hip_check(launch_batch_start<float>(g.jobs, d32, (uint32_t)s32.size(), b->stream), "batch start");
+        hip_check(launch_batch_start<double>(g.jobs, d64, (uint32_t)s64.size(), b->stream), "batch start");
+        hip_check(launch_batch_finish(g.jobs, dl, (uint32_t)idx.size(), 0, kBatchStart, b->stream), "batch finish");
+        hip_check(hipStreamSynchronize(b->stream), "sync");  // sd is freed at scope end
+      }
+    }
+    const int S = std::max(1, b->cfg.steps_per_round);
+    for (int t = 0; t < S; ++t) {
+      hip_check(launch_batch_hv<float>(g.jobs, g.hv32, g.n_hv32, t, g.seg32, b->stream), "batch hv");
+      hip_check(launch_batch_hv<double>(g.jobs, g.hv64, g.n_hv64, t, g.seg64, b->stream), "batch hv");
+      hip_check(launch_batch_finish(g.jobs, g.list, g.n_list, t, kBatchAlpha, b->stream), "batch finish");
+      hip_check(launch_batch_update<float>(g.jobs, g.up32, g.n_up32, t, b->stream), "batch update");
+      hip_check(launch_batch_update<double>(g.jobs, g.up64, g.n_up64, t, b->stream), "batch update");
+      hip_check(launch_batch_finish(g.jobs, g.list, g.n_list, t, kBatchBeta, b->stream), "batch finish");
+    }
+    hip_check(hipStreamSynchronize(b->stream), "sync");
+    b->steps += S;
+    for (Problem* p : run) p->j = std::min(p->max_iter, p->j + S);
+    b->pool->parallel_for(run.size(), [&](size_t i) {
+      Problem& p = *run[i];
+      decide(p);
+      if (p.final_n >= 0) conclude(p);
+    });
+  }
+  refine(b);
+  retire(b);
+}
+
+uint64_t submit_one(alps_ed_batch b, const alps_ed_model* m, const alps_ed_solve_params* prm) {
+  require(prm != nullptr, "params is null");
+  std::unique_ptr<Problem> p(new Problem);
+  p->spec = model_spec(m);
+  p->layout = build_layout(p->spec);
+  p->D = p->layout.D;
+  p->fp32 = b->cfg.dtype == ALPS_ED_F32 && p->D >= kFp32MinDim;
+  p->max_iter = prm->max_iter > 0 ? prm->max_iter : (int)std::min<uint64_t>(10 * std::max<uint64_t>(p->D, 1), 1000);
+  p->nev = prm->nev > 0 ? prm->nev : 1;
+  require(p->nev <= 64, "nev must be <= 64");
+  p->fixed = prm->fixed_iterations != 0;
+  p->seed = prm->seed;
+  p->n_chunks = (uint32_t)std::max<uint64_t>(1, (p->D + kBatchChunk - 1) / kBatchChunk);
+  p->cap = p->fp32 ? (uint32_t)(p->max_iter + 2) : 3u;
+  p->nvr = p->fp32 ? std::min(p->nev, kMaxRitz) : 0;
+  p->ticket = b->next_ticket++;
+  p->t_submit = now_s();
+  const uint64_t t = p->ticket;
+  b->pending.push_back(std::move(p));
+  return t;
+}
+
+}  // namespace
+
+extern "C" {
+
+int alps_ed_batch_create(const alps_ed_batch_config* cfg, alps_ed_batch* out) {
+  return guarded([&] {
+    require(cfg && out, "null argument");
+    require(cfg->dtype == ALPS_ED_F32 || cfg->dtype == ALPS_ED_F64, "unknown dtype");
+    *out = nullptr;
+    std::unique_ptr<alps_ed_batch_s> b(new alps_ed_batch_s);
+    b->cfg = *cfg;
+    if (b->cfg.steps_per_round <= 0) b->cfg.steps_per_round = 8;
+    if (b->cfg.host_threads <= 0) b->cfg.host_threads = 8;
+    size_t budget = cfg->hbm_bytes;
+    if (budget == 0) {
+      size_t fr = 0, tot = 0;
+      hip_check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+      budget = fr / 2;
+    }
+    budget = align_up(budget);
+    hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipMalloc(&b->dev_base, budget), "hipMalloc arena");
+    b->arena.init(budget);
+    const size_t hbytes = align_up(cfg->host_bytes ? cfg->host_bytes : ((size_t)256 << 20));
+    void* hp = nullptr;
+    hip_check(hipHostMalloc(&hp, hbytes, hipHostMallocMapped), "hipHostMalloc");
+    b->host_base = static_cast<char*>(hp);
+    void* dp = nullptr;
+    hip_check(hipHostGetDevicePointer(&dp, hp, 0), "hipHostGetDevicePointer");
+    b->host_dev = static_cast<char*>(dp);
+    b->harena.init(hbytes);
+    b->pool.reset(new Pool(b->cfg.host_threads - 1));
+    *out = b.release();
+  });
+}
+
+int alps_ed_batch_destroy(alps_ed_batch b) {
+  return guarded([&] { delete b; });
+}
+
+int alps_ed_batch_submit(alps_ed_batch b, const alps_ed_model* model, const alps_ed_solve_params* params,
+                         uint64_t* ticket) {
+  return guarded([&] {
+    require(b != nullptr, "batch is null");
+    const uint64_t t = submit_one(b, model, params);
+    if (ticket) *ticket = t;
+  });
+}
+
+int alps_ed_batch_progress(alps_ed_batch b, int* n_active, int* n_pending) {
+  return guarded([&] {
+    require(b != nullptr, "batch is null");
+    if (!b->active.empty() || !b->pending.empty()) round(b);
+    if (n_active) *n_active = (int)b->active.size();
+    if (n_pending) *n_pending = (int)b->pending.size();
+  });
+}
+
+int alps_ed_batch_run(alps_ed_batch b) {
+  return guarded([&] {
+    require(b != nullptr, "batch is null");
+    while (!b->active.empty() || !b->pending.empty()) round(b);
+  });
+}
+
+int alps_ed_batch_completed(alps_ed_batch b, uint64_t* tickets, int max, int* n) {
+  return guarded([&] {
+    require(b != nullptr && n != nullptr, "null argument");
+    const int cnt = (int)std::min<size_t>(b->completed.size(), (size_t)std::max(0, max));
+    for (int i = 0; i < cnt; ++i) tickets[i] = b->completed[i];
+    b->completed.erase(b->completed.begin(), b->completed.begin() + cnt);
+    *n = cnt;
+  });
+}
+
+int alps_ed_batch_result(alps_ed_batch b, uint64_t ticket, alps_ed_solve_result* result, int* ready) {
+  return guarded([&] {
+    require(b != nullptr && result != nullptr, "null argument");
+    auto it = b->results.find(ticket);
+    if (ready) *ready = it != b->results.end();
+    if (it == b->results.end()) {
+      std::memset(result, 0, sizeof(*result));
+      return;
+    }
+    *result = it->second;
+  });
+}
+
+int alps_ed_batch_stats(alps_ed_batch b, alps_ed_batch_status* s) {
+  return guarded([&] {
+    require(b != nullptr && s != nullptr, "null argument");
+    s->n_active = (int)b->active.size();
+    s->n_pending = (int)b->pending.size();
+    s->hbm_used = b->arena.used();
+    s->hbm_bytes = b->arena.size();
+    s->rounds = b->rounds;
+    s->steps = b->steps;
+    s->solves_done = b->solves_done;
+  });
+}
+
+int alps_ed_batch_footprint(const alps_ed_model* model, const alps_ed_solve_params* params, alps_ed_dtype dtype,
+                            uint64_t* bytes) {
+  return guarded([&] {
+    require(params && bytes, "null argument");
+    Problem p;
+    p.spec = model_spec(model);
+    p.layout = build_layout(p.spec);
+    p.D = p.layout.D;
+    p.fp32 = dtype == ALPS_ED_F32 && p.D >= kFp32MinDim;
+    p.max_iter = params->max_iter > 0 ? params->max_iter : (int)std::min<uint64_t>(10 * std::max<uint64_t>(p.D, 1), 1000);
+    p.nev = params->nev > 0 ? params->nev : 1;
+    p.n_chunks = (uint32_t)std::max<uint64_t>(1, (p.D + kBatchChunk - 1) / kBatchChunk);
+    p.cap = p.fp32 ? (uint32_t)(p.max_iter + 2) : 3u;
+    p.nvr = p.fp32 ? std::min(p.nev, kMaxRitz) : 0;
+    *bytes = footprint(p).dev;
+  });
+}
+
+}  // extern "C"

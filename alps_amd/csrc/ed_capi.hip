@@ -15,90 +15,14 @@
 #include <ietl/tmatrix.h>
 
 #include "alps_ed.h"
+#include "ed_host.hpp"
 #include "ed_kernels.hpp"
 #include "ed_sparsediag.hpp"
 
 using namespace alps_ed;
+using namespace alps_ed::host;
 
 namespace {
-
-thread_local std::string g_last_error;
-thread_local hipStream_t g_stream = nullptr;
-
-struct DeviceError : std::runtime_error {
-  explicit DeviceError(const std::string& s) : std::runtime_error(s) {}
-};
-struct MemoryError : std::runtime_error {
-  explicit MemoryError(const std::string& s) : std::runtime_error(s) {}
-};
-struct StateError : std::runtime_error {
-  explicit StateError(const std::string& s) : std::runtime_error(s) {}
-};
-
-void hip_check(hipError_t e, const char* what) {
-  if (e == hipSuccess) return;
-  if (e == hipErrorOutOfMemory) throw MemoryError(std::string(what) + ": " + hipGetErrorString(e));
-  throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
-}
-
-template <class F>
-int guarded(F&& f) {
-  try {
-    f();
-    return ALPS_ED_OK;
-  } catch (const std::invalid_argument& e) {
-    g_last_error = e.what();
-    return ALPS_ED_ERR_ARGUMENT;
-  } catch (const MemoryError& e) {
-    g_last_error = e.what();
-    return ALPS_ED_ERR_MEMORY;
-  } catch (const DeviceError& e) {
-    g_last_error = e.what();
-    return ALPS_ED_ERR_DEVICE;
-  } catch (const StateError& e) {
-    g_last_error = e.what();
-    return ALPS_ED_ERR_STATE;
-  } catch (const std::bad_alloc&) {
-    g_last_error = "host allocation failed";
-    return ALPS_ED_ERR_MEMORY;
-  } catch (const std::runtime_error& e) {
-    g_last_error = e.what();
-    return ALPS_ED_ERR_SOLVER;
-  } catch (...) {
-    g_last_error = "unknown error";
-    return ALPS_ED_ERR_DEVICE;
-  }
-}
-
-void require(bool c, const char* msg) {
-  if (!c) throw std::invalid_argument(msg);
-}
-
-// RAII device buffer
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  DevBuf() = default;
-  explicit DevBuf(size_t b) { alloc(b); }
-  void alloc(size_t b) {
-    release();
-    if (b == 0) return;
-    hip_check(hipMalloc(&p, b), "hipMalloc");
-    bytes = b;
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  ~DevBuf() { release(); }
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  template <class T>
-  T* as() const {
-    return static_cast<T*>(p);
-  }
-};
 
 template <class T>
 void upload(DevBuf& b, const std::vector<T>& v) {
@@ -177,25 +101,6 @@ struct alps_ed_lanczos_s {
 
 namespace {
 
-ModelSpec model_spec(const alps_ed_model* m) {
-  require(m != nullptr, "model is null");
-  require(m->n_sites >= 1 && m->n_sites <= kMaxSites, "n_sites must be in [1, 32]");
-  require(m->n_bonds >= 0, "n_bonds must be >= 0");
-  require(m->n_bonds == 0 || (m->bond_site1 && m->bond_site2), "bond site arrays are null");
-  ModelSpec s;
-  s.n_sites = m->n_sites;
-  s.n_up = m->n_up;
-  for (int b = 0; b < m->n_bonds; ++b) {
-    require(m->bond_site1[b] >= 0 && m->bond_site1[b] < m->n_sites && m->bond_site2[b] >= 0 &&
-                m->bond_site2[b] < m->n_sites && m->bond_site1[b] != m->bond_site2[b],
-            "bond with invalid sites");
-    s.bonds.push_back(make_bond(m->bond_site1[b], m->bond_site2[b], m->jz ? m->jz[b] : 0.0,
-                                m->jxy ? m->jxy[b] : 0.0, m->k ? m->k[b] : 0.0));
-  }
-  for (int i = 0; i < m->n_sites; ++i) s.sites.push_back(make_site(m->h ? m->h[i] : 0.0, m->d ? m->d[i] : 0.0));
-  return s;
-}
-
 void operator_upload(alps_ed_operator op) {
   SectorLayout& L = op->layout;
   DevLayout& d = op->dev;
@@ -265,11 +170,11 @@ void with_dtype(alps_ed_dtype t, F&& f) {
 template <class V>
 double dot_sync(uint64_t n, const V* x, const V* y) {
   Reducer& r = thread_reducer();
-  hip_check(launch_dot<V>(n, x, y, r.a(), g_stream), "dot");
-  hip_check(launch_finish(Partials{r.a(), stream_blocks(n)}, nullptr, r.result(0), nullptr, kSum, g_stream), "finish");
+  hip_check(launch_dot<V>(n, x, y, r.a(), current_stream()), "dot");
+  hip_check(launch_finish(Partials{r.a(), stream_blocks(n)}, nullptr, r.result(0), nullptr, kSum, current_stream()), "finish");
   double h = 0.0;
-  hip_check(hipMemcpyAsync(&h, r.result(0), sizeof(double), hipMemcpyDeviceToHost, g_stream), "copy");
-  hip_check(hipStreamSynchronize(g_stream), "sync");
+  hip_check(hipMemcpyAsync(&h, r.result(0), sizeof(double), hipMemcpyDeviceToHost, current_stream()), "copy");
+  hip_check(hipStreamSynchronize(current_stream()), "sync");
   return h;
 }
 
@@ -309,7 +214,7 @@ extern "C" {
 
 int alps_ed_version(void) { return 100; }
 
-const char* alps_ed_last_error(void) { return g_last_error.c_str(); }
+const char* alps_ed_last_error(void) { return last_error().c_str(); }
 
 int alps_ed_device_info(char* name, size_t name_len, uint64_t* hbm_bytes, int* n_cu) {
   return guarded([&] {
@@ -328,12 +233,12 @@ int alps_ed_device_info(char* name, size_t name_len, uint64_t* hbm_bytes, int* n
 }
 
 int alps_ed_set_stream(void* s) {
-  g_stream = static_cast<hipStream_t>(s);
+  current_stream() = static_cast<hipStream_t>(s);
   return ALPS_ED_OK;
 }
 
 int alps_ed_synchronize(void) {
-  return guarded([&] { hip_check(hipStreamSynchronize(g_stream), "hipStreamSynchronize"); });
+  return guarded([&] { hip_check(hipStreamSynchronize(current_stream()), "hipStreamSynchronize"); });
 }
 
 // ---------------------------------------------------------------------------
@@ -375,9 +280,9 @@ int alps_ed_operator_states(alps_ed_operator op, uint64_t first, uint64_t count,
     require(first + count <= op->layout.D, "state range beyond the dimension");
     if (count == 0) return;
     DevBuf d(count * sizeof(uint32_t));
-    hip_check(launch_unrank(op->dev, first, count, d.as<uint32_t>(), g_stream), "unrank");
-    hip_check(hipMemcpyAsync(states_out, d.p, count * sizeof(uint32_t), hipMemcpyDeviceToHost, g_stream), "copy");
-    hip_check(hipStreamSynchronize(g_stream), "sync");
+    hip_check(launch_unrank(op->dev, first, count, d.as<uint32_t>(), current_stream()), "unrank");
+    hip_check(hipMemcpyAsync(states_out, d.p, count * sizeof(uint32_t), hipMemcpyDeviceToHost, current_stream()), "copy");
+    hip_check(hipStreamSynchronize(current_stream()), "sync");
   });
 }
 
@@ -386,10 +291,10 @@ int alps_ed_operator_index(alps_ed_operator op, const uint32_t* states, uint64_t
     require(op && ((states && index_out) || count == 0), "null argument");
     if (count == 0) return;
     DevBuf ds(count * sizeof(uint32_t)), di(count * sizeof(uint64_t));
-    hip_check(hipMemcpyAsync(ds.p, states, count * sizeof(uint32_t), hipMemcpyHostToDevice, g_stream), "copy");
-    hip_check(launch_rank(op->dev, ds.as<uint32_t>(), count, di.as<uint64_t>(), g_stream), "rank");
-    hip_check(hipMemcpyAsync(index_out, di.p, count * sizeof(uint64_t), hipMemcpyDeviceToHost, g_stream), "copy");
-    hip_check(hipStreamSynchronize(g_stream), "sync");
+    hip_check(hipMemcpyAsync(ds.p, states, count * sizeof(uint32_t), hipMemcpyHostToDevice, current_stream()), "copy");
+    hip_check(launch_rank(op->dev, ds.as<uint32_t>(), count, di.as<uint64_t>(), current_stream()), "rank");
+    hip_check(hipMemcpyAsync(index_out, di.p, count * sizeof(uint64_t), hipMemcpyDeviceToHost, current_stream()), "copy");
+    hip_check(hipStreamSynchronize(current_stream()), "sync");
   });
 }
 
@@ -403,12 +308,12 @@ int alps_ed_operator_elements(alps_ed_operator op, uint64_t row0, uint64_t nrows
     const uint64_t ne = nrows * (uint64_t)max_per_row;
     DevBuf dc(ne * sizeof(uint32_t)), dv(ne * sizeof(double)), dn(nrows * sizeof(int));
     hip_check(launch_elements(op->dev, row0, nrows, max_per_row, dc.as<uint32_t>(), dv.as<double>(),
-                              dn.as<int>(), g_stream),
+                              dn.as<int>(), current_stream()),
               "elements");
-    hip_check(hipMemcpyAsync(cols, dc.p, ne * sizeof(uint32_t), hipMemcpyDeviceToHost, g_stream), "copy");
-    hip_check(hipMemcpyAsync(vals, dv.p, ne * sizeof(double), hipMemcpyDeviceToHost, g_stream), "copy");
-    hip_check(hipMemcpyAsync(counts, dn.p, nrows * sizeof(int), hipMemcpyDeviceToHost, g_stream), "copy");
-    hip_check(hipStreamSynchronize(g_stream), "sync");
+    hip_check(hipMemcpyAsync(cols, dc.p, ne * sizeof(uint32_t), hipMemcpyDeviceToHost, current_stream()), "copy");
+    hip_check(hipMemcpyAsync(vals, dv.p, ne * sizeof(double), hipMemcpyDeviceToHost, current_stream()), "copy");
+    hip_check(hipMemcpyAsync(counts, dn.p, nrows * sizeof(int), hipMemcpyDeviceToHost, current_stream()), "copy");
+    hip_check(hipStreamSynchronize(current_stream()), "sync");
   });
 }
 
@@ -425,7 +330,7 @@ int alps_ed_vector_create(uint64_t n, alps_ed_dtype dtype, alps_ed_vector* v) {
     p->buf.alloc(std::max<uint64_t>(n, 1) * dtype_size(dtype));
     with_dtype(dtype, [&](auto z) {
       using V = decltype(z);
-      hip_check(launch_fill<V>(n, p->buf.as<V>(), 0.0, g_stream), "fill");
+      hip_check(launch_fill<V>(n, p->buf.as<V>(), 0.0, current_stream()), "fill");
     });
     *v = p.release();
   });
@@ -466,14 +371,14 @@ int alps_ed_vector_upload(alps_ed_vector v, const double* host, uint64_t n) {
     require(n == v->n && (host || n == 0), "size mismatch");
     if (n == 0) return;
     if (v->dtype == ALPS_ED_F64) {
-      hip_check(hipMemcpyAsync(v->buf.p, host, n * 8, hipMemcpyHostToDevice, g_stream), "copy");
+      hip_check(hipMemcpyAsync(v->buf.p, host, n * 8, hipMemcpyHostToDevice, current_stream()), "copy");
     } else {
       DevBuf tmp(n * 8);
-      hip_check(hipMemcpyAsync(tmp.p, host, n * 8, hipMemcpyHostToDevice, g_stream), "copy");
-      hip_check(launch_convert<float, double>(n, v->buf.as<float>(), tmp.as<double>(), nullptr, g_stream), "convert");
-      hip_check(hipStreamSynchronize(g_stream), "sync");
+      hip_check(hipMemcpyAsync(tmp.p, host, n * 8, hipMemcpyHostToDevice, current_stream()), "copy");
+      hip_check(launch_convert<float, double>(n, v->buf.as<float>(), tmp.as<double>(), nullptr, current_stream()), "convert");
+      hip_check(hipStreamSynchronize(current_stream()), "sync");
     }
-    hip_check(hipStreamSynchronize(g_stream), "sync");
+    hip_check(hipStreamSynchronize(current_stream()), "sync");
   });
 }
 
@@ -483,14 +388,14 @@ int alps_ed_vector_download(alps_ed_vector v, double* host, uint64_t n) {
     require(n == v->n && (host || n == 0), "size mismatch");
     if (n == 0) return;
     if (v->dtype == ALPS_ED_F64) {
-      hip_check(hipMemcpyAsync(host, v->buf.p, n * 8, hipMemcpyDeviceToHost, g_stream), "copy");
+      hip_check(hipMemcpyAsync(host, v->buf.p, n * 8, hipMemcpyDeviceToHost, current_stream()), "copy");
     } else {
       DevBuf tmp(n * 8);
-      hip_check(launch_convert<double, float>(n, tmp.as<double>(), v->buf.as<float>(), nullptr, g_stream), "convert");
-      hip_check(hipMemcpyAsync(host, tmp.p, n * 8, hipMemcpyDeviceToHost, g_stream), "copy");
-      hip_check(hipStreamSynchronize(g_stream), "sync");
+      hip_check(launch_convert<double, float>(n, tmp.as<double>(), v->buf.as<float>(), nullptr, current_stream()), "convert");
+      hip_check(hipMemcpyAsync(host, tmp.p, n * 8, hipMemcpyDeviceToHost, current_stream()), "copy");
+      hip_check(hipStreamSynchronize(current_stream()), "sync");
     }
-    hip_check(hipStreamSynchronize(g_stream), "sync");
+    hip_check(hipStreamSynchronize(current_stream()), "sync");
   });
 }
 
@@ -499,7 +404,7 @@ int alps_ed_vector_fill_random(alps_ed_vector v, uint64_t seed, double lo, doubl
     check_vec(v);
     with_dtype(v->dtype, [&](auto z) {
       using V = decltype(z);
-      hip_check(launch_fill_random<V>(v->n, v->buf.as<V>(), seed, lo, hi, g_stream), "fill_random");
+      hip_check(launch_fill_random<V>(v->n, v->buf.as<V>(), seed, lo, hi, current_stream()), "fill_random");
     });
   });
 }
@@ -509,7 +414,7 @@ int alps_ed_vector_fill(alps_ed_vector v, double value) {
     check_vec(v);
     with_dtype(v->dtype, [&](auto z) {
       using V = decltype(z);
-      hip_check(launch_fill<V>(v->n, v->buf.as<V>(), value, g_stream), "fill");
+      hip_check(launch_fill<V>(v->n, v->buf.as<V>(), value, current_stream()), "fill");
     });
   });
 }
@@ -521,11 +426,11 @@ int alps_ed_vector_copy(alps_ed_vector dst, alps_ed_vector src) {
     require(dst->n == src->n, "size mismatch");
     if (dst == src) return;
     if (dst->dtype == src->dtype) {
-      hip_check(hipMemcpyAsync(dst->buf.p, src->buf.p, src->n * dtype_size(src->dtype), hipMemcpyDeviceToDevice, g_stream), "copy");
+      hip_check(hipMemcpyAsync(dst->buf.p, src->buf.p, src->n * dtype_size(src->dtype), hipMemcpyDeviceToDevice, current_stream()), "copy");
     } else if (dst->dtype == ALPS_ED_F32) {
-      hip_check(launch_convert<float, double>(src->n, dst->buf.as<float>(), src->buf.as<double>(), nullptr, g_stream), "convert");
+      hip_check(launch_convert<float, double>(src->n, dst->buf.as<float>(), src->buf.as<double>(), nullptr, current_stream()), "convert");
     } else {
-      hip_check(launch_convert<double, float>(src->n, dst->buf.as<double>(), src->buf.as<float>(), nullptr, g_stream), "convert");
+      hip_check(launch_convert<double, float>(src->n, dst->buf.as<double>(), src->buf.as<float>(), nullptr, current_stream()), "convert");
     }
   });
 }
@@ -535,7 +440,7 @@ int alps_ed_vector_scale(alps_ed_vector v, double a) {
     check_vec(v);
     with_dtype(v->dtype, [&](auto z) {
       using V = decltype(z);
-      hip_check(launch_axpby<V>(v->n, v->buf.as<V>(), 0.0, nullptr, a, g_stream), "scale");
+      hip_check(launch_axpby<V>(v->n, v->buf.as<V>(), 0.0, nullptr, a, current_stream()), "scale");
     });
   });
 }
@@ -547,7 +452,7 @@ int alps_ed_vector_axpby(alps_ed_vector y, double a, alps_ed_vector x, double b)
     require(x->n == y->n && x->dtype == y->dtype, "size or dtype mismatch");
     with_dtype(y->dtype, [&](auto z) {
       using V = decltype(z);
-      hip_check(launch_axpby<V>(y->n, y->buf.as<V>(), a, x->buf.as<V>(), b, g_stream), "axpby");
+      hip_check(launch_axpby<V>(y->n, y->buf.as<V>(), a, x->buf.as<V>(), b, current_stream()), "axpby");
     });
   });
 }
@@ -586,7 +491,7 @@ int alps_ed_matvec(alps_ed_operator op, alps_ed_vector x, alps_ed_vector y) {
     require(x != y, "in-place product is not supported");
     with_dtype(x->dtype, [&](auto z) {
       using V = decltype(z);
-      hip_check(launch_hv<V>(op->dev, x->buf.as<V>(), y->buf.as<V>(), nullptr, g_stream), "hv");
+      hip_check(launch_hv<V>(op->dev, x->buf.as<V>(), y->buf.as<V>(), nullptr, current_stream()), "hv");
     });
   });
 }
@@ -607,20 +512,20 @@ int alps_ed_lanczos_step_vectors(alps_ed_operator op, alps_ed_vector v_prev, alp
     Reducer& r = thread_reducer();
     r.ensure(std::max<size_t>(hv_blocks(op->dev), stream_blocks(n)));
     double* res = r.result(0);
-    hip_check(hipMemcpyAsync(res + 2, &beta_prev, sizeof(double), hipMemcpyHostToDevice, g_stream), "copy");
+    hip_check(hipMemcpyAsync(res + 2, &beta_prev, sizeof(double), hipMemcpyHostToDevice, current_stream()), "copy");
     with_dtype(w->dtype, [&](auto z) {
       using V = decltype(z);
       V* wv = w->buf.as<V>();
-      hip_check(launch_hv<V>(op->dev, v_cur->buf.as<V>(), wv, r.a(), g_stream), "hv");
-      hip_check(launch_finish(Partials{r.a(), hv_blocks(op->dev)}, nullptr, res, nullptr, kAlpha, g_stream), "finish");
+      hip_check(launch_hv<V>(op->dev, v_cur->buf.as<V>(), wv, r.a(), current_stream()), "hv");
+      hip_check(launch_finish(Partials{r.a(), hv_blocks(op->dev)}, nullptr, res, nullptr, kAlpha, current_stream()), "finish");
       hip_check(launch_update<V>(n, wv, v_cur->buf.as<V>(), v_prev->buf.as<V>(), wv, res, nullptr, res + 2, nullptr,
-                                 r.b(), g_stream),
+                                 r.b(), current_stream()),
                 "update");
-      hip_check(launch_normalize<V>(n, wv, Partials{r.b(), stream_blocks(n)}, res + 1, g_stream), "normalize");
+      hip_check(launch_normalize<V>(n, wv, Partials{r.b(), stream_blocks(n)}, res + 1, current_stream()), "normalize");
     });
     double h[2];
-    hip_check(hipMemcpyAsync(h, res, 2 * sizeof(double), hipMemcpyDeviceToHost, g_stream), "copy");
-    hip_check(hipStreamSynchronize(g_stream), "sync");
+    hip_check(hipMemcpyAsync(h, res, 2 * sizeof(double), hipMemcpyDeviceToHost, current_stream()), "copy");
+    hip_check(hipStreamSynchronize(current_stream()), "sync");
     *alpha = h[0];
     *beta = h[1];
   });
@@ -652,7 +557,7 @@ int alps_ed_lanczos_create(alps_ed_operator op, alps_ed_dtype dtype, int max_ste
     lz->beta.alloc((max_steps + 2) * sizeof(double));
     lz->scale.alloc((max_steps + 3) * sizeof(double));
     lz->red.ensure(std::max<size_t>(hv_blocks(op->dev), stream_blocks(n)));
-    lz->stream = g_stream;
+    lz->stream = current_stream();
     *out = lz.release();
   });
 }
@@ -666,7 +571,7 @@ int alps_ed_lanczos_start(alps_ed_lanczos lz, uint64_t seed, alps_ed_vector star
   return guarded([&] {
     require(lz != nullptr, "null argument");
     const uint64_t n = lz->op->layout.D;
-    lz->stream = g_stream;
+    lz->stream = current_stream();
     with_dtype(lz->dtype, [&](auto z) {
       using V = decltype(z);
       V* u0 = static_cast<V*>(lz->slot(0));
@@ -807,51 +712,8 @@ int alps_ed_tmatrix_analyze(int n, const double* alpha, const double* beta, doub
 
 namespace {
 
-// The lowest distinct non-ghost eigenvalues of T[0..m) with their error
-// bounds and eigenvector positions.  With FP32-stored Lanczos vectors the
-// recurrence produces near-copies of converged eigenvalues that differ by
-// O(eps_f32 |T|), above the FP64 merge threshold; they are merged here at
-// 64 eps_f32 |lambda| (a merged group has multiplicity > 1, error 0, as in
-// tmatrix.h:164-183).
-struct TValues {
-  std::vector<double> ev, err;
-  std::vector<int> index;
-};
-
-TValues analyze_prefix(const std::vector<double>& a, const std::vector<double>& b, int m, bool fp32) {
-  double amax = a[0], bmax = b[0];
-  for (int i = 1; i < m; ++i) {
-    amax = std::max(amax, a[i]);
-    bmax = std::max(bmax, b[i]);
-  }
-  ietl::tmatrix_analysis an;
-  an.run(a.data(), b.data(), m, amax, bmax, true);
-  TValues t;
-  const double dup = 64.0 * std::numeric_limits<float>::epsilon();
-  for (size_t i = 0; i < an.eigval_noghost.size(); ++i) {
-    const double lam = an.eigval_noghost[i];
-    if (fp32 && !t.ev.empty() && lam - t.ev.back() <= dup * std::max(1.0, std::fabs(t.ev.back()))) {
-      t.err.back() = 0.0;
-      continue;
-    }
-    t.ev.push_back(lam);
-    t.err.push_back(an.err_noghost[i]);
-    t.index.push_back(an.index_noghost[i]);
-  }
-  return t;
-}
-
-// lanczos_iteration_nlowest::converged (iteration.h:116-130), rtol = atol = 100 eps.
-bool nlowest_converged(const TValues& t, int nev) {
-  const double tol = 100. * std::numeric_limits<double>::epsilon();
-  if ((int)t.ev.size() < nev) return false;
-  for (int i = 0; i < nev; ++i)
-    if (t.err[i] > std::max(tol, tol * std::fabs(t.ev[i]))) return false;
-  return true;
-}
-
 void check_status(int st) {
-  if (st != ALPS_ED_OK) throw DeviceError(g_last_error);
+  if (st != ALPS_ED_OK) throw DeviceError(last_error());
 }
 
 // Rayleigh quotients (FP64 accumulation) of the Ritz vectors of T[0..m)

@@ -50,6 +50,33 @@ __device__ __forceinline__ T ldc(const T* p) {
   return *(const __attribute__((address_space(4))) T*)(p);
 }
 
+// Start-vector element i of stream `seed`: uniform on [lo, hi) from the
+// splitmix64 output of the counter seed + (i+1) * golden gamma.
+__device__ __forceinline__ uint64_t splitmix64_mix(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ double start_value(uint64_t seed, uint64_t i, double lo, double hi) {
+  const uint64_t z = splitmix64_mix(seed + (i + 1) * 0x9e3779b97f4a7c15ULL);
+  const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  return lo + (hi - lo) * u;
+}
+
+// Whole struct read through the constant address space (scalar loads when
+// the address is wave-uniform); T is plain data of 8-byte multiple size.
+template <class T>
+__device__ __forceinline__ T ldc_struct(const T* p) {
+  static_assert(sizeof(T) % 8 == 0, "ldc_struct needs an 8-byte multiple");
+  T out;
+  uint64_t* o = reinterpret_cast<uint64_t*>(&out);
+  const __attribute__((address_space(4))) uint64_t* s = (const __attribute__((address_space(4))) uint64_t*)(p);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 8); ++i) o[i] = s[i];
+  return out;
+}
+
 // H.v over the row segment of high word h (one workgroup of kBlock threads;
 // dynamic LDS of at least (L.max_seg + 1) elements of V):
 //   y[r] = sum_c H[r][c] x[c] for the segment's rows r;

@@ -104,20 +104,12 @@ __global__ __launch_bounds__(kBlock) void normalize_kernel(uint64_t n, V* v, Par
     v[i] = (V)(inv * (double)v[i]);
 }
 
-__device__ __forceinline__ uint64_t splitmix64_mix(uint64_t z) {
-  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-  return z ^ (z >> 31);
-}
-
 template <class V>
 __global__ __launch_bounds__(kBlock) void fill_random_kernel(uint64_t n, V* v, uint64_t seed, double lo,
                                                              double hi) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const uint64_t z = splitmix64_mix(seed + (i + 1) * 0x9e3779b97f4a7c15ULL);
-    const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
-    v[i] = (V)(lo + (hi - lo) * u);
+    v[i] = (V)start_value(seed, i, lo, hi);
   }
 }
 

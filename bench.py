@@ -85,6 +85,125 @@ def cpu_baseline(L, rows):
     }
 
 
+def xxz_batch(n, seed, Ls, field=False, any_sector=False):
+    """Random-bond XXZ chains: L uniform over Ls, J_i ~ U[0.5,1.5],
+    Delta ~ U[0,2] (Jz_i = Delta J_i, Jxy_i = J_i), open or periodic with
+    probability 1/2, optional uniform field h ~ U[0,0.5].  Sector Sz=0, or
+    Sz=+1/2 for odd L; with any_sector, n_up uniform over 0..L."""
+    import numpy as np
+
+    import alps_amd as A
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        L = int(rng.choice(Ls))
+        periodic = bool(rng.integers(2))
+        nb = L if periodic else L - 1
+        s = np.arange(nb)
+        J = rng.uniform(0.5, 1.5, nb)
+        delta = rng.uniform(0.0, 2.0)
+        h = float(rng.uniform(0.0, 0.5)) if field else 0.0
+        n_up = int(rng.integers(0, L + 1)) if any_sector else (L + 1) // 2
+        out.append(A.SpinModel(L, s, (s + 1) % L, delta * J, J, n_up, h=h))
+    return out
+
+
+def batch_leg(n, seq_sample, seed=1):
+    """1024 random XXZ chains (L in 12..20) solved in one batch, against the
+    same solves one at a time (alps_ed_sparsediag) on a sample."""
+    import numpy as np
+
+    import alps_amd as A
+
+    models = xxz_batch(n, seed, list(range(12, 21)))
+    free = A.device_info()["hbm_bytes"]
+    b = A.Batch("f32", hbm_bytes=int(free * 0.75), steps_per_round=8, host_threads=12)
+    t0 = time.perf_counter()
+    tickets = [b.submit(m, nev=1, seed=i) for i, m in enumerate(models)]
+    b.run()
+    t_batch = time.perf_counter() - t0
+    st = b.stats()
+    res = [b.result(t) for t in tickets]
+    b.close()
+    # the FP64 batch on the same problems is the accuracy reference
+    b64 = A.Batch("f64", hbm_bytes=int(free * 0.5), steps_per_round=8, host_threads=12)
+    t1 = time.perf_counter()
+    t64 = [b64.submit(m, nev=1, seed=i) for i, m in enumerate(models)]
+    b64.run()
+    t_batch64 = time.perf_counter() - t1
+    r64 = [b64.result(t) for t in t64]
+    b64.close()
+    rel = max(abs(r["ev"][0] - q["ev"][0]) / abs(q["ev"][0]) for r, q in zip(res, r64))
+    # one solve at a time on a sample of the same problems
+    t2 = time.perf_counter()
+    for i in range(seq_sample):
+        A.sparsediag(models[i], nev=1, seed=i, dtype="f32")
+    t_seq = (time.perf_counter() - t2) / seq_sample
+    dims = [r["dim"] for r in res]
+    return {
+        "solves": n,
+        "states_total": int(sum(dims)),
+        "batch_s": round(t_batch, 4),
+        "solves_per_s": round(n / t_batch, 1),
+        "fp64_batch_solves_per_s": round(n / t_batch64, 1),
+        "one_at_a_time_solves_per_s": round(1.0 / t_seq, 2),
+        "speedup_vs_one_at_a_time": round((n / t_batch) * t_seq, 2),
+        "one_at_a_time_sample": seq_sample,
+        "max_rel_err_f32_vs_f64": rel,
+        "mean_iterations": round(float(np.mean([r["iterations"] for r in res])), 1),
+        "rounds": st["rounds"],
+        "workload": "random-bond XXZ chains, L uniform 12..20, J_i~U[0.5,1.5], Delta~U[0,2], open/periodic 1:1, "
+                    "Sz=0 (Sz=1/2 for odd L), lowest eigenvalue, FP32 vectors + FP64 refinement",
+    }
+
+
+def serve_leg(n, rate, seed=2):
+    """Poisson request stream (XXZ chains, L uniform 12..26, field h ~ U[0,0.5],
+    sector uniform over n_up = 0..L), continuous batching with HBM admission."""
+    import numpy as np
+
+    import alps_amd as A
+
+    models = xxz_batch(n, seed, list(range(12, 27)), field=True, any_sector=True)
+    rng = np.random.default_rng(seed + 1)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    free = A.device_info()["hbm_bytes"]
+    b = A.Batch("f32", hbm_bytes=int(free * 0.75), steps_per_round=4, host_threads=12)
+    t0 = time.perf_counter()
+    i = 0
+    arrival_of = {}
+    lat = []
+    last_print = t0
+    while len(lat) < n:
+        now = time.perf_counter() - t0
+        while i < n and arrivals[i] <= now:
+            arrival_of[b.submit(models[i], nev=1, seed=i)] = arrivals[i]
+            i += 1
+        a, p = b.progress()
+        if a == 0 and p == 0 and i < n:
+            time.sleep(max(0.0, arrivals[i] - (time.perf_counter() - t0)))
+        done_t = time.perf_counter() - t0
+        for t in b.completed():
+            lat.append(done_t - arrival_of[t])
+        if time.perf_counter() - last_print > 20:
+            print(f"serve: {len(lat)}/{n} done", file=sys.stderr, flush=True)
+            last_print = time.perf_counter()
+    wall = time.perf_counter() - t0
+    b.close()
+    lat = np.array(lat)
+    return {
+        "requests": n,
+        "offered_rate_per_s": rate,
+        "solves_per_s": round(n / wall, 1),
+        "p50_latency_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+        "p99_latency_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+        "wall_s": round(wall, 3),
+        "workload": "Poisson arrivals; XXZ chains L uniform 12..26, Delta~U[0,2], h~U[0,0.5], n_up uniform 0..L, "
+                    "FP32 vectors + FP64 refinement, HBM-footprint admission",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +213,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-eigen", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=600000)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-size", type=int, default=1024)
+    ap.add_argument("--seq-sample", type=int, default=64)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs one solve per GPU; --gpus {args.gpus} is not supported "
@@ -185,6 +310,10 @@ def main():
         out["eigen"] = {"f64": r64["ev"].tolist(), "f32": r32["ev"].tolist(), "rel_err_f32": rel,
                         "iterations_f64": r64["iterations"], "iterations_f32": r32["iterations"],
                         "solve_s_f64": round(r64["seconds"], 4), "solve_s_f32": round(r32["seconds"], 4)}
+    if not args.no_batch:
+        out["batch"] = batch_leg(args.batch_size, args.seq_sample)
+    if not args.no_serve:
+        out["serve"] = serve_leg(args.serve_requests, args.serve_rate)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(L, min(args.cpu_rows, D))
     print(json.dumps(out), flush=True)

@@ -205,6 +205,59 @@ typedef struct {
 int alps_ed_sparsediag(const alps_ed_model* model, const alps_ed_solve_params* params,
                        alps_ed_solve_result* result);
 
+/* ---------------------------------------------------------------------------
+ * Batched solves and the request loop.
+ *
+ * Many independent do_subspace solves (each its own Hamiltonian and sector,
+ * as DiagMatrix::dostep produces one per sector, diag.h:86-138, and as the
+ * reference's task farm runs one per task) advance together: one launch of
+ * each batched kernel serves every problem in the batch, so small sectors
+ * fill the GPU.  Requests are admitted first come first served while their
+ * HBM footprint (tables, Lanczos vectors -- the whole basis for FP32 storage
+ * -- scalars and refinement buffers) fits the batch's arena; requests
+ * submitted while others run join at the next round.  Each problem's
+ * arithmetic is independent of the rest of the batch: its eigenvalues are
+ * bit-identical alone and in any mix.  The stopping rule, precision policy
+ * and results are those of alps_ed_sparsediag; the start vector uses the
+ * request's seed.  A request that can never fit is completed with
+ * error_code -1 and no eigenvalues.
+ * ------------------------------------------------------------------------- */
+typedef struct alps_ed_batch_s* alps_ed_batch;
+
+typedef struct {
+  alps_ed_dtype dtype;    /* storage of the Lanczos vectors (FP64 below 1000 states) */
+  uint64_t hbm_bytes;     /* arena size; 0 = half of the free device memory */
+  uint64_t host_bytes;    /* mapped host memory for alpha/beta mirrors; 0 = 256 MiB */
+  int steps_per_round;    /* Lanczos steps between stopping-rule tests; 0 = 8 */
+  int max_active;         /* problems admitted at once; 0 = no limit */
+  int host_threads;       /* threads of the host analysis; 0 = 8 */
+} alps_ed_batch_config;
+
+typedef struct {
+  int n_active, n_pending;
+  uint64_t hbm_used, hbm_bytes;
+  uint64_t rounds, steps, solves_done;
+} alps_ed_batch_status;
+
+int alps_ed_batch_create(const alps_ed_batch_config* cfg, alps_ed_batch* out);
+int alps_ed_batch_destroy(alps_ed_batch b);
+/* Queues one solve; the model arrays are copied.  *ticket identifies it. */
+int alps_ed_batch_submit(alps_ed_batch b, const alps_ed_model* model, const alps_ed_solve_params* params,
+                         uint64_t* ticket);
+/* One round: admit, start, steps_per_round steps, stopping-rule tests,
+ * refinement, retirement.  Returns the counts after the round. */
+int alps_ed_batch_progress(alps_ed_batch b, int* n_active, int* n_pending);
+/* Rounds until every submitted request is complete. */
+int alps_ed_batch_run(alps_ed_batch b);
+/* Tickets completed since the last call (at most max), oldest first. */
+int alps_ed_batch_completed(alps_ed_batch b, uint64_t* tickets, int max, int* n);
+/* Result of a ticket; *ready = 0 while it is not complete. */
+int alps_ed_batch_result(alps_ed_batch b, uint64_t ticket, alps_ed_solve_result* result, int* ready);
+int alps_ed_batch_stats(alps_ed_batch b, alps_ed_batch_status* stats);
+/* HBM bytes a request would hold while admitted. */
+int alps_ed_batch_footprint(const alps_ed_model* model, const alps_ed_solve_params* params, alps_ed_dtype dtype,
+                            uint64_t* bytes);
+
 /* The sparsediag task for the spin-1/2 path: runs every task of a parameter
  * file (the grammar of src/alps/parameter/parameterlist.h:20-36; LATTICE,
  * L, W, H, MODEL="spin", local_S=1/2, J/Jz/Jxy/J'/J1/Jz#/Jxy#/K#/h#/D#,
