@@ -204,6 +204,45 @@ def serve_leg(n, rate, seed=2):
     }
 
 
+def j1j2_leg(ev, iterations=300):
+    """5x6 J1-J2 square lattice (periodic, J1=1, J2=0.5, Sz=0,
+    D=155,117,520): H.v rate and a fixed-length Lanczos run with FP32
+    vectors (basis kept in HBM, FP64 refinement) next to FP64 vectors."""
+    import numpy as np
+
+    import alps_amd as A
+
+    (t,) = A.describe_parameter_file('LATTICE="frustrated square lattice"; L=6; W=5; J=1; J1=0.5; '
+                                     "Sz_total=0; {}")
+    m = A.SpinModel(t["n_sites"], np.array(t["bond_s"]), np.array(t["bond_t"]), np.array(t["jz"]),
+                    np.array(t["jxy"]), 15)
+    op = A.Operator(m)
+    D = op.dim
+    x, y = A.Vector(D, "f32"), A.Vector(D, "f32")
+    x.fill_random(1)
+    for _ in range(2):
+        op.matvec(x, y)
+    ev.sync()
+    e0, e1 = ev.event(), ev.event()
+    ev.record(e0)
+    for _ in range(10):
+        op.matvec(x, y)
+    ev.record(e1)
+    ms = ev.elapsed_ms(e0, e1) / 10
+    del x, y
+    r32 = A.sparsediag(m, max_iter=iterations, fixed=True, nev=2, seed=1, dtype="f32")
+    r64 = A.sparsediag(m, max_iter=iterations, fixed=True, nev=2, seed=1, dtype="f64")
+    rel = (np.abs(r32["ev"] - r64["ev"]) / np.abs(r64["ev"])).tolist()
+    return {
+        "workload": "J1-J2 Heisenberg, 6x5 periodic square lattice, J1=1, J2=0.5, Sz=0",
+        "dim": D, "bonds": len(t["bond_s"]),
+        "hv_ms": round(ms, 4), "hv_gbs_8D": round(8.0 * D / (ms * 1e-3) / 1e9, 1),
+        "iterations": iterations,
+        "f32_eigenvalues": r32["ev"].tolist(), "f64_eigenvalues": r64["ev"].tolist(), "rel_err_f32": rel,
+        "solve_s_f32": round(r32["seconds"], 3), "solve_s_f64": round(r64["seconds"], 3),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -213,6 +252,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-eigen", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=600000)
+    ap.add_argument("--j1j2", action="store_true", help="also run the 5x6 J1-J2 leg (about 200 GB of HBM)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-size", type=int, default=1024)
     ap.add_argument("--seq-sample", type=int, default=64)
@@ -310,6 +350,8 @@ def main():
         out["eigen"] = {"f64": r64["ev"].tolist(), "f32": r32["ev"].tolist(), "rel_err_f32": rel,
                         "iterations_f64": r64["iterations"], "iterations_f32": r32["iterations"],
                         "solve_s_f64": round(r64["seconds"], 4), "solve_s_f32": round(r32["seconds"], 4)}
+    if args.j1j2:
+        out["j1j2"] = j1j2_leg(ev)
     if not args.no_batch:
         out["batch"] = batch_leg(args.batch_size, args.seq_sample)
     if not args.no_serve:

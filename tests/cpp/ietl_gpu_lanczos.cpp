@@ -1,11 +1,14 @@
 // ietl::lanczos<alps_ed::hamiltonian, alps_ed::gpu_vectorspace>: the
 // sparsediag Lanczos call (applications/diag/sparsediag/sparsediag.h:104-118)
 // with the device operator and device vectors.
-//   usage: ietl_gpu_lanczos <L> <n_up> <f32|f64> <max_iter> <nev> <seed>
+//   usage: ietl_gpu_lanczos <L> <n_up> <f32|f64> <max_iter> <nev> <seed> [vectors]
+// With "vectors", also the Ritz vectors (lanczos::eigenvectors, replaying the
+// device recurrence) with their Info and the first components.
 // Heisenberg ring J=1 on L sites; prints iterations and eigenvalues.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <iterator>
 #include <vector>
 
 #include <alps_ed/ietl_gpu.hpp>
@@ -33,6 +36,20 @@ int main(int argc, char** argv) {
     printf("dim %llu iterations %u error %d\n", (unsigned long long)H.dimension(), it.iterations(), it.error_code());
     const std::vector<double>& ev = lz.eigenvalues();
     for (size_t i = 0; i < ev.size() && i < (size_t)nev; ++i) printf("ev %.17g\n", ev[i]);
+    if (argc > 7) {
+      std::vector<alps_ed::gpu_vector> vecs;
+      ietl::Info<double> info;
+      const size_t k = std::min(ev.size(), (size_t)nev);
+      lz.eigenvectors(ev.begin(), ev.begin() + k, std::back_inserter(vecs), info, alps_ed::uniform_start{seed});
+      for (size_t i = 0; i < vecs.size(); ++i) {
+        printf("info %d %d %d %d %.17g %.17g %.17g\n", info.m1(i), info.m2(i), info.ma(i), (int)info.error_info(i),
+               info.eigenvalue(i), info.residual(i), alps_ed::two_norm(vecs[i]));
+        const std::vector<double> h = vecs[i].to_host();
+        printf("vec");
+        for (size_t q = 0; q < h.size() && q < 4096; ++q) printf(" %.17g", h[q]);
+        printf("\n");
+      }
+    }
   } catch (const std::exception& e) {
     printf("exception %s\n", e.what());
     return 1;

@@ -1,12 +1,15 @@
 // Host-only check of include/ietl/{lanczos,tmatrix,iteration,vectorspace}.h:
 // reads a CSR matrix, runs ietl::lanczos with std::vector<double> vectors
 // and prints alpha/beta and the distinct non-ghost eigenvalues.
-//   usage: ietl_host_lanczos <csr file> <max_iter> <nev> <seed> <fixed 0|1>
+//   usage: ietl_host_lanczos <csr file> <max_iter> <nev> <seed> <fixed 0|1> [vectors]
+// With "vectors", also the Ritz vectors of the reported eigenvalues
+// (lanczos::eigenvectors) with their Info.
 // CSR file: D nnz, then D+1 row pointers, nnz columns, nnz values (text).
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
+#include <iterator>
 #include <vector>
 
 #include <ietl/vectorspace.h>
@@ -77,5 +80,18 @@ int main(int argc, char** argv) {
   for (size_t i = 0; i < lz.diagonal().size(); ++i) printf("ab %.17g %.17g\n", lz.diagonal()[i], lz.subdiagonal()[i]);
   const std::vector<double>& ev = lz.eigenvalues();
   for (size_t i = 0; i < ev.size() && i < (size_t)nev; ++i) printf("ev %.17g\n", ev[i]);
+  if (argc > 6) {
+    std::vector<std::vector<double> > vecs;
+    ietl::Info<double> info;
+    const size_t k = std::min(ev.size(), (size_t)nev);
+    lz.eigenvectors(ev.begin(), ev.begin() + k, std::back_inserter(vecs), info, start_gen{seed});
+    for (size_t i = 0; i < vecs.size(); ++i) {
+      printf("info %d %d %d %d %.17g %.17g\n", info.m1(i), info.m2(i), info.ma(i), (int)info.error_info(i),
+             info.eigenvalue(i), info.residual(i));
+      printf("vec");
+      for (double v : vecs[i]) printf(" %.17g", v);
+      printf("\n");
+    }
+  }
   return 0;
 }

@@ -260,6 +260,29 @@ def test_ietl_lanczos_on_device_vectorspace(dtype, rtol):
         np.testing.assert_allclose(ev, ref["ev"], rtol=rtol, atol=0)
 
 
+@pytest.mark.parametrize("dtype,res_tol", [("f64", 1e-8), ("f32", 1e-4)])
+def test_ietl_eigenvectors_on_device_vectorspace(dtype, res_tol):
+    """lanczos::eigenvectors with device vectors: the recurrence is replayed
+    on the GPU (it must reproduce alpha and beta bit for bit) and the Ritz
+    vectors are checked against the dense eigenvectors."""
+    om = O.lattice_model("chain lattice", 12)   # the program's Heisenberg ring
+    H = O.dense(om, 6)
+    w, Z = np.linalg.eigh(H)
+    out = subprocess.run([IETL_GPU, "12", "6", dtype, "1000", "2", "9", "vectors"], capture_output=True, text=True,
+                         timeout=120)
+    say(out.stdout[:2000])
+    assert out.returncode == 0, out.stdout + out.stderr
+    infos = [l.split()[1:] for l in out.stdout.split("\n") if l.startswith("info ")]
+    vecs = [np.array(list(map(float, l.split()[1:]))) for l in out.stdout.split("\n") if l.startswith("vec")]
+    assert len(infos) == 2 and len(vecs) == 2
+    for inf, v in zip(infos, vecs):
+        status, lam, res, nrm = int(inf[3]), float(inf[4]), float(inf[5]), float(inf[6])
+        assert status == 0 and res < res_tol and abs(nrm - 1) < res_tol
+        j = np.argmin(abs(w - lam))
+        assert abs(lam - w[j]) < 1e-9
+        assert np.linalg.norm(H @ v - lam * v) < 10 * res_tol
+
+
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_parameter_file_sectors_ground_and_gap(prec):
     # a sparsediag parameter file with Sz conserved: every sector's spectrum,

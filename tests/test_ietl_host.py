@@ -39,3 +39,34 @@ def test_host_ietl_lanczos_matches_oracle(name, L, k, fixed, tmp_path):
     np.testing.assert_allclose(ab[:, 0], ref["alpha"], rtol=1e-12, atol=1e-12)
     np.testing.assert_allclose(ab[:, 1], ref["beta"], rtol=1e-12, atol=1e-12)
     np.testing.assert_allclose(ev, ref["ev"], rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("name,L,k", [("chain lattice", 10, 5), ("open chain lattice", 11, 5)])
+def test_host_ietl_eigenvectors(name, L, k, tmp_path):
+    # lanczos::eigenvectors: Ritz vectors of the reported eigenvalues, checked
+    # against the dense eigenvectors (up to sign) and by their residuals
+    m = O.lattice_model(name, L, jz=0.7)
+    rp, col, val = O.build_csr(m, k)
+    f = tmp_path / "csr.txt"
+    with open(f, "w") as fh:
+        fh.write(f"{len(rp) - 1} {len(col)}\n")
+        fh.write(" ".join(map(str, rp.tolist())) + "\n")
+        fh.write(" ".join(map(str, col.tolist())) + "\n")
+        fh.write(" ".join(repr(float(v)) for v in val) + "\n")
+    out = subprocess.run([BIN, str(f), "1000", "2", "3", "0", "vectors"], capture_output=True, text=True,
+                         check=True).stdout.split("\n")
+    infos = [l.split()[1:] for l in out if l.startswith("info ")]
+    vecs = [np.array(list(map(float, l.split()[1:]))) for l in out if l.startswith("vec")]
+    H = O.dense(m, k)
+    w, Z = np.linalg.eigh(H)
+    assert len(vecs) == 2
+    for i, (inf, v) in enumerate(zip(infos, vecs)):
+        m1, m2, ma, status = map(int, inf[:4])
+        lam, res = float(inf[4]), float(inf[5])
+        assert status == 0 and 2 <= m1 <= ma
+        assert abs(lam - w[np.argmin(abs(w - lam))]) < 1e-10
+        assert res < 1e-8 and abs(np.linalg.norm(v) - 1) < 1e-8
+        assert np.linalg.norm(H @ v - lam * v) < 1e-8
+        j = np.argmin(abs(w - lam))
+        if np.min(abs(np.delete(w, j) - w[j])) > 1e-6:   # non-degenerate: the eigenvector itself
+            assert abs(abs(v @ Z[:, j]) - 1) < 1e-8
