@@ -448,15 +448,15 @@ void decide(Problem& p) {
     return;
   }
   std::vector<double> a(p.host_alpha, p.host_alpha + n), bb(p.host_beta, p.host_beta + n);
-  const double tol = 100. * std::numeric_limits<double>::epsilon();
+  const double tol = stop_tolerance(p.fp32);
   for (int m = std::max(2, p.scan_from); m <= n; ++m) {
     double lam = 0.0;
     const double e = lowest_ritz_error(a.data(), bb.data(), m, &lam);
     if (!(e <= 4.0 * std::max(tol, tol * std::fabs(lam)))) continue;
-    if (!nlowest_converged(analyze_prefix(a, bb, m, p.fp32), p.nev)) continue;
+    if (!nlowest_converged(analyze_prefix(a, bb, m, p.fp32), p.nev, p.fp32)) continue;
     // the full test holds at m; step back while it holds earlier too
     int first = m;
-    while (first - 1 >= std::max(2, p.scan_from) && nlowest_converged(analyze_prefix(a, bb, first - 1, p.fp32), p.nev))
+    while (first - 1 >= std::max(2, p.scan_from) && nlowest_converged(analyze_prefix(a, bb, first - 1, p.fp32), p.nev, p.fp32))
       --first;
     p.final_n = first;
     return;

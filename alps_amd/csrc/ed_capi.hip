@@ -34,17 +34,19 @@ void upload(DevBuf& b, const std::vector<T>& v) {
 // so a consumer never reads an array that is being rewritten) and result
 // slots.  Partial counts follow the producing kernel's grid.
 struct Reducer {
-  DevBuf pa, pb, results;
+  DevBuf pa, pb, pc, results;
   size_t cap = 0;
   void ensure(size_t nblocks) {
     if (nblocks <= cap) return;
     pa.alloc(nblocks * sizeof(double));
     pb.alloc(nblocks * sizeof(double));
+    pc.alloc(nblocks * sizeof(double));
     cap = nblocks;
     if (!results.p) results.alloc(16 * sizeof(double));
   }
   double* a() { return pa.as<double>(); }
   double* b() { return pb.as<double>(); }
+  double* c() { return pc.as<double>(); }
   double* result(int i) { return results.as<double>() + i; }
 };
 
@@ -85,6 +87,7 @@ struct alps_ed_lanczos_s {
   // basis is kept, chunks of kChunk consecutive vectors allocated on demand.
   static constexpr int kChunk = 32;
   DevBuf u[2], w;
+  unsigned norm_count[2] = {0, 0};  // partials in each norm buffer
   std::vector<std::unique_ptr<DevBuf>> chunks;
   DevBuf alpha, beta, scale;  // FP64 on device
   Reducer red;
@@ -178,9 +181,16 @@ double dot_sync(uint64_t n, const V* x, const V* y) {
   return h;
 }
 
-// One Lanczos step of the device-resident recurrence (see alps_ed_lanczos_*):
-// w = scale[j] H u_j with alpha_j; r = w - alpha_j v_j - beta_{j-1} v_{j-1}
-// stored as u_{j+1} with scale[j+1] = 1/|r| = 1/beta_j.
+// One Lanczos step of the device-resident recurrence (see alps_ed_lanczos_*),
+// two launches: w = H u_j with the partials of <u_j, w>; then the fused
+// update, which forms scale_j = 1/|u_j| and beta_{j-1} from the norm
+// partials of the previous stage, alpha_j = scale_j^2 <u_j, w>, and stores
+// r = scale_j w - alpha_j v_j - beta_{j-1} v_{j-1} as u_{j+1} with its norm
+// partials.  The norm partials alternate between two buffers: stage j (the
+// start for j = 0, the update of step j-1 otherwise) wrote norm_buf(j).
+// beta_j and scale_{j+1} of the last step are formed by finish_norm.
+double* norm_buf(alps_ed_lanczos lz, int j) { return (j & 1) ? lz->red.c() : lz->red.b(); }
+
 template <class V>
 void lanczos_step_dev(alps_ed_lanczos lz, int j) {
   const DevLayout& L = lz->op->dev;
@@ -191,17 +201,19 @@ void lanczos_step_dev(alps_ed_lanczos lz, int j) {
   double* alpha = lz->alpha.as<double>();
   double* beta = lz->beta.as<double>();
   double* scale = lz->scale.as<double>();
-  // w = H u_j; alpha_j = scale_j^2 <u_j, w>
   hip_check(launch_hv<V>(L, cur, w, lz->red.a(), lz->stream), "hv");
-  hip_check(launch_finish(Partials{lz->red.a(), hv_blocks(L)}, scale + j, alpha + j, nullptr, kAlpha, lz->stream),
-            "finish");
-  // u_{j+1} = scale_j w - alpha_j v_j - beta_{j-1} v_{j-1}
-  hip_check(launch_update<V>(L.D, w, cur, prev, next, alpha + j, scale + j, j > 0 ? beta + j - 1 : nullptr,
-                             j > 0 ? scale + j - 1 : nullptr, lz->red.b(), lz->stream),
+  hip_check(launch_update_fused<V>(L.D, w, cur, prev, next, Partials{lz->red.a(), hv_blocks(L)},
+                                   Partials{norm_buf(lz, j), lz->norm_count[j & 1]}, alpha + j,
+                                   j > 0 ? beta + j - 1 : nullptr, scale + j, j > 0 ? scale + j - 1 : nullptr,
+                                   norm_buf(lz, j + 1), lz->stream),
             "update");
-  // beta_j = |u_{j+1}|, scale_{j+1} = 1/beta_j
-  hip_check(launch_finish(Partials{lz->red.b(), stream_blocks(L.D)}, nullptr, beta + j, scale + j + 1, kNorm,
-                          lz->stream),
+  lz->norm_count[(j + 1) & 1] = fused_blocks(L.D);
+}
+
+// beta_{n-1} and scale_n after n steps (the host reads beta).
+void finish_norm(alps_ed_lanczos lz, int n) {
+  hip_check(launch_finish(Partials{norm_buf(lz, n), lz->norm_count[n & 1]}, nullptr, lz->beta.as<double>() + n - 1,
+                          lz->scale.as<double>() + n, kNorm, lz->stream),
             "finish");
 }
 
@@ -587,11 +599,13 @@ int alps_ed_lanczos_start(alps_ed_lanczos lz, uint64_t seed, alps_ed_vector star
         hip_check(launch_fill_random<V>(n, u0, seed, -1.0, 1.0, lz->stream), "fill_random");
       }
       // scale[0] = 1/|u0|; |u0| goes to a spare slot of beta
-      hip_check(launch_dot<V>(n, u0, u0, lz->red.b(), lz->stream), "norm");
-      hip_check(launch_finish(Partials{lz->red.b(), stream_blocks(n)}, nullptr, lz->beta.as<double>() + lz->max_steps + 1,
-                              lz->scale.as<double>(), kNorm, lz->stream),
+      hip_check(launch_dot<V>(n, u0, u0, norm_buf(lz, 0), lz->stream), "norm");
+      lz->norm_count[0] = stream_blocks(n);
+      hip_check(launch_finish(Partials{norm_buf(lz, 0), lz->norm_count[0]}, nullptr,
+                              lz->beta.as<double>() + lz->max_steps + 1, lz->scale.as<double>(), kNorm, lz->stream),
                 "finish");
       lanczos_step_dev<V>(lz, 0);
+      finish_norm(lz, 1);
     });
     lz->steps = 1;
   });
@@ -606,6 +620,7 @@ int alps_ed_lanczos_steps(alps_ed_lanczos lz, int nsteps) {
     with_dtype(lz->dtype, [&](auto z) {
       using V = decltype(z);
       for (int s = 0; s < nsteps; ++s) lanczos_step_dev<V>(lz, lz->steps + s);
+      finish_norm(lz, lz->steps + nsteps);
     });
     lz->steps += nsteps;
   });
@@ -812,9 +827,9 @@ extern "C" int alps_ed_sparsediag(const alps_ed_model* model, const alps_ed_solv
       }
       int got = 0;
       check_status(alps_ed_lanczos_coefficients(lz.get(), a.data(), b.data(), n, &got));
-      if (!params->fixed_iterations && n >= 2 && nlowest_converged(analyze_prefix(a, b, n, fp32), nev)) {
+      if (!params->fixed_iterations && n >= 2 && nlowest_converged(analyze_prefix(a, b, n, fp32), nev, fp32)) {
         for (int m = std::max(2, checked + 1); m <= n; ++m)
-          if (m == n || nlowest_converged(analyze_prefix(a, b, m, fp32), nev)) {
+          if (m == n || nlowest_converged(analyze_prefix(a, b, m, fp32), nev, fp32)) {
             final_n = m;
             break;
           }

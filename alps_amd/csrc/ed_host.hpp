@@ -127,10 +127,12 @@ inline ModelSpec model_spec(const alps_ed_model* m) {
 
 // The lowest distinct non-ghost eigenvalues of T[0..m) with their error
 // bounds and eigenvector positions.  With FP32-stored Lanczos vectors the
-// recurrence produces near-copies of converged eigenvalues that differ by
+// recurrence produces near-copies of eigenvalues that differ by
 // O(eps_f32 |T|), above the FP64 merge threshold; they are merged here at
-// 64 eps_f32 |lambda| (a merged group has multiplicity > 1, error 0, as in
-// tmatrix.h:164-183).
+// 64 eps_f32 |lambda|.  Unlike the FP64 rule (tmatrix.h:164-183, where an
+// exact copy means convergence and the error is 0), FP32 near-copies also
+// appear before convergence, so a merged group keeps the smallest error
+// bound of its members and the position of its lowest member.
 struct TValues {
   std::vector<double> ev, err;
   std::vector<int> index;
@@ -149,7 +151,7 @@ inline TValues analyze_prefix(const std::vector<double>& a, const std::vector<do
   for (size_t i = 0; i < an.eigval_noghost.size(); ++i) {
     const double lam = an.eigval_noghost[i];
     if (fp32 && !t.ev.empty() && lam - t.ev.back() <= dup * std::max(1.0, std::fabs(t.ev.back()))) {
-      t.err.back() = 0.0;
+      t.err.back() = std::min(t.err.back(), an.err_noghost[i]);
       continue;
     }
     t.ev.push_back(lam);
@@ -159,9 +161,19 @@ inline TValues analyze_prefix(const std::vector<double>& a, const std::vector<do
   return t;
 }
 
-// lanczos_iteration_nlowest::converged (iteration.h:116-130), rtol = atol = 100 eps.
-inline bool nlowest_converged(const TValues& t, int nev) {
-  const double tol = 100. * std::numeric_limits<double>::epsilon();
+// Stopping tolerance (rtol = atol) of the nlowest rule: 100 eps, as
+// lanczos_iteration_nlowest (iteration.h:110-113), for FP64 storage.  With
+// FP32 storage the T-matrix error bounds stall near the FP32 noise level
+// long before 100 eps; the solve stops at 1e-8 and reports FP64 Rayleigh
+// quotients of the Ritz vectors, whose error is about the square of the
+// residual (1e-14 and below on the tested chains and lattices).
+inline double stop_tolerance(bool fp32) {
+  return fp32 ? 1e-8 : 100. * std::numeric_limits<double>::epsilon();
+}
+
+// lanczos_iteration_nlowest::converged (iteration.h:116-130).
+inline bool nlowest_converged(const TValues& t, int nev, bool fp32 = false) {
+  const double tol = stop_tolerance(fp32);
   if ((int)t.ev.size() < nev) return false;
   for (int i = 0; i < nev; ++i)
     if (t.err[i] > std::max(tol, tol * std::fabs(t.ev[i]))) return false;

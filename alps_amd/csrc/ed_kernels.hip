@@ -68,6 +68,42 @@ __global__ __launch_bounds__(kBlock) void update_kernel(uint64_t n, const V* w, 
   store_partial(norm_out, acc);
 }
 
+// One Lanczos update with its scalars: every block first forms, in the same
+// order, scale_j = 1/sqrt(sum(norm_prev)) (the norm of u_j), beta_{j-1} =
+// sqrt(sum(norm_prev)) when j > 0, and alpha_j = scale_j^2 * sum(hv_part);
+// block 0 stores them.  Then as update_kernel.  This replaces the two
+// one-block reduction launches of a step.
+template <class V>
+__global__ __launch_bounds__(kBlock) void update_fused_kernel(uint64_t n, const V* w, const V* __restrict__ u_cur,
+                                                              const V* u_prev, V* out, Partials hv_part,
+                                                              Partials norm_prev, double* __restrict__ alpha_j,
+                                                              double* __restrict__ beta_jm1,
+                                                              double* __restrict__ scale_j,
+                                                              const double* __restrict__ scale_jm1,
+                                                              double* __restrict__ norm_out) {
+  const double nrm = sqrt(sum_partials(norm_prev));
+  const double sc = 1.0 / nrm;
+  const double a = sc * sc * sum_partials(hv_part);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *alpha_j = a;
+    *scale_j = sc;
+    if (beta_jm1) *beta_jm1 = nrm;
+  }
+  const double ca = a * sc;
+  const double cb = beta_jm1 ? nrm * (*scale_jm1) : 0.0;
+  double acc = 0.0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double r = sc * (double)w[i] - ca * (double)u_cur[i];
+    if (beta_jm1) r -= cb * (double)u_prev[i];
+    const V rs = (V)r;
+    out[i] = rs;
+    const double rd = (double)rs;
+    acc += rd * rd;
+  }
+  store_partial(norm_out, acc);
+}
+
 template <class V>
 __global__ __launch_bounds__(kBlock) void dot_kernel(uint64_t n, const V* __restrict__ x,
                                                      const V* __restrict__ y, double* partials) {
@@ -276,6 +312,15 @@ hipError_t launch_update(uint64_t n, const V* w, const V* u_cur, const V* u_prev
 }
 
 template <class V>
+hipError_t launch_update_fused(uint64_t n, const V* w, const V* u_cur, const V* u_prev, V* out, Partials hv_part,
+                               Partials norm_prev, double* alpha_j, double* beta_jm1, double* scale_j,
+                               const double* scale_jm1, double* norm_out, hipStream_t st) {
+  hipLaunchKernelGGL((update_fused_kernel<V>), dim3(fused_blocks(n)), dim3(kBlock), 0, st, n, w, u_cur, u_prev, out,
+                     hv_part, norm_prev, alpha_j, beta_jm1, scale_j, scale_jm1, norm_out);
+  return hipGetLastError();
+}
+
+template <class V>
 hipError_t launch_dot(uint64_t n, const V* x, const V* y, double* partials, hipStream_t st) {
   hipLaunchKernelGGL((dot_kernel<V>), dim3(stream_blocks(n)), dim3(kBlock), 0, st, n, x, y, partials);
   return hipGetLastError();
@@ -358,6 +403,8 @@ hipError_t launch_elements(const DevLayout& L, uint64_t row0, uint64_t nrows, in
   template hipError_t launch_update<V>(uint64_t, const V*, const V*, const V*, V*, const double*,           \
                                        const double*, const double*, const double*, double*, hipStream_t); \
   template hipError_t launch_dot<V>(uint64_t, const V*, const V*, double*, hipStream_t);                    \
+  template hipError_t launch_update_fused<V>(uint64_t, const V*, const V*, const V*, V*, Partials, Partials,   \
+                                             double*, double*, double*, const double*, double*, hipStream_t);  \
   template hipError_t launch_normalize<V>(uint64_t, V*, Partials, double*, hipStream_t);                    \
   template hipError_t launch_fill_random<V>(uint64_t, V*, uint64_t, double, double, hipStream_t);           \
   template hipError_t launch_fill<V>(uint64_t, V*, double, hipStream_t);                                    \

@@ -70,6 +70,10 @@ inline unsigned stream_blocks(uint64_t n) {
   return (unsigned)b;
 }
 
+// Blocks of the fused update: every block also reads all partials, so the
+// grid is kept smaller than the streaming kernels'.
+inline unsigned fused_blocks(uint64_t n) { return stream_blocks(n) < 512 ? stream_blocks(n) : 512; }
+
 // Blocks of the H.v kernel: one per segment.
 inline unsigned hv_blocks(const DevLayout& L) { return L.n_seg ? L.n_seg : 1; }
 
@@ -92,6 +96,17 @@ template <class V>
 hipError_t launch_update(uint64_t n, const V* w, const V* u_cur, const V* u_prev, V* out, const double* alpha,
                          const double* s_cur, const double* beta_prev, const double* s_prev, double* norm_out,
                          hipStream_t st);
+
+// The update with its scalars (one launch instead of three): scale_j =
+// 1/sqrt(sum(norm_prev)), beta_{j-1} = sqrt(sum(norm_prev)) (when beta_jm1 is
+// set; it is null at j = 0), alpha_j = scale_j^2 * sum(hv_part); then
+// r = scale_j w - alpha_j scale_j u_cur - beta_{j-1} scale_{j-1} u_prev into
+// `out` with per-block partials of |r|^2 into norm_out (fused_blocks(n) of
+// them).  The scalars are bit-identical to launch_finish + launch_update.
+template <class V>
+hipError_t launch_update_fused(uint64_t n, const V* w, const V* u_cur, const V* u_prev, V* out, Partials hv_part,
+                               Partials norm_prev, double* alpha_j, double* beta_jm1, double* scale_j,
+                               const double* scale_jm1, double* norm_out, hipStream_t st);
 
 // per-block partials of sum x_i y_i
 template <class V>

@@ -37,6 +37,7 @@
 namespace alps_ed {
 
 constexpr int kMaxSites = 32;
+constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
 
 inline uint64_t binom(int n, int k) {
   if (k < 0 || n < 0 || k > n) return 0;
@@ -203,6 +204,22 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1) {
   std::stable_sort(L.proc_hi.begin(), L.proc_hi.end(), [&](uint32_t a, uint32_t b) {
     return binom(B, L.k - __builtin_popcount(a)) > binom(B, L.k - __builtin_popcount(b));
   });
+  // Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD
+  // b % 8), each XCD with its own L2.  When the segments split evenly by the
+  // top three bits of the high word, block b takes its segment from group
+  // b % 8: most high-high neighbours (hi ^ mask with the top bits unchanged)
+  // of a segment are then read on the XCD that owns them, from its L2.
+  if (L.NH >= 3) {
+    std::vector<std::vector<uint32_t>> grp(kXcds);
+    for (uint32_t h : L.proc_hi) grp[h >> (L.NH - 3)].push_back(h);
+    bool even = true;
+    for (const auto& g : grp) even = even && g.size() == grp[0].size();
+    if (even && !grp[0].empty()) {
+      L.proc_hi.clear();
+      for (size_t i = 0; i < grp[0].size(); ++i)
+        for (int g = 0; g < kXcds; ++g) L.proc_hi.push_back(grp[g][i]);
+    }
+  }
   // terms in bit coordinates
   auto pos = [N](int s) { return (uint32_t)(N - 1 - s); };
   L.diag_lo.assign(nlo, 0.0);

@@ -179,9 +179,12 @@ int alps_ed_tmatrix_analyze(int n, const double* alpha, const double* beta, doub
  * `nev` distinct non-ghost eigenvalues.  max_iter <= 0 selects the
  * reference default min(10*D, 1000); D == 1 returns H(0,0).
  * With FP64 storage the eigenvalues are those of T, as in the reference.
- * With FP32 storage the basis is kept in HBM, near-copies of converged
- * eigenvalues (within 64 eps_f32 |lambda|) are merged, and each reported
- * eigenvalue is the FP64-accumulated Rayleigh quotient of its Ritz vector.
+ * With FP32 storage the basis is kept in HBM, near-copies of eigenvalues
+ * (within 64 eps_f32 |lambda|) are merged keeping their smallest error bound,
+ * the stopping tolerance is 1e-8 (the FP32 T-matrix bounds stall above
+ * 100 eps), and each reported eigenvalue is the FP64-accumulated Rayleigh
+ * quotient of its Ritz vector, whose error is about the square of the
+ * residual.
  * Sectors with D < 1000 are stored in FP64 whatever `dtype` asks: there the
  * Krylov space the solve builds is a large part of the sector and FP32
  * storage would cost accuracy for no measurable saving. */
