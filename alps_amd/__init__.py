@@ -130,7 +130,9 @@ class _BatchConfig(ctypes.Structure):
 class _BatchStatus(ctypes.Structure):
     _fields_ = [("n_active", ctypes.c_int), ("n_pending", ctypes.c_int), ("hbm_used", ctypes.c_uint64),
                 ("hbm_bytes", ctypes.c_uint64), ("rounds", ctypes.c_uint64), ("steps", ctypes.c_uint64),
-                ("solves_done", ctypes.c_uint64)]
+                ("solves_done", ctypes.c_uint64), ("seconds_admit", ctypes.c_double),
+                ("seconds_steps", ctypes.c_double), ("seconds_decide", ctypes.c_double),
+                ("seconds_refine", ctypes.c_double)]
 
 
 def _result_dict(r):

@@ -203,6 +203,7 @@ struct Problem {
   size_t next_cand = 0;     // next candidate of tv to refine
   std::vector<int> cand;    // tv positions in the current refinement pass
   double t_submit = 0, t_admit = 0, t_done = 0;
+  bool sized = false;  // layout built
   alps_ed_solve_result result{};
 };
 
@@ -222,18 +223,22 @@ struct alps_ed_batch_s {
   std::vector<uint64_t> completed;
   uint64_t next_ticket = 1;
   uint64_t rounds = 0, steps = 0, solves_done = 0;
+  double t_admit = 0, t_steps = 0, t_decide = 0, t_refine = 0;
   // device work lists (rebuilt every round)
   DevBuf jobs_d, items_d, list_d;
   size_t jobs_cap = 0, items_cap = 0, list_cap = 0;
-  // pinned staging for the work lists
+  // pinned staging for the work lists and the tables of new problems
   void* stage = nullptr;
   size_t stage_cap = 0;
+  void* tstage = nullptr;
+  size_t tstage_cap = 0;
 
   ~alps_ed_batch_s() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (dev_base) (void)hipFree(dev_base);
     if (host_base) (void)hipHostFree(host_base);
     if (stage) (void)hipHostFree(stage);
+    if (tstage) (void)hipHostFree(tstage);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -274,8 +279,10 @@ size_t table_bytes(const SectorLayout& L) {
   return s;
 }
 
-// Copies the tables into [base, ...) and returns the device layout.
-DevLayout place_tables(const SectorLayout& L, char* base, hipStream_t st) {
+// Lays the tables out at [base, base + table_bytes) of the device, writing
+// their bytes to the same offsets of the host staging area `stage`, and
+// returns the device layout; one copy of the staging area uploads them.
+DevLayout place_tables(const SectorLayout& L, char* base, char* stage) {
   DevLayout d{};
   d.N = L.N;
   d.k = L.k;
@@ -288,14 +295,13 @@ DevLayout place_tables(const SectorLayout& L, char* base, hipStream_t st) {
   d.n_hh_p = L.n_hh_p;
   d.n_xb_p = L.n_xb_p;
   d.n_bonds = (int)L.bond_p1.size();
-  char* p = base;
+  size_t o = 0;
   auto put = [&](const auto& v) {
     using T = typename std::decay_t<decltype(v)>::value_type;
-    T* dst = reinterpret_cast<T*>(p);
-    if (!v.empty())
-      hip_check(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st), "upload tables");
-    p += align_up(std::max<size_t>(v.size(), 1) * sizeof(T));
-    return static_cast<const T*>(dst);
+    if (!v.empty()) std::memcpy(stage + o, v.data(), v.size() * sizeof(T));
+    const T* dst = reinterpret_cast<const T*>(base + o);
+    o += align_up(std::max<size_t>(v.size(), 1) * sizeof(T));
+    return dst;
   };
   d.lo_rank = put(L.lo_rank);
   d.lo_state = put(L.lo_state);
@@ -368,12 +374,13 @@ T* dev_ptr(alps_ed_batch b, size_t off) {
   return reinterpret_cast<T*>(static_cast<char*>(b->dev_base) + off);
 }
 
-// Places an admitted problem in its regions and uploads its tables.
-void place(alps_ed_batch b, Problem& p) {
+// Places an admitted problem in its regions; its tables are staged at
+// `stage` (table_bytes of them) for the caller to upload.
+void place(alps_ed_batch b, Problem& p, char* stage) {
   char* base = static_cast<char*>(b->dev_base) + p.dev_off;
   BatchJob& J = p.job;
   J = BatchJob{};
-  J.L = place_tables(p.layout, base, b->stream);
+  J.L = place_tables(p.layout, base, stage);
   char* q = base + table_bytes(p.layout);
   const size_t es = p.fp32 ? 4 : 8;
   const uint64_t stride = (p.D + 63) & ~uint64_t(63);
@@ -580,10 +587,42 @@ Group upload_group(alps_ed_batch b, const std::vector<Problem*>& run) {
   return g;
 }
 
+// Per-problem sizes that follow from its layout.
+void size_problem(alps_ed_batch b, Problem& p) {
+  p.layout = build_layout(p.spec);
+  p.D = p.layout.D;
+  p.fp32 = b->cfg.dtype == ALPS_ED_F32 && p.D >= kFp32MinDim;
+  if (p.max_iter <= 0) p.max_iter = (int)std::min<uint64_t>(10 * std::max<uint64_t>(p.D, 1), 1000);
+  p.n_chunks = (uint32_t)std::max<uint64_t>(1, (p.D + kBatchChunk - 1) / kBatchChunk);
+  p.cap = p.fp32 ? (uint32_t)(p.max_iter + 2) : 3u;
+  p.nvr = p.fp32 ? std::min(p.nev, kMaxRitz) : 0;
+  p.sized = true;
+}
+
+void ensure_tstage(alps_ed_batch b, size_t bytes) {
+  if (bytes <= b->tstage_cap) return;
+  if (b->tstage) hip_check(hipHostFree(b->tstage), "hipHostFree");
+  b->tstage = nullptr;
+  const size_t cap = std::max<size_t>(bytes, 2 * b->tstage_cap);
+  hip_check(hipHostMalloc(&b->tstage, cap, hipHostMallocDefault), "hipHostMalloc");
+  b->tstage_cap = cap;
+}
+
 void admit(alps_ed_batch b, std::vector<Problem*>& started) {
+  {
+    // layouts of the requests at the head of the queue, built in parallel
+    std::vector<Problem*> todo;
+    for (auto& p : b->pending) {
+      if (todo.size() >= 4096) break;
+      if (!p->sized) todo.push_back(p.get());
+    }
+    b->pool->parallel_for(todo.size(), [&](size_t i) { size_problem(b, *todo[i]); });
+  }
+  std::vector<Problem*> placed;
   while (!b->pending.empty()) {
     if (b->cfg.max_active > 0 && (int)b->active.size() >= b->cfg.max_active) break;
     Problem& p = *b->pending.front();
+    if (!p.sized) size_problem(b, p);
     const Footprint f = footprint(p);
     size_t doff = 0, hoff = 0;
     if (f.dev > b->arena.size() || f.host > b->harena.size()) {
@@ -608,11 +647,26 @@ void admit(alps_ed_batch b, std::vector<Problem*>& started) {
     q->host_off = hoff;
     q->host_bytes = f.host;
     q->t_admit = now_s();
-    place(b, *q);
     q->state = kRunning;
+    placed.push_back(q.get());
     started.push_back(q.get());
     b->active.push_back(std::move(q));
   }
+  if (placed.empty()) return;
+  // stage every new problem's tables in pinned memory, one copy per problem
+  size_t total = 0;
+  std::vector<size_t> at;
+  for (Problem* p : placed) {
+    at.push_back(total);
+    total += align_up(table_bytes(p->layout));
+  }
+  ensure_tstage(b, total);
+  char* st = static_cast<char*>(b->tstage);
+  b->pool->parallel_for(placed.size(), [&](size_t i) { place(b, *placed[i], st + at[i]); });
+  for (size_t i = 0; i < placed.size(); ++i)
+    hip_check(hipMemcpyAsync(static_cast<char*>(b->dev_base) + placed[i]->dev_off, st + at[i],
+                             table_bytes(placed[i]->layout), hipMemcpyHostToDevice, b->stream),
+              "upload tables");
 }
 
 void retire(alps_ed_batch b) {
@@ -732,6 +786,7 @@ void refine(alps_ed_batch b) {
 void round(alps_ed_batch b) {
   ++b->rounds;
   std::vector<Problem*> started;
+  double t0 = now_s();
   admit(b, started);
   // D = 0 and D = 1 sectors need no Lanczos (sparsediag.h:101-102,120-124)
   for (Problem* p : started) {
@@ -746,6 +801,8 @@ void round(alps_ed_batch b) {
   std::vector<Problem*> run;
   for (auto& p : b->active)
     if (p->state == kRunning) run.push_back(p.get());
+  double t1 = now_s();
+  b->t_admit += t1 - t0;
   if (!run.empty()) {
     for (Problem* p : run) p->job.j0 = p->j;
     Group g = upload_group(b, run);
@@ -788,6 +845,8 @@ void round(alps_ed_batch b) {
       hip_check(launch_batch_finish(g.jobs, g.list, g.n_list, t, kBatchBeta, b->stream), "batch finish");
     }
     hip_check(hipStreamSynchronize(b->stream), "sync");
+    const double t2 = now_s();
+    b->t_steps += t2 - t1;
     b->steps += S;
     for (Problem* p : run) p->j = std::min(p->max_iter, p->j + S);
     b->pool->parallel_for(run.size(), [&](size_t i) {
@@ -795,26 +854,26 @@ void round(alps_ed_batch b) {
       decide(p);
       if (p.final_n >= 0) conclude(p);
     });
+    t1 = now_s();
+    b->t_decide += t1 - t2;
   }
   refine(b);
+  b->t_refine += now_s() - t1;
   retire(b);
+  hip_check(hipStreamSynchronize(b->stream), "sync");  // staging buffers are reused next round
 }
 
 uint64_t submit_one(alps_ed_batch b, const alps_ed_model* m, const alps_ed_solve_params* prm) {
   require(prm != nullptr, "params is null");
   std::unique_ptr<Problem> p(new Problem);
   p->spec = model_spec(m);
-  p->layout = build_layout(p->spec);
-  p->D = p->layout.D;
-  p->fp32 = b->cfg.dtype == ALPS_ED_F32 && p->D >= kFp32MinDim;
-  p->max_iter = prm->max_iter > 0 ? prm->max_iter : (int)std::min<uint64_t>(10 * std::max<uint64_t>(p->D, 1), 1000);
+  if (p->spec.n_up >= 0 && p->spec.n_up <= p->spec.n_sites)
+    require(binom(p->spec.n_sites, p->spec.n_up) < 0xffffffffULL, "sector dimension exceeds 2^32-1");
+  p->max_iter = prm->max_iter > 0 ? prm->max_iter : 0;  // default set when sized
   p->nev = prm->nev > 0 ? prm->nev : 1;
   require(p->nev <= 64, "nev must be <= 64");
   p->fixed = prm->fixed_iterations != 0;
   p->seed = prm->seed;
-  p->n_chunks = (uint32_t)std::max<uint64_t>(1, (p->D + kBatchChunk - 1) / kBatchChunk);
-  p->cap = p->fp32 ? (uint32_t)(p->max_iter + 2) : 3u;
-  p->nvr = p->fp32 ? std::min(p->nev, kMaxRitz) : 0;
   p->ticket = b->next_ticket++;
   p->t_submit = now_s();
   const uint64_t t = p->ticket;
@@ -920,6 +979,10 @@ int alps_ed_batch_stats(alps_ed_batch b, alps_ed_batch_status* s) {
     s->rounds = b->rounds;
     s->steps = b->steps;
     s->solves_done = b->solves_done;
+    s->seconds_admit = b->t_admit;
+    s->seconds_steps = b->t_steps;
+    s->seconds_decide = b->t_decide;
+    s->seconds_refine = b->t_refine;
   });
 }
 

@@ -30,6 +30,9 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -39,11 +42,19 @@ namespace alps_ed {
 constexpr int kMaxSites = 32;
 constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
 
+// Binomial coefficients C(n, k) for 0 <= n <= 64 (Pascal's triangle, built
+// once); 0 outside 0 <= k <= n.
 inline uint64_t binom(int n, int k) {
-  if (k < 0 || n < 0 || k > n) return 0;
-  uint64_t r = 1;
-  for (int i = 1; i <= k; ++i) r = r * (uint64_t)(n - k + i) / (uint64_t)i;
-  return r;
+  struct Table {
+    uint64_t c[65][65];
+    Table() {
+      for (int i = 0; i <= 64; ++i)
+        for (int j = 0; j <= 64; ++j) c[i][j] = j == 0 ? 1 : (i == 0 ? 0 : c[i - 1][j - 1] + c[i - 1][j]);
+    }
+  };
+  static const Table t;
+  if (k < 0 || n < 0 || k > n || n > 64) return 0;
+  return t.c[n][k];
 }
 
 struct BondTerm {
@@ -148,6 +159,99 @@ inline int choose_low_bits(int N) {
   return N < 12 ? N : 12;
 }
 
+// Tables of the B-bit low word, the same for every sector with that B:
+// colex rank of each word within its popcount class, the words of each class
+// in ascending order, and the class offsets.  Built once per B.
+struct LowTables {
+  std::vector<uint32_t> lo_rank, lo_state, lo_class_off;
+};
+
+inline const LowTables& low_tables(int B) {
+  static std::mutex mu;
+  static std::map<int, std::unique_ptr<LowTables>> cache;
+  std::lock_guard<std::mutex> g(mu);
+  std::unique_ptr<LowTables>& slot = cache[B];
+  if (!slot) {
+    slot.reset(new LowTables);
+    LowTables& T = *slot;
+    const uint32_t nlo = 1u << B;
+    T.lo_rank.assign(nlo, 0);
+    T.lo_class_off.assign(B + 2, 0);
+    for (uint32_t v = 0; v < nlo; ++v) {
+      uint64_t r = 0;
+      int m_idx = 0;
+      for (int p = 0; p < B; ++p)
+        if (v >> p & 1u) r += binom(p, ++m_idx);
+      T.lo_rank[v] = (uint32_t)r;
+      T.lo_class_off[__builtin_popcount(v) + 1]++;
+    }
+    for (int c = 0; c <= B; ++c) T.lo_class_off[c + 1] += T.lo_class_off[c];
+    T.lo_state.assign(nlo, 0);
+    for (uint32_t v = 0; v < nlo; ++v) T.lo_state[T.lo_class_off[__builtin_popcount(v)] + T.lo_rank[v]] = v;
+  }
+  return *slot;
+}
+
+// Per-class flip tables of a set of low-word masks (low-low bonds, padded
+// to n_pad with null bonds) and of cross-bond low bits, cached by their key:
+// many sectors of one batch share the lattice and so these tables.
+struct FlipTables {
+  std::vector<uint32_t> nbp_off, xn_off;
+  std::vector<uint16_t> nbp16, xn16;
+};
+
+inline std::shared_ptr<const FlipTables> flip_tables(int B, const std::vector<uint32_t>& ll_masks, int n_ll_p,
+                                                     const std::vector<uint32_t>& x_lbits, int n_xb_p) {
+  std::vector<uint32_t> key{(uint32_t)B, (uint32_t)n_ll_p, (uint32_t)n_xb_p, (uint32_t)ll_masks.size()};
+  key.insert(key.end(), ll_masks.begin(), ll_masks.end());
+  key.insert(key.end(), x_lbits.begin(), x_lbits.end());
+  static std::mutex mu;
+  static std::map<std::vector<uint32_t>, std::shared_ptr<const FlipTables>> cache;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
+  const LowTables& T = low_tables(B);
+  std::shared_ptr<FlipTables> F(new FlipTables);
+  F->nbp_off.assign(B + 2, 0);
+  F->xn_off.assign(B + 2, 0);
+  for (int c = 0; c <= B; ++c) {
+    F->nbp_off[c + 1] = F->nbp_off[c] + (uint32_t)(binom(B, c) * n_ll_p);
+    F->xn_off[c + 1] = F->xn_off[c] + (uint32_t)(binom(B, c) * n_xb_p);
+  }
+  F->nbp16.assign(F->nbp_off[B + 1], 0);
+  F->xn16.assign(F->xn_off[B + 1], 0);
+  for (int c = 0; c <= B; ++c) {
+    const uint32_t len = (uint32_t)binom(B, c);
+    const uint32_t* ls = T.lo_state.data() + T.lo_class_off[c];
+    for (int b = 0; b < n_ll_p; ++b) {
+      uint16_t* out = F->nbp16.data() + F->nbp_off[c] + (size_t)b * len;
+      if (b >= (int)ll_masks.size()) {
+        for (uint32_t r = 0; r < len; ++r) out[r] = (uint16_t)len;
+        continue;
+      }
+      const uint32_t m = ll_masks[b];
+      for (uint32_t r = 0; r < len; ++r) {
+        const uint32_t lo = ls[r], bits = lo & m;
+        out[r] = (uint16_t)((bits != 0u && bits != m) ? T.lo_rank[lo ^ m] : len);
+      }
+    }
+    for (int b = 0; b < (int)x_lbits.size(); ++b) {
+      uint16_t* out = F->xn16.data() + F->xn_off[c] + (size_t)b * len;
+      const uint32_t lb = x_lbits[b];
+      for (uint32_t r = 0; r < len; ++r) {
+        const uint32_t lo = ls[r];
+        out[r] = (uint16_t)(T.lo_rank[lo ^ lb] | (((lo & lb) != 0u ? 1u : 0u) << 15));
+      }
+    }
+  }
+  std::lock_guard<std::mutex> g(mu);
+  if (cache.size() > 256) cache.clear();  // bounded: a batch uses a few lattices
+  cache[key] = F;
+  return F;
+}
+
 inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1) {
   SectorLayout L;
   const int N = m.n_sites;
@@ -167,22 +271,12 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1) {
   L.NH = N - B;
   const uint32_t nlo = 1u << B;
   const uint32_t nhi = 1u << L.NH;
-  // low tables
-  L.lo_rank.assign(nlo, 0);
-  L.lo_class_off.assign(B + 2, 0);
-  for (uint32_t v = 0; v < nlo; ++v) {
-    uint64_t r = 0;
-    int m_idx = 0;
-    for (int p = 0; p < B; ++p)
-      if (v >> p & 1u) r += binom(p, ++m_idx);
-    L.lo_rank[v] = (uint32_t)r;
-    L.lo_class_off[__builtin_popcount(v) + 1]++;
-  }
-  for (int c = 0; c <= B; ++c) L.lo_class_off[c + 1] += L.lo_class_off[c];
-  L.lo_state.assign(nlo, 0);
-  for (uint32_t v = 0; v < nlo; ++v) {
-    int c = __builtin_popcount(v);
-    L.lo_state[L.lo_class_off[c] + L.lo_rank[v]] = v;
+  // low tables (depend on B only; shared)
+  {
+    const LowTables& T = low_tables(B);
+    L.lo_rank = T.lo_rank;
+    L.lo_state = T.lo_state;
+    L.lo_class_off = T.lo_class_off;
   }
   // high tables and segments
   L.hi_base.assign(nhi, 0xffffffffu);
@@ -201,8 +295,11 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1) {
   L.proc_hi = L.seg_hi;
   for (uint32_t h : L.seg_hi) L.max_seg = std::max<uint32_t>(L.max_seg, (uint32_t)binom(B, L.k - __builtin_popcount(h)));
   // longest segments first, so that short ones fill the tail of the launch
+  // (popcount(hi) picks the class; a longer class has kl nearer B/2)
   std::stable_sort(L.proc_hi.begin(), L.proc_hi.end(), [&](uint32_t a, uint32_t b) {
-    return binom(B, L.k - __builtin_popcount(a)) > binom(B, L.k - __builtin_popcount(b));
+    const uint64_t la = L.lo_class_off[L.k - __builtin_popcount(a) + 1] - L.lo_class_off[L.k - __builtin_popcount(a)];
+    const uint64_t lb = L.lo_class_off[L.k - __builtin_popcount(b) + 1] - L.lo_class_off[L.k - __builtin_popcount(b)];
+    return la > lb;
   });
   // Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD
   // b % 8), each XCD with its own L2.  When the segments split evenly by the
@@ -263,32 +360,15 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1) {
   L.n_ll_p = round_up((int)L.ll.size(), kChunkLL);
   L.n_hh_p = round_up((int)L.hh.size(), kChunkHH);
   L.n_xb_p = round_up((int)L.xb.size(), kChunkX);
-  L.nbp_off.assign(B + 2, 0);
-  L.xn_off.assign(B + 2, 0);
-  for (int c = 0; c <= B; ++c) {
-    L.nbp_off[c + 1] = L.nbp_off[c] + (uint32_t)(binom(B, c) * L.n_ll_p);
-    L.xn_off[c + 1] = L.xn_off[c] + (uint32_t)(binom(B, c) * L.n_xb_p);
-  }
-  L.nbp16.assign(L.nbp_off[B + 1], 0);
-  L.xn16.assign(L.xn_off[B + 1], 0);
-  for (int c = 0; c <= B; ++c) {
-    const uint32_t len = (uint32_t)binom(B, c);
-    for (int b = 0; b < L.n_ll_p; ++b)
-      for (uint32_t r = 0; r < len; ++r) {
-        uint32_t v = len;
-        if (b < (int)L.ll.size()) {
-          const uint32_t lo = L.lo_state[L.lo_class_off[c] + r];
-          const uint32_t m = L.ll[b].mask, bits = lo & m;
-          if (bits != 0u && bits != m) v = L.lo_rank[lo ^ m];
-        }
-        L.nbp16[L.nbp_off[c] + b * len + r] = (uint16_t)v;
-      }
-    for (int b = 0; b < (int)L.xb.size(); ++b)
-      for (uint32_t r = 0; r < len; ++r) {
-        const uint32_t lo = L.lo_state[L.lo_class_off[c] + r];
-        const uint32_t lb = (lo & L.xb[b].lbit) != 0u;
-        L.xn16[L.xn_off[c] + b * len + r] = (uint16_t)(L.lo_rank[lo ^ L.xb[b].lbit] | (lb << 15));
-      }
+  {
+    std::vector<uint32_t> llm, xlb;
+    for (const FlipBond& f : L.ll) llm.push_back(f.mask);
+    for (const CrossBond& x : L.xb) xlb.push_back(x.lbit);
+    std::shared_ptr<const FlipTables> F = flip_tables(B, llm, L.n_ll_p, xlb, L.n_xb_p);
+    L.nbp_off = F->nbp_off;
+    L.xn_off = F->xn_off;
+    L.nbp16 = F->nbp16;
+    L.xn16 = F->xn16;
   }
   L.ll_t.assign(L.n_ll_p, 0.0);
   for (size_t b = 0; b < L.ll.size(); ++b) L.ll_t[b] = L.ll[b].t;

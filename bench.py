@@ -153,6 +153,7 @@ def batch_leg(n, seq_sample, seed=1):
         "max_rel_err_f32_vs_f64": rel,
         "mean_iterations": round(float(np.mean([r["iterations"] for r in res])), 1),
         "rounds": st["rounds"],
+        "seconds_split": {k: round(st["seconds_" + k], 4) for k in ("admit", "steps", "decide", "refine")},
         "workload": "random-bond XXZ chains, L uniform 12..20, J_i~U[0.5,1.5], Delta~U[0,2], open/periodic 1:1, "
                     "Sz=0 (Sz=1/2 for odd L), lowest eigenvalue, FP32 vectors + FP64 refinement",
     }

@@ -240,6 +240,9 @@ typedef struct {
   int n_active, n_pending;
   uint64_t hbm_used, hbm_bytes;
   uint64_t rounds, steps, solves_done;
+  /* wall seconds spent in: admission (layouts, uploads), the Lanczos steps
+   * (launch to completion), the stopping-rule tests, refinement */
+  double seconds_admit, seconds_steps, seconds_decide, seconds_refine;
 } alps_ed_batch_status;
 
 int alps_ed_batch_create(const alps_ed_batch_config* cfg, alps_ed_batch* out);
