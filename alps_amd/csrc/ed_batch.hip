@@ -446,29 +446,31 @@ void finish_problem(alps_ed_batch b, Problem& p) {
 }
 
 // Stopping rule on the alpha/beta produced so far (see alps_ed_sparsediag):
-// the first prefix length at which lanczos_iteration_nlowest holds, with a
-// cheap test of the lowest value gating the full T-matrix analysis.
+// the first prefix length at which lanczos_iteration_nlowest holds.  Per
+// round, a cheap O(m) test of the lowest value at the current length gates
+// the full T-matrix analysis; when the full rule holds there, the lengths
+// of this round are scanned back to the first one at which it holds.
 void decide(Problem& p) {
   const int n = p.j;
   if (p.fixed) {
     if (n >= p.max_iter) p.final_n = p.max_iter;
     return;
   }
-  std::vector<double> a(p.host_alpha, p.host_alpha + n), bb(p.host_beta, p.host_beta + n);
-  const double tol = stop_tolerance(p.fp32);
-  for (int m = std::max(2, p.scan_from); m <= n; ++m) {
-    double lam = 0.0;
-    const double e = lowest_ritz_error(a.data(), bb.data(), m, &lam);
-    if (!(e <= 4.0 * std::max(tol, tol * std::fabs(lam)))) continue;
-    if (!nlowest_converged(analyze_prefix(a, bb, m, p.fp32), p.nev, p.fp32)) continue;
-    // the full test holds at m; step back while it holds earlier too
-    int first = m;
-    while (first - 1 >= std::max(2, p.scan_from) && nlowest_converged(analyze_prefix(a, bb, first - 1, p.fp32), p.nev, p.fp32))
-      --first;
-    p.final_n = first;
-    return;
-  }
+  const int lo = std::max(2, p.scan_from);
   p.scan_from = std::max(p.scan_from, n + 1);
+  if (n >= lo) {
+    std::vector<double> a(p.host_alpha, p.host_alpha + n), bb(p.host_beta, p.host_beta + n);
+    const double tol = stop_tolerance(p.fp32);
+    double lam = 0.0;
+    const double e = lowest_ritz_error(a.data(), bb.data(), n, &lam);
+    if (e <= 4.0 * std::max(tol, tol * std::fabs(lam)) &&
+        nlowest_converged(analyze_prefix(a, bb, n, p.fp32), p.nev, p.fp32)) {
+      int first = n;
+      while (first - 1 >= lo && nlowest_converged(analyze_prefix(a, bb, first - 1, p.fp32), p.nev, p.fp32)) --first;
+      p.final_n = first;
+      return;
+    }
+  }
   if (n >= p.max_iter) {
     p.final_n = n;
     p.error_code = 1;  // iteration.h:80
@@ -591,7 +593,7 @@ Group upload_group(alps_ed_batch b, const std::vector<Problem*>& run) {
 void size_problem(alps_ed_batch b, Problem& p) {
   p.layout = build_layout(p.spec);
   p.D = p.layout.D;
-  p.fp32 = b->cfg.dtype == ALPS_ED_F32 && p.D >= kFp32MinDim;
+  p.fp32 = use_fp32(b->cfg.dtype == ALPS_ED_F32, p.D, p.nev);
   if (p.max_iter <= 0) p.max_iter = (int)std::min<uint64_t>(10 * std::max<uint64_t>(p.D, 1), 1000);
   p.n_chunks = (uint32_t)std::max<uint64_t>(1, (p.D + kBatchChunk - 1) / kBatchChunk);
   p.cap = p.fp32 ? (uint32_t)(p.max_iter + 2) : 3u;
@@ -994,9 +996,9 @@ int alps_ed_batch_footprint(const alps_ed_model* model, const alps_ed_solve_para
     p.spec = model_spec(model);
     p.layout = build_layout(p.spec);
     p.D = p.layout.D;
-    p.fp32 = dtype == ALPS_ED_F32 && p.D >= kFp32MinDim;
-    p.max_iter = params->max_iter > 0 ? params->max_iter : (int)std::min<uint64_t>(10 * std::max<uint64_t>(p.D, 1), 1000);
     p.nev = params->nev > 0 ? params->nev : 1;
+    p.fp32 = use_fp32(dtype == ALPS_ED_F32, p.D, p.nev);
+    p.max_iter = params->max_iter > 0 ? params->max_iter : (int)std::min<uint64_t>(10 * std::max<uint64_t>(p.D, 1), 1000);
     p.n_chunks = (uint32_t)std::max<uint64_t>(1, (p.D + kBatchChunk - 1) / kBatchChunk);
     p.cap = p.fp32 ? (uint32_t)(p.max_iter + 2) : 3u;
     p.nvr = p.fp32 ? std::min(p.nev, kMaxRitz) : 0;

@@ -802,10 +802,7 @@ extern "C" int alps_ed_sparsediag(const alps_ed_model* model, const alps_ed_solv
       result->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       return;
     }
-    // FP32 storage only pays, and only keeps the refined eigenvalues at
-    // ~1e-11, where the sector is much larger than the Krylov space the
-    // solve builds; small sectors are solved with FP64 vectors.
-    const bool fp32 = params->dtype == ALPS_ED_F32 && D >= kFp32MinDim;
+    const bool fp32 = use_fp32(params->dtype == ALPS_ED_F32, D, params->nev > 0 ? params->nev : 1);
     const int max_iter = params->max_iter > 0 ? params->max_iter : (int)std::min<uint64_t>(10 * D, 1000);
     const int nev = params->nev > 0 ? params->nev : 1;
     require(nev <= 64, "nev must be <= 64");

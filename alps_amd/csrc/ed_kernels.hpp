@@ -55,9 +55,16 @@ struct Partials {
 };
 
 constexpr int kBlock = 256;
-// Smallest sector whose Lanczos vectors a solve stores in FP32 when asked to
-// (smaller sectors use FP64 storage).
+// FP32 storage of the Lanczos vectors is used, when asked for, only where it
+// pays: sectors of at least kFp32MinDim states and at most kFp32MaxNev
+// requested eigenvalues.  In small sectors the Krylov space is a large part
+// of the sector; for more eigenvalues, the copies of converged levels that
+// FP32 rounding creates early delay the higher levels (about twice the FP64
+// iteration count for four levels), which costs more than the halved bytes
+// save.  Elsewhere the vectors are stored in FP64.
 constexpr uint64_t kFp32MinDim = 1000;
+constexpr int kFp32MaxNev = 2;
+inline bool use_fp32(bool asked, uint64_t dim, int nev) { return asked && dim >= kFp32MinDim && nev <= kFp32MaxNev; }
 constexpr int kMaxRitz = 8;     // Ritz vectors per pass of launch_ritz
 // H.v rows per thread: a segment of up to kRows*kBlock rows is one workgroup
 constexpr int kRows = 4;

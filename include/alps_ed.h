@@ -185,9 +185,12 @@ int alps_ed_tmatrix_analyze(int n, const double* alpha, const double* beta, doub
  * 100 eps), and each reported eigenvalue is the FP64-accumulated Rayleigh
  * quotient of its Ritz vector, whose error is about the square of the
  * residual.
- * Sectors with D < 1000 are stored in FP64 whatever `dtype` asks: there the
- * Krylov space the solve builds is a large part of the sector and FP32
- * storage would cost accuracy for no measurable saving. */
+ * FP32 storage is used only for sectors with D >= 1000 and nev <= 2; other
+ * solves store FP64 whatever `dtype` asks: in small sectors the Krylov space
+ * is a large part of the sector, and for more eigenvalues the copies of
+ * converged levels that FP32 rounding creates early delay the higher ones
+ * (about twice the FP64 iteration count for four), costing more than the
+ * halved bytes save. */
 typedef struct {
   int max_iter;
   int nev;
@@ -228,7 +231,7 @@ int alps_ed_sparsediag(const alps_ed_model* model, const alps_ed_solve_params* p
 typedef struct alps_ed_batch_s* alps_ed_batch;
 
 typedef struct {
-  alps_ed_dtype dtype;    /* storage of the Lanczos vectors (FP64 below 1000 states) */
+  alps_ed_dtype dtype;    /* storage of the Lanczos vectors (FP32 as in alps_ed_sparsediag) */
   uint64_t hbm_bytes;     /* arena size; 0 = half of the free device memory */
   uint64_t host_bytes;    /* mapped host memory for alpha/beta mirrors; 0 = 256 MiB */
   int steps_per_round;    /* Lanczos steps between stopping-rule tests; 0 = 8 */
