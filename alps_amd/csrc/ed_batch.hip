@@ -195,7 +195,8 @@ struct Problem {
   double* host_beta = nullptr;
   // recurrence and stopping rule
   int j = 0;           // alpha values produced
-  int scan_from = 2;   // smallest prefix length not yet decided
+  double lam_prev = std::numeric_limits<double>::quiet_NaN();  // lowest Ritz value at the last test
+  double lam_drop = 0;  // its decrease between the last two tests
   int final_n = -1;
   int error_code = 0;
   TValues tv;
@@ -445,30 +446,36 @@ void finish_problem(alps_ed_batch b, Problem& p) {
   ++b->solves_done;
 }
 
-// Stopping rule on the alpha/beta produced so far (see alps_ed_sparsediag):
-// the first prefix length at which lanczos_iteration_nlowest holds.  Per
-// round, a cheap O(m) test of the lowest value at the current length gates
-// the full T-matrix analysis; when the full rule holds there, the lengths
-// of this round are scanned back to the first one at which it holds.
+// Stopping rule on the alpha/beta produced so far.  The batch tests
+// lanczos_iteration_nlowest at the end of each round, i.e. at step counts
+// that are multiples of steps_per_round for every problem (each starts at
+// step 0 when admitted), and reports the T matrix of the first such count
+// at which the rule holds.  A problem's result therefore does not depend on
+// when it was admitted or on the rest of the batch.  A cheap O(m) test of
+// the lowest value gates the full T-matrix analysis.
 void decide(Problem& p) {
   const int n = p.j;
   if (p.fixed) {
     if (n >= p.max_iter) p.final_n = p.max_iter;
     return;
   }
-  const int lo = std::max(2, p.scan_from);
-  p.scan_from = std::max(p.scan_from, n + 1);
-  if (n >= lo) {
-    std::vector<double> a(p.host_alpha, p.host_alpha + n), bb(p.host_beta, p.host_beta + n);
+  if (n >= 2) {
     const double tol = stop_tolerance(p.fp32);
-    double lam = 0.0;
-    const double e = lowest_ritz_error(a.data(), bb.data(), n, &lam);
-    if (e <= 4.0 * std::max(tol, tol * std::fabs(lam)) &&
-        nlowest_converged(analyze_prefix(a, bb, n, p.fp32), p.nev, p.fp32)) {
-      int first = n;
-      while (first - 1 >= lo && nlowest_converged(analyze_prefix(a, bb, first - 1, p.fp32), p.nev, p.fp32)) --first;
-      p.final_n = first;
-      return;
+    double lam = 0.0, lo = 0.0, hi = -1.0;
+    if (p.lam_prev == p.lam_prev) {  // bracket from the previous test
+      const double drop = std::max(p.lam_drop, 64 * std::numeric_limits<double>::epsilon() * std::fabs(p.lam_prev));
+      hi = p.lam_prev + 4 * std::numeric_limits<double>::epsilon() * std::fabs(p.lam_prev);
+      lo = p.lam_prev - 4 * drop;
+    }
+    const double e = lowest_ritz_error(p.host_alpha, p.host_beta, n, &lam, lo, hi);
+    if (p.lam_prev == p.lam_prev) p.lam_drop = std::max(0.0, p.lam_prev - lam);
+    p.lam_prev = lam;
+    if (e <= 4.0 * std::max(tol, tol * std::fabs(lam))) {
+      std::vector<double> a(p.host_alpha, p.host_alpha + n), bb(p.host_beta, p.host_beta + n);
+      if (nlowest_converged(analyze_prefix(a, bb, n, p.fp32), p.nev, p.fp32)) {
+        p.final_n = n;
+        return;
+      }
     }
   }
   if (n >= p.max_iter) {

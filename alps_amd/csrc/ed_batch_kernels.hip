@@ -79,16 +79,34 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(const BatchJob* __
   const V* up = has_prev ? slot_ptr<V>(vec, stride, cap, j - 1) : u;
   const double cb = has_prev ? ldc(beta + j - 1) * ldc(scale + j - 1) : 0.0;
   double* part = ldc(&J->part_up);
-  const uint32_t r0 = it.idx * kBatchChunk;
-  const uint32_t r1 = min(D, r0 + kBatchChunk);
+  // four consecutive rows per thread (slots are 256-byte aligned, chunks
+  // start at multiples of kBatchChunk rows)
+  const uint32_t i0 = it.idx * kBatchChunk + 4u * threadIdx.x;
   double acc = 0.0;
-  for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
-    double r = sc * (double)w[i] - ca * (double)u[i];
-    if (has_prev) r -= cb * (double)up[i];
-    const V rs = (V)r;
-    w[i] = rs;
-    const double rd = (double)rs;
-    acc += rd * rd;
+  if (i0 + 4u <= D) {
+    typedef V v4 __attribute__((ext_vector_type(4), aligned(sizeof(V))));
+    const v4 wv = *reinterpret_cast<const v4*>(w + i0);
+    const v4 uv = *reinterpret_cast<const v4*>(u + i0);
+    const v4 pv = has_prev ? *reinterpret_cast<const v4*>(up + i0) : uv;
+    v4 o;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      double r = sc * (double)wv[c] - ca * (double)uv[c];
+      if (has_prev) r -= cb * (double)pv[c];
+      o[c] = (V)r;
+      const double rd = (double)o[c];
+      acc += rd * rd;
+    }
+    *reinterpret_cast<v4*>(w + i0) = o;
+  } else {
+    for (uint32_t i = i0; i < D && i < i0 + 4u; ++i) {
+      double r = sc * (double)w[i] - ca * (double)u[i];
+      if (has_prev) r -= cb * (double)up[i];
+      const V rs = (V)r;
+      w[i] = rs;
+      const double rd = (double)rs;
+      acc += rd * rd;
+    }
   }
   const double s = block_sum_all(acc);
   if (threadIdx.x == 0) part[it.idx] = s;

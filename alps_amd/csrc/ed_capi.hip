@@ -93,7 +93,8 @@ struct alps_ed_lanczos_s {
   Reducer red;
   hipStream_t stream = nullptr;
 
-  size_t vec_bytes() const { return op->layout.D * (dtype == ALPS_ED_F32 ? 4 : 8); }
+  // slots are padded to 64 elements, so every slot is 256-byte aligned
+  size_t vec_bytes() const { return ((op->layout.D + 63) & ~uint64_t(63)) * (dtype == ALPS_ED_F32 ? 4 : 8); }
   void* slot(int j) {
     if (!keep_basis) return u[j & 1].p;
     const size_t c = (size_t)j / kChunk;

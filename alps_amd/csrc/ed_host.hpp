@@ -185,7 +185,11 @@ inline bool nlowest_converged(const TValues& t, int nev, bool fp32 = false) {
 // Sturm-sequence bisection for the eigenvalue and the backward three-term
 // recurrence for its eigenvector (stable exactly when the Ritz value has
 // converged, i.e. when the last components are small).  O(m) per call.
-inline double lowest_ritz_error(const double* a, const double* b, int m, double* lambda_out) {
+// An optional bracket [hint_lo, hint_hi] (e.g. from the same test a few
+// steps earlier: by interlacing the lowest value only moves down) shortens
+// the bisection; it is used only if the Sturm counts confirm it.
+inline double lowest_ritz_error(const double* a, const double* b, int m, double* lambda_out,
+                                double hint_lo = 0.0, double hint_hi = -1.0) {
   double lo = a[0], hi = a[0];
   for (int i = 0; i < m; ++i) {
     const double r = (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < m ? std::fabs(b[i]) : 0.0);
@@ -205,6 +209,10 @@ inline double lowest_ritz_error(const double* a, const double* b, int m, double*
     }
     return c;
   };
+  if (hint_lo < hint_hi && hint_lo >= lo && hint_hi <= hi && count_below(hint_lo) == 0 && count_below(hint_hi) >= 1) {
+    lo = hint_lo;
+    hi = hint_hi;
+  }
   for (int it = 0; it < 200 && hi - lo > 2 * eps * std::max(std::fabs(lo), std::fabs(hi)); ++it) {
     const double mid = 0.5 * (lo + hi);
     if (mid <= lo || mid >= hi) break;

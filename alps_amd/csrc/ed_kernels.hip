@@ -90,12 +90,32 @@ __global__ __launch_bounds__(kBlock) void update_fused_kernel(uint64_t n, const 
     if (beta_jm1) *beta_jm1 = nrm;
   }
   const double ca = a * sc;
-  const double cb = beta_jm1 ? nrm * (*scale_jm1) : 0.0;
+  const bool has_prev = beta_jm1 != nullptr;
+  const double cb = has_prev ? nrm * (*scale_jm1) : 0.0;
   double acc = 0.0;
+  // four consecutive elements per thread and iteration (vector loads)
+  typedef V v4 __attribute__((ext_vector_type(4), aligned(sizeof(V))));
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t n4 = n / 4;
+  for (uint64_t q = gid; q < n4; q += stride) {
+    const v4 wv = *reinterpret_cast<const v4*>(w + 4 * q);
+    const v4 uv = *reinterpret_cast<const v4*>(u_cur + 4 * q);
+    const v4 pv = has_prev ? *reinterpret_cast<const v4*>(u_prev + 4 * q) : uv;
+    v4 o;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      double r = sc * (double)wv[c] - ca * (double)uv[c];
+      if (has_prev) r -= cb * (double)pv[c];
+      o[c] = (V)r;
+      const double rd = (double)o[c];
+      acc += rd * rd;
+    }
+    *reinterpret_cast<v4*>(out + 4 * q) = o;
+  }
+  for (uint64_t i = 4 * n4 + gid; i < n; i += stride) {
     double r = sc * (double)w[i] - ca * (double)u_cur[i];
-    if (beta_jm1) r -= cb * (double)u_prev[i];
+    if (has_prev) r -= cb * (double)u_prev[i];
     const V rs = (V)r;
     out[i] = rs;
     const double rd = (double)rs;

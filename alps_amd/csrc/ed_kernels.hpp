@@ -79,7 +79,7 @@ inline unsigned stream_blocks(uint64_t n) {
 
 // Blocks of the fused update: every block also reads all partials, so the
 // grid is kept smaller than the streaming kernels'.
-inline unsigned fused_blocks(uint64_t n) { return stream_blocks(n) < 512 ? stream_blocks(n) : 512; }
+inline unsigned fused_blocks(uint64_t n) { return stream_blocks(n) < 256 ? stream_blocks(n) : 256; }
 
 // Blocks of the H.v kernel: one per segment.
 inline unsigned hv_blocks(const DevLayout& L) { return L.n_seg ? L.n_seg : 1; }

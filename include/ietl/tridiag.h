@@ -20,6 +20,17 @@
 namespace ietl {
 namespace tridiag {
 
+// sqrt(a^2 + b^2) without the cost of std::hypot on the common range; the
+// scaled form guards against overflow and underflow outside it.
+inline double pythag(double a, double b) {
+  const double fa = std::fabs(a), fb = std::fabs(b);
+  const double m = fa > fb ? fa : fb;
+  if (m > 1e-150 && m < 1e150) return std::sqrt(a * a + b * b);
+  if (m == 0.0) return 0.0;
+  const double x = fa / m, y = fb / m;
+  return m * std::sqrt(x * x + y * y);
+}
+
 // d[0..n): diagonal (overwritten by ascending eigenvalues).
 // e[0..n-1): off-diagonal (not modified).
 // zlast: if non-null, receives z(n-1, i) of the orthonormal eigenvectors,
@@ -40,14 +51,14 @@ inline int eig(int n, double* d, const double* e_in, double* zlast) {
       if (m == l) break;
       if (iter >= 60) return l + 1;
       double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-      double r = std::hypot(g, 1.0);
+      double r = pythag(g, 1.0);
       g = d[m] - d[l] + e[l] / (g + std::copysign(r, g));
       double s = 1.0, c = 1.0, p = 0.0;
       bool split = false;
       for (int i = m - 1; i >= l; --i) {
         const double f = s * e[i];
         const double b = c * e[i];
-        r = std::hypot(f, g);
+        r = pythag(f, g);
         e[i + 1] = r;
         if (r == 0.0) {  // the matrix splits; restart the sweep
           d[i + 1] -= p;
@@ -108,14 +119,14 @@ inline int eig_full(int n, double* d, const double* e_in, double* z) {
       if (m == l) break;
       if (iter >= 60) return l + 1;
       double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-      double r = std::hypot(g, 1.0);
+      double r = pythag(g, 1.0);
       g = d[m] - d[l] + e[l] / (g + std::copysign(r, g));
       double s = 1.0, c = 1.0, p = 0.0;
       bool split = false;
       for (int i = m - 1; i >= l; --i) {
         const double f = s * e[i];
         const double b = c * e[i];
-        r = std::hypot(f, g);
+        r = pythag(f, g);
         e[i + 1] = r;
         if (r == 0.0) {
           d[i + 1] -= p;
