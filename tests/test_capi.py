@@ -49,7 +49,8 @@ def test_tmatrix_analysis_matches_oracle(n):
     t2 = A.tmatrix_analyze(a, b)
     np.testing.assert_array_equal(t1["mult"], t2["mult"])
     np.testing.assert_allclose(t2["eig"], t1["eig"], atol=1e-13, rtol=0)
-    np.testing.assert_allclose(t2["err"], t1["err"], atol=1e-14, rtol=0)
+    # error bounds are last-row eigenvector entries times beta: rounding-level
+    np.testing.assert_allclose(t2["err"], t1["err"], atol=5e-14, rtol=0)
 
 
 def test_tmatrix_ghosts_removed_after_convergence():
