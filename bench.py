@@ -24,6 +24,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of hv_kernel<float,false> on this workload (L=24),
+# from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE scaled by a copy kernel of
+# known size (profiles/hv_kernel_heisenberg_L24.md); valid for that kernel
+# version and L only.
+PMC_TRAFFIC_BYTES = {24: 52.03e6}
 
 
 class HipEvents:
@@ -337,7 +342,7 @@ def main():
         "config": {"workload": f"Heisenberg ring L={L}, J=1, Sz=0, D={D}, matrix-free H.v, FP32 vectors",
                    "L": L, "dim": D, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": PMC_TRAFFIC_BYTES.get(L),
                      "kernel": "hv_kernel<float,false>", "bytes_per_launch": bytes_per_hv},
         "timed_wall_s": round(wall, 4),
         "copy_gbs": round(copy_gbs, 2),
