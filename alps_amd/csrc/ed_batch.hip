@@ -197,6 +197,7 @@ struct Problem {
   int j = 0;           // alpha values produced
   double lam_prev = std::numeric_limits<double>::quiet_NaN();  // lowest Ritz value at the last test
   double lam_drop = 0;  // its decrease between the last two tests
+  int tested_j = 0;     // value of j at the last stopping test
   int final_n = -1;
   int error_code = 0;
   TValues tv;
@@ -556,21 +557,30 @@ struct Group {
   BatchItem* hv64 = nullptr;
   BatchItem* up32 = nullptr;
   BatchItem* up64 = nullptr;
+  BatchItem* st32 = nullptr;  // start-vector items of the problems new this round
+  BatchItem* st64 = nullptr;
   uint32_t* list = nullptr;
-  uint32_t n_hv32 = 0, n_hv64 = 0, n_up32 = 0, n_up64 = 0, n_list = 0;
+  uint32_t* st_list = nullptr;
+  uint32_t n_hv32 = 0, n_hv64 = 0, n_up32 = 0, n_up64 = 0, n_list = 0, n_st32 = 0, n_st64 = 0, n_st_list = 0;
   uint32_t seg32 = 0, seg64 = 0;
 };
 
 Group upload_group(alps_ed_batch b, const std::vector<Problem*>& run) {
-  std::vector<BatchItem> hv32, hv64, up32, up64;
+  std::vector<BatchItem> hv32, hv64, up32, up64, st32, st64;
+  std::vector<uint32_t> st_list;
   Group g;
   rebuild_lists(b, run, hv32, hv64, up32, up64, g.seg32, g.seg64);
+  for (uint32_t i = 0; i < run.size(); ++i)
+    if (run[i]->j == 0) {
+      st_list.push_back(i);
+      for (uint32_t c = 0; c < run[i]->n_chunks; ++c) (run[i]->fp32 ? st32 : st64).push_back({i, c});
+    }
   const size_t nj = run.size();
+  auto ib = [](size_t n) { return align_up((n + 1) * sizeof(BatchItem)); };
   const size_t bj = align_up(nj * sizeof(BatchJob));
-  const size_t bi = align_up((hv32.size() + 1) * sizeof(BatchItem)) + align_up((hv64.size() + 1) * sizeof(BatchItem)) +
-                    align_up((up32.size() + 1) * sizeof(BatchItem)) + align_up((up64.size() + 1) * sizeof(BatchItem));
   const size_t bl = align_up((nj + 1) * sizeof(uint32_t));
-  const size_t total = bj + bi + bl;
+  const size_t total = bj + ib(hv32.size()) + ib(hv64.size()) + ib(up32.size()) + ib(up64.size()) + ib(st32.size()) +
+                       ib(st64.size()) + 2 * bl;
   ensure_buf(b->jobs_d, b->jobs_cap, total);
   ensure_stage(b, total);
   char* s = static_cast<char*>(b->stage);
@@ -582,16 +592,22 @@ Group upload_group(alps_ed_batch b, const std::vector<Problem*>& run) {
     if (!v.empty()) std::memcpy(s + o, v.data(), v.size() * sizeof(BatchItem));
     dst = reinterpret_cast<BatchItem*>(d + o);
     n = (uint32_t)v.size();
-    o += align_up((v.size() + 1) * sizeof(BatchItem));
+    o += ib(v.size());
   };
   put_items(hv32, g.hv32, g.n_hv32);
   put_items(hv64, g.hv64, g.n_hv64);
   put_items(up32, g.up32, g.n_up32);
   put_items(up64, g.up64, g.n_up64);
+  put_items(st32, g.st32, g.n_st32);
+  put_items(st64, g.st64, g.n_st64);
   uint32_t* lst = reinterpret_cast<uint32_t*>(s + o);
   for (uint32_t i = 0; i < nj; ++i) lst[i] = i;
   g.list = reinterpret_cast<uint32_t*>(d + o);
   g.n_list = (uint32_t)nj;
+  o += bl;
+  if (!st_list.empty()) std::memcpy(s + o, st_list.data(), st_list.size() * sizeof(uint32_t));
+  g.st_list = reinterpret_cast<uint32_t*>(d + o);
+  g.n_st_list = (uint32_t)st_list.size();
   hip_check(hipMemcpyAsync(d, s, total, hipMemcpyHostToDevice, b->stream), "upload work lists");
   return g;
 }
@@ -792,10 +808,14 @@ void refine(alps_ed_batch b) {
   }
 }
 
+// One round.  The host tests the stopping rule on the previous round's
+// alpha/beta while the device runs this round's steps, so the tests cost
+// no device time; a problem found converged has then run one more round of
+// steps, which changes nothing it reports.
 void round(alps_ed_batch b) {
   ++b->rounds;
   std::vector<Problem*> started;
-  double t0 = now_s();
+  const double t0 = now_s();
   admit(b, started);
   // D = 0 and D = 1 sectors need no Lanczos (sparsediag.h:101-102,120-124)
   for (Problem* p : started) {
@@ -807,44 +827,23 @@ void round(alps_ed_batch b) {
       p->state = kDone;
     }
   }
-  std::vector<Problem*> run;
+  std::vector<Problem*> run, test;
   for (auto& p : b->active)
-    if (p->state == kRunning) run.push_back(p.get());
-  double t1 = now_s();
+    if (p->state == kRunning) {
+      run.push_back(p.get());
+      if (p->j > p->tested_j) test.push_back(p.get());
+    }
+  const double t1 = now_s();
   b->t_admit += t1 - t0;
+  const int S = std::max(1, b->cfg.steps_per_round);
   if (!run.empty()) {
     for (Problem* p : run) p->job.j0 = p->j;
     Group g = upload_group(b, run);
-    // start vectors of the problems admitted in this round
-    {
-      std::vector<uint32_t> idx;
-      for (uint32_t i = 0; i < run.size(); ++i)
-        if (run[i]->j == 0 && std::find(started.begin(), started.end(), run[i]) != started.end()) idx.push_back(i);
-      if (!idx.empty()) {
-        std::vector<BatchItem> s32, s64;
-        for (uint32_t i : idx)
-          for (uint32_t c = 0; c < run[i]->n_chunks; ++c) (run[i]->fp32 ? s32 : s64).push_back({i, c});
-        DevBuf sd(align_up((s32.size() + 1) * sizeof(BatchItem)) + align_up((s64.size() + 1) * sizeof(BatchItem)) +
-                  align_up((idx.size() + 1) * 4));
-        char* base = sd.as<char>();
-        BatchItem* d32 = reinterpret_cast<BatchItem*>(base);
-        BatchItem* d64 = reinterpret_cast<BatchItem*>(base + align_up((s32.size() + 1) * sizeof(BatchItem)));
-        uint32_t* dl = reinterpret_cast<uint32_t*>(base + align_up((s32.size() + 1) * sizeof(BatchItem)) +
-                                                   align_up((s64.size() + 1) * sizeof(BatchItem)));
-        if (!s32.empty())
-          hip_check(hipMemcpyAsync(d32, s32.data(), s32.size() * sizeof(BatchItem), hipMemcpyHostToDevice, b->stream),
-                    "upload");
-        if (!s64.empty())
-          hip_check(hipMemcpyAsync(d64, s64.data(), s64.size() * sizeof(BatchItem), hipMemcpyHostToDevice, b->stream),
-                    "upload");
-        hip_check(hipMemcpyAsync(dl, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, b->stream), "upload");
-        hip_check(launch_batch_start<float>(g.jobs, d32, (uint32_t)s32.size(), b->stream), "batch start");
-        hip_check(launch_batch_start<double>(g.jobs, d64, (uint32_t)s64.size(), b->stream), "batch start");
-        hip_check(launch_batch_finish(g.jobs, dl, (uint32_t)idx.size(), 0, kBatchStart, b->stream), "batch finish");
-        hip_check(hipStreamSynchronize(b->stream), "sync");  // sd is freed at scope end
-      }
+    if (g.n_st_list) {
+      hip_check(launch_batch_start<float>(g.jobs, g.st32, g.n_st32, b->stream), "batch start");
+      hip_check(launch_batch_start<double>(g.jobs, g.st64, g.n_st64, b->stream), "batch start");
+      hip_check(launch_batch_finish(g.jobs, g.st_list, g.n_st_list, 0, kBatchStart, b->stream), "batch finish");
     }
-    const int S = std::max(1, b->cfg.steps_per_round);
     for (int t = 0; t < S; ++t) {
       hip_check(launch_batch_hv<float>(g.jobs, g.hv32, g.n_hv32, t, g.seg32, b->stream), "batch hv");
       hip_check(launch_batch_hv<double>(g.jobs, g.hv64, g.n_hv64, t, g.seg64, b->stream), "batch hv");
@@ -853,21 +852,25 @@ void round(alps_ed_batch b) {
       hip_check(launch_batch_update<double>(g.jobs, g.up64, g.n_up64, t, b->stream), "batch update");
       hip_check(launch_batch_finish(g.jobs, g.list, g.n_list, t, kBatchBeta, b->stream), "batch finish");
     }
+  }
+  // stopping tests on results of earlier rounds (entries below p->j are final)
+  b->pool->parallel_for(test.size(), [&](size_t i) {
+    Problem& p = *test[i];
+    decide(p);
+    p.tested_j = p.j;
+    if (p.final_n >= 0) conclude(p);
+  });
+  const double t2 = now_s();
+  b->t_decide += t2 - t1;
+  if (!run.empty()) {
     hip_check(hipStreamSynchronize(b->stream), "sync");
-    const double t2 = now_s();
-    b->t_steps += t2 - t1;
     b->steps += S;
     for (Problem* p : run) p->j = std::min(p->max_iter, p->j + S);
-    b->pool->parallel_for(run.size(), [&](size_t i) {
-      Problem& p = *run[i];
-      decide(p);
-      if (p.final_n >= 0) conclude(p);
-    });
-    t1 = now_s();
-    b->t_decide += t1 - t2;
   }
+  const double t3 = now_s();
+  b->t_steps += t3 - t2;
   refine(b);
-  b->t_refine += now_s() - t1;
+  b->t_refine += now_s() - t3;
   retire(b);
   hip_check(hipStreamSynchronize(b->stream), "sync");  // staging buffers are reused next round
 }
