@@ -86,56 +86,41 @@ __device__ __forceinline__ T ldc_struct(const T* p) {
 // The segment's slice of x is staged in LDS.  Each thread owns up to kRows
 // rows (r = tid + q*kBlock); row slots past the segment are skipped with a
 // uniform test, so every chunk's scalar data is loaded once per thread.
-template <class V, bool DOT>
-__device__ __forceinline__ void hv_segment(const DevLayout& L, uint32_t h, const V* __restrict__ x,
-                                           V* __restrict__ y, double* __restrict__ dot_slot) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  V* xs = reinterpret_cast<V*>(smem);  // segment slice; entry len is the zero slot
+// The bond loops of hv_segment for NQ row slots per thread (rows tid,
+// tid + kBlock, ...), NQ fixed at compile time so that the loads of all
+// slots are issued together; rows past the segment are clamped to row 0 and
+// not stored.  Returns the thread's share of x.Hx when DOT.
+template <class V, bool DOT, int NQ>
+__device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const V* __restrict__ x,
+                                          V* __restrict__ y, const V* xs, uint32_t len, uint32_t base, double dh,
+                                          const uint16_t* nbp, const uint16_t* xnp, const double* dlp) {
   const uint32_t tid = threadIdx.x;
-  const int kl = L.k - __builtin_popcount(h);
-  const uint32_t cls0 = ldc(L.lo_class_off + kl);
-  const uint32_t len = ldc(L.lo_class_off + kl + 1) - cls0;
-  const uint32_t base = ldc(L.hi_base + h);
-  const double dh = ldc(L.diag_hi + h);
   const V* xb = x + base;
-  for (uint32_t r = tid; r < len; r += kBlock) xs[r] = xb[r];
-  if (tid == 0) xs[len] = (V)0;
-  const uint16_t* nbp = L.nbp16 + ldc(L.nbp_off + kl);
-  const uint16_t* xnp = L.xn16 + ldc(L.xn_off + kl);
-  const double* dlp = L.diag_lo_cls + cls0;
-  __syncthreads();
-
-  const uint32_t nq = (len + kBlock - 1) / kBlock;
-  uint32_t rc[kRows];
-  bool ok[kRows];
-  double d[kRows], off[kRows];
+  uint32_t rc[NQ];
+  bool ok[NQ];
+  double d[NQ], off[NQ];
 #pragma unroll
-  for (int q = 0; q < kRows; ++q) {
+  for (int q = 0; q < NQ; ++q) {
     const uint32_t r = tid + q * kBlock;
     ok[q] = r < len;
     rc[q] = ok[q] ? r : 0u;  // clamped row: every read stays in bounds
-    d[q] = dh;
+    d[q] = dh + dlp[rc[q]];
     off[q] = 0.0;
-    if (q < (int)nq) d[q] += dlp[rc[q]];
   }
   // low-low flips: target rows from the class table, values from LDS
   for (int b0 = 0; b0 < L.n_ll_p; b0 += kChunkLL) {
     double t[kChunkLL];
 #pragma unroll
     for (int c = 0; c < kChunkLL; ++c) t[c] = ldc(L.ll_t + b0 + c);
+    uint32_t j[NQ][kChunkLL];
 #pragma unroll
-    for (int q = 0; q < kRows; ++q) {
-      if (q < (int)nq) {
-        uint32_t j[kChunkLL];
+    for (int q = 0; q < NQ; ++q)
 #pragma unroll
-        for (int c = 0; c < kChunkLL; ++c) j[c] = (nbp + (b0 + c) * len)[rc[q]];
-        V v[kChunkLL];
+      for (int c = 0; c < kChunkLL; ++c) j[q][c] = (nbp + (b0 + c) * len)[rc[q]];
 #pragma unroll
-        for (int c = 0; c < kChunkLL; ++c) v[c] = xs[j[c]];
+    for (int q = 0; q < NQ; ++q)
 #pragma unroll
-        for (int c = 0; c < kChunkLL; ++c) off[q] += t[c] * (double)v[c];
-      }
-    }
+      for (int c = 0; c < kChunkLL; ++c) off[q] += t[c] * (double)xs[j[q][c]];
   }
   // high-high flips: uniform decision, contiguous target run
   for (int b0 = 0; b0 < L.n_hh_p; b0 += kChunkHH) {
@@ -149,16 +134,15 @@ __device__ __forceinline__ void hv_segment(const DevLayout& L, uint32_t h, const
       xt[c] = act ? x + ldc(L.hi_base + (h ^ m)) : xb;
       tt[c] = act ? ldc(L.hh_t + b0 + c) : 0.0;
     }
+    V v[NQ][kChunkHH];
 #pragma unroll
-    for (int q = 0; q < kRows; ++q) {
-      if (q < (int)nq) {
-        V v[kChunkHH];
+    for (int q = 0; q < NQ; ++q)
 #pragma unroll
-        for (int c = 0; c < kChunkHH; ++c) v[c] = xt[c][rc[q]];
+      for (int c = 0; c < kChunkHH; ++c) v[q][c] = xt[c][rc[q]];
 #pragma unroll
-        for (int c = 0; c < kChunkHH; ++c) off[q] += tt[c] * (double)v[c];
-      }
-    }
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int c = 0; c < kChunkHH; ++c) off[q] += tt[c] * (double)v[q][c];
   }
   // cross bonds: diagonal always, flip when the high and low bits differ.
   // If the neighbouring high word has no segment no row can flip (its low
@@ -178,30 +162,31 @@ __device__ __forceinline__ void hv_segment(const DevLayout& L, uint32_t h, const
       ep[c] = ldc(L.xb_par + b0 + c);
       tx[c] = ldc(L.xb_t + b0 + c);
     }
+    uint32_t e[NQ][kChunkX];
 #pragma unroll
-    for (int q = 0; q < kRows; ++q) {
-      if (q < (int)nq) {
-        uint32_t e[kChunkX];
+    for (int q = 0; q < NQ; ++q)
 #pragma unroll
-        for (int c = 0; c < kChunkX; ++c) e[c] = (xnp + (b0 + c) * len)[rc[q]];
-        V v[kChunkX];
-        bool f[kChunkX];
+      for (int c = 0; c < kChunkX; ++c) e[q][c] = (xnp + (b0 + c) * len)[rc[q]];
+    V v[NQ][kChunkX];
+    bool f[NQ][kChunkX];
 #pragma unroll
-        for (int c = 0; c < kChunkX; ++c) {
-          f[c] = ((e[c] >> 15) != 0u) != hb[c];
-          v[c] = xn[c][f[c] ? (e[c] & 0x7fffu) : 0u];
-        }
+    for (int q = 0; q < NQ; ++q)
 #pragma unroll
-        for (int c = 0; c < kChunkX; ++c) {
-          d[q] += f[c] ? ea[c] : ep[c];
-          off[q] += (f[c] ? tx[c] : 0.0) * (double)v[c];
-        }
+      for (int c = 0; c < kChunkX; ++c) {
+        f[q][c] = ((e[q][c] >> 15) != 0u) != hb[c];
+        v[q][c] = xn[c][f[q][c] ? (e[q][c] & 0x7fffu) : 0u];
       }
-    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int c = 0; c < kChunkX; ++c) {
+        d[q] += f[q][c] ? ea[c] : ep[c];
+        off[q] += (f[q][c] ? tx[c] : 0.0) * (double)v[q][c];
+      }
   }
   double dotp = 0.0;
 #pragma unroll
-  for (int q = 0; q < kRows; ++q) {
+  for (int q = 0; q < NQ; ++q) {
     if (ok[q]) {
       const double xi = (double)xs[rc[q]];
       const double yv = d[q] * xi + off[q];
@@ -209,6 +194,47 @@ __device__ __forceinline__ void hv_segment(const DevLayout& L, uint32_t h, const
       if (DOT) dotp += xi * yv;  // FP64 value before the storage rounding
     }
   }
+  return dotp;
+}
+
+// H.v over the row segment of high word h (one workgroup of kBlock threads;
+// dynamic LDS of at least (L.max_seg + 1) elements of V):
+//   y[r] = sum_c H[r][c] x[c] for the segment's rows r;
+// with DOT, *dot_slot receives the segment's sum of x_r (H x)_r (FP64, before
+// y is rounded to the storage type).
+//
+// The segment's slice of x is staged in LDS.  Each thread owns up to kRows
+// rows (r = tid + q*kBlock); the number of row slots is uniform over the
+// workgroup and selects a fixed-size instance of the bond loops.
+template <class V, bool DOT>
+__device__ __forceinline__ void hv_segment(const DevLayout& L, uint32_t h, const V* __restrict__ x,
+                                           V* __restrict__ y, double* __restrict__ dot_slot) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  V* xs = reinterpret_cast<V*>(smem);  // segment slice; entry len is the zero slot
+  const uint32_t tid = threadIdx.x;
+  const int kl = L.k - __builtin_popcount(h);
+  const uint32_t cls0 = ldc(L.lo_class_off + kl);
+  const uint32_t len = ldc(L.lo_class_off + kl + 1) - cls0;
+  const uint32_t base = ldc(L.hi_base + h);
+  const double dh = ldc(L.diag_hi + h);
+  const V* xb = x + base;
+  for (uint32_t r = tid; r < len; r += kBlock) xs[r] = xb[r];
+  if (tid == 0) xs[len] = (V)0;
+  const uint16_t* nbp = L.nbp16 + ldc(L.nbp_off + kl);
+  const uint16_t* xnp = L.xn16 + ldc(L.xn_off + kl);
+  const double* dlp = L.diag_lo_cls + cls0;
+  __syncthreads();
+  const uint32_t nq = (len + kBlock - 1) / kBlock;
+  static_assert(kRows == 4, "hv_segment dispatches one to four row slots");
+  double dotp;
+  if (nq <= 1)
+    dotp = hv_rows<V, DOT, 1>(L, h, x, y, xs, len, base, dh, nbp, xnp, dlp);
+  else if (nq == 2)
+    dotp = hv_rows<V, DOT, 2>(L, h, x, y, xs, len, base, dh, nbp, xnp, dlp);
+  else if (nq == 3)
+    dotp = hv_rows<V, DOT, 3>(L, h, x, y, xs, len, base, dh, nbp, xnp, dlp);
+  else
+    dotp = hv_rows<V, DOT, 4>(L, h, x, y, xs, len, base, dh, nbp, xnp, dlp);
   if (DOT) {
     const double bs = block_sum_all(dotp);
     if (tid == 0) *dot_slot = bs;

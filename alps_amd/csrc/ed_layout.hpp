@@ -145,10 +145,21 @@ struct SectorLayout {
   std::vector<double> bond_par, bond_anti, bond_t;
 };
 
-// Chunk sizes of the H.v flip loops (bond lists are padded to multiples).
-constexpr int kChunkLL = 4;
-constexpr int kChunkHH = 4;
-constexpr int kChunkX = 2;
+// Chunk sizes of the H.v flip loops (bond lists are padded to multiples):
+// the loads of one chunk for all of a thread's rows are in flight together.
+// Build-time tuning parameters.
+#ifndef ALPS_ED_CHUNK_LL
+#define ALPS_ED_CHUNK_LL 4
+#endif
+#ifndef ALPS_ED_CHUNK_HH
+#define ALPS_ED_CHUNK_HH 4
+#endif
+#ifndef ALPS_ED_CHUNK_X
+#define ALPS_ED_CHUNK_X 2
+#endif
+constexpr int kChunkLL = ALPS_ED_CHUNK_LL;
+constexpr int kChunkHH = ALPS_ED_CHUNK_HH;
+constexpr int kChunkX = ALPS_ED_CHUNK_X;
 
 inline int round_up(int n, int m) { return (n + m - 1) / m * m; }
 
