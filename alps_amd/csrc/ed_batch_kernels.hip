@@ -40,7 +40,7 @@ __global__ __launch_bounds__(kBlock) void batch_start_kernel(const BatchJob* __r
 }
 
 template <class V>
-__global__ __launch_bounds__(kBlock) void batch_hv_kernel(const BatchJob* __restrict__ jobs,
+__global__ __launch_bounds__(kBlock, HvWaves<V>::value) void batch_hv_kernel(const BatchJob* __restrict__ jobs,
                                                           const BatchItem* __restrict__ items, int t) {
   const BatchItem it = ldc_struct(items + blockIdx.x);
   const BatchJob* J = jobs + it.job;

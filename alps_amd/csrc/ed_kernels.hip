@@ -34,7 +34,7 @@ using namespace dev;
 // H.v: one workgroup per segment, longest segments first
 // ---------------------------------------------------------------------------
 template <class V, bool DOT>
-__global__ __launch_bounds__(kBlock) void hv_kernel(DevLayout L, const V* __restrict__ x, V* __restrict__ y,
+__global__ __launch_bounds__(kBlock, HvWaves<V>::value) void hv_kernel(DevLayout L, const V* __restrict__ x, V* __restrict__ y,
                                                      double* __restrict__ dot_out) {
   const uint32_t h = ldc(L.proc_hi + blockIdx.x);
   hv_segment<V, DOT>(L, h, x, y, DOT ? dot_out + blockIdx.x : nullptr);

@@ -55,6 +55,13 @@ struct Partials {
 };
 
 constexpr int kBlock = 256;
+// H.v kernels on FP32 vectors are compiled for at least 8 waves per SIMD
+// (at most 64 registers): more waves in flight hide the gather latency.
+// The FP64 instances keep the default (8 would spill).
+template <class V>
+struct HvWaves {
+  static constexpr int value = sizeof(V) == 4 ? 8 : 1;
+};
 // FP32 storage of the Lanczos vectors is used, when asked for, only where it
 // pays: sectors of at least kFp32MinDim states and at most kFp32MaxNev
 // requested eigenvalues.  In small sectors the Krylov space is a large part
