@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE scaled by a copy kernel of
 # known size (profiles/hv_kernel_heisenberg_L24.md); valid for that kernel
 # version and L only.
-PMC_TRAFFIC_BYTES = {24: 52.03e6}
+PMC_TRAFFIC_BYTES = {24: 51.89e6}
 
 
 class HipEvents:
