@@ -168,6 +168,16 @@ class Pool {
   bool stop_ = false;
 };
 
+// Device copy of the tables that depend only on the low-word size and the
+// bond topology (low-word ranks and states, class offsets, flip tables).
+struct SharedTables {
+  DevBuf buf;
+  std::shared_ptr<const FlipTables> keep;  // keeps the key alive
+  const uint32_t *lo_rank = nullptr, *lo_state = nullptr, *lo_class_off = nullptr, *nbp_off = nullptr,
+                 *xn_off = nullptr;
+  const uint16_t *nbp16 = nullptr, *xn16 = nullptr;
+};
+
 enum State { kPending, kRunning, kRefining, kDone };
 
 struct Problem {
@@ -234,6 +244,8 @@ struct alps_ed_batch_s {
   size_t stage_cap = 0;
   void* tstage = nullptr;
   size_t tstage_cap = 0;
+  // shared low-word and flip tables, one device copy per lattice topology
+  std::map<std::pair<const void*, const void*>, std::unique_ptr<SharedTables>> shared;
 
   ~alps_ed_batch_s() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -247,23 +259,17 @@ struct alps_ed_batch_s {
 
 namespace {
 
-// Bytes of the layout tables a problem uploads.
+// Bytes of the per-problem layout tables (the low-word and flip tables are
+// shared by all problems of a lattice and uploaded once, see SharedTables).
 size_t table_bytes(const SectorLayout& L) {
   size_t s = 0;
   auto add = [&](size_t n, size_t es) { s += align_up(std::max<size_t>(n, 1) * es); };
-  add(L.lo_rank.size(), 4);
-  add(L.lo_state.size(), 4);
-  add(L.lo_class_off.size(), 4);
   add(L.hi_base.size(), 4);
   add(L.seg_hi.size(), 4);
   add(L.seg_start.size(), 4);
   add(L.proc_hi.size(), 4);
   add(L.diag_lo_cls.size(), 8);
   add(L.diag_hi.size(), 8);
-  add(L.nbp16.size(), 2);
-  add(L.nbp_off.size(), 4);
-  add(L.xn16.size(), 2);
-  add(L.xn_off.size(), 4);
   add(L.ll_t.size(), 8);
   add(L.hh_mask.size(), 4);
   add(L.hh_t.size(), 8);
@@ -281,10 +287,11 @@ size_t table_bytes(const SectorLayout& L) {
   return s;
 }
 
+
 // Lays the tables out at [base, base + table_bytes) of the device, writing
 // their bytes to the same offsets of the host staging area `stage`, and
 // returns the device layout; one copy of the staging area uploads them.
-DevLayout place_tables(const SectorLayout& L, char* base, char* stage) {
+DevLayout place_tables(const SectorLayout& L, char* base, char* stage, const SharedTables& sh) {
   DevLayout d{};
   d.N = L.N;
   d.k = L.k;
@@ -297,6 +304,13 @@ DevLayout place_tables(const SectorLayout& L, char* base, char* stage) {
   d.n_hh_p = L.n_hh_p;
   d.n_xb_p = L.n_xb_p;
   d.n_bonds = (int)L.bond_p1.size();
+  d.lo_rank = sh.lo_rank;
+  d.lo_state = sh.lo_state;
+  d.lo_class_off = sh.lo_class_off;
+  d.nbp16 = sh.nbp16;
+  d.nbp_off = sh.nbp_off;
+  d.xn16 = sh.xn16;
+  d.xn_off = sh.xn_off;
   size_t o = 0;
   auto put = [&](const auto& v) {
     using T = typename std::decay_t<decltype(v)>::value_type;
@@ -305,19 +319,12 @@ DevLayout place_tables(const SectorLayout& L, char* base, char* stage) {
     o += align_up(std::max<size_t>(v.size(), 1) * sizeof(T));
     return dst;
   };
-  d.lo_rank = put(L.lo_rank);
-  d.lo_state = put(L.lo_state);
-  d.lo_class_off = put(L.lo_class_off);
   d.hi_base = put(L.hi_base);
   d.seg_hi = put(L.seg_hi);
   d.seg_start = put(L.seg_start);
   d.proc_hi = put(L.proc_hi);
   d.diag_lo_cls = put(L.diag_lo_cls);
   d.diag_hi = put(L.diag_hi);
-  d.nbp16 = put(L.nbp16);
-  d.nbp_off = put(L.nbp_off);
-  d.xn16 = put(L.xn16);
-  d.xn_off = put(L.xn_off);
   d.ll_t = put(L.ll_t);
   d.hh_mask = put(L.hh_mask);
   d.hh_t = put(L.hh_t);
@@ -378,11 +385,11 @@ T* dev_ptr(alps_ed_batch b, size_t off) {
 
 // Places an admitted problem in its regions; its tables are staged at
 // `stage` (table_bytes of them) for the caller to upload.
-void place(alps_ed_batch b, Problem& p, char* stage) {
+void place(alps_ed_batch b, Problem& p, char* stage, const SharedTables& sh) {
   char* base = static_cast<char*>(b->dev_base) + p.dev_off;
   BatchJob& J = p.job;
   J = BatchJob{};
-  J.L = place_tables(p.layout, base, stage);
+  J.L = place_tables(p.layout, base, stage, sh);
   char* q = base + table_bytes(p.layout);
   const size_t es = p.fp32 ? 4 : 8;
   const uint64_t stride = (p.D + 63) & ~uint64_t(63);
@@ -624,6 +631,42 @@ void size_problem(alps_ed_batch b, Problem& p) {
   p.sized = true;
 }
 
+// The device copy of a layout's shared tables, uploaded on first use.
+const SharedTables& shared_tables(alps_ed_batch b, const SectorLayout& L) {
+  const auto key = std::make_pair(static_cast<const void*>(L.low), static_cast<const void*>(L.flip.get()));
+  std::unique_ptr<SharedTables>& slot = b->shared[key];
+  if (slot) return *slot;
+  slot.reset(new SharedTables);
+  SharedTables& t = *slot;
+  t.keep = L.flip;
+  size_t total = 0;
+  auto sz = [&](size_t n, size_t es) {
+    const size_t o = total;
+    total += align_up(std::max<size_t>(n, 1) * es);
+    return o;
+  };
+  const size_t o1 = sz(L.lo_rank.size(), 4), o2 = sz(L.lo_state.size(), 4), o3 = sz(L.lo_class_off.size(), 4),
+               o4 = sz(L.nbp16.size(), 2), o5 = sz(L.nbp_off.size(), 4), o6 = sz(L.xn16.size(), 2),
+               o7 = sz(L.xn_off.size(), 4);
+  t.buf.alloc(total);
+  char* d = t.buf.as<char>();
+  auto up = [&](size_t off, const auto& v) {
+    using T = typename std::decay_t<decltype(v)>::value_type;
+    if (!v.empty())
+      hip_check(hipMemcpyAsync(d + off, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, b->stream),
+                "upload shared tables");
+    return reinterpret_cast<const T*>(d + off);
+  };
+  t.lo_rank = up(o1, L.lo_rank);
+  t.lo_state = up(o2, L.lo_state);
+  t.lo_class_off = up(o3, L.lo_class_off);
+  t.nbp16 = up(o4, L.nbp16);
+  t.nbp_off = up(o5, L.nbp_off);
+  t.xn16 = up(o6, L.xn16);
+  t.xn_off = up(o7, L.xn_off);
+  return t;
+}
+
 void ensure_tstage(alps_ed_batch b, size_t bytes) {
   if (bytes <= b->tstage_cap) return;
   if (b->tstage) hip_check(hipHostFree(b->tstage), "hipHostFree");
@@ -687,7 +730,9 @@ void admit(alps_ed_batch b, std::vector<Problem*>& started) {
   }
   ensure_tstage(b, total);
   char* st = static_cast<char*>(b->tstage);
-  b->pool->parallel_for(placed.size(), [&](size_t i) { place(b, *placed[i], st + at[i]); });
+  std::vector<const SharedTables*> sh(placed.size());
+  for (size_t i = 0; i < placed.size(); ++i) sh[i] = &shared_tables(b, placed[i]->layout);
+  b->pool->parallel_for(placed.size(), [&](size_t i) { place(b, *placed[i], st + at[i], *sh[i]); });
   for (size_t i = 0; i < placed.size(); ++i)
     hip_check(hipMemcpyAsync(static_cast<char*>(b->dev_base) + placed[i]->dev_off, st + at[i],
                              table_bytes(placed[i]->layout), hipMemcpyHostToDevice, b->stream),
@@ -808,16 +853,8 @@ void refine(alps_ed_batch b) {
   }
 }
 
-// One round.  The host tests the stopping rule on the previous round's
-// alpha/beta while the device runs this round's steps, so the tests cost
-// no device time; a problem found converged has then run one more round of
-// steps, which changes nothing it reports.
-void round(alps_ed_batch b) {
-  ++b->rounds;
-  std::vector<Problem*> started;
-  const double t0 = now_s();
-  admit(b, started);
-  // D = 0 and D = 1 sectors need no Lanczos (sparsediag.h:101-102,120-124)
+// D = 0 and D = 1 sectors need no Lanczos (sparsediag.h:101-102,120-124).
+void settle_trivial(const std::vector<Problem*>& started) {
   for (Problem* p : started) {
     if (p->D == 0) {
       p->state = kDone;
@@ -827,12 +864,34 @@ void round(alps_ed_batch b) {
       p->state = kDone;
     }
   }
-  std::vector<Problem*> run, test;
-  for (auto& p : b->active)
-    if (p->state == kRunning) {
-      run.push_back(p.get());
-      if (p->j > p->tested_j) test.push_back(p.get());
-    }
+}
+
+// One round.  While the device runs this round's steps the host admits new
+// requests (they start next round) and tests the stopping rule on the
+// previous round's alpha/beta, so neither costs device time; a problem found
+// converged has then run one more round of steps, which changes nothing it
+// reports.  When nothing runs, requests are admitted first and start at once.
+void round(alps_ed_batch b) {
+  ++b->rounds;
+  const double t0 = now_s();
+  auto running = [&](std::vector<Problem*>& run, std::vector<Problem*>& test) {
+    run.clear();
+    test.clear();
+    for (auto& p : b->active)
+      if (p->state == kRunning) {
+        run.push_back(p.get());
+        if (p->j > p->tested_j) test.push_back(p.get());
+      }
+  };
+  std::vector<Problem*> run, test, started;
+  running(run, test);
+  bool admitted = false;
+  if (run.empty()) {
+    admit(b, started);
+    settle_trivial(started);
+    running(run, test);
+    admitted = true;
+  }
   const double t1 = now_s();
   b->t_admit += t1 - t0;
   const int S = std::max(1, b->cfg.steps_per_round);
@@ -853,24 +912,33 @@ void round(alps_ed_batch b) {
       hip_check(launch_batch_finish(g.jobs, g.list, g.n_list, t, kBatchBeta, b->stream), "batch finish");
     }
   }
-  // stopping tests on results of earlier rounds (entries below p->j are final)
+  // host work overlapping the steps: admission (table copies queue behind
+  // the steps on the stream), then the stopping tests on results of earlier
+  // rounds (entries below p->j are final)
+  const double t2 = now_s();
+  if (!admitted) {
+    admit(b, started);
+    settle_trivial(started);
+  }
+  const double t3 = now_s();
+  b->t_admit += t3 - t2;
   b->pool->parallel_for(test.size(), [&](size_t i) {
     Problem& p = *test[i];
     decide(p);
     p.tested_j = p.j;
     if (p.final_n >= 0) conclude(p);
   });
-  const double t2 = now_s();
-  b->t_decide += t2 - t1;
+  const double t4 = now_s();
+  b->t_decide += t4 - t3;
   if (!run.empty()) {
     hip_check(hipStreamSynchronize(b->stream), "sync");
     b->steps += S;
     for (Problem* p : run) p->j = std::min(p->max_iter, p->j + S);
   }
-  const double t3 = now_s();
-  b->t_steps += t3 - t2;
+  const double t5 = now_s();
+  b->t_steps += t5 - t4;
   refine(b);
-  b->t_refine += now_s() - t3;
+  b->t_refine += now_s() - t5;
   retire(b);
   hip_check(hipStreamSynchronize(b->stream), "sync");  // staging buffers are reused next round
 }

@@ -109,9 +109,17 @@ struct CrossBond {
   double e_par, e_anti, t;
 };
 
+struct LowTables;
+struct FlipTables;
+
 struct SectorLayout {
   int N = 0, k = 0, B = 0, NH = 0;
   uint64_t D = 0;
+  // the shared tables the low-word and flip-table vectors below copy: the
+  // same objects for every sector with the same low-word size and low-low /
+  // cross-bond topology (lets a batch upload them once)
+  const LowTables* low = nullptr;
+  std::shared_ptr<const FlipTables> flip;
   std::vector<uint32_t> lo_rank;       // [2^B]
   std::vector<uint32_t> lo_state;      // [2^B] grouped by popcount, ascending
   std::vector<uint32_t> lo_class_off;  // [B+2]
@@ -285,6 +293,7 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1) {
   // low tables (depend on B only; shared)
   {
     const LowTables& T = low_tables(B);
+    L.low = &T;
     L.lo_rank = T.lo_rank;
     L.lo_state = T.lo_state;
     L.lo_class_off = T.lo_class_off;
@@ -376,6 +385,7 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1) {
     for (const FlipBond& f : L.ll) llm.push_back(f.mask);
     for (const CrossBond& x : L.xb) xlb.push_back(x.lbit);
     std::shared_ptr<const FlipTables> F = flip_tables(B, llm, L.n_ll_p, xlb, L.n_xb_p);
+    L.flip = F;
     L.nbp_off = F->nbp_off;
     L.xn_off = F->xn_off;
     L.nbp16 = F->nbp16;
