@@ -963,11 +963,41 @@ extern "C" int alps_ed_run_parameter_file(const char* text, char* out, size_t ou
       uint64_t dim_sum = 0;
       int it_sum = 0, err_max = 0;
       double sec_sum = 0.0;
-      for (int n_up : ts.n_up) {
-        alps_ed_model am{m.n_sites, n_up, (int)m.s1.size(), m.s1.data(), m.s2.data(), m.jz.data(),
-                         m.jxy.data(),  m.k.data(), m.h.data(), m.d.data()};
-        alps_ed_solve_result r{};
-        check_status(alps_ed_sparsediag(&am, &sp, &r));
+      // All sectors of the task are solved together as one batch
+      // (DiagMatrix::dostep's sector loop, diag.h:86-138) when their
+      // footprint fits the free device memory, else one after the other.
+      std::vector<alps_ed_model> models;
+      for (int n_up : ts.n_up)
+        models.push_back(alps_ed_model{m.n_sites, n_up, (int)m.s1.size(), m.s1.data(), m.s2.data(), m.jz.data(),
+                                       m.jxy.data(), m.k.data(), m.h.data(), m.d.data()});
+      std::vector<alps_ed_solve_result> res(models.size());
+      uint64_t need = 0;
+      for (const alps_ed_model& am : models) {
+        uint64_t f = 0;
+        check_status(alps_ed_batch_footprint(&am, &sp, sp.dtype, &f));
+        need += f + (1u << 20);
+      }
+      size_t free_b = 0, total_b = 0;
+      hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+      if (models.size() > 1 && need < 0.8 * (double)free_b) {
+        alps_ed_batch_config cfg{sp.dtype, need, 0, 8, 0, 0};
+        alps_ed_batch bh = nullptr;
+        check_status(alps_ed_batch_create(&cfg, &bh));
+        std::unique_ptr<alps_ed_batch_s, int (*)(alps_ed_batch)> hold(bh, alps_ed_batch_destroy);
+        std::vector<uint64_t> tickets(models.size());
+        for (size_t i = 0; i < models.size(); ++i) check_status(alps_ed_batch_submit(bh, &models[i], &sp, &tickets[i]));
+        check_status(alps_ed_batch_run(bh));
+        for (size_t i = 0; i < models.size(); ++i) {
+          int ready = 0;
+          check_status(alps_ed_batch_result(bh, tickets[i], &res[i], &ready));
+          if (!ready || res[i].error_code < 0) throw DeviceError("sector solve did not complete");
+        }
+      } else {
+        for (size_t i = 0; i < models.size(); ++i) check_status(alps_ed_sparsediag(&models[i], &sp, &res[i]));
+      }
+      for (size_t i = 0; i < models.size(); ++i) {
+        const alps_ed_solve_result& r = res[i];
+        const int n_up = ts.n_up[i];
         if (r.dim == 0) continue;  // empty subspaces are skipped (diag.h:115)
         std::vector<double> e(r.eigenvalues, r.eigenvalues + r.n_eigenvalues);
         evs.push_back(e);

@@ -345,11 +345,15 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1) {
     uint32_t p = pos(s);
     L.site_dn.push_back(m.sites[s].e_dn);
     L.site_up.push_back(m.sites[s].e_up);
+    // (locals and raw pointers: the loops vectorize; same sums as before)
+    const double up = m.sites[s].e_up, dn = m.sites[s].e_dn;
     if ((int)p < B) {
-      for (uint32_t v = 0; v < nlo; ++v) L.diag_lo[v] += (v >> p & 1u) ? m.sites[s].e_up : m.sites[s].e_dn;
+      double* dl = L.diag_lo.data();
+      for (uint32_t v = 0; v < nlo; ++v) dl[v] += (v >> p & 1u) ? up : dn;
     } else {
-      uint32_t q = p - B;
-      for (uint32_t h = 0; h < nhi; ++h) L.diag_hi[h] += (h >> q & 1u) ? m.sites[s].e_up : m.sites[s].e_dn;
+      const uint32_t q = p - B;
+      double* dh = L.diag_hi.data();
+      for (uint32_t h = 0; h < nhi; ++h) dh[h] += (h >> q & 1u) ? up : dn;
     }
   }
   for (const BondTerm& b : m.bonds) {
@@ -360,14 +364,15 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1) {
     L.bond_anti.push_back(b.e_anti);
     L.bond_t.push_back(b.t);
     bool lo1 = (int)p1 < B, lo2 = (int)p2 < B;
+    const double anti = b.e_anti, par = b.e_par;
     if (lo1 && lo2) {
-      for (uint32_t v = 0; v < nlo; ++v)
-        L.diag_lo[v] += ((v >> p1 ^ v >> p2) & 1u) ? b.e_anti : b.e_par;
+      double* dl = L.diag_lo.data();
+      for (uint32_t v = 0; v < nlo; ++v) dl[v] += ((v >> p1 ^ v >> p2) & 1u) ? anti : par;
       if (b.t != 0.0) L.ll.push_back({(1u << p1) | (1u << p2), 0u, b.t});
     } else if (!lo1 && !lo2) {
       uint32_t q1 = p1 - B, q2 = p2 - B;
-      for (uint32_t h = 0; h < nhi; ++h)
-        L.diag_hi[h] += ((h >> q1 ^ h >> q2) & 1u) ? b.e_anti : b.e_par;
+      double* dh = L.diag_hi.data();
+      for (uint32_t h = 0; h < nhi; ++h) dh[h] += ((h >> q1 ^ h >> q2) & 1u) ? anti : par;
       if (b.t != 0.0) L.hh.push_back({(1u << q1) | (1u << q2), 0u, b.t});
     } else {
       uint32_t ph = lo1 ? p2 : p1, pl = lo1 ? p1 : p2;

@@ -273,7 +273,9 @@ int alps_ed_batch_footprint(const alps_ed_model* model, const alps_ed_solve_para
  * CONSERVED_QUANTUMNUMBERS, Sz_total, MAX_ITERATIONS, NUMBER_EIGENVALUES,
  * plus ED_PRECISION = "f64" | "f32" and ED_SEED) as
  * DiagMatrix::dostep does (applications/diag/diag.h:85-175): one Lanczos
- * solve per Sz sector, then the ground-state energy and gap of
+ * solve per Sz sector (all sectors of a task advance together as one batch,
+ * see alps_ed_batch_*, when their footprint fits the free device memory;
+ * else one after the other), then the ground-state energy and gap of
  * SparseDiagMatrix::write_xml_body (sparsediag.h:59-90).  The results are
  * written to `out` as a JSON document (at most out_len bytes including the
  * terminating zero; *needed receives the full length). */
