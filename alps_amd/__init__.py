@@ -132,7 +132,8 @@ class _BatchStatus(ctypes.Structure):
                 ("hbm_bytes", ctypes.c_uint64), ("rounds", ctypes.c_uint64), ("steps", ctypes.c_uint64),
                 ("solves_done", ctypes.c_uint64), ("seconds_admit", ctypes.c_double),
                 ("seconds_steps", ctypes.c_double), ("seconds_decide", ctypes.c_double),
-                ("seconds_refine", ctypes.c_double)]
+                ("seconds_refine", ctypes.c_double), ("hv_rows_f32", ctypes.c_uint64),
+                ("hv_rows_f64", ctypes.c_uint64)]
 
 
 def _result_dict(r):

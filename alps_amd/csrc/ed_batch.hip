@@ -235,6 +235,7 @@ struct alps_ed_batch_s {
   std::vector<uint64_t> completed;
   uint64_t next_ticket = 1;
   uint64_t rounds = 0, steps = 0, solves_done = 0;
+  uint64_t hv_rows[2] = {0, 0};  // rows of Lanczos-step H.v, FP64 / FP32 storage
   double t_admit = 0, t_steps = 0, t_decide = 0, t_refine = 0;
   // device work lists (rebuilt every round)
   DevBuf jobs_d, items_d, list_d;
@@ -933,7 +934,11 @@ void round(alps_ed_batch b) {
   if (!run.empty()) {
     hip_check(hipStreamSynchronize(b->stream), "sync");
     b->steps += S;
-    for (Problem* p : run) p->j = std::min(p->max_iter, p->j + S);
+    for (Problem* p : run) {
+      const int j1 = std::min(p->max_iter, p->j + S);
+      b->hv_rows[p->fp32 ? 1 : 0] += p->D * (uint64_t)(j1 - p->j);
+      p->j = j1;
+    }
   }
   const double t5 = now_s();
   b->t_steps += t5 - t4;
@@ -1063,6 +1068,8 @@ int alps_ed_batch_stats(alps_ed_batch b, alps_ed_batch_status* s) {
     s->seconds_steps = b->t_steps;
     s->seconds_decide = b->t_decide;
     s->seconds_refine = b->t_refine;
+    s->hv_rows_f32 = b->hv_rows[1];
+    s->hv_rows_f64 = b->hv_rows[0];
   });
 }
 

@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE scaled by a copy kernel of
 # known size (profiles/hv_kernel_heisenberg_L24.md); valid for that kernel
 # version and L only.
-PMC_TRAFFIC_BYTES = {24: 51.89e6}
+PMC_TRAFFIC_BYTES = {24: 51.94e6}
 
 
 class HipEvents:
@@ -159,6 +159,7 @@ def batch_leg(n, seq_sample, seed=1):
         "mean_iterations": round(float(np.mean([r["iterations"] for r in res])), 1),
         "rounds": st["rounds"],
         "seconds_split": {k: round(st["seconds_" + k], 4) for k in ("admit", "steps", "decide", "refine")},
+        "hv_rows_f32": st["hv_rows_f32"],
         "workload": "random-bond XXZ chains, L uniform 12..20, J_i~U[0.5,1.5], Delta~U[0,2], open/periodic 1:1, "
                     "Sz=0 (Sz=1/2 for odd L), lowest eigenvalue, FP32 vectors + FP64 refinement",
     }

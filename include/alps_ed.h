@@ -246,6 +246,10 @@ typedef struct {
   /* wall seconds spent in: admission (layouts, uploads), the Lanczos steps
    * (launch to completion), the stopping-rule tests, refinement */
   double seconds_admit, seconds_steps, seconds_decide, seconds_refine;
+  /* rows processed by the Lanczos steps' H.v launches, by storage type
+   * (sum over steps of the dimension of each problem stepped): the
+   * algorithmic H.v bytes are 8 x hv_rows_f32 + 16 x hv_rows_f64 */
+  uint64_t hv_rows_f32, hv_rows_f64;
 } alps_ed_batch_status;
 
 int alps_ed_batch_create(const alps_ed_batch_config* cfg, alps_ed_batch* out);
