@@ -65,6 +65,8 @@ def lib():
             "alps_ed_vector_dot": ([P, P, P], i32),
             "alps_ed_vector_nrm2": ([P, P], i32),
             "alps_ed_matvec": ([P, P, P], i32),
+            "alps_ed_vector_spin_moments": ([P, P, P, P, P], i32),
+            "alps_ed_vector_flip_pairs": ([P, P, i32, P, P, P], i32),
             "alps_ed_lanczos_step_vectors": ([P, P, P, P, f64, P, P], i32),
             "alps_ed_lanczos_create": ([P, i32, i32, i32, P], i32),
             "alps_ed_lanczos_ritz": ([P, i32, i32, P, P], i32),
@@ -242,6 +244,23 @@ class Operator:
     def matvec(self, x: "Vector", y: "Vector"):
         _check(lib().alps_ed_matvec(self.handle, x.handle, y.handle))
         return y
+
+    def spin_moments(self, v: "Vector"):
+        """(sum v^2, sum v^2 Sz_i [N], sum v^2 Sz_i Sz_j [N, N]) on the device."""
+        N = self.model.n_sites
+        n2 = ctypes.c_double()
+        sz = np.zeros(N)
+        szsz = np.zeros(N * N)
+        _check(lib().alps_ed_vector_spin_moments(self.handle, v.handle, ctypes.byref(n2), _p(sz), _p(szsz)))
+        return n2.value, sz, szsz.reshape(N, N)
+
+    def flip_pairs(self, v: "Vector", site_i, site_j):
+        """<v| S+_i S-_j |v> for each pair (site_i[p], site_j[p])."""
+        a = np.ascontiguousarray(site_i, dtype=np.int32)
+        b = np.ascontiguousarray(site_j, dtype=np.int32)
+        out = np.zeros(max(len(a), 1))
+        _check(lib().alps_ed_vector_flip_pairs(self.handle, v.handle, len(a), _p(a), _p(b), _p(out)))
+        return out[: len(a)]
 
 
 class Vector:

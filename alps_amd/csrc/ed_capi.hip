@@ -17,6 +17,7 @@
 #include "alps_ed.h"
 #include "ed_host.hpp"
 #include "ed_kernels.hpp"
+#include "ed_measure.hpp"
 #include "ed_sparsediag.hpp"
 
 using namespace alps_ed;
@@ -494,6 +495,65 @@ int alps_ed_vector_nrm2(alps_ed_vector x, double* result) {
   });
 }
 
+int alps_ed_vector_spin_moments(alps_ed_operator op, alps_ed_vector v, double* norm2, double* sz, double* szsz) {
+  return guarded([&] {
+    require(op != nullptr, "operator is null");
+    check_vec(v);
+    require(v->n == op->layout.D, "vector size != operator dimension");
+    require(op->layout.D >= 1, "empty sector");
+    const DevLayout& L = op->dev;
+    const int N = L.N;
+    DevBuf stats((size_t)hv_blocks(L) * moment_stat_count(L) * sizeof(double));
+    DevBuf out((size_t)(1 + N + N * N) * sizeof(double));
+    hipStream_t st = current_stream();
+    with_dtype(v->dtype, [&](auto z) {
+      using V = decltype(z);
+      hip_check(launch_spin_moments<V>(L, v->buf.as<V>(), stats.as<double>(), out.as<double>(), st), "moments");
+    });
+    std::vector<double> h(1 + N + N * N);
+    hip_check(hipMemcpyAsync(h.data(), out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, st), "copy");
+    hip_check(hipStreamSynchronize(st), "sync");
+    if (norm2) *norm2 = h[0];
+    if (sz) std::copy(h.begin() + 1, h.begin() + 1 + N, sz);
+    if (szsz) std::copy(h.begin() + 1 + N, h.end(), szsz);
+  });
+}
+
+int alps_ed_vector_flip_pairs(alps_ed_operator op, alps_ed_vector v, int n_pairs, const int* site_i,
+                              const int* site_j, double* out) {
+  return guarded([&] {
+    require(op != nullptr, "operator is null");
+    check_vec(v);
+    require(v->n == op->layout.D, "vector size != operator dimension");
+    require(n_pairs >= 0 && (n_pairs == 0 || (site_i && site_j && out)), "null argument");
+    if (n_pairs == 0) return;
+    require(n_pairs <= (1 << 19), "at most 2^19 pairs per call");
+    require(op->layout.D >= 1, "empty sector");
+    const DevLayout& L = op->dev;
+    std::vector<uint32_t> masks(2 * (size_t)n_pairs);
+    for (int p = 0; p < n_pairs; ++p) {
+      require(site_i[p] >= 0 && site_i[p] < L.N && site_j[p] >= 0 && site_j[p] < L.N, "site out of range");
+      require(site_i[p] != site_j[p], "a flip pair needs two different sites");
+      masks[2 * p] = 1u << (L.N - 1 - site_j[p]);
+      masks[2 * p + 1] = 1u << (L.N - 1 - site_i[p]);
+    }
+    hipStream_t st = current_stream();
+    DevBuf dm(masks.size() * sizeof(uint32_t));
+    DevBuf part((size_t)n_pairs * flip_pair_blocks(L) * sizeof(double));
+    DevBuf dout((size_t)n_pairs * sizeof(double));
+    hip_check(hipMemcpyAsync(dm.p, masks.data(), masks.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st),
+              "copy");
+    with_dtype(v->dtype, [&](auto z) {
+      using V = decltype(z);
+      hip_check(launch_flip_pairs<V>(L, v->buf.as<V>(), dm.as<uint32_t>(), n_pairs, part.as<double>(),
+                                     dout.as<double>(), st),
+                "flip pairs");
+    });
+    hip_check(hipMemcpyAsync(out, dout.p, (size_t)n_pairs * sizeof(double), hipMemcpyDeviceToHost, st), "copy");
+    hip_check(hipStreamSynchronize(st), "sync");
+  });
+}
+
 int alps_ed_matvec(alps_ed_operator op, alps_ed_vector x, alps_ed_vector y) {
   return guarded([&] {
     require(op != nullptr, "operator is null");
@@ -732,48 +792,179 @@ void check_status(int st) {
   if (st != ALPS_ED_OK) throw DeviceError(last_error());
 }
 
-// Rayleigh quotients (FP64 accumulation) of the Ritz vectors of T[0..m)
-// eigenvalues with the given positions, from the stored FP32 basis.
-std::vector<double> refine_ritz(alps_ed_lanczos lz, const std::vector<double>& a, const std::vector<double>& b,
-                                int m, const std::vector<int>& index) {
+// Ritz vectors (FP64, device) of the T[0..m) eigenvalues at the given
+// positions, from the stored basis.
+std::vector<std::unique_ptr<alps_ed_vector_s>> ritz_vectors(alps_ed_lanczos lz, const std::vector<double>& a,
+                                                            const std::vector<double>& b, int m,
+                                                            const std::vector<int>& index) {
   std::vector<double> d(a.begin(), a.begin() + m), z((size_t)m * m);
   if (ietl::tridiag::eig_full(m, d.data(), b.data(), z.data()) > 0)
     throw std::runtime_error("LAPACK error, stev function failed.");
   const uint64_t n = lz->op->layout.D;
-  std::vector<double> rq;
+  std::vector<std::unique_ptr<alps_ed_vector_s>> out;
   for (size_t i0 = 0; i0 < index.size(); i0 += kMaxRitz) {
     const int nv = (int)std::min<size_t>(kMaxRitz, index.size() - i0);
     std::vector<double> coef((size_t)nv * m);
     for (int i = 0; i < nv; ++i)
       for (int j = 0; j < m; ++j) coef[(size_t)i * m + j] = z[j + (size_t)index[i0 + i] * m];
     std::vector<alps_ed_vector> ys(nv, nullptr);
-    std::vector<std::unique_ptr<alps_ed_vector_s>> hold;
     for (int i = 0; i < nv; ++i) {
       check_status(alps_ed_vector_create(n, ALPS_ED_F64, &ys[i]));
-      hold.emplace_back(ys[i]);
+      out.emplace_back(ys[i]);
     }
     check_status(alps_ed_lanczos_ritz(lz, nv, m, coef.data(), ys.data()));
-    alps_ed_vector hy = nullptr;
-    check_status(alps_ed_vector_create(n, ALPS_ED_F64, &hy));
-    std::unique_ptr<alps_ed_vector_s> hold_hy(hy);
-    for (int i = 0; i < nv; ++i) {
-      double* res = lz->red.result(0);
-      hip_check(launch_hv<double>(lz->op->dev, ys[i]->buf.as<double>(), hy->buf.as<double>(), lz->red.a(),
-                                  lz->stream),
-                "hv");
-      hip_check(launch_finish(Partials{lz->red.a(), hv_blocks(lz->op->dev)}, nullptr, res, nullptr, kSum, lz->stream),
-                "finish");
-      hip_check(launch_dot<double>(n, ys[i]->buf.as<double>(), ys[i]->buf.as<double>(), lz->red.b(), lz->stream),
-                "dot");
-      hip_check(launch_finish(Partials{lz->red.b(), stream_blocks(n)}, nullptr, res + 1, nullptr, kSum, lz->stream),
-                "finish");
-      double h[2];
-      hip_check(hipMemcpyAsync(h, res, 2 * sizeof(double), hipMemcpyDeviceToHost, lz->stream), "copy");
-      hip_check(hipStreamSynchronize(lz->stream), "sync");
-      rq.push_back(h[0] / h[1]);
+  }
+  return out;
+}
+
+// Rayleigh quotient y.Hy / y.y (FP64 accumulation) of a device FP64 vector.
+double rayleigh_quotient(alps_ed_lanczos lz, alps_ed_vector y, alps_ed_vector hy) {
+  const uint64_t n = lz->op->layout.D;
+  double* res = lz->red.result(0);
+  hip_check(launch_hv<double>(lz->op->dev, y->buf.as<double>(), hy->buf.as<double>(), lz->red.a(), lz->stream), "hv");
+  hip_check(launch_finish(Partials{lz->red.a(), hv_blocks(lz->op->dev)}, nullptr, res, nullptr, kSum, lz->stream),
+            "finish");
+  hip_check(launch_dot<double>(n, y->buf.as<double>(), y->buf.as<double>(), lz->red.b(), lz->stream), "dot");
+  hip_check(launch_finish(Partials{lz->red.b(), stream_blocks(n)}, nullptr, res + 1, nullptr, kSum, lz->stream),
+            "finish");
+  double h[2];
+  hip_check(hipMemcpyAsync(h, res, 2 * sizeof(double), hipMemcpyDeviceToHost, lz->stream), "copy");
+  hip_check(hipStreamSynchronize(lz->stream), "sync");
+  return h[0] / h[1];
+}
+
+// Rayleigh quotients of the Ritz vectors of T[0..m) eigenvalues with the
+// given positions, from the stored FP32 basis; the vectors are handed to
+// `keep` (in the same order) when it is set.
+std::vector<double> refine_ritz(alps_ed_lanczos lz, const std::vector<double>& a, const std::vector<double>& b,
+                                int m, const std::vector<int>& index,
+                                std::vector<std::unique_ptr<alps_ed_vector_s>>* keep) {
+  std::vector<std::unique_ptr<alps_ed_vector_s>> ys = ritz_vectors(lz, a, b, m, index);
+  alps_ed_vector hy = nullptr;
+  check_status(alps_ed_vector_create(lz->op->layout.D, ALPS_ED_F64, &hy));
+  std::unique_ptr<alps_ed_vector_s> hold_hy(hy);
+  std::vector<double> rq;
+  for (auto& y : ys) rq.push_back(rayleigh_quotient(lz, y.get(), hy));
+  if (keep)
+    for (auto& y : ys) keep->push_back(std::move(y));
+  return rq;
+}
+
+// One sector as SparseDiagMatrix::do_subspace solves it (sparsediag.h:93-149),
+// keeping the operator and, when `vectors` is set, the eigenvectors of the
+// reported eigenvalues (FP64 Ritz vectors from the kept basis, as
+// lanczos::eigenvectors provides them for the measurements, :125-144).
+struct SectorSolve {
+  std::unique_ptr<alps_ed_operator_s> op;
+  alps_ed_solve_result r{};
+  std::vector<std::unique_ptr<alps_ed_vector_s>> vectors;
+};
+
+SectorSolve solve_sector(const alps_ed_model* model, const alps_ed_solve_params* params, bool want_vectors) {
+  auto t0 = std::chrono::steady_clock::now();
+  SectorSolve out;
+  alps_ed_solve_result* result = &out.r;
+  std::memset(result, 0, sizeof(*result));
+  out.op.reset(new alps_ed_operator_s);
+  alps_ed_operator_s* op = out.op.get();
+  op->layout = build_layout(model_spec(model));
+  const uint64_t D = op->layout.D;
+  result->dim = D;
+  auto seconds = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+  if (D == 0) return out;  // sparsediag.h:101-102
+  operator_upload(op);
+  if (D == 1) {  // sparsediag.h:120-124: the single matrix element
+    std::vector<uint32_t> cols(op->layout.bond_p1.size() + 1);
+    std::vector<double> vals(cols.size());
+    int cnt = 0;
+    check_status(alps_ed_operator_elements(op, 0, 1, (int)cols.size(), cols.data(), vals.data(), &cnt));
+    double v = 0.0;
+    for (int q = 0; q < cnt; ++q)
+      if (cols[q] == 0) v = vals[q];
+    result->n_eigenvalues = 1;
+    result->eigenvalues[0] = v;
+    if (want_vectors) {
+      alps_ed_vector one = nullptr;
+      check_status(alps_ed_vector_create(1, ALPS_ED_F64, &one));
+      out.vectors.emplace_back(one);
+      check_status(alps_ed_vector_fill(one, 1.0));
+    }
+    result->seconds = seconds();
+    return out;
+  }
+  const bool fp32 = use_fp32(params->dtype == ALPS_ED_F32, D, params->nev > 0 ? params->nev : 1);
+  const int max_iter = params->max_iter > 0 ? params->max_iter : (int)std::min<uint64_t>(10 * D, 1000);
+  const int nev = params->nev > 0 ? params->nev : 1;
+  require(nev <= 64, "nev must be <= 64");
+  alps_ed_lanczos lzh = nullptr;
+  check_status(
+      alps_ed_lanczos_create(op, fp32 ? ALPS_ED_F32 : ALPS_ED_F64, max_iter, (fp32 || want_vectors) ? 1 : 0, &lzh));
+  std::unique_ptr<alps_ed_lanczos_s> lz(lzh);
+  check_status(alps_ed_lanczos_start(lz.get(), params->seed, nullptr));
+  std::vector<double> a(max_iter + 1), b(max_iter + 1);
+  int n = 1, checked = 1, final_n = -1, error_code = 0;
+  // The stopping rule is tested on chunk boundaries; when it holds, the
+  // first prefix length at which it holds is found by scanning forward, so
+  // the T matrix is the one the per-step test of the reference would stop at.
+  while (final_n < 0) {
+    const int target =
+        params->fixed_iterations ? max_iter : std::min(max_iter, n + std::max(1, std::min(n / 8, 64)));
+    if (target > n) {
+      check_status(alps_ed_lanczos_steps(lz.get(), target - n));
+      n = target;
+    }
+    int got = 0;
+    check_status(alps_ed_lanczos_coefficients(lz.get(), a.data(), b.data(), n, &got));
+    if (!params->fixed_iterations && n >= 2 && nlowest_converged(analyze_prefix(a, b, n, fp32), nev, fp32)) {
+      for (int m = std::max(2, checked + 1); m <= n; ++m)
+        if (m == n || nlowest_converged(analyze_prefix(a, b, m, fp32), nev, fp32)) {
+          final_n = m;
+          break;
+        }
+    } else if (n >= max_iter) {
+      final_n = n;
+      error_code = 1;  // iteration.h:80
+    }
+    checked = n;
+  }
+  TValues t = analyze_prefix(a, b, final_n, fp32);
+  std::vector<double> ev;
+  if (!fp32) {
+    ev.assign(t.ev.begin(), t.ev.begin() + std::min<size_t>(nev, t.ev.size()));
+    if (want_vectors)
+      out.vectors = ritz_vectors(lz.get(), a, b, final_n, std::vector<int>(t.index.begin(), t.index.begin() + ev.size()));
+  } else {
+    // Rayleigh quotients of the Ritz vectors; values that agree to 1e-9
+    // relative are the same level reached twice and are reported once.
+    size_t next = 0;
+    std::vector<std::pair<double, std::unique_ptr<alps_ed_vector_s>>> kept;
+    while ((int)kept.size() < nev && next < t.ev.size()) {
+      const size_t cnt = std::min<size_t>(t.ev.size() - next, (size_t)(nev - (int)kept.size()));
+      std::vector<int> idx(t.index.begin() + next, t.index.begin() + next + cnt);
+      std::vector<std::unique_ptr<alps_ed_vector_s>> vs;
+      std::vector<double> r = refine_ritz(lz.get(), a, b, final_n, idx, want_vectors ? &vs : nullptr);
+      next += cnt;
+      for (size_t i = 0; i < r.size(); ++i) {
+        bool dup = false;
+        for (const auto& e : kept)
+          if (std::fabs(r[i] - e.first) <= 1e-9 * std::max(1.0, std::fabs(e.first))) dup = true;
+        if (!dup) kept.emplace_back(r[i], want_vectors ? std::move(vs[i]) : nullptr);
+      }
+      std::sort(kept.begin(), kept.end(),
+                [](const std::pair<double, std::unique_ptr<alps_ed_vector_s>>& x,
+                   const std::pair<double, std::unique_ptr<alps_ed_vector_s>>& y) { return x.first < y.first; });
+    }
+    for (auto& e : kept) {
+      ev.push_back(e.first);
+      if (want_vectors) out.vectors.push_back(std::move(e.second));
     }
   }
-  return rq;
+  result->n_eigenvalues = (int)ev.size();
+  for (size_t i = 0; i < ev.size(); ++i) result->eigenvalues[i] = ev[i];
+  result->iterations = final_n;
+  result->error_code = error_code;
+  result->seconds = seconds();
+  return out;
 }
 
 }  // namespace
@@ -782,110 +973,12 @@ extern "C" int alps_ed_sparsediag(const alps_ed_model* model, const alps_ed_solv
                                   alps_ed_solve_result* result) {
   return guarded([&] {
     require(params && result, "null argument");
-    auto t0 = std::chrono::steady_clock::now();
-    std::memset(result, 0, sizeof(*result));
-    std::unique_ptr<alps_ed_operator_s> op(new alps_ed_operator_s);
-    op->layout = build_layout(model_spec(model));
-    const uint64_t D = op->layout.D;
-    result->dim = D;
-    if (D == 0) return;  // sparsediag.h:101-102
-    operator_upload(op.get());
-    if (D == 1) {  // sparsediag.h:120-124: the single matrix element
-      std::vector<uint32_t> cols(op->layout.bond_p1.size() + 1);
-      std::vector<double> vals(cols.size());
-      int cnt = 0;
-      check_status(alps_ed_operator_elements(op.get(), 0, 1, (int)cols.size(), cols.data(), vals.data(), &cnt));
-      double v = 0.0;
-      for (int q = 0; q < cnt; ++q)
-        if (cols[q] == 0) v = vals[q];
-      result->n_eigenvalues = 1;
-      result->eigenvalues[0] = v;
-      result->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-      return;
-    }
-    const bool fp32 = use_fp32(params->dtype == ALPS_ED_F32, D, params->nev > 0 ? params->nev : 1);
-    const int max_iter = params->max_iter > 0 ? params->max_iter : (int)std::min<uint64_t>(10 * D, 1000);
-    const int nev = params->nev > 0 ? params->nev : 1;
-    require(nev <= 64, "nev must be <= 64");
-    alps_ed_lanczos lzh = nullptr;
-    check_status(alps_ed_lanczos_create(op.get(), fp32 ? ALPS_ED_F32 : ALPS_ED_F64, max_iter, fp32 ? 1 : 0, &lzh));
-    std::unique_ptr<alps_ed_lanczos_s> lz(lzh);
-    check_status(alps_ed_lanczos_start(lz.get(), params->seed, nullptr));
-    std::vector<double> a(max_iter + 1), b(max_iter + 1);
-    int n = 1, checked = 1, final_n = -1, error_code = 0;
-    // The stopping rule is tested on chunk boundaries; when it holds, the
-    // first prefix length at which it holds is found by scanning forward, so
-    // the T matrix is the one the per-step test of the reference would stop at.
-    while (final_n < 0) {
-      const int target =
-          params->fixed_iterations ? max_iter : std::min(max_iter, n + std::max(1, std::min(n / 8, 64)));
-      if (target > n) {
-        check_status(alps_ed_lanczos_steps(lz.get(), target - n));
-        n = target;
-      }
-      int got = 0;
-      check_status(alps_ed_lanczos_coefficients(lz.get(), a.data(), b.data(), n, &got));
-      if (!params->fixed_iterations && n >= 2 && nlowest_converged(analyze_prefix(a, b, n, fp32), nev, fp32)) {
-        for (int m = std::max(2, checked + 1); m <= n; ++m)
-          if (m == n || nlowest_converged(analyze_prefix(a, b, m, fp32), nev, fp32)) {
-            final_n = m;
-            break;
-          }
-      } else if (n >= max_iter) {
-        final_n = n;
-        error_code = 1;  // iteration.h:80
-      }
-      checked = n;
-    }
-    TValues t = analyze_prefix(a, b, final_n, fp32);
-    std::vector<double> ev;
-    if (!fp32) {
-      ev.assign(t.ev.begin(), t.ev.begin() + std::min<size_t>(nev, t.ev.size()));
-    } else {
-      // Rayleigh quotients of the Ritz vectors; values that agree to 1e-9
-      // relative are the same level reached twice and are reported once.
-      size_t next = 0;
-      std::vector<double> rq;
-      while ((int)ev.size() < nev && next < t.ev.size()) {
-        const size_t cnt = std::min<size_t>(t.ev.size() - next, (size_t)(nev - (int)ev.size()));
-        std::vector<int> idx(t.index.begin() + next, t.index.begin() + next + cnt);
-        std::vector<double> r = refine_ritz(lz.get(), a, b, final_n, idx);
-        next += cnt;
-        for (double v : r) {
-          bool dup = false;
-          for (double e : ev)
-            if (std::fabs(v - e) <= 1e-9 * std::max(1.0, std::fabs(e))) dup = true;
-          if (!dup) ev.push_back(v);
-        }
-        std::sort(ev.begin(), ev.end());
-      }
-    }
-    result->n_eigenvalues = (int)ev.size();
-    for (size_t i = 0; i < ev.size(); ++i) result->eigenvalues[i] = ev[i];
-    result->iterations = final_n;
-    result->error_code = error_code;
-    result->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    SectorSolve s = solve_sector(model, params, false);
+    *result = s.r;
   });
 }
 
 namespace {
-
-std::string json_number(double v) {
-  if (std::isnan(v) || std::isinf(v)) return "null";
-  char b[40];
-  std::snprintf(b, sizeof b, "%.17g", v);
-  return b;
-}
-
-std::string json_string(const std::string& v) {
-  std::string o = "\"";
-  for (char c : v) {
-    if (c == '"' || c == '\\') o += '\\';
-    if ((unsigned char)c < 0x20) continue;
-    o += c;
-  }
-  return o + "\"";
-}
 
 std::string json_list(const std::vector<double>& v) {
   std::string o = "[";
@@ -901,6 +994,57 @@ std::string json_params(const Parameters& p) {
     first = false;
   }
   return js + "}";
+}
+
+// Device sums of one eigenvector for the measurements.
+SpinSums spin_sums(alps_ed_operator op, alps_ed_vector v, const std::vector<std::pair<int, int>>& flips) {
+  SpinSums s;
+  const int N = op->layout.N;
+  s.sz.resize(N);
+  s.szsz.resize((size_t)N * N);
+  check_status(alps_ed_vector_spin_moments(op, v, &s.n2, s.sz.data(), s.szsz.data()));
+  if (!flips.empty()) {
+    std::vector<int> a, b;
+    for (const auto& f : flips) {
+      a.push_back(f.first);
+      b.push_back(f.second);
+    }
+    std::vector<double> f(flips.size());
+    check_status(alps_ed_vector_flip_pairs(op, v, (int)flips.size(), a.data(), b.data(), f.data()));
+    for (size_t i = 0; i < flips.size(); ++i) s.flip[flips[i]] = f[i];
+  }
+  return s;
+}
+
+// What a task measures, resolved on the host: operator pairs, labels and
+// the unsupported requests.
+std::string json_measure_plan(const MeasureSpec& spec, const LatticeGraph& g) {
+  const MeasurePlan plan = measure_plan(spec, g);
+  auto names = [](const std::map<std::string, std::string>& m) {
+    std::string o = "{";
+    for (const auto& kv : m) o += (o.size() > 1 ? ", " : "") + json_string(kv.first) + ": " + json_string(kv.second);
+    return o + "}";
+  };
+  auto pairs = [](const std::map<std::string, std::pair<std::string, std::string>>& m) {
+    std::string o = "{";
+    for (const auto& kv : m)
+      o += (o.size() > 1 ? ", " : "") + json_string(kv.first) + ": [" + json_string(kv.second.first) + ", " +
+           json_string(kv.second.second) + "]";
+    return o + "}";
+  };
+  std::vector<std::string> dl, kl;
+  for (const auto& pr : distance_pairs(g))
+    dl.push_back(pr.first < 0 ? std::string()
+                              : coordinate_string(site_coordinate(g, pr.first)) + " -- " +
+                                    coordinate_string(site_coordinate(g, pr.second)));
+  for (const auto& k : momenta(g)) kl.push_back(coordinate_string(k));
+  std::string un = "[";
+  for (size_t i = 0; i < plan.unsupported.size(); ++i) un += (i ? ", " : "") + plan.unsupported[i];
+  return "{\"average\": " + names(spec.average) + ", \"local\": " + names(spec.local) +
+         ", \"correlations\": " + pairs(spec.correlation) + ", \"structure_factor\": " +
+         pairs(spec.structure_factor) + ", \"flip_pairs\": " + std::to_string(plan.flips.size()) +
+         ", \"distance_labels\": " + json_labels(dl) + ", \"momentum_labels\": " + json_labels(kl) +
+         ", \"unsupported\": " + un + "]}";
 }
 
 void emit(const std::string& js, char* out, size_t out_len, size_t* needed) {
@@ -932,6 +1076,8 @@ extern "C" int alps_ed_describe_parameter_file(const char* text, char* out, size
               ", \"bond_t\": " + json_list(s2) + ", \"jz\": " + json_list(m.jz) + ", \"jxy\": " + json_list(m.jxy) +
               ", \"k\": " + json_list(m.k) + ", \"h\": " + json_list(m.h) + ", \"d\": " + json_list(m.d) +
               ", \"n_up\": " + json_list(up) + ", \"whole_space\": " + (ts.whole_space ? "true" : "false");
+        const MeasureSpec ms = measure_spec(p);
+        if (ms.any()) js += ", \"measure\": " + json_measure_plan(ms, m.graph);
       }
       js += "}";
     }
@@ -971,6 +1117,12 @@ extern "C" int alps_ed_run_parameter_file(const char* text, char* out, size_t ou
         models.push_back(alps_ed_model{m.n_sites, n_up, (int)m.s1.size(), m.s1.data(), m.s2.data(), m.jz.data(),
                                        m.jxy.data(), m.k.data(), m.h.data(), m.d.data()});
       std::vector<alps_ed_solve_result> res(models.size());
+      // Eigenvector measurements (DiagMatrix::perform_measurements,
+      // diag.h:191-314) need the eigenvectors of every sector: those tasks
+      // solve their sectors one by one with the Lanczos basis kept.
+      const MeasureSpec ms = measure_spec(p);
+      const bool measure = ms.any() && !ts.whole_space;
+      std::vector<std::string> meas_json(models.size());
       uint64_t need = 0;
       for (const alps_ed_model& am : models) {
         uint64_t f = 0;
@@ -979,7 +1131,17 @@ extern "C" int alps_ed_run_parameter_file(const char* text, char* out, size_t ou
       }
       size_t free_b = 0, total_b = 0;
       hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-      if (models.size() > 1 && need < 0.8 * (double)free_b) {
+      if (measure) {
+        const MeasurePlan plan = measure_plan(ms, m.graph);
+        for (size_t i = 0; i < models.size(); ++i) {
+          SectorSolve sv = solve_sector(&models[i], &sp, true);
+          res[i] = sv.r;
+          if (sv.r.dim == 0) continue;
+          std::vector<SpinSums> sums;
+          for (auto& v : sv.vectors) sums.push_back(spin_sums(sv.op.get(), v.get(), plan.flips));
+          meas_json[i] = measurements_json(ms, m.graph, plan, sums);
+        }
+      } else if (models.size() > 1 && need < 0.8 * (double)free_b) {
         alps_ed_batch_config cfg{sp.dtype, need, 0, 8, 0, 0};
         alps_ed_batch bh = nullptr;
         check_status(alps_ed_batch_create(&cfg, &bh));
@@ -1008,7 +1170,8 @@ extern "C" int alps_ed_run_parameter_file(const char* text, char* out, size_t ou
         entries.push_back("\"Sz_total\": " + json_number(n_up - 0.5 * m.n_sites) + ", \"n_up\": " +
                           std::to_string(n_up) + ", \"dim\": " + std::to_string(r.dim) + ", \"iterations\": " +
                           std::to_string(r.iterations) + ", \"error_code\": " + std::to_string(r.error_code) +
-                          ", \"seconds\": " + json_number(r.seconds) + ", \"eigenvalues\": " + json_list(e));
+                          ", \"seconds\": " + json_number(r.seconds) + ", \"eigenvalues\": " + json_list(e) +
+                          (measure ? ", \"measurements\": " + meas_json[i] : std::string()));
       }
       if (ts.whole_space) {
         // one subspace without quantum numbers: the merged spectrum
@@ -1022,7 +1185,11 @@ extern "C" int alps_ed_run_parameter_file(const char* text, char* out, size_t ou
       for (size_t i = 0; i < entries.size(); ++i) js += (i ? ", {" : "{") + entries[i] + "}";
       double g = 0, gap = 0;
       ground_and_gap(evs, g, gap);
-      js += "], \"ground_state_energy\": " + json_number(g) + ", \"energy_gap\": " + json_number(gap) + "}";
+      js += "], \"ground_state_energy\": " + json_number(g) + ", \"energy_gap\": " + json_number(gap);
+      if (ms.any() && ts.whole_space)
+        js += ", \"measurement_error\": \"eigenvector measurements need Sz-conserved sectors "
+              "(CONSERVED_QUANTUMNUMBERS=\\\"Sz\\\" or Sz_total)\"";
+      js += "}";
     }
     js += "]}";
     emit(js, out, out_len, needed);

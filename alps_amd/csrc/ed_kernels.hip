@@ -221,29 +221,6 @@ __global__ __launch_bounds__(kBlock) void ritz_kernel(uint64_t n, const V* const
 // ---------------------------------------------------------------------------
 // Basis: unrank / rank / explicit elements
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t unrank(const DevLayout& L, uint32_t i) {
-  uint32_t lo = 0, hi = L.n_seg - 1;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi + 1) >> 1;
-    if (L.seg_start[mid] <= i) lo = mid;
-    else hi = mid - 1;
-  }
-  const uint32_t h = L.seg_hi[lo];
-  const uint32_t r = i - L.seg_start[lo];
-  const int kl = L.k - __builtin_popcount(h);
-  return (h << L.B) | L.lo_state[L.lo_class_off[kl] + r];
-}
-
-__device__ __forceinline__ uint32_t rank_of(const DevLayout& L, uint32_t s) {
-  if (__builtin_popcount(s) != L.k) return L.D;
-  if (L.N < 32 && (s >> L.N) != 0u) return L.D;
-  const uint32_t h = L.NH > 0 ? (s >> L.B) : 0u;
-  const uint32_t lo = L.B >= 32 ? s : (s & ((1u << L.B) - 1u));
-  const uint32_t hb = L.hi_base[h];
-  if (hb == 0xffffffffu) return L.D;
-  return hb + L.lo_rank[lo];
-}
-
 __global__ __launch_bounds__(kBlock) void unrank_kernel(DevLayout L, uint64_t first, uint64_t count,
                                                         uint32_t* out) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;

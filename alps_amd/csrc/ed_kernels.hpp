@@ -143,6 +143,20 @@ template <class V>
 hipError_t launch_ritz(uint64_t n, const V* const* slots, const double* scale, int m, int nvec,
                        const double* coef, double* const* outs, hipStream_t st);
 
+// Diagonal spin moments of v (ed_measure.hip): out[0] = sum v_r^2,
+// out[1 + i] = sum v_r^2 Sz_i(r), out[1 + N + i*N + j] = sum v_r^2 Sz_i Sz_j;
+// `stats` holds n_seg * moment_stat_count(L) doubles of scratch.
+int moment_stat_count(const DevLayout& L);
+template <class V>
+hipError_t launch_spin_moments(const DevLayout& L, const V* v, double* stats, double* out, hipStream_t st);
+// Flip amplitudes out[p] = <v| S+_i S-_j |v> for masks[2p] = bit of site j,
+// masks[2p+1] = bit of site i (i != j); `part` holds
+// n_pairs * flip_pair_blocks(L) doubles of scratch.
+uint32_t flip_pair_blocks(const DevLayout& L);
+template <class V>
+hipError_t launch_flip_pairs(const DevLayout& L, const V* v, const uint32_t* masks, int n_pairs, double* part,
+                             double* out, hipStream_t st);
+
 hipError_t launch_unrank(const DevLayout& L, uint64_t first, uint64_t count, uint32_t* out, hipStream_t st);
 hipError_t launch_rank(const DevLayout& L, const uint32_t* states, uint64_t count, uint64_t* out,
                        hipStream_t st);

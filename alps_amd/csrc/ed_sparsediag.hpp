@@ -35,6 +35,12 @@ struct LatticeGraph {
   int n = 0;
   std::vector<int> site_type;
   std::vector<LatticeBond> bonds;
+  // geometry of the finite lattice (one vertex per cell): extents and
+  // boundaries per dimension, each site's cell offset, the basis vectors
+  std::vector<int> extent;
+  std::vector<bool> periodic;
+  std::vector<std::vector<int>> offset;
+  std::vector<std::vector<double>> basis;
 };
 
 // A finite lattice: extents, boundary per dimension, one-vertex unit cell
@@ -55,11 +61,16 @@ inline LatticeGraph hypercubic_graph(const std::vector<int>& extent, const std::
   }
   g.n = volume;
   g.site_type.assign(volume, 0);
+  g.extent = extent;
+  g.periodic = periodic;
+  g.basis.assign(dim, std::vector<double>(dim, 0.0));
+  for (int d = 0; d < dim; ++d) g.basis[d][d] = 1.0;
   std::vector<int> cell(dim, 0);
   for (int c = 0; c < volume; ++c) {
     // cells in odometer order, last dimension fastest; index row-major
     int ci = 0;
     for (int d = 0; d < dim; ++d) ci = ci * extent[d] + cell[d];
+    g.offset.push_back(cell);
     for (const CellEdge& e : edges) {
       int ti = 0;
       bool inside = true;
@@ -83,6 +94,8 @@ inline LatticeGraph hypercubic_graph(const std::vector<int>& extent, const std::
 
 // The LATTICEGRAPHs of lib/xml/lattices.xml.in with one-vertex unit cells.
 inline LatticeGraph named_lattice_graph(const std::string& name, const Evaluator& ev) {
+  if (name == "dimer")  // an open chain of two sites (lattices.xml.in:295-303)
+    return hypercubic_graph({2}, {false}, {{0, {1}}});
   const int L = (int)std::lround(ev.param("L"));
   auto get = [&](const char* p, int def) { return ev.defined(p) ? (int)std::lround(ev.param(p)) : def; };
   std::vector<CellEdge> e;
@@ -114,7 +127,10 @@ inline LatticeGraph named_lattice_graph(const std::string& name, const Evaluator
   if (name == "triangular lattice" || name == "anisotropic triangular lattice") {
     const int W = get("W", L);
     const bool an = name != "triangular lattice";
-    return hypercubic_graph({L, W}, {true, true}, {{0, {0, 1}}, {an ? 1 : 0, {1, 0}}, {an ? 2 : 0, {-1, 1}}});
+    LatticeGraph g =
+        hypercubic_graph({L, W}, {true, true}, {{0, {0, 1}}, {an ? 1 : 0, {1, 0}}, {an ? 2 : 0, {-1, 1}}});
+    g.basis = {{1.0, 0.0}, {0.5, std::sqrt(3.0) / 2.0}};  // lattices.xml.in:45-49
+    return g;
   }
   if (name == "simple cubic lattice" || name == "open simple cubic lattice") {
     const int W = get("W", L);
@@ -160,6 +176,7 @@ struct TaskModel {
   int n_sites = 0;
   std::vector<int> s1, s2;
   std::vector<double> jz, jxy, k, h, d;
+  LatticeGraph graph;
 };
 
 inline TaskModel task_model(const Parameters& in) {
@@ -174,6 +191,7 @@ inline TaskModel task_model(const Parameters& in) {
   Evaluator ev(p);
   TaskModel m;
   m.n_sites = g.n;
+  m.graph = g;
   for (const LatticeBond& b : g.bonds) {
     const std::string t = std::to_string(b.type);
     m.s1.push_back(b.s);

@@ -131,6 +131,21 @@ int alps_ed_vector_nrm2(alps_ed_vector x, double* result);
 /* ietl::mult(m, x, y) -> y = prod(m, x) (interface/ublas.h:57-63). */
 int alps_ed_matvec(alps_ed_operator op, alps_ed_vector x, alps_ed_vector y);
 
+/* Spin expectation sums of a vector v of the operator's sector, the parts
+ * from which SparseDiagMatrix::calculate (sparsediag.h:151-159) evaluates
+ * <v|O|v> for the measurements of DiagMatrix::perform_measurements
+ * (applications/diag/diag.h:191-314).  With w_r = v_r^2 and Sz_i(r) = +-1/2:
+ *   *norm2 = sum_r w_r;  sz[i] = sum_r w_r Sz_i(r);
+ *   szsz[i*n_sites + j] = sum_r w_r Sz_i(r) Sz_j(r)   (i, j model sites).
+ * Any output may be null.  Sums are FP64 and deterministic; divide by
+ * *norm2 for expectation values of an unnormalised v. */
+int alps_ed_vector_spin_moments(alps_ed_operator op, alps_ed_vector v, double* norm2, double* sz, double* szsz);
+/* out[p] = <v| S+_{site_i[p]} S-_{site_j[p]} |v> = sum over the rows r with
+ * site j up and site i down of v_r v_r', r' the row with both turned
+ * (site_i[p] != site_j[p]).  <v|S-_i S+_j|v> is the pair (j, i). */
+int alps_ed_vector_flip_pairs(alps_ed_operator op, alps_ed_vector v, int n_pairs, const int* site_i,
+                              const int* site_j, double* out);
+
 /* One ietl::lanczos::make_step on caller vectors (lanczos.h:392-406), fused:
  * w = H v_cur; alpha = <v_cur,w>; w -= alpha v_cur + beta_prev v_prev;
  * beta = |w|; w /= beta.  v_prev and v_cur are not modified.  Used by the

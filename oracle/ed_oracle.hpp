@@ -388,6 +388,79 @@ inline void spmv_csr(const CSR& c, const double* x, double* y, size_t nrows) {
 }
 
 // ---------------------------------------------------------------------------
+// Eigenvector measurements (applications/diag/diag.h:191-314,
+// sparsediag.h:151-159): <v| O |v> = inner_prod(v, prod(M, v)) with M the
+// operator matrix of a site term A(i), a same-site product A(i)*B(i) or a
+// two-site term A(i)*B(j), built as the Hamiltonian is (basis lookup of
+// every target state, matrix_as_vector's M(row, target) += element,
+// hamiltonian_matrix.hpp:269-345).
+// ---------------------------------------------------------------------------
+// Local 2x2 matrices m[out][in] = <out|A|in> of the spin-1/2 site operators
+// of models.xml.in:24-36 (Splus, Sminus, Sz) and :283-285 (Sx =
+// (Splus+Sminus)/2); index 0 is Sz=-1/2.
+enum SiteOp { kOpSz = 0, kOpSplus = 1, kOpSminus = 2, kOpSx = 3 };
+
+inline void site_op_matrix(int op, double m[2][2]) {
+  m[0][0] = m[0][1] = m[1][0] = m[1][1] = 0.0;
+  if (op == kOpSz) {
+    m[0][0] = -0.5;
+    m[1][1] = 0.5;
+  } else if (op == kOpSplus) {
+    m[1][0] = 1.0;  // sqrt(S(S+1) - Sz(Sz+1)) at Sz = -1/2
+  } else if (op == kOpSminus) {
+    m[0][1] = 1.0;
+  } else if (op == kOpSx) {
+    m[1][0] = 0.5;
+    m[0][1] = 0.5;
+  } else {
+    throw std::invalid_argument("unknown site operator");
+  }
+}
+
+// <v|O|v> for O = A(i) (j < 0), A(i)*B(i) (j == i, B applied first) or
+// A(i)*B(j) (j != i).
+inline double expect_site_term(const std::vector<State>& states, const double* v, int opA, int i, int opB,
+                               int j) {
+  const size_t D = states.size();
+  double a[2][2], b[2][2];
+  site_op_matrix(opA, a);
+  if (j >= 0) site_op_matrix(opB, b);
+  MapMatrix M;
+  for (size_t r = 0; r < D; ++r) {
+    const State& st = states[r];
+    if (j < 0 || j == i) {
+      double m[2][2];
+      for (int o = 0; o < 2; ++o)
+        for (int n = 0; n < 2; ++n) m[o][n] = j < 0 ? a[o][n] : a[o][0] * b[0][n] + a[o][1] * b[1][n];
+      for (int js = 0; js < 2; ++js) {
+        const double val = m[js][st[i]];
+        if (val == 0.0) continue;
+        State ns = st;
+        ns[i] = (short)js;
+        const size_t c = basis_index(states, ns);
+        if (c < D) M[(uint32_t)r][(uint32_t)c] += val;
+      }
+    } else {
+      for (int ji = 0; ji < 2; ++ji)
+        for (int jj = 0; jj < 2; ++jj) {
+          const double val = a[ji][st[i]] * b[jj][st[j]];
+          if (val == 0.0) continue;
+          State ns = st;
+          ns[i] = (short)ji;
+          ns[j] = (short)jj;
+          const size_t c = basis_index(states, ns);
+          if (c < D) M[(uint32_t)r][(uint32_t)c] += val;
+        }
+    }
+  }
+  std::vector<double> y(D);
+  spmv(M, v, y.data(), D);
+  double t = 0.0;
+  for (size_t r = 0; r < D; ++r) t += v[r] * y[r];
+  return t;
+}
+
+// ---------------------------------------------------------------------------
 // Start vector: counter-based splitmix64, uniform on [lo, hi)
 // ---------------------------------------------------------------------------
 // value_i = lo + (hi-lo) * u_i,  u_i = (mix(seed + (i+1)*G) >> 11) * 2^-53,

@@ -215,3 +215,17 @@ double oracle_bench_hv(int N, int n_up, int n_bonds, const int* bs, const int* b
 }
 
 }  // extern "C"
+
+// <v| O |v> of a site term on the fixed-Sz basis of N sites with n_up up
+// spins (v in basis order): O = A(i) when j < 0, A(i)*B(i) when j == i,
+// A(i)*B(j) otherwise; operators 0 Sz, 1 Splus, 2 Sminus, 3 Sx.
+extern "C" int oracle_expect_site_term(int N, int n_up, const double* v, int opA, int i, int opB, int j,
+                                       double* out) {
+  try {
+    std::vector<edo::State> st = edo::build_basis(N, n_up);
+    *out = edo::expect_site_term(st, v, opA, i, opB, j);
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}

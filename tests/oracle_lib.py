@@ -45,6 +45,8 @@ def lib():
         L.oracle_bench_hv.restype = f64
         L.oracle_ground_and_gap.argtypes = [i32, P, P, P, P]
         L.oracle_ground_and_gap.restype = i32
+        L.oracle_expect_site_term.argtypes = [i32, i32, P, i32, i32, i32, i32, P]
+        L.oracle_expect_site_term.restype = i32
         _lib = L
     return _lib
 
@@ -211,3 +213,17 @@ def bench_hv(m: Model, n_up, rows, reps=3):
     nnz = ctypes.c_long()
     t = lib().oracle_bench_hv(m.n_sites, n_up, *m.cargs(), rows, reps, ctypes.byref(bs), ctypes.byref(nnz))
     return dict(seconds=t, build_seconds=bs.value, nnz=nnz.value)
+
+
+SITE_OPS = {"Sz": 0, "Splus": 1, "Sminus": 2, "Sx": 3}
+
+
+def expect_site_term(N, n_up, v, a, i, b=None, j=-1):
+    """<v|O|v> by the oracle's operator-matrix restatement: O = a(i) when
+    j < 0, a(i)*b(i) when j == i, a(i)*b(j) otherwise (names of SITE_OPS)."""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = ctypes.c_double()
+    rc = lib().oracle_expect_site_term(N, n_up, _p(v), SITE_OPS[a], i, SITE_OPS[b] if b else 0, j,
+                                       ctypes.byref(out))
+    assert rc == 0
+    return out.value

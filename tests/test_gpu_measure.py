@@ -1,0 +1,166 @@
+"""GPU parity of the eigenvector measurements (DiagMatrix::perform_measurements,
+applications/diag/diag.h:191-314): the device spin sums through the C ABI
+against the oracle's operator-matrix restatement (expect_site_term), and the
+parameter-file measurements against the oracle on dense eigenvectors, with
+size-independent sum rules at L=24.
+
+Bars: spin sums of a given vector to 1e-12 relative (FP64 accumulation of
+the same stored values); measurements of Lanczos eigenvectors to 1e-8 (FP64
+vectors) and 1e-6 (FP32-stored basis, FP64 Ritz vectors) absolute."""
+import sys
+
+import numpy as np
+import pytest
+
+import alps_amd as A
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def to_product(m: O.Model, n_up) -> A.SpinModel:
+    return A.SpinModel(m.n_sites, m.bond_s, m.bond_t, m.jz, m.jxy, n_up, k=m.k, h=m.h, d=m.d)
+
+
+def _open_xxz(L, seed):
+    rng = np.random.default_rng(seed)
+    s = np.arange(L - 1)
+    J = rng.uniform(0.5, 1.5, L - 1)
+    return O.Model(L, s, s + 1, rng.uniform(0, 2) * J, J, h=rng.uniform(0, 0.5))
+
+
+CASES = {
+    "chain10": (lambda: O.lattice_model("chain lattice", 10), 5),
+    "chain16": (lambda: O.lattice_model("chain lattice", 16), 8),
+    "ladder7": (lambda: O.lattice_model("ladder", 7), 6),
+    "square4x3": (lambda: O.lattice_model("open square lattice", 4, 3), 5),
+    "xxz_open13": (lambda: _open_xxz(13, 3), 4),
+}
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("name", list(CASES))
+def test_spin_sums_match_oracle(name, dtype):
+    mk, k = CASES[name]
+    m = mk()
+    N = m.n_sites
+    say("spin sums", name, dtype)
+    op = A.Operator(to_product(m, k))
+    v = A.Vector(op.dim, dtype)
+    v.fill_random(7)
+    x = v.numpy()
+    n2, sz, szsz = op.spin_moments(v)
+    scale = float(x @ x)
+    assert abs(n2 - scale) <= 1e-12 * scale
+    for i in range(N):
+        assert abs(sz[i] - O.expect_site_term(N, k, x, "Sz", i)) <= 1e-12 * scale, i
+        for j in range(N):
+            want = O.expect_site_term(N, k, x, "Sz", i, "Sz", j)
+            assert abs(szsz[i, j] - want) <= 1e-12 * scale, (i, j)
+    pi, pj = np.array([(i, j) for i in range(N) for j in range(N) if i != j]).T
+    f = op.flip_pairs(v, pi, pj)
+    for a, b, got in zip(pi, pj, f):
+        want = O.expect_site_term(N, k, x, "Splus", int(a), "Sminus", int(b))
+        assert abs(got - want) <= 1e-12 * scale, (a, b)
+    # the same sums of the vector's transpose pairs: <S+_i S-_j> = <S+_j S-_i> for a real vector
+    np.testing.assert_allclose(f, op.flip_pairs(v, pj, pi), rtol=0, atol=1e-12 * scale)
+
+
+MEASURES = """MEASURE_AVERAGE[Magnetization]=Sz
+MEASURE_LOCAL[Local magnetization]=Sz
+MEASURE_LOCAL[Bond energy]=exchange
+MEASURE_CORRELATIONS[Diagonal spin correlations]=Sz
+MEASURE_CORRELATIONS[Offdiagonal spin correlations]="Splus:Sminus"
+MEASURE_CORRELATIONS[Sx correlations]="Sx:Sx"
+MEASURE_STRUCTURE_FACTOR[Structure Factor S]=Sz
+"""
+
+
+def _oracle_values(m, k, v, graph_pairs, bonds):
+    N = m.n_sites
+    loc = [O.expect_site_term(N, k, v, "Sz", i) for i in range(N)]
+    ex = [O.expect_site_term(N, k, v, "Sz", i, "Sz", j)
+          + 0.5 * (O.expect_site_term(N, k, v, "Splus", i, "Sminus", j)
+                   + O.expect_site_term(N, k, v, "Sminus", i, "Splus", j)) for i, j in bonds]
+    corr = {op: [O.expect_site_term(N, k, v, a, i, b, j) for i, j in graph_pairs]
+            for op, (a, b) in {"Diagonal spin correlations": ("Sz", "Sz"),
+                               "Offdiagonal spin correlations": ("Splus", "Sminus"),
+                               "Sx correlations": ("Sx", "Sx")}.items()}
+    return loc, ex, corr
+
+
+@pytest.mark.parametrize("precision,tol", [("f64", 1e-8), ("f32", 1e-6)])
+@pytest.mark.parametrize("lattice,size,sz", [("chain lattice", "L=10", 0), ("open chain lattice", "L=9", 0.5),
+                                             ("ladder", "L=6", 0)])
+def test_parameter_file_measurements_match_oracle(lattice, size, sz, precision, tol):
+    say("measurements", lattice, size, sz, precision)
+    text = (f'LATTICE="{lattice}"; {size}; J=1; Jz=0.8; h=0.1; Sz_total={sz}; ED_PRECISION="{precision}";\n'
+            + MEASURES + "{}\n")
+    (t,) = A.run_parameter_file(text)
+    (d,) = A.describe_parameter_file(text)
+    (sec,) = t["sectors"]
+    N = t["n_sites"]
+    k = int(round(sz + N / 2))
+    m = O.Model(N, np.array(d["bond_s"]), np.array(d["bond_t"]), np.array(d["jz"]), np.array(d["jxy"]),
+                h=np.array(d["h"]))
+    w, V = np.linalg.eigh(O.dense(m, k))
+    assert w[1] - w[0] > 1e-6, "the test needs a non-degenerate ground state"
+    assert abs(sec["eigenvalues"][0] - w[0]) < 1e-9
+    v = V[:, 0]
+    meas = sec["measurements"]
+    # first site pair per distance label, and bonds in edge order
+    labels = meas["correlations"]["Diagonal spin correlations"]["labels"]
+    coords = [lab for lab in meas["local"]["Local magnetization"]["labels"]]
+    pairs = [(coords.index(lab.split(" -- ")[0]), coords.index(lab.split(" -- ")[1])) for lab in labels]
+    bonds = list(zip(d["bond_s"], d["bond_t"]))
+    loc, ex, corr = _oracle_values(m, k, v, pairs, bonds)
+    np.testing.assert_allclose(meas["local"]["Local magnetization"]["values"][0], loc, atol=tol, rtol=0)
+    np.testing.assert_allclose(meas["local"]["Bond energy"]["values"][0], ex, atol=tol, rtol=0)
+    for name, want in corr.items():
+        np.testing.assert_allclose(meas["correlations"][name]["values"][0], want, atol=tol, rtol=0, err_msg=name)
+    assert abs(meas["average"]["Magnetization"][0] - sz / N) < tol
+    sf = np.array(meas["structure_factor"]["Structure Factor S"]["values"][0])
+    # sum rules of the Fourier sum over the N momenta: sum_k S(k) = sum_i <Sz_i^2> = N/4
+    assert abs(sf.sum() - N / 4) < tol * N
+    if lattice != "open chain lattice":
+        # k = 0 on a periodic lattice: (1/N) sum_ij <Sz_i Sz_j> = (Sz_total)^2 / N
+        assert abs(sf[0] - sz * sz / N) < tol * N
+    assert meas["unsupported"] == []
+
+
+def test_measurements_full_size_ring24_sum_rules():
+    # L=24 Heisenberg ring, Sz=0, FP32 storage: the ground state is a
+    # translation-invariant singlet, so every bond energy is E0/24, the
+    # correlations from site 0 sum to zero, <Sz_0^2> = 1/4, and the Splus:Sminus
+    # structure factor sums to sum_i <S+_i S-_i> = N/2.
+    say("measurements ring24")
+    text = ('LATTICE="chain lattice"; L=24; J=1; Sz_total=0; ED_PRECISION="f32";\n'
+            "MEASURE_LOCAL[Bond energy]=exchange\nMEASURE_CORRELATIONS[SzSz]=Sz\n"
+            'MEASURE_STRUCTURE_FACTOR[Sxy]="Splus:Sminus"\n{}\n')
+    (t,) = A.run_parameter_file(text)
+    (sec,) = t["sectors"]
+    e0 = sec["eigenvalues"][0]
+    assert abs(e0 - (-10.67001451653723)) < 1e-10 * 10.7
+    meas = sec["measurements"]
+    be = np.array(meas["local"]["Bond energy"]["values"][0])
+    assert abs(be.sum() - e0) < 1e-6
+    assert be.max() - be.min() < 1e-6
+    c = np.array(meas["correlations"]["SzSz"]["values"][0])
+    assert len(c) == 24 and abs(c[0] - 0.25) < 1e-12 and abs(c.sum()) < 1e-6
+    assert abs(c[1] - e0 / 24 / 3) < 1e-6  # singlet: <Sz Sz> = <S.S>/3
+    sf = np.array(meas["structure_factor"]["Sxy"]["values"][0])
+    assert abs(sf.sum() - 12.0) < 1e-5
+    assert sf.argmax() == 12  # k = pi (the 13th momentum label)
+    assert meas["structure_factor"]["Sxy"]["labels"][12] == "( 3.14159 )"
+
+
+def test_measurements_need_sz_sectors():
+    text = 'LATTICE="chain lattice"; L=6; J=1;\nMEASURE_LOCAL[M]=Sz\n{}\n'
+    (t,) = A.run_parameter_file(text)
+    assert "measurement_error" in t
+    assert all("measurements" not in s for s in t["sectors"])

@@ -381,3 +381,87 @@ def test_sparsediag_edge_dimensions():
     assert r["ev"][0] == 1.5 + 6 * 0.15   # 6 parallel bonds Jz/4, 6 sites -h*(-1/2)
     r = O.sparsediag(m, 7)           # D = 0: nothing
     assert r["dim"] == 0 and len(r["ev"]) == 0
+
+
+# ---------------------------------------------------------------------------
+# eigenvector measurements (expect_site_term): Kronecker products, the ring
+# of four and the open XX chain as free fermions
+# ---------------------------------------------------------------------------
+_LOCAL = {"Sz": _SZ, "Splus": _SP, "Sminus": _SM, "Sx": 0.5 * (_SP + _SM)}
+
+
+def _embed(N, k, v):
+    full = np.zeros(2 ** N)
+    full[O.basis(N, k).astype(np.int64)] = v
+    return full
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_measurement_terms_match_kronecker_products(seed):
+    rng = np.random.default_rng(100 + seed)
+    N, k = 7, 3 + seed
+    D = len(O.basis(N, k))
+    v = rng.uniform(-1, 1, D)
+    f = _embed(N, k, v)
+    for a in _LOCAL:
+        for i in range(N):
+            want = f @ _site_op(_LOCAL[a], i, N) @ f
+            assert abs(O.expect_site_term(N, k, v, a, i) - want) < 1e-13
+        for b in _LOCAL:
+            for i in range(N):
+                for j in range(N):
+                    op = (_site_op(_LOCAL[a] @ _LOCAL[b], i, N) if i == j
+                          else _site_op(_LOCAL[a], i, N) @ _site_op(_LOCAL[b], j, N))
+                    got = O.expect_site_term(N, k, v, a, i, b, j)
+                    assert abs(got - f @ op @ f) < 1e-13, (a, b, i, j)
+
+
+def test_measurement_ring_of_four_singlet():
+    # ground state of the J=1 ring of four: E0 = -2, a singlet, so
+    # <S_i.S_j> = -1/2 (neighbours), +1/4 (across); <SzSz> = <S.S>/3,
+    # <S+_i S-_j> = 2<S.S>/3
+    N, s, t, _ = O.lattice("chain lattice", 4)
+    m = O.Model(N, s, t, 1.0, 1.0)
+    w, V = np.linalg.eigh(O.dense(m, 2))
+    assert abs(w[0] + 2) < 1e-12 and w[1] - w[0] > 0.5
+    v = V[:, 0]
+    for i in range(4):
+        for d, ss in ((1, -0.5), (2, 0.25), (3, -0.5)):
+            j = (i + d) % 4
+            assert abs(O.expect_site_term(4, 2, v, "Sz", i, "Sz", j) - ss / 3) < 1e-13
+            assert abs(O.expect_site_term(4, 2, v, "Splus", i, "Sminus", j) - 2 * ss / 3) < 1e-13
+            assert abs(O.expect_site_term(4, 2, v, "Sx", i, "Sx", j) - ss / 3) < 1e-13
+        assert abs(O.expect_site_term(4, 2, v, "Sz", i)) < 1e-13
+        assert abs(O.expect_site_term(4, 2, v, "Splus", i, "Sminus", i) - 0.5) < 1e-13  # n_up = 1/2
+        assert abs(O.expect_site_term(4, 2, v, "Sz", i, "Sz", i) - 0.25) < 1e-13
+
+
+def test_measurement_open_xx_chain_free_fermions():
+    # Jordan-Wigner: the open XX chain (H = sum J/2 (S+S- + h.c.)) is free
+    # fermions with hopping J/2; in the ground state with M particles
+    # G_ij = sum_{occupied} phi(i) phi(j), <n_i> = G_ii,
+    # <Sz_i Sz_j> = (G_ii - 1/2)(G_jj - 1/2) - G_ij^2 (i != j, Wick),
+    # <S+_i S-_{i+1}> = <c+_i c_{i+1}> = G_{i,i+1} (no string between neighbours).
+    rng = np.random.default_rng(5)
+    L, M = 10, 4
+    J = rng.uniform(0.5, 1.5, L - 1)
+    s = np.arange(L - 1)
+    m = O.Model(L, s, s + 1, 0.0, J)
+    T = np.zeros((L, L))
+    for i in range(L - 1):
+        T[i, i + 1] = T[i + 1, i] = J[i] / 2
+    eps, phi = np.linalg.eigh(T)
+    assert eps[M] - eps[M - 1] > 1e-3  # non-degenerate Fermi level
+    G = phi[:, :M] @ phi[:, :M].T
+    w, V = np.linalg.eigh(O.dense(m, M))
+    assert abs(w[0] - eps[:M].sum()) < 1e-11 and w[1] - w[0] > 1e-3
+    v = V[:, 0]
+    for i in range(L):
+        assert abs(O.expect_site_term(L, M, v, "Sz", i) - (G[i, i] - 0.5)) < 1e-11
+        for j in range(L):
+            if i != j:
+                want = (G[i, i] - 0.5) * (G[j, j] - 0.5) - G[i, j] ** 2
+                assert abs(O.expect_site_term(L, M, v, "Sz", i, "Sz", j) - want) < 1e-11, (i, j)
+        if i + 1 < L:
+            assert abs(O.expect_site_term(L, M, v, "Splus", i, "Sminus", i + 1) - G[i, i + 1]) < 1e-11
+            assert abs(O.expect_site_term(L, M, v, "Sminus", i, "Splus", i + 1) - G[i, i + 1]) < 1e-11

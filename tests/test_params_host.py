@@ -168,3 +168,52 @@ def test_cli_describe(tmp_path):
     (t,) = json.loads(out)["tasks"]
     assert t["bond_s"] == [0, 1] and t["bond_t"] == [1, 2]
     assert math.isclose(sum(t["jz"]), 2.0)
+
+
+# ---------------------------------------------------------------------------
+# measurement requests (MeasurementOperators, measurement_operators.C:25-66)
+# and the lattice labels they are reported with
+# ---------------------------------------------------------------------------
+_MEAS = """MEASURE_AVERAGE[Magnetization]=Sz
+MEASURE_AVERAGE[Exchange]=exchange
+MEASURE_LOCAL[Local magnetization]=Sz
+MEASURE_CORRELATIONS[Diagonal spin correlations]=Sz
+MEASURE_CORRELATIONS[Offdiagonal spin correlations]="Splus:Sminus"
+MEASURE_CORRELATIONS[Broken]=Sz=
+MEASURE_STRUCTURE_FACTOR[Structure Factor S]=Sz
+"""
+
+
+def test_measure_requests_and_labels():
+    text = ('MODEL="spin"; CONSERVED_QUANTUMNUMBERS="Sz"; J=1;\n' + _MEAS +
+            '{LATTICE="dimer";}\n{LATTICE="chain lattice";L=4;}\n{LATTICE="open chain lattice";L=3;}\n'
+            '{LATTICE="triangular lattice";L=3;W=2;}\n')
+    dimer, ring, open3, tri = A.describe_parameter_file(text)
+    m = ring["measure"]
+    assert m["average"] == {"Exchange": "exchange", "Magnetization": "Sz"}
+    assert m["correlations"]["Offdiagonal spin correlations"] == ["Splus", "Sminus"]
+    assert m["correlations"]["Diagonal spin correlations"] == ["Sz", "Sz"]
+    assert m["correlations"]["Broken"] == ["Sz=", "Sz="]  # evaluated as unsupported
+    assert {u["name"] for u in m["unsupported"]} == {"Exchange", "Broken"}
+    # periodic chain: one distance per offset, from site 0; momenta 2 pi x / L
+    # shifted to (-pi, pi]
+    assert m["distance_labels"] == ["( 0 ) -- ( 0 )", "( 0 ) -- ( 1 )", "( 0 ) -- ( 2 )", "( 0 ) -- ( 3 )"]
+    assert m["momentum_labels"] == ["( 0 )", "( 1.5708 )", "( 3.14159 )", "( -1.5708 )"]
+    assert m["flip_pairs"] == 3  # (0,1), (0,2), (0,3)
+    # open boundaries: every ordered pair is its own distance (L^2 of them)
+    assert dimer["n_sites"] == 2 and dimer["bond_s"] == [0] and dimer["bond_t"] == [1]
+    assert dimer["measure"]["distance_labels"] == ["( 0 ) -- ( 0 )", "( 0 ) -- ( 1 )", "( 1 ) -- ( 0 )",
+                                                   "( 1 ) -- ( 1 )"]
+    assert len(open3["measure"]["distance_labels"]) == 9 and open3["measure"]["flip_pairs"] == 6
+    assert open3["measure"]["momentum_labels"] == ["( 0 )", "( 2.0944 )", "( -2.0944 )"]
+    # triangular lattice: Cartesian coordinates a1 = (1,0), a2 = (1/2, sqrt(3)/2)
+    # and momenta in the reciprocal basis (lattices.xml.in:45-49)
+    t = tri["measure"]
+    assert t["distance_labels"][1] == "( 0,0 ) -- ( 0.5,0.866025 )"
+    assert t["momentum_labels"][1] == "( 0,3.6276 )"  # offset (0,1): k = (0, pi) -> b2 / 2
+    assert t["momentum_labels"][2] == "( 2.0944,-1.2092 )"  # offset (1,0): b1 / 3
+
+
+def test_measure_absent_without_requests():
+    (t,) = A.describe_parameter_file('LATTICE="chain lattice"; L=4; {}')
+    assert "measure" not in t
