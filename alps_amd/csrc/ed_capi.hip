@@ -19,6 +19,7 @@
 #include "ed_kernels.hpp"
 #include "ed_measure.hpp"
 #include "ed_sparsediag.hpp"
+#include "ed_xml.hpp"
 
 using namespace alps_ed;
 using namespace alps_ed::host;
@@ -1123,6 +1124,10 @@ extern "C" int alps_ed_run_parameter_file(const char* text, char* out, size_t ou
       const MeasureSpec ms = measure_spec(p);
       const bool measure = ms.any() && !ts.whole_space;
       std::vector<std::string> meas_json(models.size());
+      std::vector<MeasureValues> meas_val(models.size());
+      std::vector<XmlSector> xsec;
+      // sectors of a quantum-number loop carry their Sz (diag.h:100-105)
+      const bool qn_loop = !ts.whole_space && !ev.defined("Sz_total");
       uint64_t need = 0;
       for (const alps_ed_model& am : models) {
         uint64_t f = 0;
@@ -1139,7 +1144,8 @@ extern "C" int alps_ed_run_parameter_file(const char* text, char* out, size_t ou
           if (sv.r.dim == 0) continue;
           std::vector<SpinSums> sums;
           for (auto& v : sv.vectors) sums.push_back(spin_sums(sv.op.get(), v.get(), plan.flips));
-          meas_json[i] = measurements_json(ms, m.graph, plan, sums);
+          meas_val[i] = evaluate_measurements(ms, m.graph, sums);
+          meas_json[i] = measurements_json(meas_val[i], plan);
         }
       } else if (models.size() > 1 && need < 0.8 * (double)free_b) {
         alps_ed_batch_config cfg{sp.dtype, need, 0, 8, 0, 0};
@@ -1172,11 +1178,17 @@ extern "C" int alps_ed_run_parameter_file(const char* text, char* out, size_t ou
                           std::to_string(r.iterations) + ", \"error_code\": " + std::to_string(r.error_code) +
                           ", \"seconds\": " + json_number(r.seconds) + ", \"eigenvalues\": " + json_list(e) +
                           (measure ? ", \"measurements\": " + meas_json[i] : std::string()));
+        XmlSector xs;
+        if (qn_loop) xs.qn.push_back({"Sz", half_integer_string(2 * n_up - m.n_sites)});
+        xs.ev = e;
+        if (measure) xs.meas = &meas_val[i];
+        xsec.push_back(xs);
       }
       if (ts.whole_space) {
         // one subspace without quantum numbers: the merged spectrum
         const std::vector<double> e = merge_spectra(evs, sp.nev, 1e-9);
         evs.assign(1, e);
+        xsec.assign(1, XmlSector{{}, e, nullptr});
         entries.assign(1, "\"Sz_total\": null, \"n_up\": null, \"dim\": " + std::to_string(dim_sum) +
                               ", \"iterations\": " + std::to_string(it_sum) + ", \"error_code\": " +
                               std::to_string(err_max) + ", \"seconds\": " + json_number(sec_sum) +
@@ -1189,6 +1201,9 @@ extern "C" int alps_ed_run_parameter_file(const char* text, char* out, size_t ou
       if (ms.any() && ts.whole_space)
         js += ", \"measurement_error\": \"eigenvector measurements need Sz-conserved sectors "
               "(CONSERVED_QUANTUMNUMBERS=\\\"Sz\\\" or Sz_total)\"";
+      const int nev_xml = ev.defined("NUMBER_EIGENVALUES") ? sp.nev : 0;
+      js += ", \"xml\": " +
+            json_string(task_xml(p, xsec, nev_xml, !std::isnan(g), g, !std::isnan(gap), gap));
       js += "}";
     }
     js += "]}";

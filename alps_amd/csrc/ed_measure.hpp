@@ -236,9 +236,18 @@ struct MeasurePlan {
 inline std::string json_string(const std::string& v) {
   std::string o = "\"";
   for (char c : v) {
-    if (c == '"' || c == '\\') o += '\\';
-    if ((unsigned char)c < 0x20) continue;
-    o += c;
+    if (c == '"' || c == '\\') {
+      o += '\\';
+      o += c;
+    } else if (c == '\n') {
+      o += "\\n";
+    } else if ((unsigned char)c < 0x20) {
+      char b[8];
+      std::snprintf(b, sizeof b, "\\u%04x", (unsigned)(unsigned char)c);
+      o += b;
+    } else {
+      o += c;
+    }
   }
   return o + "\"";
 }
@@ -325,25 +334,33 @@ inline std::string json_labels(const std::vector<std::string>& v) {
   return o + "]";
 }
 
-// The measurements of one sector's eigenvectors as a JSON object.
-inline std::string measurements_json(const MeasureSpec& spec, const LatticeGraph& g, const MeasurePlan& plan,
-                                     const std::vector<SpinSums>& vecs) {
+// Measured values of one sector's eigenvectors (EigenvectorMeasurements,
+// measurement_operators.h:74-98): per name, one value or one labelled list
+// per eigenvector.
+struct MeasureValues {
+  struct Series {
+    std::vector<std::string> labels;
+    std::vector<std::vector<double>> values;  // [eigenvector][label]
+  };
+  std::map<std::string, std::vector<double>> average;  // [eigenvector]
+  std::map<std::string, Series> local, correlation, structure_factor;
+};
+
+inline MeasureValues evaluate_measurements(const MeasureSpec& spec, const LatticeGraph& g,
+                                           const std::vector<SpinSums>& vecs) {
   const int N = g.n;
+  MeasureValues out;
   std::vector<SpinAlgebra> al;
   for (const SpinSums& s : vecs) al.push_back(SpinAlgebra{s, N});
-  std::string avg, loc, cor, sf;
   SpinOp a, b;
   for (const auto& kv : spec.average) {
     if (!site_operator(kv.second, a)) continue;
-    std::vector<double> v;
+    std::vector<double>& v = out.average[kv.first];
     for (const SpinAlgebra& x : al) {
       double t = 0.0;
       for (int i = 0; i < N; ++i) t += x.one(a, i);
       v.push_back(t / N);
     }
-    std::string list = "[";
-    for (size_t i = 0; i < v.size(); ++i) list += (i ? ", " : "") + json_number(v[i]);
-    avg += (avg.empty() ? "" : ", ") + json_string(kv.first) + ": " + list + "]";
   }
   std::vector<std::string> site_labels, bond_labels;
   for (int s = 0; s < N; ++s) site_labels.push_back(coordinate_string(site_coordinate(g, s)));
@@ -351,33 +368,30 @@ inline std::string measurements_json(const MeasureSpec& spec, const LatticeGraph
   for (const auto& kv : spec.local) {
     const bool site = site_operator(kv.second, a);
     if (!site && !bond_operator(kv.second)) continue;
-    std::vector<std::vector<double>> rows;
+    MeasureValues::Series& ser = out.local[kv.first];
+    ser.labels = site ? site_labels : bond_labels;
     for (const SpinAlgebra& x : al) {
       std::vector<double> r;
       if (site)
         for (int i = 0; i < N; ++i) r.push_back(x.one(a, i));
       else
         for (const LatticeBond& e : g.bonds) r.push_back(x.bond(kv.second, e.s, e.t));
-      rows.push_back(r);
+      ser.values.push_back(r);
     }
-    loc += (loc.empty() ? "" : ", ") + json_string(kv.first) + ": {\"labels\": " +
-           json_labels(site ? site_labels : bond_labels) + ", \"values\": " + json_rows(rows) + "}";
   }
   const std::vector<std::pair<int, int>> dp = distance_pairs(g);
   std::vector<std::string> dist_labels;
   for (const auto& pr : dp)
-    dist_labels.push_back(pr.first < 0 ? std::string()
-                                       : site_labels[pr.first] + " -- " + site_labels[pr.second]);
+    dist_labels.push_back(pr.first < 0 ? std::string() : site_labels[pr.first] + " -- " + site_labels[pr.second]);
   for (const auto& kv : spec.correlation) {
     if (!site_operator(kv.second.first, a) || !site_operator(kv.second.second, b)) continue;
-    std::vector<std::vector<double>> rows;
+    MeasureValues::Series& ser = out.correlation[kv.first];
+    ser.labels = dist_labels;
     for (const SpinAlgebra& x : al) {
       std::vector<double> r;
       for (const auto& pr : dp) r.push_back(pr.first < 0 ? 0.0 : x.pair(a, pr.first, b, pr.second));
-      rows.push_back(r);
+      ser.values.push_back(r);
     }
-    cor += (cor.empty() ? "" : ", ") + json_string(kv.first) + ": {\"labels\": " + json_labels(dist_labels) +
-           ", \"values\": " + json_rows(rows) + "}";
   }
   if (!spec.structure_factor.empty()) {
     const std::vector<std::vector<double>> ks = momenta(g);
@@ -387,7 +401,8 @@ inline std::string measurements_json(const MeasureSpec& spec, const LatticeGraph
     for (const auto& k : ks) k_labels.push_back(coordinate_string(k));
     for (const auto& kv : spec.structure_factor) {
       if (!site_operator(kv.second.first, a) || !site_operator(kv.second.second, b)) continue;
-      std::vector<std::vector<double>> rows;
+      MeasureValues::Series& ser = out.structure_factor[kv.first];
+      ser.labels = k_labels;
       for (const SpinAlgebra& x : al) {
         std::vector<double> C((size_t)N * N);
         for (int i = 0; i < N; ++i)
@@ -407,17 +422,34 @@ inline std::string measurements_json(const MeasureSpec& spec, const LatticeGraph
             }
           r.push_back(t);
         }
-        rows.push_back(r);
+        ser.values.push_back(r);
       }
-      sf += (sf.empty() ? "" : ", ") + json_string(kv.first) + ": {\"labels\": " + json_labels(k_labels) +
-            ", \"values\": " + json_rows(rows) + "}";
     }
+  }
+  return out;
+}
+
+// The measurements of one sector as a JSON object.
+inline std::string measurements_json(const MeasureValues& mv, const MeasurePlan& plan) {
+  auto series = [](const std::map<std::string, MeasureValues::Series>& m) {
+    std::string o;
+    for (const auto& kv : m)
+      o += (o.empty() ? "" : ", ") + json_string(kv.first) + ": {\"labels\": " + json_labels(kv.second.labels) +
+           ", \"values\": " + json_rows(kv.second.values) + "}";
+    return "{" + o + "}";
+  };
+  std::string avg;
+  for (const auto& kv : mv.average) {
+    std::string list = "[";
+    for (size_t i = 0; i < kv.second.size(); ++i) list += (i ? ", " : "") + json_number(kv.second[i]);
+    avg += (avg.empty() ? "" : ", ") + json_string(kv.first) + ": " + list + "]";
   }
   std::string un = "[";
   for (size_t i = 0; i < plan.unsupported.size(); ++i) un += (i ? ", " : "") + plan.unsupported[i];
   un += "]";
-  return "{\"average\": {" + avg + "}, \"local\": {" + loc + "}, \"correlations\": {" + cor +
-         "}, \"structure_factor\": {" + sf + "}, \"unsupported\": " + un + "}";
+  return "{\"average\": {" + avg + "}, \"local\": " + series(mv.local) + ", \"correlations\": " +
+         series(mv.correlation) + ", \"structure_factor\": " + series(mv.structure_factor) +
+         ", \"unsupported\": " + un + "}";
 }
 
 }  // namespace alps_ed

@@ -164,3 +164,63 @@ def test_measurements_need_sz_sectors():
     (t,) = A.run_parameter_file(text)
     assert "measurement_error" in t
     assert all("measurements" not in s for s in t["sectors"])
+
+
+def _xml_check(doc, task):
+    """The task result document against the same run's JSON."""
+    import xml.etree.ElementTree as ET
+
+    root = ET.fromstring(doc)
+    assert root.tag == "SIMULATION"
+    params = {p.get("name"): p.text or "" for p in root.find("PARAMETERS")}
+    assert params == task["parameters"]
+    avg = {a.get("name"): float(a.find("MEAN").text) for a in root.find("AVERAGES")}
+    assert avg["Ground State Energy"] == task["ground_state_energy"]
+    # no gap with a single level (sparsediag.h:80-84 writes it only when set)
+    assert avg.get("Energy Gap") == task["energy_gap"]
+    evs = root.findall("EIGENVALUES")
+    states = root.findall("EIGENSTATES")
+    assert len(evs) == len(states) == len(task["sectors"])
+    return evs, states
+
+
+def test_task_xml_matches_results(tmp_path):
+    # quantum-number loop over all Sz of an odd ring: sector values are half
+    # integers written as the reference's half_integer prints them
+    say("task xml")
+    text = ('LATTICE="chain lattice"; L=5; J=1; Jz=0.7; CONSERVED_QUANTUMNUMBERS="Sz"; NUMBER_EIGENVALUES=2;\n'
+            "MEASURE_LOCAL[Local magnetization]=Sz\nMEASURE_AVERAGE[Magnetization]=Sz\n{}\n")
+    (t,) = A.run_parameter_file(text)
+    evs, states = _xml_check(t["xml"], t)
+    want_sz = ["-5/2", "-3/2", "-1/2", "1/2", "3/2", "5/2"]
+    for e, st, sec, sz in zip(evs, states, t["sectors"], want_sz):
+        assert [(q.get("name"), q.get("value")) for q in e.findall("QUANTUMNUMBER")] == [("Sz", sz)]
+        vals = [float(x) for x in "".join(e.itertext()).split()]
+        assert int(e.get("number")) == len(vals) == min(2, len(sec["eigenvalues"]))
+        assert vals == sec["eigenvalues"][: len(vals)]
+        for j, es in enumerate(st.findall("EIGENSTATE")):
+            sc = {a.get("name"): float(a.find("MEAN").text) for a in es.findall("SCALAR_AVERAGE")}
+            assert sc["Energy"] == vals[j]
+            assert sc["Magnetization"] == sec["measurements"]["average"]["Magnetization"][j]
+            (va,) = es.findall("VECTOR_AVERAGE")
+            loc = sec["measurements"]["local"]["Local magnetization"]
+            assert [s.get("indexvalue") for s in va] == loc["labels"]
+            assert [float(s.find("MEAN").text) for s in va] == loc["values"][j]
+            # a fixed-Sz eigenvector: the local magnetisations sum to Sz
+            assert abs(sum(loc["values"][j]) - float(eval(sz))) < 1e-9
+
+
+def test_cli_writes_task_xml(tmp_path):
+    import os
+    import subprocess
+
+    f = tmp_path / "parm.txt"
+    f.write_text('LATTICE="chain lattice"\nL=8\nJ=1\nSz_total=0\n{}\n{J=0.5;}\n')
+    cli = os.path.join(os.path.dirname(A.LIB_PATH), "alps_ed_sparsediag")
+    say("cli xml", cli)
+    subprocess.run([cli, "--write-xml", str(f)], check=True, capture_output=True, text=True, timeout=60)
+    tasks = A.run_parameter_file(f.read_text())
+    for i, t in enumerate(tasks):
+        doc = (tmp_path / f"parm.task{i + 1}.out.xml").read_text()
+        evs, _ = _xml_check(doc, t)
+        assert evs[0].findall("QUANTUMNUMBER") == []  # Sz_total given: no loop
