@@ -36,6 +36,7 @@
 #pragma once
 
 #include <algorithm>
+#include <complex>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -75,6 +76,9 @@ struct Graph {
   std::vector<int> stype;
   std::vector<Bond> bonds;
   std::vector<std::vector<int>> coord;  // cell offset of each site
+  std::vector<int> ext;                 // extents of the finite lattice
+  std::vector<bool> periodic;
+  int cell_vertices = 1;
 };
 
 // Shift an offset by s on a finite lattice; false when it leaves an open
@@ -115,6 +119,9 @@ inline Graph lattice_graph(const UnitCell& uc, const std::vector<int>& ext,
   g.n = volume * nv;
   g.stype.resize(g.n);
   g.coord.resize(g.n);
+  g.ext = ext;
+  g.periodic = periodic;
+  g.cell_vertices = nv;
   std::vector<int> cell(ext.size(), 0);
   for (int c = 0; c < volume; ++c) {
     for (int v = 0; v < nv; ++v) {
@@ -458,6 +465,175 @@ inline double expect_site_term(const std::vector<State>& states, const double* v
   double t = 0.0;
   for (size_t r = 0; r < D; ++r) t += v[r] * y[r];
   return t;
+}
+
+// ---------------------------------------------------------------------------
+// Momentum sectors: lattice translations (hypercubic.h:425-462 over the
+// periodic directions, latticegraph.h:252-268), the Bloch basis
+// (blochbasisstates.h:97-271) and the Hamiltonian in it
+// (hamiltonian_matrix.hpp:268-345: element * phase * norm(j) / norm(i)).
+// ---------------------------------------------------------------------------
+struct Translation {
+  std::complex<double> phase;
+  std::vector<size_t> perm;  // site i goes to perm[i]
+};
+
+// One translation per shift of the periodic directions (first direction
+// fastest), phase exp(i sum shift_d k_d); k has one entry per periodic
+// direction.
+inline std::vector<Translation> translations(const Graph& g, const std::vector<double>& k) {
+  std::vector<int> dirs;
+  for (size_t d = 0; d < g.ext.size(); ++d)
+    if (g.periodic[d]) dirs.push_back((int)d);
+  if (k.size() != dirs.size()) throw std::runtime_error("Incorrect number of momenta specified");
+  std::vector<Translation> out;
+  std::vector<int> shift(dirs.size(), 0);
+  const int nv = g.cell_vertices;
+  while (true) {
+    double phase = 0.;
+    std::vector<int> off(g.ext.size(), 0);
+    for (size_t i = 0; i < dirs.size(); ++i) {
+      off[dirs[i]] = shift[i];
+      phase += shift[i] * k[i];
+    }
+    Translation t;
+    t.phase = std::exp(std::complex<double>(0., phase));
+    for (int c = 0; c < g.n / nv; ++c) {
+      std::vector<int> o = g.coord[c * nv];
+      if (!lattice_shift(o, off, g.ext, g.periodic)) throw std::logic_error("shift leaves the lattice");
+      for (int v = 0; v < nv; ++v) t.perm.push_back((size_t)(cell_index(o, g.ext) * nv + v));
+    }
+    out.push_back(t);
+    bool done = true;
+    for (size_t i = 0; i < shift.size() && done; ++i) {
+      if (++shift[i] >= g.ext[dirs[i]]) shift[i] = 0;
+      else done = false;
+    }
+    if (done) break;
+  }
+  return out;
+}
+
+// translation_momenta (hypercubic.h:403-423): 2 pi j / extent per periodic
+// direction, the first direction outermost.
+inline std::vector<std::vector<double>> translation_momenta(const Graph& g) {
+  std::vector<std::vector<double>> ks;
+  for (size_t d = 0; d < g.ext.size(); ++d) {
+    if (!g.periodic[d]) continue;
+    if (ks.empty()) {
+      for (int j = 0; j < g.ext[d]; ++j) ks.push_back({2. * j * M_PI / g.ext[d]});
+    } else {
+      std::vector<std::vector<double>> nk;
+      for (const auto& k : ks)
+        for (int j = 0; j < g.ext[d]; ++j) {
+          std::vector<double> v = k;
+          v.push_back(2. * j * M_PI / g.ext[d]);
+          nk.push_back(v);
+        }
+      ks.swap(nk);
+    }
+  }
+  return ks;
+}
+
+struct BlochBasis {
+  std::vector<State> reps;                  // representatives, ascending
+  std::vector<double> norm;                 // normalization per representative
+  std::vector<State> full;                  // every kept state, ascending
+  std::vector<size_t> rep;                  // its representative
+  std::vector<std::complex<double>> phase;  // phase of the translation to it
+};
+
+inline std::pair<size_t, std::complex<double>> index_and_phase(const BlochBasis& b, const State& x) {
+  auto it = std::lower_bound(b.full.begin(), b.full.end(), x);
+  if (it == b.full.end() || !(*it == x)) return {b.reps.size(), std::complex<double>(1.)};
+  const size_t i = (size_t)(it - b.full.begin());
+  return {b.rep[i], b.phase[i]};
+}
+
+// n_up < 0: no constraint (every state of the N sites).
+inline BlochBasis build_bloch(int N, int n_up, const std::vector<Translation>& trans) {
+  std::vector<State> states;
+  if (n_up >= 0) {
+    states = build_basis(N, n_up);
+  } else {
+    for (int u = 0; u <= N; ++u) {
+      std::vector<State> s = build_basis(N, u);
+      states.insert(states.end(), s.begin(), s.end());
+    }
+    std::sort(states.begin(), states.end());
+  }
+  BlochBasis b;
+  const double l = std::sqrt(double(N));
+  for (const State& idx : states) {
+    size_t representative_number = b.reps.size();
+    std::complex<double> ph = 1., phase_sum = 1.;
+    for (size_t t = 1; t < trans.size(); ++t) {
+      State tr(idx.size());
+      for (size_t i = 0; i < trans[t].perm.size(); ++i) tr[trans[t].perm[i]] = idx[i];
+      if (tr > idx) continue;
+      if (tr == idx) {
+        phase_sum += trans[t].phase;
+        continue;
+      }
+      const size_t i = index_and_phase(b, tr).first;
+      if (i < b.reps.size() && b.reps[i] == tr) {
+        representative_number = i;
+        ph = trans[t].phase;
+        break;
+      }
+    }
+    bool storeit = true;
+    if (representative_number == b.reps.size()) {
+      const double w = std::sqrt(std::abs(phase_sum));
+      if (w < 0.01) {
+        storeit = false;
+      } else {
+        b.norm.push_back(w / l);
+        b.reps.push_back(idx);
+      }
+    }
+    if (storeit) {
+      b.rep.push_back(representative_number);
+      b.full.push_back(idx);
+      b.phase.push_back(ph);
+    }
+  }
+  return b;
+}
+
+// Dense complex H in the Bloch basis (row i, column j), built as the real
+// matrix is: explicit zero diagonal, all site terms, then all bonds.
+inline std::vector<std::complex<double>> bloch_matrix(const SpinModel& m, const BlochBasis& b) {
+  const size_t D = b.reps.size();
+  std::vector<std::complex<double>> H(D * D, std::complex<double>(0.));
+  auto add = [&](size_t i, const State& ns, double val) {
+    const std::pair<size_t, std::complex<double>> jp = index_and_phase(b, ns);
+    if (jp.first < D) H[i * D + jp.first] += val * (jp.second * b.norm[jp.first] / b.norm[i]);
+  };
+  for (int s = 0; s < m.n_sites; ++s)
+    for (size_t i = 0; i < D; ++i)
+      for (int js = 0; js < 2; ++js) {
+        const double val = site_element(m, s, b.reps[i][s], js);
+        if (val == 0.0) continue;
+        State ns = b.reps[i];
+        ns[s] = (short)js;
+        add(i, ns, val);
+      }
+  for (size_t bd = 0; bd < m.bonds.size(); ++bd) {
+    const int s1 = m.bonds[bd].s, s2 = m.bonds[bd].t;
+    for (size_t i = 0; i < D; ++i)
+      for (int j1 = 0; j1 < 2; ++j1)
+        for (int j2 = 0; j2 < 2; ++j2) {
+          const double val = bond_element(m, (int)bd, b.reps[i][s1], b.reps[i][s2], j1, j2);
+          if (val == 0.0) continue;
+          State ns = b.reps[i];
+          ns[s1] = (short)j1;
+          ns[s2] = (short)j2;
+          add(i, ns, val);
+        }
+  }
+  return H;
 }
 
 // ---------------------------------------------------------------------------

@@ -229,3 +229,90 @@ extern "C" int oracle_expect_site_term(int N, int n_up, const double* v, int opA
     return -1;
   }
 }
+
+namespace {
+
+std::vector<edo::Translation> read_translations(int N, int n_t, const long* perm, const double* re,
+                                                const double* im) {
+  std::vector<edo::Translation> t(n_t);
+  for (int i = 0; i < n_t; ++i) {
+    t[i].phase = std::complex<double>(re[i], im[i]);
+    t[i].perm.assign(perm + (size_t)i * N, perm + (size_t)(i + 1) * N);
+  }
+  return t;
+}
+
+}  // namespace
+
+// Momenta of a named lattice (one row of n_dirs values per momentum).
+// Returns the number of momenta (-1 on error); *n_dirs = periodic directions.
+extern "C" int oracle_translation_momenta(const char* name, int L, int W, int H, int max, double* out,
+                                          int* n_dirs) {
+  try {
+    const edo::Graph g = edo::named_lattice(name, L, W, H);
+    const std::vector<std::vector<double>> ks = edo::translation_momenta(g);
+    *n_dirs = ks.empty() ? 0 : (int)ks[0].size();
+    for (int i = 0; i < (int)ks.size() && i < max; ++i)
+      for (int d = 0; d < *n_dirs; ++d) out[i * *n_dirs + d] = ks[i][d];
+    return (int)ks.size();
+  } catch (...) {
+    return -1;
+  }
+}
+
+// Translations of a named lattice at momentum k: phases and site maps
+// (perm[t*N + i] = image of site i).  Returns their number (-1 on error).
+extern "C" int oracle_translations(const char* name, int L, int W, int H, int n_k, const double* k, int max,
+                                   long* perm, double* re, double* im) {
+  try {
+    const edo::Graph g = edo::named_lattice(name, L, W, H);
+    const std::vector<edo::Translation> t = edo::translations(g, std::vector<double>(k, k + n_k));
+    for (int i = 0; i < (int)t.size() && i < max; ++i) {
+      re[i] = t[i].phase.real();
+      im[i] = t[i].phase.imag();
+      for (int s = 0; s < g.n; ++s) perm[(size_t)i * g.n + s] = (long)t[i].perm[s];
+    }
+    return (int)t.size();
+  } catch (...) {
+    return -1;
+  }
+}
+
+// Representatives (bit-encoded) and normalizations of the Bloch basis of N
+// sites (n_up < 0: no constraint).  Returns their number (-1 on error).
+extern "C" long oracle_bloch_basis(int N, int n_up, int n_t, const long* perm, const double* re, const double* im,
+                                   long max, uint32_t* reps, double* norm) {
+  try {
+    const edo::BlochBasis b = edo::build_bloch(N, n_up, read_translations(N, n_t, perm, re, im));
+    for (size_t i = 0; i < b.reps.size() && (long)i < max; ++i) {
+      reps[i] = edo::state_bits(b.reps[i]);
+      norm[i] = b.norm[i];
+    }
+    return (long)b.reps.size();
+  } catch (...) {
+    return -1;
+  }
+}
+
+// Dense complex H of the model in the Bloch basis (row-major, real and
+// imaginary parts).  Returns its dimension (-1 on error; -(dim) if it
+// exceeds max_dim).
+extern "C" long oracle_bloch_dense(int N, int n_up, int n_bonds, const int* bs, const int* bt, const double* jz,
+                                   const double* jxy, const double* k, const double* h, const double* d, int n_t,
+                                   const long* perm, const double* re, const double* im, long max_dim,
+                                   double* out_re, double* out_im) {
+  try {
+    const edo::SpinModel m = make_model(N, n_bonds, bs, bt, jz, jxy, k, h, d);
+    const edo::BlochBasis b = edo::build_bloch(N, n_up, read_translations(N, n_t, perm, re, im));
+    const long D = (long)b.reps.size();
+    if (D > max_dim) return -D;
+    const std::vector<std::complex<double>> H = edo::bloch_matrix(m, b);
+    for (size_t i = 0; i < H.size(); ++i) {
+      out_re[i] = H[i].real();
+      out_im[i] = H[i].imag();
+    }
+    return D;
+  } catch (...) {
+    return -1;
+  }
+}

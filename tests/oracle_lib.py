@@ -47,6 +47,14 @@ def lib():
         L.oracle_ground_and_gap.restype = i32
         L.oracle_expect_site_term.argtypes = [i32, i32, P, i32, i32, i32, i32, P]
         L.oracle_expect_site_term.restype = i32
+        L.oracle_translation_momenta.argtypes = [ctypes.c_char_p, i32, i32, i32, i32, P, P]
+        L.oracle_translation_momenta.restype = i32
+        L.oracle_translations.argtypes = [ctypes.c_char_p, i32, i32, i32, i32, P, i32, P, P, P]
+        L.oracle_translations.restype = i32
+        L.oracle_bloch_basis.argtypes = [i32, i32, i32, P, P, P, i64, P, P]
+        L.oracle_bloch_basis.restype = i64
+        L.oracle_bloch_dense.argtypes = [i32, i32] + model + [i32, P, P, P, i64, P, P]
+        L.oracle_bloch_dense.restype = i64
         _lib = L
     return _lib
 
@@ -227,3 +235,55 @@ def expect_site_term(N, n_up, v, a, i, b=None, j=-1):
                                        ctypes.byref(out))
     assert rc == 0
     return out.value
+
+
+def translation_momenta(name, L, W=0, H=0):
+    out = np.zeros(4096)
+    nd = ctypes.c_int()
+    n = lib().oracle_translation_momenta(name.encode(), L, W, H, 4096 // 3, _p(out), ctypes.byref(nd))
+    assert n >= 0
+    return out[: n * nd.value].reshape(n, nd.value)
+
+
+def translations(name, L, W=0, H=0, k=()):
+    """(phases [n_t] complex, perms [n_t, N]) of the lattice at momentum k."""
+    N = lattice(name, L, W, H)[0]
+    k = np.ascontiguousarray(k, dtype=np.float64)
+    perm = np.zeros((1024, N), dtype=np.int64)
+    re, im = np.zeros(1024), np.zeros(1024)
+    n = lib().oracle_translations(name.encode(), L, W, H, len(k), _p(k), 1024, _p(perm), _p(re), _p(im))
+    assert n > 0
+    return re[:n] + 1j * im[:n], perm[:n].copy()
+
+
+def _trans_args(phases, perms):
+    perms = np.ascontiguousarray(perms, dtype=np.int64)
+    re = np.ascontiguousarray(np.real(phases), dtype=np.float64)
+    im = np.ascontiguousarray(np.imag(phases), dtype=np.float64)
+    return len(re), perms, re, im
+
+
+def bloch_basis(N, n_up, phases, perms):
+    """Representatives (bit-encoded) and normalizations; n_up < 0 for the
+    whole space."""
+    nt, perms, re, im = _trans_args(phases, perms)
+    cap = 1 << min(N, 20)
+    reps = np.zeros(cap, dtype=np.uint32)
+    norm = np.zeros(cap)
+    n = lib().oracle_bloch_basis(N, n_up, nt, _p(perms), _p(re), _p(im), cap, _p(reps), _p(norm))
+    assert n >= 0
+    return reps[:n].copy(), norm[:n].copy()
+
+
+def bloch_dense(m: Model, n_up, phases, perms):
+    """Dense complex H of the model in the Bloch basis of n_up up spins."""
+    nt, perms, re, im = _trans_args(phases, perms)
+    one = np.zeros(1)
+    D = -lib().oracle_bloch_dense(m.n_sites, n_up, *m.cargs(), nt, _p(perms), _p(re), _p(im), 0, _p(one), _p(one))
+    assert D >= 0, D
+    out_re = np.zeros(max(D * D, 1))
+    out_im = np.zeros(max(D * D, 1))
+    D = lib().oracle_bloch_dense(m.n_sites, n_up, *m.cargs(), nt, _p(perms), _p(re), _p(im), D,
+                                 _p(out_re), _p(out_im))
+    assert D >= 0, D
+    return (out_re[: D * D] + 1j * out_im[: D * D]).reshape(D, D)

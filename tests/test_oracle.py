@@ -465,3 +465,82 @@ def test_measurement_open_xx_chain_free_fermions():
         if i + 1 < L:
             assert abs(O.expect_site_term(L, M, v, "Splus", i, "Sminus", i + 1) - G[i, i + 1]) < 1e-11
             assert abs(O.expect_site_term(L, M, v, "Sminus", i, "Splus", i + 1) - G[i, i + 1]) < 1e-11
+
+
+# ---------------------------------------------------------------------------
+# momentum sectors: translations and Bloch basis against the reference's
+# test/model/example13.output (4-site ring) and example15.output (2x2 square,
+# translation_momenta); Hamiltonians Hermitian and their spectra, over all
+# momenta, the spectrum of the Sz sector
+# ---------------------------------------------------------------------------
+def _parse_bloch_fixture(path):
+    blocks, cur = [], None
+    for line in open(path):
+        line = line.strip()
+        if line.startswith("Momentum:"):
+            cur = {"k": [float(x) for x in line.split()[1:]], "trans": [], "states": []}
+            blocks.append(cur)
+        elif line.startswith("Translation"):
+            if cur is None or (cur["states"] and line.startswith("Translation 0 ")):
+                cur = {"k": None, "trans": [], "states": []}
+                blocks.append(cur)
+            m = re.match(r"Translation (\d+) with phase \(([-\d.e]+),([-\d.e]+)\) maps (.*)$", line)
+            maps = [int(x.split("->")[1]) for x in m.group(4).split()]
+            cur["trans"].append((complex(float(m.group(2)), float(m.group(3))), maps))
+        elif "] + translations" in line:
+            norm = float(line.split("*[")[0])
+            spins = re.findall(r"\|1/2 (-?1/2) >", line)
+            bits = "".join("1" if s == "1/2" else "0" for s in spins)
+            cur["states"].append((int(bits, 2), norm))
+    return blocks
+
+
+def _check_bloch_block(name, L, W, k, block):
+    N = L * (W if W else 1)
+    ph, pm = O.translations(name, L, W, k=k)
+    assert len(ph) == len(block["trans"])
+    for (want_ph, want_map), p, m in zip(block["trans"], ph, pm):
+        assert abs(round(p.real, 1) - want_ph.real) < 1e-12 and abs(round(p.imag, 1) - want_ph.imag) < 1e-12
+        assert list(m) == want_map
+    reps, norm = O.bloch_basis(N, -1, ph, pm)
+    assert [int(r) for r in reps] == [s for s, _ in block["states"]]
+    for got, (_, want) in zip(norm, block["states"]):
+        assert abs(got - want) <= 1e-5 * want
+
+
+def test_bloch_ring_example13(golden_dir):
+    blocks = _parse_bloch_fixture(os.path.join(golden_dir, "model_example13.output"))
+    assert len(blocks) == 4
+    for ik, b in enumerate(blocks):
+        _check_bloch_block("chain lattice", 4, 0, [2 * ik * math.pi / 4], b)
+
+
+def test_bloch_square_example15(golden_dir):
+    blocks = _parse_bloch_fixture(os.path.join(golden_dir, "model_example15.output"))
+    ks = O.translation_momenta("square lattice", 2)
+    assert len(blocks) == len(ks) == 4
+    for k, b in zip(ks, blocks):
+        np.testing.assert_allclose(k, b["k"], atol=1e-5)
+        _check_bloch_block("square lattice", 2, 2, k, b)
+
+
+@pytest.mark.parametrize("name,L,W,k_up", [("chain lattice", 10, 0, [5, 4]), ("square lattice", 4, 2, [4, 3]),
+                                           ("ladder", 5, 0, [5]), ("triangular lattice", 3, 3, [4]),
+                                           ("frustrated square lattice", 3, 3, [4])])
+def test_bloch_sectors_reproduce_sz_spectrum(name, L, W, k_up):
+    rng = np.random.default_rng(L + W)
+    N, s, t, types = O.lattice(name, L, W)
+    # couplings by bond type keep the lattice translation invariant
+    jz_t, jxy_t = rng.uniform(0.3, 1.5, 3), rng.uniform(0.3, 1.5, 3)
+    m = O.Model(N, s, t, jz_t[types], jxy_t[types], h=0.3)
+    for n_up in k_up:
+        ev = []
+        for k in O.translation_momenta(name, L, W):
+            ph, pm = O.translations(name, L, W, k=k)
+            H = O.bloch_dense(m, n_up, ph, pm)
+            if H.size:
+                assert np.abs(H - H.conj().T).max() < 1e-12
+                ev += list(np.linalg.eigvalsh(H))
+        full = np.linalg.eigvalsh(O.dense(m, n_up))
+        assert len(ev) == len(full)
+        np.testing.assert_allclose(np.sort(ev), full, atol=1e-11)
