@@ -44,6 +44,8 @@ def lib():
             "alps_ed_set_stream": ([P], i32),
             "alps_ed_synchronize": ([], i32),
             "alps_ed_operator_create": ([P, P], i32),
+            "alps_ed_operator_create_momentum": ([P, i32, P, P, P, P], i32),
+            "alps_ed_sparsediag_momentum": ([P, i32, P, P, P, P, P], i32),
             "alps_ed_operator_destroy": ([P], i32),
             "alps_ed_operator_dimension": ([P, P], i32),
             "alps_ed_operator_bytes": ([P, P], i32),
@@ -195,14 +197,31 @@ def synchronize():
     _check(lib().alps_ed_synchronize())
 
 
-class Operator:
-    """Matrix-free device Hamiltonian of one sector (alps_ed_operator)."""
+def _translation_args(translations, n_sites):
+    """(phases, site maps) as lattice translations list them -> C arrays."""
+    phases, perms = translations
+    perms = np.ascontiguousarray(perms, dtype=np.int32).reshape(-1, n_sites)
+    re = np.ascontiguousarray(np.real(phases), dtype=np.float64)
+    im = np.ascontiguousarray(np.imag(phases), dtype=np.float64)
+    assert len(re) == perms.shape[0]
+    return len(re), perms, re, im
 
-    def __init__(self, model: SpinModel):
+
+class Operator:
+    """Matrix-free device Hamiltonian of one sector (alps_ed_operator).  With
+    translations=(phases, site maps) it is the momentum sector of the
+    model's Sz sector, whose vectors are realified (length 2 * sector_dim)."""
+
+    def __init__(self, model: SpinModel, translations=None):
         self.model = model
         self._m = model.cstruct()
         h = ctypes.c_void_p()
-        _check(lib().alps_ed_operator_create(ctypes.byref(self._m), ctypes.byref(h)))
+        if translations is None:
+            _check(lib().alps_ed_operator_create(ctypes.byref(self._m), ctypes.byref(h)))
+        else:
+            nt, perms, re, im = _translation_args(translations, model.n_sites)
+            _check(lib().alps_ed_operator_create_momentum(ctypes.byref(self._m), nt, _p(perms), _p(re), _p(im),
+                                                          ctypes.byref(h)))
         self.handle = h
         d = ctypes.c_uint64()
         _check(lib().alps_ed_operator_dimension(self.handle, ctypes.byref(d)))
@@ -389,12 +408,18 @@ def tmatrix_analyze(alpha, beta):
     return dict(eig=ev[: k.value], err=err[: k.value], mult=mult[: k.value])
 
 
-def sparsediag(model: SpinModel, max_iter=0, nev=1, seed=0, dtype="f64", fixed=False):
-    """One sector of SparseDiagMatrix::do_subspace on the device."""
+def sparsediag(model: SpinModel, max_iter=0, nev=1, seed=0, dtype="f64", fixed=False, translations=None):
+    """One sector of SparseDiagMatrix::do_subspace on the device (the
+    momentum sector when translations=(phases, site maps) is given)."""
     m = model.cstruct()
     p = _SolveParams(max_iter, nev, int(fixed), seed, _DTYPES[dtype])
     r = _SolveResult()
-    _check(lib().alps_ed_sparsediag(ctypes.byref(m), ctypes.byref(p), ctypes.byref(r)))
+    if translations is None:
+        _check(lib().alps_ed_sparsediag(ctypes.byref(m), ctypes.byref(p), ctypes.byref(r)))
+    else:
+        nt, perms, re, im = _translation_args(translations, model.n_sites)
+        _check(lib().alps_ed_sparsediag_momentum(ctypes.byref(m), nt, _p(perms), _p(re), _p(im), ctypes.byref(p),
+                                                 ctypes.byref(r)))
     return dict(dim=r.dim, ev=np.array(r.eigenvalues[: r.n_eigenvalues]), iterations=r.iterations,
                 error_code=r.error_code, seconds=r.seconds)
 

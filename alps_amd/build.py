@@ -14,7 +14,7 @@ OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libalps_ed.so")
 CLI = os.path.join(OUT_DIR, "alps_ed_sparsediag")
 
-SOURCES = ["ed_kernels.hip", "ed_capi.hip", "ed_batch_kernels.hip", "ed_batch.hip", "ed_measure.hip"]
+SOURCES = ["ed_kernels.hip", "ed_capi.hip", "ed_batch_kernels.hip", "ed_batch.hip", "ed_measure.hip", "ed_bloch.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -1,0 +1,221 @@
+// ed_bloch.hpp -- momentum (Bloch) sectors of the spin-1/2 path: lattice
+// translations, the Bloch basis of a fixed-Sz sector and its device form.
+//
+// Mirrors:
+//   - hypercubic_lattice::translations (src/alps/lattice/hypercubic.h:425-462)
+//     and translation_momenta (:403-423): one translation per shift of the
+//     periodic directions, the first direction fastest, with phase
+//     exp(i sum_d shift_d k_d); momenta 2 pi j / extent per periodic
+//     direction, the first direction outermost;
+//   - bloch_basis_states::build (src/alps/model/blochbasisstates.h:97-271):
+//     states in ascending order; a state's representative is the smallest
+//     state of its orbit, reached by the first translation (in order) that
+//     maps it there; a new representative is kept when
+//     w = sqrt|1 + sum of the phases of the translations that fix it| >= 0.01,
+//     with normalization w / sqrt(N); states of dropped orbits are not in
+//     the basis;
+//   - the matrix in that basis (hamiltonian_matrix.hpp:268-345): the element
+//     from representative i to a state s' is
+//     element * phase(s') * norm(rep(s')) / norm(i).
+// The orbit structure (smallest state and the first translation reaching
+// it, the translations fixing each representative) does not depend on the
+// momentum; it is built once per sector (BlochOrbits) and each momentum
+// selects its kept orbits (BlochSector).
+#pragma once
+
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ed_layout.hpp"
+#include "ed_sparsediag.hpp"
+
+namespace alps_ed {
+
+struct Translations {
+  std::vector<std::vector<int>> perm;  // perm[t][i]: image of site i
+  std::vector<std::complex<double>> phase;
+};
+
+inline std::vector<int> periodic_directions(const LatticeGraph& g) {
+  std::vector<int> dirs;
+  for (size_t d = 0; d < g.extent.size(); ++d)
+    if (g.periodic[d]) dirs.push_back((int)d);
+  return dirs;
+}
+
+inline std::vector<std::vector<double>> translation_momenta(const LatticeGraph& g) {
+  std::vector<std::vector<double>> ks;
+  for (int d : periodic_directions(g)) {
+    std::vector<std::vector<double>> next;
+    if (ks.empty()) ks.push_back({});
+    for (const auto& k : ks)
+      for (int j = 0; j < g.extent[d]; ++j) {
+        std::vector<double> v = k;
+        v.push_back(2. * j * M_PI / g.extent[d]);
+        next.push_back(v);
+      }
+    ks.swap(next);
+  }
+  return ks;
+}
+
+inline Translations lattice_translations(const LatticeGraph& g, const std::vector<double>& k) {
+  const std::vector<int> dirs = periodic_directions(g);
+  if (k.size() != dirs.size()) throw std::invalid_argument("the momentum needs one entry per periodic direction");
+  const size_t dim = g.extent.size();
+  Translations t;
+  std::vector<int> shift(dirs.size(), 0);
+  while (true) {
+    double phase = 0.;
+    std::vector<int> off(dim, 0);
+    for (size_t i = 0; i < dirs.size(); ++i) {
+      off[dirs[i]] = shift[i];
+      phase += shift[i] * k[i];
+    }
+    std::vector<int> perm(g.n);
+    for (int s = 0; s < g.n; ++s) {
+      int idx = 0;
+      for (size_t d = 0; d < dim; ++d) idx = idx * g.extent[d] + (g.offset[s][d] + off[d]) % g.extent[d];
+      perm[s] = idx;
+    }
+    t.perm.push_back(perm);
+    t.phase.push_back(std::exp(std::complex<double>(0., phase)));
+    bool done = true;
+    for (size_t i = 0; i < shift.size() && done; ++i) {
+      if (++shift[i] >= g.extent[dirs[i]]) shift[i] = 0;
+      else done = false;
+    }
+    if (done) break;
+  }
+  return t;
+}
+
+// Site permutation applied to bit-encoded states (site i is bit N-1-i)
+// through one table per byte.
+struct BitPermutation {
+  std::vector<uint32_t> tab;  // [4][256]
+  BitPermutation(int N, const std::vector<int>& perm) : tab(4 * 256, 0u) {
+    for (int byte = 0; byte < 4; ++byte)
+      for (uint32_t v = 0; v < 256; ++v) {
+        uint32_t out = 0;
+        for (int b = 0; b < 8; ++b) {
+          const int bit = byte * 8 + b;
+          if (((v >> b) & 1u) && bit < N) out |= 1u << (N - 1 - perm[N - 1 - bit]);
+        }
+        tab[byte * 256 + v] = out;
+      }
+  }
+  uint32_t operator()(uint32_t s) const {
+    return tab[s & 255u] | tab[256 + ((s >> 8) & 255u)] | tab[512 + ((s >> 16) & 255u)] | tab[768 + (s >> 24)];
+  }
+};
+
+inline uint32_t split_rank(const SectorLayout& L, uint32_t s) {
+  const uint32_t h = L.NH > 0 ? (s >> L.B) : 0u;
+  return L.hi_base[h] + L.lo_rank[s & ((1u << L.B) - 1u)];
+}
+
+// Momentum-independent orbit data of a fixed-Sz sector.
+struct BlochOrbits {
+  std::vector<uint32_t> state;     // per rank: the state (ascending)
+  std::vector<uint32_t> rep_rank;  // per rank: rank of its orbit's smallest state
+  std::vector<uint8_t> tr;         // per rank: first translation mapping it there (0 for the smallest)
+  std::vector<uint32_t> reps;      // ranks of the smallest states, ascending
+  std::vector<uint64_t> fixes;     // per entry of reps: translations t >= 1 that fix it (bit t)
+};
+
+inline BlochOrbits bloch_orbits(const SectorLayout& L, const Translations& t) {
+  const int N = L.N;
+  if (N > 32) throw std::invalid_argument("momentum sectors support at most 32 sites");
+  if (t.perm.size() > 64) throw std::invalid_argument("at most 64 translations");
+  for (size_t i = 0; i < t.perm.size(); ++i) {
+    if ((int)t.perm[i].size() != N) throw std::invalid_argument("translation does not map every site");
+    std::vector<int> seen(N, 0);
+    for (int s : t.perm[i]) {
+      if (s < 0 || s >= N || seen[s]) throw std::invalid_argument("translation is not a permutation of the sites");
+      seen[s] = 1;
+    }
+  }
+  std::vector<BitPermutation> bp;
+  for (size_t i = 1; i < t.perm.size(); ++i) bp.emplace_back(N, t.perm[i]);
+  const uint64_t D = L.D;
+  BlochOrbits o;
+  o.state.resize(D);
+  o.rep_rank.resize(D);
+  o.tr.resize(D);
+  // fixed-Sz states in ascending order (the rank order)
+  uint32_t s = L.k == 0 ? 0u : ((L.k >= 32 ? 0xffffffffu : ((1u << L.k) - 1u)));
+  for (uint64_t r = 0; r < D; ++r) {
+    o.state[r] = s;
+    if (s != 0u) {  // next larger state with the same number of bits (Gosper)
+      const uint32_t c = s & (0u - s), rr = s + c;
+      s = (((rr ^ s) >> 2) / c) | rr;
+    }
+  }
+  for (uint64_t r = 0; r < D; ++r) {
+    const uint32_t x = o.state[r];
+    if (split_rank(L, x) != r) throw std::logic_error("sector states are not in rank order");
+    uint32_t best = x;
+    int best_t = 0;
+    uint64_t fix = 0;
+    for (size_t i = 0; i < bp.size(); ++i) {
+      const uint32_t y = bp[i](x);
+      if (y == x) fix |= uint64_t(1) << (i + 1);
+      else if (y < best) {  // the first translation reaching the smallest state
+        best = y;
+        best_t = (int)i + 1;
+      }
+    }
+    if (best == x) {
+      o.rep_rank[r] = (uint32_t)r;
+      o.tr[r] = 0;
+      o.reps.push_back((uint32_t)r);
+      o.fixes.push_back(fix);
+    } else {
+      o.rep_rank[r] = split_rank(L, best);
+      o.tr[r] = (uint8_t)best_t;
+    }
+  }
+  return o;
+}
+
+constexpr uint32_t kNoState = 0xffffffffu;
+
+// The Bloch basis at one momentum.
+struct BlochSector {
+  std::vector<uint32_t> rep_state;  // kept representatives, ascending
+  std::vector<double> norm;
+  std::vector<uint32_t> full_rep;   // per rank: kept representative index, or kNoState
+  std::vector<uint8_t> full_tr;     // per rank: translation index of its phase
+  std::vector<double> ph_re, ph_im; // per translation
+};
+
+inline BlochSector bloch_sector(const BlochOrbits& o, const Translations& t, int N) {
+  BlochSector b;
+  const double l = std::sqrt(double(N));
+  std::vector<uint32_t> kept(o.state.size(), kNoState);  // by representative rank
+  for (size_t q = 0; q < o.reps.size(); ++q) {
+    std::complex<double> sum = 1.;
+    for (size_t i = 1; i < t.phase.size(); ++i)
+      if ((o.fixes[q] >> i) & 1u) sum += t.phase[i];
+    const double w = std::sqrt(std::abs(sum));
+    if (w < 0.01) continue;
+    kept[o.reps[q]] = (uint32_t)b.rep_state.size();
+    b.rep_state.push_back(o.state[o.reps[q]]);
+    b.norm.push_back(w / l);
+  }
+  b.full_rep.resize(o.state.size());
+  b.full_tr = o.tr;
+  for (size_t r = 0; r < o.state.size(); ++r) b.full_rep[r] = kept[o.rep_rank[r]];
+  for (const auto& p : t.phase) {
+    b.ph_re.push_back(p.real());
+    b.ph_im.push_back(p.imag());
+  }
+  return b;
+}
+
+}  // namespace alps_ed

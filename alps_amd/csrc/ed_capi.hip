@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -9,6 +10,7 @@
 #include <memory>
 #include <new>
 #include <stdexcept>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -17,6 +19,7 @@
 #include "alps_ed.h"
 #include "ed_host.hpp"
 #include "ed_kernels.hpp"
+#include "ed_bloch.hpp"
 #include "ed_measure.hpp"
 #include "ed_sparsediag.hpp"
 #include "ed_xml.hpp"
@@ -71,6 +74,11 @@ struct alps_ed_operator_s {
       nbp_off, xn16, xn_off, ll_t, hh_mask, hh_t, xb_hbit, xb_par, xb_anti, xb_t, site_dn, site_up, bond_p1,
       bond_p2, bond_par, bond_anti, bond_t;
   uint64_t bytes = 0;
+  uint64_t n = 0;  // vector length: D, or 2 * dk for a momentum sector (realified)
+  // momentum sector (ed_bloch.hpp): rows are the kept representatives
+  bool bloch = false;
+  DevBloch bdev{};
+  DevBuf b_rep_state, b_norm, b_full_rep, b_full_tr, b_ph_re, b_ph_im;
 };
 
 struct alps_ed_vector_s {
@@ -96,7 +104,7 @@ struct alps_ed_lanczos_s {
   hipStream_t stream = nullptr;
 
   // slots are padded to 64 elements, so every slot is 256-byte aligned
-  size_t vec_bytes() const { return ((op->layout.D + 63) & ~uint64_t(63)) * (dtype == ALPS_ED_F32 ? 4 : 8); }
+  size_t vec_bytes() const { return ((op->n + 63) & ~uint64_t(63)) * (dtype == ALPS_ED_F32 ? 4 : 8); }
   void* slot(int j) {
     if (!keep_basis) return u[j & 1].p;
     const size_t c = (size_t)j / kChunk;
@@ -106,6 +114,16 @@ struct alps_ed_lanczos_s {
 };
 
 namespace {
+
+// H.v of an operator of either kind, with the number of partials its DOT
+// form writes.
+unsigned op_hv_blocks(const alps_ed_operator_s* op) { return op->bloch ? bloch_blocks(op->bdev) : hv_blocks(op->dev); }
+
+template <class V>
+hipError_t op_hv(const alps_ed_operator_s* op, const V* x, V* y, double* dot_out, hipStream_t st) {
+  return op->bloch ? launch_bloch_hv<V>(op->dev, op->bdev, x, y, dot_out, st)
+                   : launch_hv<V>(op->dev, x, y, dot_out, st);
+}
 
 void operator_upload(alps_ed_operator op) {
   SectorLayout& L = op->layout;
@@ -160,6 +178,24 @@ void operator_upload(alps_ed_operator op) {
                     &op->site_dn, &op->site_up, &op->bond_p1, &op->bond_p2, &op->bond_par, &op->bond_anti,
                     &op->bond_t})
     op->bytes += b->bytes;
+  op->n = L.D;
+}
+
+// Turns an uploaded operator into its momentum sector.
+void bloch_upload(alps_ed_operator op, const BlochSector& b) {
+  upload(op->b_rep_state, b.rep_state);
+  upload(op->b_norm, b.norm);
+  upload(op->b_full_rep, b.full_rep);
+  upload(op->b_full_tr, b.full_tr);
+  upload(op->b_ph_re, b.ph_re);
+  upload(op->b_ph_im, b.ph_im);
+  op->bdev = DevBloch{(uint32_t)b.rep_state.size(), op->b_rep_state.as<uint32_t>(), op->b_norm.as<double>(),
+                      op->b_full_rep.as<uint32_t>(), op->b_full_tr.as<uint8_t>(), op->b_ph_re.as<double>(),
+                      op->b_ph_im.as<double>()};
+  op->bloch = true;
+  op->n = 2 * (uint64_t)b.rep_state.size();
+  for (DevBuf* x : {&op->b_rep_state, &op->b_norm, &op->b_full_rep, &op->b_full_tr, &op->b_ph_re, &op->b_ph_im})
+    op->bytes += x->bytes;
 }
 
 size_t dtype_size(alps_ed_dtype t) { return t == ALPS_ED_F32 ? 4 : 8; }
@@ -196,7 +232,6 @@ double* norm_buf(alps_ed_lanczos lz, int j) { return (j & 1) ? lz->red.c() : lz-
 
 template <class V>
 void lanczos_step_dev(alps_ed_lanczos lz, int j) {
-  const DevLayout& L = lz->op->dev;
   V* cur = static_cast<V*>(lz->slot(j));
   V* prev = j > 0 ? static_cast<V*>(lz->slot(j - 1)) : cur;
   V* next = static_cast<V*>(lz->slot(j + 1));
@@ -204,13 +239,13 @@ void lanczos_step_dev(alps_ed_lanczos lz, int j) {
   double* alpha = lz->alpha.as<double>();
   double* beta = lz->beta.as<double>();
   double* scale = lz->scale.as<double>();
-  hip_check(launch_hv<V>(L, cur, w, lz->red.a(), lz->stream), "hv");
-  hip_check(launch_update_fused<V>(L.D, w, cur, prev, next, Partials{lz->red.a(), hv_blocks(L)},
+  hip_check(op_hv<V>(lz->op, cur, w, lz->red.a(), lz->stream), "hv");
+  hip_check(launch_update_fused<V>(lz->op->n, w, cur, prev, next, Partials{lz->red.a(), op_hv_blocks(lz->op)},
                                    Partials{norm_buf(lz, j), lz->norm_count[j & 1]}, alpha + j,
                                    j > 0 ? beta + j - 1 : nullptr, scale + j, j > 0 ? scale + j - 1 : nullptr,
                                    norm_buf(lz, j + 1), lz->stream),
             "update");
-  lz->norm_count[(j + 1) & 1] = fused_blocks(L.D);
+  lz->norm_count[(j + 1) & 1] = fused_blocks(lz->op->n);
 }
 
 // beta_{n-1} and scale_n after n steps (the host reads beta).
@@ -270,6 +305,40 @@ int alps_ed_operator_create(const alps_ed_model* model, alps_ed_operator* out) {
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+Translations read_translations(int N, int n_t, const int* perm, const double* re, const double* im) {
+  require(n_t >= 1 && n_t <= 64, "between 1 and 64 translations");
+  require(perm && re && im, "null argument");
+  Translations t;
+  for (int i = 0; i < n_t; ++i) {
+    t.perm.emplace_back(perm + (size_t)i * N, perm + (size_t)(i + 1) * N);
+    t.phase.emplace_back(re[i], im[i]);
+  }
+  for (int s = 0; s < N; ++s) require(t.perm[0][s] == s, "the first translation must be the identity");
+  return t;
+}
+
+}  // namespace
+
+extern "C" {
+
+int alps_ed_operator_create_momentum(const alps_ed_model* model, int n_translations, const int* perm,
+                                     const double* phase_re, const double* phase_im, alps_ed_operator* out) {
+  return guarded([&] {
+    require(out != nullptr && model != nullptr, "null argument");
+    const Translations t = read_translations(model->n_sites, n_translations, perm, phase_re, phase_im);
+    std::unique_ptr<alps_ed_operator_s> op(new alps_ed_operator_s);
+    op->layout = build_layout(model_spec(model));
+    require(op->layout.D >= 1, "empty sector");
+    operator_upload(op.get());
+    bloch_upload(op.get(), bloch_sector(bloch_orbits(op->layout, t), t, op->layout.N));
+    *out = op.release();
+  });
+}
+
 int alps_ed_operator_destroy(alps_ed_operator op) {
   delete op;
   return ALPS_ED_OK;
@@ -278,7 +347,7 @@ int alps_ed_operator_destroy(alps_ed_operator op) {
 int alps_ed_operator_dimension(alps_ed_operator op, uint64_t* dim) {
   return guarded([&] {
     require(op && dim, "null argument");
-    *dim = op->layout.D;
+    *dim = op->n;
   });
 }
 
@@ -292,6 +361,7 @@ int alps_ed_operator_bytes(alps_ed_operator op, uint64_t* bytes) {
 int alps_ed_operator_states(alps_ed_operator op, uint64_t first, uint64_t count, uint32_t* states_out) {
   return guarded([&] {
     require(op && (states_out || count == 0), "null argument");
+    require(!op->bloch, "explicit states are not available for a momentum sector");
     require(first + count <= op->layout.D, "state range beyond the dimension");
     if (count == 0) return;
     DevBuf d(count * sizeof(uint32_t));
@@ -317,6 +387,7 @@ int alps_ed_operator_elements(alps_ed_operator op, uint64_t row0, uint64_t nrows
                               uint32_t* cols, double* vals, int* counts) {
   return guarded([&] {
     require(op && ((cols && vals && counts) || nrows == 0), "null argument");
+    require(!op->bloch, "explicit elements are not available for a momentum sector");
     require(row0 + nrows <= op->layout.D, "row range beyond the dimension");
     require(max_per_row >= (int)op->layout.bond_p1.size() + 1, "max_per_row must be >= n_bonds + 1");
     if (nrows == 0) return;
@@ -500,6 +571,7 @@ int alps_ed_vector_spin_moments(alps_ed_operator op, alps_ed_vector v, double* n
   return guarded([&] {
     require(op != nullptr, "operator is null");
     check_vec(v);
+    require(!op->bloch, "spin sums need a fixed-Sz sector without momentum");
     require(v->n == op->layout.D, "vector size != operator dimension");
     require(op->layout.D >= 1, "empty sector");
     const DevLayout& L = op->dev;
@@ -525,6 +597,7 @@ int alps_ed_vector_flip_pairs(alps_ed_operator op, alps_ed_vector v, int n_pairs
   return guarded([&] {
     require(op != nullptr, "operator is null");
     check_vec(v);
+    require(!op->bloch, "spin sums need a fixed-Sz sector without momentum");
     require(v->n == op->layout.D, "vector size != operator dimension");
     require(n_pairs >= 0 && (n_pairs == 0 || (site_i && site_j && out)), "null argument");
     if (n_pairs == 0) return;
@@ -560,12 +633,12 @@ int alps_ed_matvec(alps_ed_operator op, alps_ed_vector x, alps_ed_vector y) {
     require(op != nullptr, "operator is null");
     check_vec(x);
     check_vec(y);
-    require(x->n == op->layout.D && y->n == op->layout.D, "vector size != operator dimension");
+    require(x->n == op->n && y->n == op->n, "vector size != operator dimension");
     require(x->dtype == y->dtype, "dtype mismatch");
     require(x != y, "in-place product is not supported");
     with_dtype(x->dtype, [&](auto z) {
       using V = decltype(z);
-      hip_check(launch_hv<V>(op->dev, x->buf.as<V>(), y->buf.as<V>(), nullptr, current_stream()), "hv");
+      hip_check(op_hv<V>(op, x->buf.as<V>(), y->buf.as<V>(), nullptr, current_stream()), "hv");
     });
   });
 }
@@ -580,18 +653,18 @@ int alps_ed_lanczos_step_vectors(alps_ed_operator op, alps_ed_vector v_prev, alp
     check_vec(v_prev);
     check_vec(v_cur);
     check_vec(w);
-    const uint64_t n = op->layout.D;
+    const uint64_t n = op->n;
     require(v_prev->n == n && v_cur->n == n && w->n == n, "vector size != operator dimension");
     require(v_prev->dtype == v_cur->dtype && v_cur->dtype == w->dtype, "dtype mismatch");
     Reducer& r = thread_reducer();
-    r.ensure(std::max<size_t>(hv_blocks(op->dev), stream_blocks(n)));
+    r.ensure(std::max<size_t>(op_hv_blocks(op), stream_blocks(n)));
     double* res = r.result(0);
     hip_check(hipMemcpyAsync(res + 2, &beta_prev, sizeof(double), hipMemcpyHostToDevice, current_stream()), "copy");
     with_dtype(w->dtype, [&](auto z) {
       using V = decltype(z);
       V* wv = w->buf.as<V>();
-      hip_check(launch_hv<V>(op->dev, v_cur->buf.as<V>(), wv, r.a(), current_stream()), "hv");
-      hip_check(launch_finish(Partials{r.a(), hv_blocks(op->dev)}, nullptr, res, nullptr, kAlpha, current_stream()), "finish");
+      hip_check(op_hv<V>(op, v_cur->buf.as<V>(), wv, r.a(), current_stream()), "hv");
+      hip_check(launch_finish(Partials{r.a(), op_hv_blocks(op)}, nullptr, res, nullptr, kAlpha, current_stream()), "finish");
       hip_check(launch_update<V>(n, wv, v_cur->buf.as<V>(), v_prev->buf.as<V>(), wv, res, nullptr, res + 2, nullptr,
                                  r.b(), current_stream()),
                 "update");
@@ -615,13 +688,13 @@ int alps_ed_lanczos_create(alps_ed_operator op, alps_ed_dtype dtype, int max_ste
     require(op && out, "null argument");
     require(dtype == ALPS_ED_F32 || dtype == ALPS_ED_F64, "unknown dtype");
     require(max_steps >= 1, "max_steps must be >= 1");
-    require(op->layout.D >= 1, "empty sector");
+    require(op->n >= 1, "empty sector");
     std::unique_ptr<alps_ed_lanczos_s> lz(new alps_ed_lanczos_s);
     lz->op = op;
     lz->dtype = dtype;
     lz->max_steps = max_steps;
     lz->keep_basis = keep_basis != 0;
-    const uint64_t n = op->layout.D;
+    const uint64_t n = op->n;
     if (!lz->keep_basis) {
       lz->u[0].alloc(lz->vec_bytes());
       lz->u[1].alloc(lz->vec_bytes());
@@ -630,7 +703,7 @@ int alps_ed_lanczos_create(alps_ed_operator op, alps_ed_dtype dtype, int max_ste
     lz->alpha.alloc((max_steps + 2) * sizeof(double));
     lz->beta.alloc((max_steps + 2) * sizeof(double));
     lz->scale.alloc((max_steps + 3) * sizeof(double));
-    lz->red.ensure(std::max<size_t>(hv_blocks(op->dev), stream_blocks(n)));
+    lz->red.ensure(std::max<size_t>(op_hv_blocks(op), stream_blocks(n)));
     lz->stream = current_stream();
     *out = lz.release();
   });
@@ -644,7 +717,7 @@ int alps_ed_lanczos_destroy(alps_ed_lanczos lz) {
 int alps_ed_lanczos_start(alps_ed_lanczos lz, uint64_t seed, alps_ed_vector start) {
   return guarded([&] {
     require(lz != nullptr, "null argument");
-    const uint64_t n = lz->op->layout.D;
+    const uint64_t n = lz->op->n;
     lz->stream = current_stream();
     with_dtype(lz->dtype, [&](auto z) {
       using V = decltype(z);
@@ -706,7 +779,7 @@ int alps_ed_lanczos_vectors(alps_ed_lanczos lz, alps_ed_vector v_prev, alps_ed_v
   return guarded([&] {
     require(lz != nullptr, "null argument");
     if (lz->steps == 0) throw StateError("no Lanczos vectors before lanczos_start");
-    const uint64_t n = lz->op->layout.D;
+    const uint64_t n = lz->op->n;
     const int j = lz->steps;  // v_j = scale[j] u_j
     alps_ed_vector outs[2] = {v_prev, v_cur};
     for (int q = 0; q < 2; ++q) {
@@ -734,7 +807,7 @@ int alps_ed_lanczos_ritz(alps_ed_lanczos lz, int nvec, int m, const double* coef
     if (!lz->keep_basis) throw StateError("Ritz vectors need a Lanczos object created with keep_basis");
     require(m >= 1 && m <= lz->steps, "m must be in [1, steps]");
     require(nvec >= 0, "nvec must be >= 0");
-    const uint64_t n = lz->op->layout.D;
+    const uint64_t n = lz->op->n;
     for (int i = 0; i < nvec; ++i) {
       check_vec(outs[i]);
       require(outs[i]->n == n && outs[i]->dtype == ALPS_ED_F64, "Ritz outputs must be FP64 of the dimension");
@@ -801,7 +874,7 @@ std::vector<std::unique_ptr<alps_ed_vector_s>> ritz_vectors(alps_ed_lanczos lz, 
   std::vector<double> d(a.begin(), a.begin() + m), z((size_t)m * m);
   if (ietl::tridiag::eig_full(m, d.data(), b.data(), z.data()) > 0)
     throw std::runtime_error("LAPACK error, stev function failed.");
-  const uint64_t n = lz->op->layout.D;
+  const uint64_t n = lz->op->n;
   std::vector<std::unique_ptr<alps_ed_vector_s>> out;
   for (size_t i0 = 0; i0 < index.size(); i0 += kMaxRitz) {
     const int nv = (int)std::min<size_t>(kMaxRitz, index.size() - i0);
@@ -820,10 +893,10 @@ std::vector<std::unique_ptr<alps_ed_vector_s>> ritz_vectors(alps_ed_lanczos lz, 
 
 // Rayleigh quotient y.Hy / y.y (FP64 accumulation) of a device FP64 vector.
 double rayleigh_quotient(alps_ed_lanczos lz, alps_ed_vector y, alps_ed_vector hy) {
-  const uint64_t n = lz->op->layout.D;
+  const uint64_t n = lz->op->n;
   double* res = lz->red.result(0);
-  hip_check(launch_hv<double>(lz->op->dev, y->buf.as<double>(), hy->buf.as<double>(), lz->red.a(), lz->stream), "hv");
-  hip_check(launch_finish(Partials{lz->red.a(), hv_blocks(lz->op->dev)}, nullptr, res, nullptr, kSum, lz->stream),
+  hip_check(op_hv<double>(lz->op, y->buf.as<double>(), hy->buf.as<double>(), lz->red.a(), lz->stream), "hv");
+  hip_check(launch_finish(Partials{lz->red.a(), op_hv_blocks(lz->op)}, nullptr, res, nullptr, kSum, lz->stream),
             "finish");
   hip_check(launch_dot<double>(n, y->buf.as<double>(), y->buf.as<double>(), lz->red.b(), lz->stream), "dot");
   hip_check(launch_finish(Partials{lz->red.b(), stream_blocks(n)}, nullptr, res + 1, nullptr, kSum, lz->stream),
@@ -842,7 +915,7 @@ std::vector<double> refine_ritz(alps_ed_lanczos lz, const std::vector<double>& a
                                 std::vector<std::unique_ptr<alps_ed_vector_s>>* keep) {
   std::vector<std::unique_ptr<alps_ed_vector_s>> ys = ritz_vectors(lz, a, b, m, index);
   alps_ed_vector hy = nullptr;
-  check_status(alps_ed_vector_create(lz->op->layout.D, ALPS_ED_F64, &hy));
+  check_status(alps_ed_vector_create(lz->op->n, ALPS_ED_F64, &hy));
   std::unique_ptr<alps_ed_vector_s> hold_hy(hy);
   std::vector<double> rq;
   for (auto& y : ys) rq.push_back(rayleigh_quotient(lz, y.get(), hy));
@@ -861,19 +934,46 @@ struct SectorSolve {
   std::vector<std::unique_ptr<alps_ed_vector_s>> vectors;
 };
 
-SectorSolve solve_sector(const alps_ed_model* model, const alps_ed_solve_params* params, bool want_vectors) {
+// A momentum sector of a fixed-Sz sector: its layout, orbits and the
+// translations at the momentum (the first must be the identity).
+struct MomentumInput {
+  const SectorLayout* layout;
+  const BlochOrbits* orbits;
+  const Translations* trans;
+};
+
+SectorSolve solve_sector(const alps_ed_model* model, const alps_ed_solve_params* params, bool want_vectors,
+                         const MomentumInput* mom = nullptr) {
   auto t0 = std::chrono::steady_clock::now();
   SectorSolve out;
   alps_ed_solve_result* result = &out.r;
   std::memset(result, 0, sizeof(*result));
   out.op.reset(new alps_ed_operator_s);
   alps_ed_operator_s* op = out.op.get();
-  op->layout = build_layout(model_spec(model));
-  const uint64_t D = op->layout.D;
+  op->layout = mom ? *mom->layout : build_layout(model_spec(model));
+  if (op->layout.D > 0) operator_upload(op);
+  if (mom && op->layout.D > 0) bloch_upload(op, bloch_sector(*mom->orbits, *mom->trans, op->layout.N));
+  const uint64_t D = mom ? op->n / 2 : op->layout.D;  // the sector's dimension
   result->dim = D;
   auto seconds = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
   if (D == 0) return out;  // sparsediag.h:101-102
-  operator_upload(op);
+  if (D == 1 && mom) {  // the single (real) element of a one-state momentum sector
+    alps_ed_vector x = nullptr, y = nullptr;
+    check_status(alps_ed_vector_create(2, ALPS_ED_F64, &x));
+    std::unique_ptr<alps_ed_vector_s> hx(x);
+    check_status(alps_ed_vector_create(2, ALPS_ED_F64, &y));
+    std::unique_ptr<alps_ed_vector_s> hy(y);
+    const double e0[2] = {1.0, 0.0};
+    check_status(alps_ed_vector_upload(x, e0, 2));
+    check_status(alps_ed_matvec(op, x, y));
+    double h[2];
+    check_status(alps_ed_vector_download(y, h, 2));
+    result->n_eigenvalues = 1;
+    result->eigenvalues[0] = h[0];
+    if (want_vectors) out.vectors.push_back(std::move(hx));
+    result->seconds = seconds();
+    return out;
+  }
   if (D == 1) {  // sparsediag.h:120-124: the single matrix element
     std::vector<uint32_t> cols(op->layout.bond_p1.size() + 1);
     std::vector<double> vals(cols.size());
@@ -979,6 +1079,20 @@ extern "C" int alps_ed_sparsediag(const alps_ed_model* model, const alps_ed_solv
   });
 }
 
+extern "C" int alps_ed_sparsediag_momentum(const alps_ed_model* model, int n_translations, const int* perm,
+                                           const double* phase_re, const double* phase_im,
+                                           const alps_ed_solve_params* params, alps_ed_solve_result* result) {
+  return guarded([&] {
+    require(model && params && result, "null argument");
+    const Translations t = read_translations(model->n_sites, n_translations, perm, phase_re, phase_im);
+    const SectorLayout L = build_layout(model_spec(model));
+    const BlochOrbits o = bloch_orbits(L, t);
+    const MomentumInput mi{&L, &o, &t};
+    SectorSolve s = solve_sector(model, params, false, &mi);
+    *result = s.r;
+  });
+}
+
 namespace {
 
 std::string json_list(const std::vector<double>& v) {
@@ -1059,6 +1173,54 @@ void emit(const std::string& js, char* out, size_t out_len, size_t* needed) {
 
 }  // namespace
 
+namespace {
+
+// TRANSLATION_SYMMETRY as a boolean parameter (default true,
+// applications/diag/diag.h:91 and factory.C:25).
+bool translation_symmetry(const Parameters& p, const Evaluator& ev) {
+  if (!p.count("TRANSLATION_SYMMETRY")) return true;
+  std::string v = p.at("TRANSLATION_SYMMETRY");
+  for (char& c : v) c = (char)std::tolower((unsigned char)c);
+  if (v == "true" || v == "yes") return true;
+  if (v == "false" || v == "no") return false;
+  return ev.value(v) != 0.0;
+}
+
+// The momenta a task loops over (DiagMatrix::dostep, diag.h:90-97,117-120):
+// every momentum of the lattice's periodic directions unless
+// TRANSLATION_SYMMETRY is false, or the one TOTAL_MOMENTUM given (then
+// `loop` is false); none on lattices without periodic directions.
+std::vector<std::vector<double>> task_momenta(const Parameters& p, const Evaluator& ev, const LatticeGraph& g,
+                                              bool& loop) {
+  loop = false;
+  std::vector<std::vector<double>> ks;
+  if (ev.defined("TOTAL_MOMENTUM")) {
+    std::vector<double> k;
+    std::istringstream is(p.at("TOTAL_MOMENTUM"));
+    for (std::string tok; is >> tok;) k.push_back(ev.value(tok));
+    ks.push_back(k);
+  } else if (translation_symmetry(p, ev)) {
+    ks = translation_momenta(g);
+    loop = !ks.empty();
+  }
+  return ks;
+}
+
+// alps::write_vector with its default precision of 20 digits
+// (src/alps/utility/vectorio.hpp:170-180): the TOTAL_MOMENTUM value of a
+// momentum sector.
+std::string momentum_string(const std::vector<double>& k) {
+  std::string o;
+  for (size_t i = 0; i < k.size(); ++i) {
+    char b[48];
+    std::snprintf(b, sizeof b, "%.20g", k[i]);
+    o += (i ? " " : "") + std::string(b);
+  }
+  return o;
+}
+
+}  // namespace
+
 extern "C" int alps_ed_describe_parameter_file(const char* text, char* out, size_t out_len, size_t* needed) {
   return guarded([&] {
     require(text != nullptr, "parameter text is null");
@@ -1077,6 +1239,13 @@ extern "C" int alps_ed_describe_parameter_file(const char* text, char* out, size
               ", \"bond_t\": " + json_list(s2) + ", \"jz\": " + json_list(m.jz) + ", \"jxy\": " + json_list(m.jxy) +
               ", \"k\": " + json_list(m.k) + ", \"h\": " + json_list(m.h) + ", \"d\": " + json_list(m.d) +
               ", \"n_up\": " + json_list(up) + ", \"whole_space\": " + (ts.whole_space ? "true" : "false");
+        bool loop = false;
+        const std::vector<std::vector<double>> ks = task_momenta(p, Evaluator(p), m.graph, loop);
+        if (!ks.empty()) {
+          js += ", \"momenta\": [";
+          for (size_t q = 0; q < ks.size(); ++q) js += (q ? ", " : "") + json_list(ks[q]);
+          js += "]";
+        }
         const MeasureSpec ms = measure_spec(p);
         if (ms.any()) js += ", \"measure\": " + json_measure_plan(ms, m.graph);
       }
@@ -1086,6 +1255,7 @@ extern "C" int alps_ed_describe_parameter_file(const char* text, char* out, size
     emit(js, out, out_len, needed);
   });
 }
+
 
 extern "C" int alps_ed_run_parameter_file(const char* text, char* out, size_t out_len, size_t* needed) {
   return guarded([&] {
@@ -1105,38 +1275,39 @@ extern "C" int alps_ed_run_parameter_file(const char* text, char* out, size_t ou
       js += ti ? ", {" : "{";
       js += "\"parameters\": " + json_params(p) + ", \"n_sites\": " + std::to_string(m.n_sites) + ", \"sectors\": [";
       const TaskSectors ts = task_sectors(p, m.n_sites);
-      std::vector<std::vector<double>> evs;
-      std::vector<std::string> entries;
-      uint64_t dim_sum = 0;
-      int it_sum = 0, err_max = 0;
-      double sec_sum = 0.0;
-      // All sectors of the task are solved together as one batch
-      // (DiagMatrix::dostep's sector loop, diag.h:86-138) when their
-      // footprint fits the free device memory, else one after the other.
+      // momenta (DiagMatrix::dostep, diag.h:90-97,117-120): every momentum of
+      // the lattice's periodic directions unless TRANSLATION_SYMMETRY is
+      // false, or the one TOTAL_MOMENTUM given; the loop runs inside the Sz
+      // loop.  Lattices without periodic directions have none.
+      bool loop_momenta = false;
+      const std::vector<std::vector<double>> ks = task_momenta(p, ev, m.graph, loop_momenta);
+      const bool momentum = !ks.empty();
       std::vector<alps_ed_model> models;
       for (int n_up : ts.n_up)
         models.push_back(alps_ed_model{m.n_sites, n_up, (int)m.s1.size(), m.s1.data(), m.s2.data(), m.jz.data(),
                                        m.jxy.data(), m.k.data(), m.h.data(), m.d.data()});
-      std::vector<alps_ed_solve_result> res(models.size());
+      const size_t nk = momentum ? ks.size() : 1;
+      // one result per (Sz sector, momentum), the momentum fastest
+      std::vector<alps_ed_solve_result> res(models.size() * nk);
       // Eigenvector measurements (DiagMatrix::perform_measurements,
       // diag.h:191-314) need the eigenvectors of every sector: those tasks
       // solve their sectors one by one with the Lanczos basis kept.
       const MeasureSpec ms = measure_spec(p);
-      const bool measure = ms.any() && !ts.whole_space;
-      std::vector<std::string> meas_json(models.size());
-      std::vector<MeasureValues> meas_val(models.size());
-      std::vector<XmlSector> xsec;
-      // sectors of a quantum-number loop carry their Sz (diag.h:100-105)
-      const bool qn_loop = !ts.whole_space && !ev.defined("Sz_total");
-      uint64_t need = 0;
-      for (const alps_ed_model& am : models) {
-        uint64_t f = 0;
-        check_status(alps_ed_batch_footprint(&am, &sp, sp.dtype, &f));
-        need += f + (1u << 20);
-      }
-      size_t free_b = 0, total_b = 0;
-      hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-      if (measure) {
+      const bool measure = ms.any() && !ts.whole_space && !momentum;
+      std::vector<std::string> meas_json(res.size());
+      std::vector<MeasureValues> meas_val(res.size());
+      if (momentum) {
+        std::vector<Translations> trans;
+        for (const auto& k : ks) trans.push_back(lattice_translations(m.graph, k));
+        for (size_t i = 0; i < models.size(); ++i) {
+          const SectorLayout L = build_layout(model_spec(&models[i]));
+          const BlochOrbits orb = bloch_orbits(L, trans[0]);
+          for (size_t q = 0; q < nk; ++q) {
+            const MomentumInput mi{&L, &orb, &trans[q]};
+            res[i * nk + q] = solve_sector(&models[i], &sp, false, &mi).r;
+          }
+        }
+      } else if (measure) {
         const MeasurePlan plan = measure_plan(ms, m.graph);
         for (size_t i = 0; i < models.size(); ++i) {
           SectorSolve sv = solve_sector(&models[i], &sp, true);
@@ -1147,60 +1318,99 @@ extern "C" int alps_ed_run_parameter_file(const char* text, char* out, size_t ou
           meas_val[i] = evaluate_measurements(ms, m.graph, sums);
           meas_json[i] = measurements_json(meas_val[i], plan);
         }
-      } else if (models.size() > 1 && need < 0.8 * (double)free_b) {
-        alps_ed_batch_config cfg{sp.dtype, need, 0, 8, 0, 0};
-        alps_ed_batch bh = nullptr;
-        check_status(alps_ed_batch_create(&cfg, &bh));
-        std::unique_ptr<alps_ed_batch_s, int (*)(alps_ed_batch)> hold(bh, alps_ed_batch_destroy);
-        std::vector<uint64_t> tickets(models.size());
-        for (size_t i = 0; i < models.size(); ++i) check_status(alps_ed_batch_submit(bh, &models[i], &sp, &tickets[i]));
-        check_status(alps_ed_batch_run(bh));
-        for (size_t i = 0; i < models.size(); ++i) {
-          int ready = 0;
-          check_status(alps_ed_batch_result(bh, tickets[i], &res[i], &ready));
-          if (!ready || res[i].error_code < 0) throw DeviceError("sector solve did not complete");
-        }
       } else {
-        for (size_t i = 0; i < models.size(); ++i) check_status(alps_ed_sparsediag(&models[i], &sp, &res[i]));
+        // All sectors of the task are solved together as one batch
+        // (DiagMatrix::dostep's sector loop, diag.h:86-138) when their
+        // footprint fits the free device memory, else one after the other.
+        uint64_t need = 0;
+        for (const alps_ed_model& am : models) {
+          uint64_t f = 0;
+          check_status(alps_ed_batch_footprint(&am, &sp, sp.dtype, &f));
+          need += f + (1u << 20);
+        }
+        size_t free_b = 0, total_b = 0;
+        hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        if (models.size() > 1 && need < 0.8 * (double)free_b) {
+          alps_ed_batch_config cfg{sp.dtype, need, 0, 8, 0, 0};
+          alps_ed_batch bh = nullptr;
+          check_status(alps_ed_batch_create(&cfg, &bh));
+          std::unique_ptr<alps_ed_batch_s, int (*)(alps_ed_batch)> hold(bh, alps_ed_batch_destroy);
+          std::vector<uint64_t> tickets(models.size());
+          for (size_t i = 0; i < models.size(); ++i)
+            check_status(alps_ed_batch_submit(bh, &models[i], &sp, &tickets[i]));
+          check_status(alps_ed_batch_run(bh));
+          for (size_t i = 0; i < models.size(); ++i) {
+            int ready = 0;
+            check_status(alps_ed_batch_result(bh, tickets[i], &res[i], &ready));
+            if (!ready || res[i].error_code < 0) throw DeviceError("sector solve did not complete");
+          }
+        } else {
+          for (size_t i = 0; i < models.size(); ++i) check_status(alps_ed_sparsediag(&models[i], &sp, &res[i]));
+        }
       }
-      for (size_t i = 0; i < models.size(); ++i) {
+      // sectors of a quantum-number loop carry their Sz (diag.h:100-105), a
+      // momentum loop its TOTAL_MOMENTUM (:117-120)
+      const bool qn_loop = !ts.whole_space && !ev.defined("Sz_total");
+      std::vector<std::vector<double>> evs;
+      std::vector<std::string> entries;
+      std::vector<XmlSector> xsec;
+      // per momentum, for the whole-space merge
+      std::vector<std::vector<std::vector<double>>> by_k(nk);
+      std::vector<uint64_t> dim_k(nk, 0);
+      std::vector<int> it_k(nk, 0), err_k(nk, 0);
+      std::vector<double> sec_k(nk, 0.0);
+      for (size_t i = 0; i < res.size(); ++i) {
         const alps_ed_solve_result& r = res[i];
-        const int n_up = ts.n_up[i];
+        const int n_up = ts.n_up[i / nk];
+        const size_t q = i % nk;
         if (r.dim == 0) continue;  // empty subspaces are skipped (diag.h:115)
         std::vector<double> e(r.eigenvalues, r.eigenvalues + r.n_eigenvalues);
         evs.push_back(e);
-        dim_sum += r.dim;
-        it_sum += r.iterations;
-        err_max = std::max(err_max, r.error_code);
-        sec_sum += r.seconds;
+        by_k[q].push_back(e);
+        dim_k[q] += r.dim;
+        it_k[q] += r.iterations;
+        err_k[q] = std::max(err_k[q], r.error_code);
+        sec_k[q] += r.seconds;
         entries.push_back("\"Sz_total\": " + json_number(n_up - 0.5 * m.n_sites) + ", \"n_up\": " +
-                          std::to_string(n_up) + ", \"dim\": " + std::to_string(r.dim) + ", \"iterations\": " +
+                          std::to_string(n_up) + (momentum ? ", \"momentum\": " + json_list(ks[q]) : std::string()) +
+                          ", \"dim\": " + std::to_string(r.dim) + ", \"iterations\": " +
                           std::to_string(r.iterations) + ", \"error_code\": " + std::to_string(r.error_code) +
                           ", \"seconds\": " + json_number(r.seconds) + ", \"eigenvalues\": " + json_list(e) +
                           (measure ? ", \"measurements\": " + meas_json[i] : std::string()));
         XmlSector xs;
         if (qn_loop) xs.qn.push_back({"Sz", half_integer_string(2 * n_up - m.n_sites)});
+        if (loop_momenta) xs.qn.push_back({"TOTAL_MOMENTUM", momentum_string(ks[q])});
         xs.ev = e;
         if (measure) xs.meas = &meas_val[i];
         xsec.push_back(xs);
       }
       if (ts.whole_space) {
-        // one subspace without quantum numbers: the merged spectrum
-        const std::vector<double> e = merge_spectra(evs, sp.nev, 1e-9);
-        evs.assign(1, e);
-        xsec.assign(1, XmlSector{{}, e, nullptr});
-        entries.assign(1, "\"Sz_total\": null, \"n_up\": null, \"dim\": " + std::to_string(dim_sum) +
-                              ", \"iterations\": " + std::to_string(it_sum) + ", \"error_code\": " +
-                              std::to_string(err_max) + ", \"seconds\": " + json_number(sec_sum) +
-                              ", \"eigenvalues\": " + json_list(e));
+        // one subspace without Sz (per momentum): the merged spectrum
+        evs.clear();
+        entries.clear();
+        xsec.clear();
+        for (size_t q = 0; q < nk; ++q) {
+          if (by_k[q].empty()) continue;
+          const std::vector<double> e = merge_spectra(by_k[q], sp.nev, 1e-9);
+          evs.push_back(e);
+          XmlSector xs;
+          if (loop_momenta) xs.qn.push_back({"TOTAL_MOMENTUM", momentum_string(ks[q])});
+          xs.ev = e;
+          xsec.push_back(xs);
+          entries.push_back("\"Sz_total\": null, \"n_up\": null" +
+                            (momentum ? ", \"momentum\": " + json_list(ks[q]) : std::string()) +
+                            ", \"dim\": " + std::to_string(dim_k[q]) + ", \"iterations\": " +
+                            std::to_string(it_k[q]) + ", \"error_code\": " + std::to_string(err_k[q]) +
+                            ", \"seconds\": " + json_number(sec_k[q]) + ", \"eigenvalues\": " + json_list(e));
+        }
       }
       for (size_t i = 0; i < entries.size(); ++i) js += (i ? ", {" : "{") + entries[i] + "}";
       double g = 0, gap = 0;
       ground_and_gap(evs, g, gap);
       js += "], \"ground_state_energy\": " + json_number(g) + ", \"energy_gap\": " + json_number(gap);
-      if (ms.any() && ts.whole_space)
-        js += ", \"measurement_error\": \"eigenvector measurements need Sz-conserved sectors "
-              "(CONSERVED_QUANTUMNUMBERS=\\\"Sz\\\" or Sz_total)\"";
+      if (ms.any() && !measure)
+        js += std::string(", \"measurement_error\": \"eigenvector measurements need Sz-conserved sectors") +
+              " (CONSERVED_QUANTUMNUMBERS=\\\"Sz\\\" or Sz_total) without momenta (TRANSLATION_SYMMETRY=false)\"";
       const int nev_xml = ev.defined("NUMBER_EIGENVALUES") ? sp.nev : 0;
       js += ", \"xml\": " +
             json_string(task_xml(p, xsec, nev_xml, !std::isnan(g), g, !std::isnan(gap), gap));

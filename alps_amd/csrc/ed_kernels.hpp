@@ -45,6 +45,18 @@ struct DevLayout {
   const double* bond_t;
 };
 
+// A momentum sector (Bloch basis, ed_bloch.hpp) on the device.  Vectors of
+// the sector are complex and stored realified: (Re v[0..dk), Im v[0..dk)).
+struct DevBloch {
+  uint32_t dk;                // kept representatives
+  const uint32_t* rep_state;  // [dk]
+  const double* norm;         // [dk]
+  const uint32_t* full_rep;   // [D] per rank of the fixed-Sz sector
+  const uint8_t* full_tr;     // [D]
+  const double* ph_re;        // [translations]
+  const double* ph_im;
+};
+
 // Reductions without atomics: the producing kernel stores one partial sum
 // per block; a consuming kernel adds the partials in index order (every
 // block of the consumer computes the same value, so results are
@@ -156,6 +168,13 @@ uint32_t flip_pair_blocks(const DevLayout& L);
 template <class V>
 hipError_t launch_flip_pairs(const DevLayout& L, const V* v, const uint32_t* masks, int n_pairs, double* part,
                              double* out, hipStream_t st);
+
+// y = H x in a momentum sector (ed_bloch.hip), x and y realified; with
+// dot_out, per-block partials of Re <x, Hx> (bloch_blocks(B) of them).
+inline unsigned bloch_blocks(const DevBloch& B) { return B.dk ? (B.dk + kBlock - 1) / kBlock : 1u; }
+template <class V>
+hipError_t launch_bloch_hv(const DevLayout& L, const DevBloch& B, const V* x, V* y, double* dot_out,
+                           hipStream_t st);
 
 hipError_t launch_unrank(const DevLayout& L, uint64_t first, uint64_t count, uint32_t* out, hipStream_t st);
 hipError_t launch_rank(const DevLayout& L, const uint32_t* states, uint64_t count, uint64_t* out,

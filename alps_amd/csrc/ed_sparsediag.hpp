@@ -11,8 +11,8 @@
 //   - SparseDiagMatrix::do_subspace (sparsediag.h:93-149) per sector;
 //   - the "Ground State Energy" and "Energy Gap" of write_xml_body
 //     (sparsediag.h:59-90).
-// Momentum (Bloch) sectors are not decomposed: every Sz sector is solved
-// whole, as with TRANSLATION_SYMMETRY=false.
+// The momentum loop inside each Sz sector and the Bloch bases are in
+// ed_bloch.hpp.
 #pragma once
 
 #include <algorithm>

@@ -85,8 +85,22 @@ typedef struct {
  * matrix-free: nothing of size nnz is stored). ---------------------------- */
 
 int alps_ed_operator_create(const alps_ed_model* model, alps_ed_operator* op);
+/* The momentum (Bloch) sector of the model's fixed-Sz sector
+ * (bloch_basis_states, src/alps/model/blochbasisstates.h:97-271, and the
+ * matrix in it, hamiltonian_matrix.hpp:268-345) under n_translations
+ * translations as graph_helper::translations(k) lists them
+ * (hypercubic.h:425-462): site maps perm[t*n_sites + i] (image of site i;
+ * t = 0 is the identity) with phases (phase_re[t], phase_im[t]).  The
+ * sector's vectors are complex and stored realified, (Re v[0..d),
+ * Im v[0..d)): the operator's dimension is 2d and every vector call
+ * (matvec, Lanczos, Ritz vectors) works on that form.  Explicit states,
+ * elements and spin sums are not available for it.  The model must be
+ * invariant under the translations. */
+int alps_ed_operator_create_momentum(const alps_ed_model* model, int n_translations, const int* perm,
+                                     const double* phase_re, const double* phase_im, alps_ed_operator* op);
 int alps_ed_operator_destroy(alps_ed_operator op);
-/* hamiltonian_matrix::dimension (hamiltonian_matrix.hpp:92). */
+/* hamiltonian_matrix::dimension (hamiltonian_matrix.hpp:92); the length of
+ * the operator's vectors (twice the sector dimension for a momentum sector). */
 int alps_ed_operator_dimension(alps_ed_operator op, uint64_t* dim);
 /* Device bytes held by the operator's tables. */
 int alps_ed_operator_bytes(alps_ed_operator op, uint64_t* bytes);
@@ -225,6 +239,11 @@ typedef struct {
 
 int alps_ed_sparsediag(const alps_ed_model* model, const alps_ed_solve_params* params,
                        alps_ed_solve_result* result);
+/* do_subspace in a momentum sector (translations as for
+ * alps_ed_operator_create_momentum); result->dim is the sector dimension. */
+int alps_ed_sparsediag_momentum(const alps_ed_model* model, int n_translations, const int* perm,
+                                const double* phase_re, const double* phase_im, const alps_ed_solve_params* params,
+                                alps_ed_solve_result* result);
 
 /* ---------------------------------------------------------------------------
  * Batched solves and the request loop.
@@ -289,23 +308,30 @@ int alps_ed_batch_footprint(const alps_ed_model* model, const alps_ed_solve_para
 /* The sparsediag task for the spin-1/2 path: runs every task of a parameter
  * file (the grammar of src/alps/parameter/parameterlist.h:20-36; LATTICE,
  * L, W, H, MODEL="spin", local_S=1/2, J/Jz/Jxy/J'/J1/Jz#/Jxy#/K#/h#/D#,
- * CONSERVED_QUANTUMNUMBERS, Sz_total, MAX_ITERATIONS, NUMBER_EIGENVALUES,
- * plus ED_PRECISION = "f64" | "f32" and ED_SEED) as
+ * CONSERVED_QUANTUMNUMBERS, Sz_total, TRANSLATION_SYMMETRY, TOTAL_MOMENTUM,
+ * MAX_ITERATIONS, NUMBER_EIGENVALUES, MEASURE_AVERAGE/LOCAL/CORRELATIONS/
+ * STRUCTURE_FACTOR[name], plus ED_PRECISION = "f64" | "f32" and ED_SEED) as
  * DiagMatrix::dostep does (applications/diag/diag.h:85-175): one Lanczos
- * solve per Sz sector (all sectors of a task advance together as one batch,
- * see alps_ed_batch_*, when their footprint fits the free device memory;
- * else one after the other), then the ground-state energy and gap of
- * SparseDiagMatrix::write_xml_body (sparsediag.h:59-90).  The results are
- * written to `out` as a JSON document (at most out_len bytes including the
- * terminating zero; *needed receives the full length). */
+ * solve per Sz sector and, on lattices with periodic directions unless
+ * TRANSLATION_SYMMETRY is false, per momentum inside it (the Bloch sectors
+ * of alps_ed_operator_create_momentum; TOTAL_MOMENTUM selects one).  Sz
+ * sectors without momenta advance together as one batch (alps_ed_batch_*)
+ * when their footprint fits the free device memory, else one after the
+ * other.  Then the ground-state energy and gap of
+ * SparseDiagMatrix::write_xml_body (sparsediag.h:59-90), the eigenvector
+ * measurements (diag.h:191-314; Sz sectors without momenta) and the task's
+ * result XML.  The results are written to `out` as a JSON document (at most
+ * out_len bytes including the terminating zero; *needed receives the full
+ * length). */
 int alps_ed_run_parameter_file(const char* text, char* out, size_t out_len, size_t* needed);
 
 /* The same parameter file resolved on the host only, without solving: per
  * task its parameters, the lattice's sites and bonds in edge order (the
  * graph of make_graph_from_lattice, src/alps/lattice/latticegraph.h:35-171),
  * the per-bond Jz, Jxy, K and per-site h, D after the spin model's defaults
- * (lib/xml/models.xml.in:342-373), and the Sz sectors to solve
- * (applications/diag/diag.h:142-175).  Needs no device. */
+ * (lib/xml/models.xml.in:342-373), the Sz sectors and momenta to solve
+ * (applications/diag/diag.h:85-175) and the measurement requests with
+ * their labels.  Needs no device. */
 int alps_ed_describe_parameter_file(const char* text, char* out, size_t out_len, size_t* needed);
 
 #ifdef __cplusplus

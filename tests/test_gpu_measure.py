@@ -100,6 +100,7 @@ def _oracle_values(m, k, v, graph_pairs, bonds):
 def test_parameter_file_measurements_match_oracle(lattice, size, sz, precision, tol):
     say("measurements", lattice, size, sz, precision)
     text = (f'LATTICE="{lattice}"; {size}; J=1; Jz=0.8; h=0.1; Sz_total={sz}; ED_PRECISION="{precision}";\n'
+            "TRANSLATION_SYMMETRY=false\n"
             + MEASURES + "{}\n")
     (t,) = A.run_parameter_file(text)
     (d,) = A.describe_parameter_file(text)
@@ -139,7 +140,7 @@ def test_measurements_full_size_ring24_sum_rules():
     # correlations from site 0 sum to zero, <Sz_0^2> = 1/4, and the Splus:Sminus
     # structure factor sums to sum_i <S+_i S-_i> = N/2.
     say("measurements ring24")
-    text = ('LATTICE="chain lattice"; L=24; J=1; Sz_total=0; ED_PRECISION="f32";\n'
+    text = ('LATTICE="chain lattice"; L=24; J=1; Sz_total=0; ED_PRECISION="f32"; TRANSLATION_SYMMETRY=false;\n'
             "MEASURE_LOCAL[Bond energy]=exchange\nMEASURE_CORRELATIONS[SzSz]=Sz\n"
             'MEASURE_STRUCTURE_FACTOR[Sxy]="Splus:Sminus"\n{}\n')
     (t,) = A.run_parameter_file(text)
@@ -160,7 +161,7 @@ def test_measurements_full_size_ring24_sum_rules():
 
 
 def test_measurements_need_sz_sectors():
-    text = 'LATTICE="chain lattice"; L=6; J=1;\nMEASURE_LOCAL[M]=Sz\n{}\n'
+    text = 'LATTICE="chain lattice"; L=6; J=1; TRANSLATION_SYMMETRY=false;\nMEASURE_LOCAL[M]=Sz\n{}\n'
     (t,) = A.run_parameter_file(text)
     assert "measurement_error" in t
     assert all("measurements" not in s for s in t["sectors"])
@@ -189,6 +190,7 @@ def test_task_xml_matches_results(tmp_path):
     # integers written as the reference's half_integer prints them
     say("task xml")
     text = ('LATTICE="chain lattice"; L=5; J=1; Jz=0.7; CONSERVED_QUANTUMNUMBERS="Sz"; NUMBER_EIGENVALUES=2;\n'
+            "TRANSLATION_SYMMETRY=false\n"
             "MEASURE_LOCAL[Local magnetization]=Sz\nMEASURE_AVERAGE[Magnetization]=Sz\n{}\n")
     (t,) = A.run_parameter_file(text)
     evs, states = _xml_check(t["xml"], t)
@@ -215,7 +217,7 @@ def test_cli_writes_task_xml(tmp_path):
     import subprocess
 
     f = tmp_path / "parm.txt"
-    f.write_text('LATTICE="chain lattice"\nL=8\nJ=1\nSz_total=0\n{}\n{J=0.5;}\n')
+    f.write_text('LATTICE="chain lattice"\nL=8\nJ=1\nSz_total=0\nTRANSLATION_SYMMETRY=false\n{}\n{J=0.5;}\n')
     cli = os.path.join(os.path.dirname(A.LIB_PATH), "alps_ed_sparsediag")
     say("cli xml", cli)
     subprocess.run([cli, "--write-xml", str(f)], check=True, capture_output=True, text=True, timeout=60)

@@ -333,7 +333,7 @@ def test_parameter_file_sectors_ground_and_gap(prec):
 def test_parameter_file_whole_space():
     # no conserved quantum numbers: the lowest distinct eigenvalues of the
     # whole 2^N space, from the dense spectra of all sectors together
-    text = 'LATTICE="chain lattice"; L=9; J=1; NUMBER_EIGENVALUES=3; {} Sz_total=0.5; {}'
+    text = 'LATTICE="chain lattice"; L=9; J=1; NUMBER_EIGENVALUES=3; TRANSLATION_SYMMETRY=false; {} Sz_total=0.5; {}'
     say("parameter file, whole space")
     whole, one = A.run_parameter_file(text)
     om = O.lattice_model("chain lattice", 9)
@@ -357,7 +357,7 @@ def test_cli_runs_parameter_file(tmp_path):
     import json
 
     f = tmp_path / "parm"
-    f.write_text('LATTICE="chain lattice"\nL=8\nJ=1\nSz_total=0\nNUMBER_EIGENVALUES=2\n{}\n')
+    f.write_text('LATTICE="chain lattice"\nL=8\nJ=1\nSz_total=0\nNUMBER_EIGENVALUES=2\nTRANSLATION_SYMMETRY=false\n{}\n')
     cli = os.path.join(os.path.dirname(A.LIB_PATH), "alps_ed_sparsediag")
     say("cli", cli)
     out = subprocess.run([cli, str(f)], check=True, capture_output=True, text=True, timeout=60).stdout
