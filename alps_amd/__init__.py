@@ -47,6 +47,8 @@ def lib():
             "alps_ed_operator_create_momentum": ([P, i32, P, P, P, P], i32),
             "alps_ed_sparsediag_momentum": ([P, i32, P, P, P, P, P], i32),
             "alps_ed_operator_destroy": ([P], i32),
+            "alps_ed_operator_schedule": ([P, P, P], i32),
+            "alps_ed_operator_set_schedule": ([P, i32], i32),
             "alps_ed_operator_dimension": ([P, P], i32),
             "alps_ed_operator_bytes": ([P, P], i32),
             "alps_ed_operator_states": ([P, u64, u64, P], i32),
@@ -128,7 +130,8 @@ class _SolveResult(ctypes.Structure):
 
 class _BatchConfig(ctypes.Structure):
     _fields_ = [("dtype", ctypes.c_int), ("hbm_bytes", ctypes.c_uint64), ("host_bytes", ctypes.c_uint64),
-                ("steps_per_round", ctypes.c_int), ("max_active", ctypes.c_int), ("host_threads", ctypes.c_int)]
+                ("steps_per_round", ctypes.c_int), ("max_active", ctypes.c_int), ("host_threads", ctypes.c_int),
+                ("graphs", ctypes.c_int)]
 
 
 class _BatchStatus(ctypes.Structure):
@@ -137,7 +140,8 @@ class _BatchStatus(ctypes.Structure):
                 ("solves_done", ctypes.c_uint64), ("seconds_admit", ctypes.c_double),
                 ("seconds_steps", ctypes.c_double), ("seconds_decide", ctypes.c_double),
                 ("seconds_refine", ctypes.c_double), ("hv_rows_f32", ctypes.c_uint64),
-                ("hv_rows_f64", ctypes.c_uint64)]
+                ("hv_rows_f64", ctypes.c_uint64), ("graph_rounds", ctypes.c_uint64),
+                ("graph_captures", ctypes.c_uint64)]
 
 
 def _result_dict(r):
@@ -263,6 +267,19 @@ class Operator:
     def matvec(self, x: "Vector", y: "Vector"):
         _check(lib().alps_ed_matvec(self.handle, x.handle, y.handle))
         return y
+
+    HV_SCHEDULES = 6
+
+    def schedule(self):
+        """(chosen H.v schedule for FP32 vectors, seconds per schedule or None
+        when the operator was not timed)."""
+        s = ctypes.c_int()
+        sec = np.zeros(self.HV_SCHEDULES)
+        _check(lib().alps_ed_operator_schedule(self.handle, ctypes.byref(s), _p(sec)))
+        return s.value, (None if sec[0] < 0 else sec)
+
+    def set_schedule(self, s):
+        _check(lib().alps_ed_operator_set_schedule(self.handle, int(s)))
 
     def spin_moments(self, v: "Vector"):
         """(sum v^2, sum v^2 Sz_i [N], sum v^2 Sz_i Sz_j [N, N]) on the device."""
@@ -451,8 +468,10 @@ class Batch:
     call progress() (one round) or run() (until all are done), read results
     by ticket."""
 
-    def __init__(self, dtype="f32", hbm_bytes=0, host_bytes=0, steps_per_round=0, max_active=0, host_threads=0):
-        cfg = _BatchConfig(_DTYPES[dtype], hbm_bytes, host_bytes, steps_per_round, max_active, host_threads)
+    def __init__(self, dtype="f32", hbm_bytes=0, host_bytes=0, steps_per_round=0, max_active=0, host_threads=0,
+                 graphs=False):
+        cfg = _BatchConfig(_DTYPES[dtype], hbm_bytes, host_bytes, steps_per_round, max_active, host_threads,
+                           int(bool(graphs)))
         h = ctypes.c_void_p()
         _check(lib().alps_ed_batch_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h

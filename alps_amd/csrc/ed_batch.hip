@@ -236,6 +236,11 @@ struct alps_ed_batch_s {
   uint64_t next_ticket = 1;
   uint64_t rounds = 0, steps = 0, solves_done = 0;
   uint64_t hv_rows[2] = {0, 0};  // rows of Lanczos-step H.v, FP64 / FP32 storage
+  // the round's steps as a HIP graph (cfg.graphs): the launch key of the
+  // last round and of the captured graph
+  std::vector<uint64_t> last_key, graph_key;
+  hipGraphExec_t graph_exec = nullptr;
+  uint64_t graph_rounds = 0, graph_captures = 0;
   double t_admit = 0, t_steps = 0, t_decide = 0, t_refine = 0;
   // device work lists (rebuilt every round)
   DevBuf jobs_d, items_d, list_d;
@@ -250,6 +255,7 @@ struct alps_ed_batch_s {
 
   ~alps_ed_batch_s() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (dev_base) (void)hipFree(dev_base);
     if (host_base) (void)hipHostFree(host_base);
     if (stage) (void)hipHostFree(stage);
@@ -304,6 +310,9 @@ DevLayout place_tables(const SectorLayout& L, char* base, char* stage, const Sha
   d.n_ll_p = L.n_ll_p;
   d.n_hh_p = L.n_hh_p;
   d.n_xb_p = L.n_xb_p;
+  d.n_ll = (int)L.ll.size();
+  d.n_hh = (int)L.hh.size();
+  d.n_xb = (int)L.xb.size();
   d.n_bonds = (int)L.bond_p1.size();
   d.lo_rank = sh.lo_rank;
   d.lo_state = sh.lo_state;
@@ -904,14 +913,47 @@ void round(alps_ed_batch b) {
       hip_check(launch_batch_start<double>(g.jobs, g.st64, g.n_st64, b->stream), "batch start");
       hip_check(launch_batch_finish(g.jobs, g.st_list, g.n_st_list, 0, kBatchStart, b->stream), "batch finish");
     }
-    for (int t = 0; t < S; ++t) {
-      hip_check(launch_batch_hv<float>(g.jobs, g.hv32, g.n_hv32, t, g.seg32, b->stream), "batch hv");
-      hip_check(launch_batch_hv<double>(g.jobs, g.hv64, g.n_hv64, t, g.seg64, b->stream), "batch hv");
-      hip_check(launch_batch_finish(g.jobs, g.list, g.n_list, t, kBatchAlpha, b->stream), "batch finish");
-      hip_check(launch_batch_update<float>(g.jobs, g.up32, g.n_up32, t, b->stream), "batch update");
-      hip_check(launch_batch_update<double>(g.jobs, g.up64, g.n_up64, t, b->stream), "batch update");
-      hip_check(launch_batch_finish(g.jobs, g.list, g.n_list, t, kBatchBeta, b->stream), "batch finish");
+    auto steps = [&] {
+      for (int t = 0; t < S; ++t) {
+        hip_check(launch_batch_hv<float>(g.jobs, g.hv32, g.n_hv32, t, g.seg32, b->stream), "batch hv");
+        hip_check(launch_batch_hv<double>(g.jobs, g.hv64, g.n_hv64, t, g.seg64, b->stream), "batch hv");
+        hip_check(launch_batch_finish(g.jobs, g.list, g.n_list, t, kBatchAlpha, b->stream), "batch finish");
+        hip_check(launch_batch_update<float>(g.jobs, g.up32, g.n_up32, t, b->stream), "batch update");
+        hip_check(launch_batch_update<double>(g.jobs, g.up64, g.n_up64, t, b->stream), "batch update");
+        hip_check(launch_batch_finish(g.jobs, g.list, g.n_list, t, kBatchBeta, b->stream), "batch finish");
+      }
+    };
+    // The launches of a round are fixed by the work-list addresses and
+    // counts and S (each job's step base j0 is read from device memory), so
+    // a round that repeats the previous one replays them from a graph.
+    bool replayed = false;
+    if (b->cfg.graphs && g.n_st_list == 0) {
+      const std::vector<uint64_t> key{(uint64_t)g.jobs, (uint64_t)g.hv32, (uint64_t)g.hv64, (uint64_t)g.up32,
+                                      (uint64_t)g.up64, (uint64_t)g.list, g.n_hv32, g.n_hv64, g.n_up32, g.n_up64,
+                                      g.n_list, g.seg32, g.seg64, (uint64_t)S};
+      if (key == b->last_key) {
+        if (!b->graph_exec || key != b->graph_key) {
+          if (b->graph_exec) hip_check(hipGraphExecDestroy(b->graph_exec), "graph destroy");
+          b->graph_exec = nullptr;
+          hipGraph_t graph = nullptr;
+          hip_check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+          steps();
+          hip_check(hipStreamEndCapture(b->stream, &graph), "graph capture");
+          const hipError_t e = hipGraphInstantiate(&b->graph_exec, graph, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(graph);
+          hip_check(e, "graph instantiate");
+          b->graph_key = key;
+          ++b->graph_captures;
+        }
+        hip_check(hipGraphLaunch(b->graph_exec, b->stream), "graph launch");
+        ++b->graph_rounds;
+        replayed = true;
+      }
+      b->last_key = key;
+    } else {
+      b->last_key.clear();
     }
+    if (!replayed) steps();
   }
   // host work overlapping the steps: admission (table copies queue behind
   // the steps on the stream), then the stopping tests on results of earlier
@@ -1070,6 +1112,8 @@ int alps_ed_batch_stats(alps_ed_batch b, alps_ed_batch_status* s) {
     s->seconds_refine = b->t_refine;
     s->hv_rows_f32 = b->hv_rows[1];
     s->hv_rows_f64 = b->hv_rows[0];
+    s->graph_rounds = b->graph_rounds;
+    s->graph_captures = b->graph_captures;
   });
 }
 

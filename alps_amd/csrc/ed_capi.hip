@@ -8,6 +8,8 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
+#include <map>
 #include <new>
 #include <stdexcept>
 #include <sstream>
@@ -75,6 +77,8 @@ struct alps_ed_operator_s {
       bond_p2, bond_par, bond_anti, bond_t;
   uint64_t bytes = 0;
   uint64_t n = 0;  // vector length: D, or 2 * dk for a momentum sector (realified)
+  int schedule = 0;  // H.v schedule for FP32 vectors (kHvSchedule)
+  std::vector<double> schedule_seconds;  // per schedule, when tuned
   // momentum sector (ed_bloch.hpp): rows are the kept representatives
   bool bloch = false;
   DevBloch bdev{};
@@ -122,7 +126,70 @@ unsigned op_hv_blocks(const alps_ed_operator_s* op) { return op->bloch ? bloch_b
 template <class V>
 hipError_t op_hv(const alps_ed_operator_s* op, const V* x, V* y, double* dot_out, hipStream_t st) {
   return op->bloch ? launch_bloch_hv<V>(op->dev, op->bdev, x, y, dot_out, st)
-                   : launch_hv<V>(op->dev, x, y, dot_out, st);
+                   : launch_hv<V>(op->dev, x, y, dot_out, st, op->schedule);
+}
+
+// The H.v autotuner: an operator of a fixed-Sz sector with at least
+// kTuneMinDim states times every FP32 schedule on its own sector (two
+// round-robin passes of one warm-up and three timed launches each, HIP
+// events on the engine's stream) and keeps the fastest.  Choices are cached per sector shape (sites, up
+// spins, low-word size and the bonds), so later operators of the same shape
+// reuse them.  Schedules differ only in speed, never in results.
+constexpr uint64_t kTuneMinDim = 200000;
+static_assert(ALPS_ED_HV_SCHEDULES == kHvSchedules, "the C header lists every schedule");
+
+std::vector<uint64_t> schedule_key(const SectorLayout& L) {
+  uint64_t h = 1469598103934665603ull;  // FNV-1a over the bond positions
+  for (size_t b = 0; b < L.bond_p1.size(); ++b)
+    for (uint32_t v : {L.bond_p1[b], L.bond_p2[b]}) h = (h ^ v) * 1099511628211ull;
+  return {(uint64_t)L.N, (uint64_t)L.k, (uint64_t)L.B, L.ll.size(), L.hh.size(), L.xb.size(), h};
+}
+
+void tune_schedule(alps_ed_operator op) {
+  if (op->bloch || op->layout.D < kTuneMinDim) return;
+  static std::mutex mu;
+  static std::map<std::vector<uint64_t>, std::pair<int, std::vector<double>>> cache;
+  const std::vector<uint64_t> key = schedule_key(op->layout);
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      op->schedule = it->second.first;
+      op->schedule_seconds = it->second.second;
+      return;
+    }
+  }
+  const uint64_t n = op->layout.D;
+  hipStream_t st = current_stream();
+  DevBuf x(n * sizeof(float)), y(n * sizeof(float));
+  hip_check(launch_fill_random<float>(n, x.as<float>(), 1, -1.0, 1.0, st), "fill_random");
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "event");
+  hip_check(hipEventCreate(&e1), "event");
+  // two round-robin passes, the faster of each schedule's two times: the
+  // first launches of a pass see cold caches whatever the schedule
+  std::vector<double> sec(kHvSchedules, 1e30);
+  for (int pass = 0; pass < 2; ++pass)
+    for (int s = 0; s < kHvSchedules; ++s) {
+      hip_check(launch_hv<float>(op->dev, x.as<float>(), y.as<float>(), nullptr, st, s), "hv");
+      hip_check(hipEventRecord(e0, st), "event");
+      for (int r = 0; r < 3; ++r)
+        hip_check(launch_hv<float>(op->dev, x.as<float>(), y.as<float>(), nullptr, st, s), "hv");
+      hip_check(hipEventRecord(e1, st), "event");
+      hip_check(hipEventSynchronize(e1), "event");
+      float ms = 0.f;
+      hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
+      sec[s] = std::min(sec[s], ms * 1e-3 / 3);
+    }
+  int best = 0;
+  for (int s = 1; s < kHvSchedules; ++s)
+    if (sec[s] < sec[best]) best = s;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  op->schedule = best;
+  op->schedule_seconds = sec;
+  std::lock_guard<std::mutex> g(mu);
+  cache[key] = {best, sec};
 }
 
 void operator_upload(alps_ed_operator op) {
@@ -139,6 +206,9 @@ void operator_upload(alps_ed_operator op) {
   d.n_ll_p = L.n_ll_p;
   d.n_hh_p = L.n_hh_p;
   d.n_xb_p = L.n_xb_p;
+  d.n_ll = (int)L.ll.size();
+  d.n_hh = (int)L.hh.size();
+  d.n_xb = (int)L.xb.size();
   d.n_bonds = (int)L.bond_p1.size();
 #define ALPS_ED_UP(field, T) \
   upload(op->field, L.field); \
@@ -301,6 +371,7 @@ int alps_ed_operator_create(const alps_ed_model* model, alps_ed_operator* out) {
     std::unique_ptr<alps_ed_operator_s> op(new alps_ed_operator_s);
     op->layout = build_layout(spec);
     operator_upload(op.get());
+    tune_schedule(op.get());
     *out = op.release();
   });
 }
@@ -336,6 +407,24 @@ int alps_ed_operator_create_momentum(const alps_ed_model* model, int n_translati
     operator_upload(op.get());
     bloch_upload(op.get(), bloch_sector(bloch_orbits(op->layout, t), t, op->layout.N));
     *out = op.release();
+  });
+}
+
+int alps_ed_operator_schedule(alps_ed_operator op, int* schedule, double* seconds) {
+  return guarded([&] {
+    require(op != nullptr && schedule != nullptr, "null argument");
+    *schedule = op->schedule;
+    if (seconds)
+      for (int s = 0; s < kHvSchedules; ++s)
+        seconds[s] = op->schedule_seconds.empty() ? -1.0 : op->schedule_seconds[s];
+  });
+}
+
+int alps_ed_operator_set_schedule(alps_ed_operator op, int schedule) {
+  return guarded([&] {
+    require(op != nullptr, "null argument");
+    require(schedule >= 0 && schedule < kHvSchedules, "unknown schedule");
+    op->schedule = schedule;
   });
 }
 
@@ -994,6 +1083,7 @@ SectorSolve solve_sector(const alps_ed_model* model, const alps_ed_solve_params*
     return out;
   }
   const bool fp32 = use_fp32(params->dtype == ALPS_ED_F32, D, params->nev > 0 ? params->nev : 1);
+  if (fp32) tune_schedule(op);
   const int max_iter = params->max_iter > 0 ? params->max_iter : (int)std::min<uint64_t>(10 * D, 1000);
   const int nev = params->nev > 0 ? params->nev : 1;
   require(nev <= 64, "nev must be <= 64");

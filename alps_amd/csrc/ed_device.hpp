@@ -115,8 +115,10 @@ __device__ __forceinline__ T ldc_struct(const T* p) {
 // The bond loops of hv_segment for NQ row slots per thread (rows tid,
 // tid + kBlock, ...), NQ fixed at compile time so that the loads of all
 // slots are issued together; rows past the segment are clamped to row 0 and
-// not stored.  Returns the thread's share of x.Hx when DOT.
-template <class V, bool DOT, int NQ>
+// not stored.  The flip bonds of each class are processed in chunks of
+// CLL / CHH / CX (the table padding is a multiple of each).  Returns the
+// thread's share of x.Hx when DOT.
+template <class V, bool DOT, int NQ, int CLL, int CHH, int CX>
 __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const V* __restrict__ x,
                                           V* __restrict__ y, const V* xs, uint32_t len, uint32_t base, double dh,
                                           const uint16_t* nbp, const uint16_t* xnp, const double* dlp) {
@@ -134,52 +136,55 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
     off[q] = 0.0;
   }
   // low-low flips: target rows from the class table, values from LDS
-  for (int b0 = 0; b0 < L.n_ll_p; b0 += kChunkLL) {
-    double t[kChunkLL];
+  const int n_ll = (L.n_ll + CLL - 1) / CLL * CLL;
+  for (int b0 = 0; b0 < n_ll; b0 += CLL) {
+    double t[CLL];
 #pragma unroll
-    for (int c = 0; c < kChunkLL; ++c) t[c] = ldc(L.ll_t + b0 + c);
-    uint32_t j[NQ][kChunkLL];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q)
-#pragma unroll
-      for (int c = 0; c < kChunkLL; ++c) j[q][c] = (nbp + (b0 + c) * len)[rc[q]];
+    for (int c = 0; c < CLL; ++c) t[c] = ldc(L.ll_t + b0 + c);
+    uint32_t j[NQ][CLL];
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
-      for (int c = 0; c < kChunkLL; ++c) off[q] += t[c] * (double)xs[j[q][c]];
+      for (int c = 0; c < CLL; ++c) j[q][c] = (nbp + (b0 + c) * len)[rc[q]];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int c = 0; c < CLL; ++c) off[q] += t[c] * (double)xs[j[q][c]];
   }
   // high-high flips: uniform decision, contiguous target run
-  for (int b0 = 0; b0 < L.n_hh_p; b0 += kChunkHH) {
-    const V* xt[kChunkHH];
-    double tt[kChunkHH];
+  const int n_hh = (L.n_hh + CHH - 1) / CHH * CHH;
+  for (int b0 = 0; b0 < n_hh; b0 += CHH) {
+    const V* xt[CHH];
+    double tt[CHH];
 #pragma unroll
-    for (int c = 0; c < kChunkHH; ++c) {
+    for (int c = 0; c < CHH; ++c) {
       const uint32_t m = ldc(L.hh_mask + b0 + c);
       const uint32_t bits = h & m;
       const bool act = bits != 0u && bits != m;
       xt[c] = act ? x + ldc(L.hi_base + (h ^ m)) : xb;
       tt[c] = act ? ldc(L.hh_t + b0 + c) : 0.0;
     }
-    V v[NQ][kChunkHH];
+    V v[NQ][CHH];
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
-      for (int c = 0; c < kChunkHH; ++c) v[q][c] = xt[c][rc[q]];
+      for (int c = 0; c < CHH; ++c) v[q][c] = xt[c][rc[q]];
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
-      for (int c = 0; c < kChunkHH; ++c) off[q] += tt[c] * (double)v[q][c];
+      for (int c = 0; c < CHH; ++c) off[q] += tt[c] * (double)v[q][c];
   }
   // cross bonds: diagonal always, flip when the high and low bits differ.
   // If the neighbouring high word has no segment no row can flip (its low
   // word would need a set bit more or fewer than it has), and the read
   // falls back to this segment.
-  for (int b0 = 0; b0 < L.n_xb_p; b0 += kChunkX) {
-    const V* xn[kChunkX];
-    bool hb[kChunkX];
-    double ea[kChunkX], ep[kChunkX], tx[kChunkX];
+  const int n_xb = (L.n_xb + CX - 1) / CX * CX;
+  for (int b0 = 0; b0 < n_xb; b0 += CX) {
+    const V* xn[CX];
+    bool hb[CX];
+    double ea[CX], ep[CX], tx[CX];
 #pragma unroll
-    for (int c = 0; c < kChunkX; ++c) {
+    for (int c = 0; c < CX; ++c) {
       const uint32_t hbit = ldc(L.xb_hbit + b0 + c);
       hb[c] = (h & hbit) != 0u;
       const uint32_t nb = ldc(L.hi_base + (h ^ hbit));
@@ -188,24 +193,24 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
       ep[c] = ldc(L.xb_par + b0 + c);
       tx[c] = ldc(L.xb_t + b0 + c);
     }
-    uint32_t e[NQ][kChunkX];
+    uint32_t e[NQ][CX];
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
-      for (int c = 0; c < kChunkX; ++c) e[q][c] = (xnp + (b0 + c) * len)[rc[q]];
-    V v[NQ][kChunkX];
-    bool f[NQ][kChunkX];
+      for (int c = 0; c < CX; ++c) e[q][c] = (xnp + (b0 + c) * len)[rc[q]];
+    V v[NQ][CX];
+    bool f[NQ][CX];
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
-      for (int c = 0; c < kChunkX; ++c) {
+      for (int c = 0; c < CX; ++c) {
         f[q][c] = ((e[q][c] >> 15) != 0u) != hb[c];
         v[q][c] = xn[c][f[q][c] ? (e[q][c] & 0x7fffu) : 0u];
       }
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
-      for (int c = 0; c < kChunkX; ++c) {
+      for (int c = 0; c < CX; ++c) {
         d[q] += f[q][c] ? ea[c] : ep[c];
         off[q] += (f[q][c] ? tx[c] : 0.0) * (double)v[q][c];
       }
@@ -231,8 +236,9 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
 //
 // The segment's slice of x is staged in LDS.  Each thread owns up to kRows
 // rows (r = tid + q*kBlock); the number of row slots is uniform over the
-// workgroup and selects a fixed-size instance of the bond loops.
-template <class V, bool DOT>
+// workgroup and selects a fixed-size instance of the bond loops.  CLL, CHH,
+// CX: chunk sizes of the flip loops (the schedule, ed_kernels.hpp).
+template <class V, bool DOT, int CLL = 4, int CHH = 4, int CX = 2>
 __device__ __forceinline__ void hv_segment(const DevLayout& L, uint32_t h, const V* __restrict__ x,
                                            V* __restrict__ y, double* __restrict__ dot_slot) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -254,13 +260,13 @@ __device__ __forceinline__ void hv_segment(const DevLayout& L, uint32_t h, const
   static_assert(kRows == 4, "hv_segment dispatches one to four row slots");
   double dotp;
   if (nq <= 1)
-    dotp = hv_rows<V, DOT, 1>(L, h, x, y, xs, len, base, dh, nbp, xnp, dlp);
+    dotp = hv_rows<V, DOT, 1, CLL, CHH, CX>(L, h, x, y, xs, len, base, dh, nbp, xnp, dlp);
   else if (nq == 2)
-    dotp = hv_rows<V, DOT, 2>(L, h, x, y, xs, len, base, dh, nbp, xnp, dlp);
+    dotp = hv_rows<V, DOT, 2, CLL, CHH, CX>(L, h, x, y, xs, len, base, dh, nbp, xnp, dlp);
   else if (nq == 3)
-    dotp = hv_rows<V, DOT, 3>(L, h, x, y, xs, len, base, dh, nbp, xnp, dlp);
+    dotp = hv_rows<V, DOT, 3, CLL, CHH, CX>(L, h, x, y, xs, len, base, dh, nbp, xnp, dlp);
   else
-    dotp = hv_rows<V, DOT, 4>(L, h, x, y, xs, len, base, dh, nbp, xnp, dlp);
+    dotp = hv_rows<V, DOT, 4, CLL, CHH, CX>(L, h, x, y, xs, len, base, dh, nbp, xnp, dlp);
   if (DOT) {
     const double bs = block_sum_all(dotp);
     if (tid == 0) *dot_slot = bs;

@@ -33,11 +33,36 @@ using namespace dev;
 // ---------------------------------------------------------------------------
 // H.v: one workgroup per segment, longest segments first
 // ---------------------------------------------------------------------------
-template <class V, bool DOT>
-__global__ __launch_bounds__(kBlock, HvWaves<V>::value) void hv_kernel(DevLayout L, const V* __restrict__ x, V* __restrict__ y,
-                                                     double* __restrict__ dot_out) {
+template <class V, bool DOT, int W, int CLL, int CHH, int CX>
+__global__ __launch_bounds__(kBlock, W) void hv_kernel(DevLayout L, const V* __restrict__ x, V* __restrict__ y,
+                                                       double* __restrict__ dot_out) {
   const uint32_t h = ldc(L.proc_hi + blockIdx.x);
-  hv_segment<V, DOT>(L, h, x, y, DOT ? dot_out + blockIdx.x : nullptr);
+  hv_segment<V, DOT, CLL, CHH, CX>(L, h, x, y, DOT ? dot_out + blockIdx.x : nullptr);
+}
+
+template <class V, bool DOT, int W, int CLL, int CHH, int CX>
+void launch_hv_with(const DevLayout& L, const V* x, V* y, double* dot_out, hipStream_t st, size_t lds) {
+  hipLaunchKernelGGL((hv_kernel<V, DOT, W, CLL, CHH, CX>), dim3(hv_blocks(L)), dim3(kBlock), lds, st, L, x, y,
+                     dot_out);
+}
+
+// The schedule table of ed_kernels.hpp as template arguments.
+template <class V, bool DOT>
+void launch_hv_schedule(const DevLayout& L, const V* x, V* y, double* dot_out, hipStream_t st, size_t lds,
+                        int schedule) {
+  constexpr int W = HvWaves<V>::value;
+  if constexpr (sizeof(V) == 4) {
+    static_assert(kHvSchedules == 6, "one case per schedule");
+    switch (schedule) {
+      case 1: launch_hv_with<V, DOT, 8, 8, 4, 2>(L, x, y, dot_out, st, lds); return;
+      case 2: launch_hv_with<V, DOT, 8, 4, 8, 2>(L, x, y, dot_out, st, lds); return;
+      case 3: launch_hv_with<V, DOT, 8, 4, 4, 4>(L, x, y, dot_out, st, lds); return;
+      case 4: launch_hv_with<V, DOT, 4, 4, 4, 2>(L, x, y, dot_out, st, lds); return;
+      case 5: launch_hv_with<V, DOT, 4, 8, 8, 4>(L, x, y, dot_out, st, lds); return;
+      default: break;
+    }
+  }
+  launch_hv_with<V, DOT, W, 4, 4, 2>(L, x, y, dot_out, st, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -289,13 +314,14 @@ __global__ __launch_bounds__(64) void elements_kernel(DevLayout L, uint64_t row0
 // launchers
 // ---------------------------------------------------------------------------
 template <class V>
-hipError_t launch_hv(const DevLayout& L, const V* x, V* y, double* dot_out, hipStream_t st) {
+hipError_t launch_hv(const DevLayout& L, const V* x, V* y, double* dot_out, hipStream_t st, int schedule) {
   if (L.n_seg == 0) return hipSuccess;
+  if (schedule < 0 || schedule >= kHvSchedules) return hipErrorInvalidValue;
   const size_t lds = (size_t)(L.max_seg + 1) * sizeof(V);
   if (dot_out)
-    hipLaunchKernelGGL((hv_kernel<V, true>), dim3(hv_blocks(L)), dim3(kBlock), lds, st, L, x, y, dot_out);
+    launch_hv_schedule<V, true>(L, x, y, dot_out, st, lds, schedule);
   else
-    hipLaunchKernelGGL((hv_kernel<V, false>), dim3(hv_blocks(L)), dim3(kBlock), lds, st, L, x, y, dot_out);
+    launch_hv_schedule<V, false>(L, x, y, dot_out, st, lds, schedule);
   return hipGetLastError();
 }
 
@@ -396,7 +422,7 @@ hipError_t launch_elements(const DevLayout& L, uint64_t row0, uint64_t nrows, in
 }
 
 #define ALPS_ED_INSTANTIATE(V)                                                                              \
-  template hipError_t launch_hv<V>(const DevLayout&, const V*, V*, double*, hipStream_t);                   \
+  template hipError_t launch_hv<V>(const DevLayout&, const V*, V*, double*, hipStream_t, int);              \
   template hipError_t launch_update<V>(uint64_t, const V*, const V*, const V*, V*, const double*,           \
                                        const double*, const double*, const double*, double*, hipStream_t); \
   template hipError_t launch_dot<V>(uint64_t, const V*, const V*, double*, hipStream_t);                    \

@@ -14,7 +14,8 @@ struct DevLayout {
   int N, k, B, NH;
   uint32_t D;
   uint32_t n_seg, max_seg;
-  int n_ll_p, n_hh_p, n_xb_p, n_bonds;
+  int n_ll_p, n_hh_p, n_xb_p, n_bonds;  // padded flip-bond counts (table strides)
+  int n_ll, n_hh, n_xb;                 // flip bonds per class
   const uint32_t* lo_rank;
   const uint32_t* lo_state;
   const uint32_t* lo_class_off;
@@ -103,10 +104,23 @@ inline unsigned fused_blocks(uint64_t n) { return stream_blocks(n) < 256 ? strea
 // Blocks of the H.v kernel: one per segment.
 inline unsigned hv_blocks(const DevLayout& L) { return L.n_seg ? L.n_seg : 1; }
 
+// H.v schedules for FP32 vectors: the waves per SIMD the kernel is compiled
+// for and the chunk sizes of its low-low, high-high and cross flip loops.
+// Schedule 0 is the default (FP64 vectors always use it); an operator of a
+// large FP32 sector picks its schedule by timing them all (the autotuner in
+// ed_capi.hip).  Every schedule adds the bonds in the same order, so all
+// give identical results.
+struct HvSchedule {
+  int waves, cll, chh, cx;
+};
+constexpr int kHvSchedules = 6;
+constexpr HvSchedule kHvSchedule[kHvSchedules] = {{8, 4, 4, 2}, {8, 8, 4, 2}, {8, 4, 8, 2},
+                                                  {8, 4, 4, 4}, {4, 4, 4, 2}, {4, 8, 8, 4}};
+
 // y = H x; with dot_out set, per-block partials of sum_i x_i (H x)_i (FP64,
 // before y is rounded to the storage type).
 template <class V>
-hipError_t launch_hv(const DevLayout& L, const V* x, V* y, double* dot_out, hipStream_t st);
+hipError_t launch_hv(const DevLayout& L, const V* x, V* y, double* dot_out, hipStream_t st, int schedule = 0);
 
 // Lanczos scalars between the kernels of one step (one block):
 //   mode kAlpha: alpha = scale^2 * sum(in)            -> out[0]

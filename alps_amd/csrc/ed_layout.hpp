@@ -153,21 +153,10 @@ struct SectorLayout {
   std::vector<double> bond_par, bond_anti, bond_t;
 };
 
-// Chunk sizes of the H.v flip loops (bond lists are padded to multiples):
-// the loads of one chunk for all of a thread's rows are in flight together.
-// Build-time tuning parameters.
-#ifndef ALPS_ED_CHUNK_LL
-#define ALPS_ED_CHUNK_LL 4
-#endif
-#ifndef ALPS_ED_CHUNK_HH
-#define ALPS_ED_CHUNK_HH 4
-#endif
-#ifndef ALPS_ED_CHUNK_X
-#define ALPS_ED_CHUNK_X 2
-#endif
-constexpr int kChunkLL = ALPS_ED_CHUNK_LL;
-constexpr int kChunkHH = ALPS_ED_CHUNK_HH;
-constexpr int kChunkX = ALPS_ED_CHUNK_X;
+// The H.v schedules (ed_kernels.hpp) process the flip bonds of each class in
+// chunks of up to 8; bond lists and flip tables are padded to these
+// multiples, which every schedule's chunk sizes divide.
+constexpr int kPadLL = 8, kPadHH = 8, kPadX = 4;
 
 inline int round_up(int n, int m) { return (n + m - 1) / m * m; }
 
@@ -382,9 +371,9 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1) {
   L.diag_lo_cls.resize(nlo);
   for (uint32_t i = 0; i < nlo; ++i) L.diag_lo_cls[i] = L.diag_lo[L.lo_state[i]];
   // padded H.v tables
-  L.n_ll_p = round_up((int)L.ll.size(), kChunkLL);
-  L.n_hh_p = round_up((int)L.hh.size(), kChunkHH);
-  L.n_xb_p = round_up((int)L.xb.size(), kChunkX);
+  L.n_ll_p = round_up((int)L.ll.size(), kPadLL);
+  L.n_hh_p = round_up((int)L.hh.size(), kPadHH);
+  L.n_xb_p = round_up((int)L.xb.size(), kPadX);
   {
     std::vector<uint32_t> llm, xlb;
     for (const FlipBond& f : L.ll) llm.push_back(f.mask);

@@ -29,6 +29,9 @@ HBM_PEAK_GBS = 8000.0
 # known size (profiles/hv_kernel_heisenberg_L24.md); valid for that kernel
 # version and L only.
 PMC_TRAFFIC_BYTES = {24: 51.94e6}
+# H.v schedules (waves per SIMD, low-low / high-high / cross chunk sizes), in
+# the order of the engine's schedule table
+SCHEDULES = [(8, 4, 4, 2), (8, 8, 4, 2), (8, 4, 8, 2), (8, 4, 4, 4), (4, 4, 4, 2), (4, 8, 8, 4)]
 
 
 class HipEvents:
@@ -114,7 +117,7 @@ def xxz_batch(n, seed, Ls, field=False, any_sector=False):
     return out
 
 
-def batch_leg(n, seq_sample, seed=1):
+def batch_leg(n, seq_sample, seed=1, graphs=False):
     """1024 random XXZ chains (L in 12..20) solved in one batch, against the
     same solves one at a time (alps_ed_sparsediag) on a sample."""
     import numpy as np
@@ -123,7 +126,7 @@ def batch_leg(n, seq_sample, seed=1):
 
     models = xxz_batch(n, seed, list(range(12, 21)))
     free = A.device_info()["hbm_bytes"]
-    b = A.Batch("f32", hbm_bytes=int(free * 0.75), steps_per_round=8, host_threads=12)
+    b = A.Batch("f32", hbm_bytes=int(free * 0.75), steps_per_round=8, host_threads=12, graphs=graphs)
     t0 = time.perf_counter()
     tickets = [b.submit(m, nev=1, seed=i) for i, m in enumerate(models)]
     b.run()
@@ -160,12 +163,13 @@ def batch_leg(n, seq_sample, seed=1):
         "rounds": st["rounds"],
         "seconds_split": {k: round(st["seconds_" + k], 4) for k in ("admit", "steps", "decide", "refine")},
         "hv_rows_f32": st["hv_rows_f32"],
+        "graphs": bool(graphs), "graph_rounds": st["graph_rounds"],
         "workload": "random-bond XXZ chains, L uniform 12..20, J_i~U[0.5,1.5], Delta~U[0,2], open/periodic 1:1, "
                     "Sz=0 (Sz=1/2 for odd L), lowest eigenvalue, FP32 vectors + FP64 refinement",
     }
 
 
-def serve_leg(n, rate, seed=2):
+def serve_leg(n, rate, seed=2, graphs=False):
     """Poisson request stream (XXZ chains, L uniform 12..26, field h ~ U[0,0.5],
     sector uniform over n_up = 0..L), continuous batching with HBM admission."""
     import numpy as np
@@ -176,7 +180,7 @@ def serve_leg(n, rate, seed=2):
     rng = np.random.default_rng(seed + 1)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
     free = A.device_info()["hbm_bytes"]
-    b = A.Batch("f32", hbm_bytes=int(free * 0.75), steps_per_round=4, host_threads=12)
+    b = A.Batch("f32", hbm_bytes=int(free * 0.75), steps_per_round=4, host_threads=12, graphs=graphs)
     t0 = time.perf_counter()
     i = 0
     arrival_of = {}
@@ -197,10 +201,12 @@ def serve_leg(n, rate, seed=2):
             print(f"serve: {len(lat)}/{n} done", file=sys.stderr, flush=True)
             last_print = time.perf_counter()
     wall = time.perf_counter() - t0
+    st = b.stats()
     b.close()
     lat = np.array(lat)
     return {
         "requests": n,
+        "graphs": bool(graphs), "graph_rounds": st["graph_rounds"], "rounds": st["rounds"],
         "offered_rate_per_s": rate,
         "solves_per_s": round(n / wall, 1),
         "p50_latency_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
@@ -224,6 +230,7 @@ def j1j2_leg(ev, iterations=300):
     m = A.SpinModel(t["n_sites"], np.array(t["bond_s"]), np.array(t["bond_t"]), np.array(t["jz"]),
                     np.array(t["jxy"]), 15)
     op = A.Operator(m)
+    sched, sched_s = op.schedule()
     D = op.dim
     x, y = A.Vector(D, "f32"), A.Vector(D, "f32")
     x.fill_random(1)
@@ -244,6 +251,8 @@ def j1j2_leg(ev, iterations=300):
         "workload": "J1-J2 Heisenberg, 6x5 periodic square lattice, J1=1, J2=0.5, Sz=0",
         "dim": D, "bonds": len(t["bond_s"]),
         "hv_ms": round(ms, 4), "hv_gbs_8D": round(8.0 * D / (ms * 1e-3) / 1e9, 1),
+        "hv_schedule": {"chosen": sched, "waves_cll_chh_cx": SCHEDULES[sched],
+                        "ms_per_schedule": None if sched_s is None else [round(t * 1e3, 3) for t in sched_s]},
         "iterations": iterations,
         "f32_eigenvalues": r32["ev"].tolist(), "f64_eigenvalues": r64["ev"].tolist(), "rel_err_f32": rel,
         "solve_s_f32": round(r32["seconds"], 3), "solve_s_f64": round(r64["seconds"], 3),
@@ -266,6 +275,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--graphs", action="store_true", help="batch and serve legs replay repeated rounds from HIP graphs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs one solve per GPU; --gpus {args.gpus} is not supported "
@@ -280,7 +290,8 @@ def main():
     info = A.device_info()
     L = args.L
     model = heisenberg_ring(L)
-    op = A.Operator(model)
+    op = A.Operator(model)  # times the FP32 H.v schedules on this sector, keeps the fastest
+    sched, sched_s = op.schedule()
     D = op.dim
     x = A.Vector(D, "f32")
     y = A.Vector(D, "f32")
@@ -344,8 +355,11 @@ def main():
                    "L": L, "dim": D, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": PMC_TRAFFIC_BYTES.get(L),
-                     "kernel": "hv_kernel<float,false>", "bytes_per_launch": bytes_per_hv},
+                     "kernel": "hv_kernel<float,false,%d,%d,%d,%d>" % SCHEDULES[sched],
+                     "bytes_per_launch": bytes_per_hv},
         "timed_wall_s": round(wall, 4),
+        "hv_schedule": {"chosen": sched, "waves_cll_chh_cx": SCHEDULES[sched],
+                        "us_per_schedule": None if sched_s is None else [round(t * 1e6, 2) for t in sched_s]},
         "copy_gbs": round(copy_gbs, 2),
         "lanczos_ms_per_step": round(ms_lanczos, 5),
         "device": info["name"],
@@ -367,9 +381,9 @@ def main():
     if args.j1j2:
         out["j1j2"] = j1j2_leg(ev)
     if not args.no_batch:
-        out["batch"] = batch_leg(args.batch_size, args.seq_sample)
+        out["batch"] = batch_leg(args.batch_size, args.seq_sample, graphs=args.graphs)
     if not args.no_serve:
-        out["serve"] = serve_leg(args.serve_requests, args.serve_rate)
+        out["serve"] = serve_leg(args.serve_requests, args.serve_rate, graphs=args.graphs)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(L, min(args.cpu_rows, D))
     print(json.dumps(out), flush=True)

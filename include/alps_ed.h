@@ -99,6 +99,16 @@ int alps_ed_operator_create(const alps_ed_model* model, alps_ed_operator* op);
 int alps_ed_operator_create_momentum(const alps_ed_model* model, int n_translations, const int* perm,
                                      const double* phase_re, const double* phase_im, alps_ed_operator* op);
 int alps_ed_operator_destroy(alps_ed_operator op);
+/* No counterpart in the reference: the H.v schedule (waves per SIMD and
+ * flip-loop chunk sizes) an operator uses with FP32 vectors.  Operators of
+ * fixed-Sz sectors with at least 200000 states pick theirs at creation by
+ * timing all ALPS_ED_HV_SCHEDULES of them; seconds[s] receives each one's
+ * time per H.v (-1 when the operator was not timed; seconds may be null).
+ * Schedules differ only in speed: every one gives the same y.
+ * set_schedule forces one (for inspection and tests). */
+#define ALPS_ED_HV_SCHEDULES 6
+int alps_ed_operator_schedule(alps_ed_operator op, int* schedule, double* seconds);
+int alps_ed_operator_set_schedule(alps_ed_operator op, int schedule);
 /* hamiltonian_matrix::dimension (hamiltonian_matrix.hpp:92); the length of
  * the operator's vectors (twice the sector dimension for a momentum sector). */
 int alps_ed_operator_dimension(alps_ed_operator op, uint64_t* dim);
@@ -271,6 +281,8 @@ typedef struct {
   int steps_per_round;    /* Lanczos steps between stopping-rule tests; 0 = 8 */
   int max_active;         /* problems admitted at once; 0 = no limit */
   int host_threads;       /* threads of the host analysis; 0 = 8 */
+  int graphs;             /* 1: a round whose launches repeat the previous round's (same work
+                           * lists, no new problem) replays them from a captured HIP graph */
 } alps_ed_batch_config;
 
 typedef struct {
@@ -284,6 +296,8 @@ typedef struct {
    * (sum over steps of the dimension of each problem stepped): the
    * algorithmic H.v bytes are 8 x hv_rows_f32 + 16 x hv_rows_f64 */
   uint64_t hv_rows_f32, hv_rows_f64;
+  /* rounds whose steps ran from a HIP graph, and graphs captured */
+  uint64_t graph_rounds, graph_captures;
 } alps_ed_batch_status;
 
 int alps_ed_batch_create(const alps_ed_batch_config* cfg, alps_ed_batch* out);

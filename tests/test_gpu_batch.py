@@ -140,3 +140,30 @@ def test_batch_fixed_iterations_and_more_eigenvalues():
     w = np.unique(np.round(np.linalg.eigvalsh(O.dense(m, 6)), 10))
     np.testing.assert_allclose(r2["ev"], w[:5], rtol=1e-10)
     b.close()
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_graph_rounds_match_direct_launches(dtype):
+    # rounds that repeat the previous round's launches run from a captured
+    # HIP graph: replayed many times, with the engine's own host work and
+    # synchronisations between replays, the results must equal the
+    # directly launched batch bit for bit
+    models = mix(12, 21, Ls=(12, 13, 14), field=True)
+    out = {}
+    for graphs in (False, True):
+        say("graph rounds", dtype, graphs)
+        b = A.Batch(dtype, steps_per_round=4, graphs=graphs)
+        # fixed iteration counts keep the active set unchanged for many rounds
+        t = [b.submit(to_product(m, k), max_iter=120, fixed=True, nev=2, seed=i) for i, (m, k) in enumerate(models)]
+        t += [b.submit(to_product(m, k), nev=1, seed=7 + i) for i, (m, k) in enumerate(models[:4])]
+        b.run()
+        out[graphs] = [b.result(x) for x in t]
+        st = b.stats()
+        b.close()
+        if graphs:
+            assert st["graph_rounds"] > 10 and st["graph_captures"] >= 1
+        else:
+            assert st["graph_rounds"] == 0
+    for r0, r1 in zip(out[False], out[True]):
+        assert r0["iterations"] == r1["iterations"]
+        assert np.array_equal(r0["ev"], r1["ev"])

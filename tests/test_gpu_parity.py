@@ -364,3 +364,46 @@ def test_cli_runs_parameter_file(tmp_path):
     (t,) = json.loads(out)["tasks"]
     r = O.sparsediag(O.lattice_model("chain lattice", 8), 4, nev=2)
     np.testing.assert_allclose(t["sectors"][0]["eigenvalues"], r["ev"], rtol=1e-10)
+
+
+@pytest.mark.parametrize("name", ["heis_chain16", "xxz_open13", "j1j2_4x4", "general_4x3", "ladder7"])
+def test_hv_schedules_give_identical_results(name):
+    # the FP32 H.v schedules differ in waves per SIMD and chunk sizes only:
+    # every one must give the same y (flip-bond counts that are not
+    # multiples of the chunks exercise the padding), and match the oracle
+    mk, sectors = CASES[name]
+    m = mk()
+    k = sectors[0]
+    say("schedules", name)
+    op = A.Operator(to_product(m, k))
+    x = A.Vector(op.dim, "f32")
+    x.fill_random(3)
+    xs = x.numpy()
+    outs = []
+    for s in range(A.Operator.HV_SCHEDULES):
+        op.set_schedule(s)
+        y = A.Vector(op.dim, "f32")
+        op.matvec(x, y)
+        outs.append(y.numpy())
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    want = O.spmv(m, k, xs)
+    np.testing.assert_allclose(outs[0], want, rtol=2e-6, atol=2e-6 * np.abs(want).max())
+
+
+def test_hv_autotuner_full_size():
+    # L=24 ring: the operator times every schedule at creation and keeps the
+    # fastest; all schedules give the same y on the full sector
+    op = A.Operator(to_product(O.lattice_model("chain lattice", 24), 12))
+    chosen, sec = op.schedule()
+    say("autotuner", chosen, sec)
+    assert sec is not None and np.all(sec > 0) and sec[chosen] == sec.min()
+    x = A.Vector(op.dim, "f32")
+    x.fill_random(5)
+    ys = []
+    for s in (0, chosen, A.Operator.HV_SCHEDULES - 1):
+        op.set_schedule(s)
+        y = A.Vector(op.dim, "f32")
+        op.matvec(x, y)
+        ys.append(y.numpy())
+    assert np.array_equal(ys[0], ys[1]) and np.array_equal(ys[0], ys[2])
