@@ -54,11 +54,11 @@ void launch_hv_schedule(const DevLayout& L, const V* x, V* y, double* dot_out, h
   if constexpr (sizeof(V) == 4) {
     static_assert(kHvSchedules == 6, "one case per schedule");
     switch (schedule) {
-      case 1: launch_hv_with<V, DOT, 8, 8, 4, 2>(L, x, y, dot_out, st, lds); return;
-      case 2: launch_hv_with<V, DOT, 8, 4, 8, 2>(L, x, y, dot_out, st, lds); return;
-      case 3: launch_hv_with<V, DOT, 8, 4, 4, 4>(L, x, y, dot_out, st, lds); return;
-      case 4: launch_hv_with<V, DOT, 4, 4, 4, 2>(L, x, y, dot_out, st, lds); return;
-      case 5: launch_hv_with<V, DOT, 4, 8, 8, 4>(L, x, y, dot_out, st, lds); return;
+      case 1: launch_hv_with<V, DOT, 8, 4, 8, 2>(L, x, y, dot_out, st, lds); return;
+      case 2: launch_hv_with<V, DOT, 6, 4, 8, 2>(L, x, y, dot_out, st, lds); return;
+      case 3: launch_hv_with<V, DOT, 6, 4, 4, 2>(L, x, y, dot_out, st, lds); return;
+      case 4: launch_hv_with<V, DOT, 8, 8, 4, 2>(L, x, y, dot_out, st, lds); return;
+      case 5: launch_hv_with<V, DOT, 6, 8, 8, 4>(L, x, y, dot_out, st, lds); return;
       default: break;
     }
   }

@@ -114,8 +114,8 @@ struct HvSchedule {
   int waves, cll, chh, cx;
 };
 constexpr int kHvSchedules = 6;
-constexpr HvSchedule kHvSchedule[kHvSchedules] = {{8, 4, 4, 2}, {8, 8, 4, 2}, {8, 4, 8, 2},
-                                                  {8, 4, 4, 4}, {4, 4, 4, 2}, {4, 8, 8, 4}};
+constexpr HvSchedule kHvSchedule[kHvSchedules] = {{8, 4, 4, 2}, {8, 4, 8, 2}, {6, 4, 8, 2},
+                                                  {6, 4, 4, 2}, {8, 8, 4, 2}, {6, 8, 8, 4}};
 
 // y = H x; with dot_out set, per-block partials of sum_i x_i (H x)_i (FP64,
 // before y is rounded to the storage type).

@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 PMC_TRAFFIC_BYTES = {24: 51.94e6}
 # H.v schedules (waves per SIMD, low-low / high-high / cross chunk sizes), in
 # the order of the engine's schedule table
-SCHEDULES = [(8, 4, 4, 2), (8, 8, 4, 2), (8, 4, 8, 2), (8, 4, 4, 4), (4, 4, 4, 2), (4, 8, 8, 4)]
+SCHEDULES = [(8, 4, 4, 2), (8, 4, 8, 2), (6, 4, 8, 2), (6, 4, 4, 2), (8, 8, 4, 2), (6, 8, 8, 4)]
 
 
 class HipEvents:
