@@ -211,6 +211,7 @@ struct Problem {
   int final_n = -1;
   int error_code = 0;
   TValues tv;
+  int tv_n = -1;            // length of T that tv analyses
   std::vector<double> ev;   // reported eigenvalues
   size_t next_cand = 0;     // next candidate of tv to refine
   std::vector<int> cand;    // tv positions in the current refinement pass
@@ -490,8 +491,11 @@ void decide(Problem& p) {
     p.lam_prev = lam;
     if (e <= 4.0 * std::max(tol, tol * std::fabs(lam))) {
       std::vector<double> a(p.host_alpha, p.host_alpha + n), bb(p.host_beta, p.host_beta + n);
-      if (nlowest_converged(analyze_prefix(a, bb, n, p.fp32), p.nev, p.fp32)) {
+      TValues t = analyze_prefix(a, bb, n, p.fp32);
+      if (nlowest_converged(t, p.nev, p.fp32)) {
         p.final_n = n;
+        p.tv = std::move(t);  // conclude() uses this analysis of T[0..n)
+        p.tv_n = n;
         return;
       }
     }
@@ -505,8 +509,11 @@ void decide(Problem& p) {
 // After the stopping rule: eigenvalues for FP64 lanes; for FP32 lanes the
 // candidates of the first refinement pass.
 void conclude(Problem& p) {
-  std::vector<double> a(p.host_alpha, p.host_alpha + p.final_n), bb(p.host_beta, p.host_beta + p.final_n);
-  p.tv = analyze_prefix(a, bb, p.final_n, p.fp32);
+  if (p.tv_n != p.final_n) {
+    std::vector<double> a(p.host_alpha, p.host_alpha + p.final_n), bb(p.host_beta, p.host_beta + p.final_n);
+    p.tv = analyze_prefix(a, bb, p.final_n, p.fp32);
+    p.tv_n = p.final_n;
+  }
   if (!p.fp32) {
     p.ev.assign(p.tv.ev.begin(), p.tv.ev.begin() + std::min<size_t>(p.nev, p.tv.ev.size()));
     p.state = kDone;
@@ -519,10 +526,7 @@ void conclude(Problem& p) {
 void prepare_refine(Problem& p) {
   const int m = p.final_n;
   std::vector<double> a(p.host_alpha, p.host_alpha + m), bb(p.host_beta, p.host_beta + m);
-  std::vector<double> d(a);
-  std::vector<double> evals(d);
-  if (ietl::tridiag::eig(m, evals.data(), bb.data(), nullptr) > 0)
-    throw std::runtime_error("LAPACK error, stev function failed.");
+  const std::vector<double>& evals = p.tv.all;  // conclude() analysed T[0..m)
   const size_t want = (size_t)std::max(0, p.nev - (int)p.ev.size());
   const size_t cnt = std::min({p.tv.ev.size() - p.next_cand, want, (size_t)p.nvr});
   p.cand.assign(p.tv.index.begin() + p.next_cand, p.tv.index.begin() + p.next_cand + cnt);
@@ -631,7 +635,7 @@ Group upload_group(alps_ed_batch b, const std::vector<Problem*>& run) {
 
 // Per-problem sizes that follow from its layout.
 void size_problem(alps_ed_batch b, Problem& p) {
-  p.layout = build_layout(p.spec);
+  p.layout = build_layout(p.spec, -1, false);  // shared tables stay in L.low / L.flip
   p.D = p.layout.D;
   p.fp32 = use_fp32(b->cfg.dtype == ALPS_ED_F32, p.D, p.nev);
   if (p.max_iter <= 0) p.max_iter = (int)std::min<uint64_t>(10 * std::max<uint64_t>(p.D, 1), 1000);
@@ -655,9 +659,11 @@ const SharedTables& shared_tables(alps_ed_batch b, const SectorLayout& L) {
     total += align_up(std::max<size_t>(n, 1) * es);
     return o;
   };
-  const size_t o1 = sz(L.lo_rank.size(), 4), o2 = sz(L.lo_state.size(), 4), o3 = sz(L.lo_class_off.size(), 4),
-               o4 = sz(L.nbp16.size(), 2), o5 = sz(L.nbp_off.size(), 4), o6 = sz(L.xn16.size(), 2),
-               o7 = sz(L.xn_off.size(), 4);
+  const LowTables& low = *L.low;
+  const FlipTables& flip = *L.flip;
+  const size_t o1 = sz(low.lo_rank.size(), 4), o2 = sz(low.lo_state.size(), 4), o3 = sz(low.lo_class_off.size(), 4),
+               o4 = sz(flip.nbp16.size(), 2), o5 = sz(flip.nbp_off.size(), 4), o6 = sz(flip.xn16.size(), 2),
+               o7 = sz(flip.xn_off.size(), 4);
   t.buf.alloc(total);
   char* d = t.buf.as<char>();
   auto up = [&](size_t off, const auto& v) {
@@ -667,13 +673,13 @@ const SharedTables& shared_tables(alps_ed_batch b, const SectorLayout& L) {
                 "upload shared tables");
     return reinterpret_cast<const T*>(d + off);
   };
-  t.lo_rank = up(o1, L.lo_rank);
-  t.lo_state = up(o2, L.lo_state);
-  t.lo_class_off = up(o3, L.lo_class_off);
-  t.nbp16 = up(o4, L.nbp16);
-  t.nbp_off = up(o5, L.nbp_off);
-  t.xn16 = up(o6, L.xn16);
-  t.xn_off = up(o7, L.xn_off);
+  t.lo_rank = up(o1, low.lo_rank);
+  t.lo_state = up(o2, low.lo_state);
+  t.lo_class_off = up(o3, low.lo_class_off);
+  t.nbp16 = up(o4, flip.nbp16);
+  t.nbp_off = up(o5, flip.nbp_off);
+  t.xn16 = up(o6, flip.xn16);
+  t.xn_off = up(o7, flip.xn_off);
   return t;
 }
 
@@ -1123,7 +1129,7 @@ int alps_ed_batch_footprint(const alps_ed_model* model, const alps_ed_solve_para
     require(params && bytes, "null argument");
     Problem p;
     p.spec = model_spec(model);
-    p.layout = build_layout(p.spec);
+    p.layout = build_layout(p.spec, -1, false);  // shared tables stay in L.low / L.flip
     p.D = p.layout.D;
     p.nev = params->nev > 0 ? params->nev : 1;
     p.fp32 = use_fp32(dtype == ALPS_ED_F32, p.D, p.nev);

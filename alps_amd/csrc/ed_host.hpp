@@ -136,6 +136,7 @@ inline ModelSpec model_spec(const alps_ed_model* m) {
 struct TValues {
   std::vector<double> ev, err;
   std::vector<int> index;
+  std::vector<double> all;  // every eigenvalue of T, ascending (index refers to it)
 };
 
 inline TValues analyze_prefix(const std::vector<double>& a, const std::vector<double>& b, int m, bool fp32) {
@@ -147,6 +148,7 @@ inline TValues analyze_prefix(const std::vector<double>& a, const std::vector<do
   ietl::tmatrix_analysis an;
   an.run(a.data(), b.data(), m, amax, bmax, true);
   TValues t;
+  t.all = std::move(an.eigval_all);
   const double dup = 64.0 * std::numeric_limits<float>::epsilon();
   for (size_t i = 0; i < an.eigval_noghost.size(); ++i) {
     const double lam = an.eigval_noghost[i];

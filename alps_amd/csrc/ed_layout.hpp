@@ -260,7 +260,10 @@ inline std::shared_ptr<const FlipTables> flip_tables(int B, const std::vector<ui
   return F;
 }
 
-inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1) {
+// copy_shared = false leaves lo_rank, lo_state, nbp16 and xn16 empty: the
+// batch engine reads them from the shared tables (L.low, L.flip), which it
+// uploads once per shape.
+inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1, bool copy_shared = true) {
   SectorLayout L;
   const int N = m.n_sites;
   if (N < 1 || N > kMaxSites) throw std::invalid_argument("number of sites must be in [1, 32]");
@@ -283,8 +286,10 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1) {
   {
     const LowTables& T = low_tables(B);
     L.low = &T;
-    L.lo_rank = T.lo_rank;
-    L.lo_state = T.lo_state;
+    if (copy_shared) {
+      L.lo_rank = T.lo_rank;
+      L.lo_state = T.lo_state;
+    }
     L.lo_class_off = T.lo_class_off;
   }
   // high tables and segments
@@ -328,21 +333,24 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1) {
   }
   // terms in bit coordinates
   auto pos = [N](int s) { return (uint32_t)(N - 1 - s); };
-  L.diag_lo.assign(nlo, 0.0);
-  L.diag_hi.assign(nhi, 0.0);
+  // Diagonal tables by a recurrence over the highest set bit t of a word w:
+  // diag[w] = diag[w ^ 2^t] + (a constant of bit t: its site term up - down
+  // and its bonds to higher bits, all down) + for each bond (t, o) with o
+  // below t the change from setting t (bit o read from w).  Every word costs
+  // 1 + (bonds to lower bits) additions instead of one per site and bond.
+  std::vector<double> su_lo(B, 0.0), su_hi(L.NH, 0.0);           // per bit constant
+  std::vector<std::vector<std::pair<uint32_t, std::pair<double, double>>>> bl_lo(B), bl_hi(L.NH);
+  double base_lo = 0.0, base_hi = 0.0;                              // every bit down
   for (int s = 0; s < N; ++s) {
     uint32_t p = pos(s);
     L.site_dn.push_back(m.sites[s].e_dn);
     L.site_up.push_back(m.sites[s].e_up);
-    // (locals and raw pointers: the loops vectorize; same sums as before)
-    const double up = m.sites[s].e_up, dn = m.sites[s].e_dn;
     if ((int)p < B) {
-      double* dl = L.diag_lo.data();
-      for (uint32_t v = 0; v < nlo; ++v) dl[v] += (v >> p & 1u) ? up : dn;
+      base_lo += m.sites[s].e_dn;
+      su_lo[p] += m.sites[s].e_up - m.sites[s].e_dn;
     } else {
-      const uint32_t q = p - B;
-      double* dh = L.diag_hi.data();
-      for (uint32_t h = 0; h < nhi; ++h) dh[h] += (h >> q & 1u) ? up : dn;
+      base_hi += m.sites[s].e_dn;
+      su_hi[p - B] += m.sites[s].e_up - m.sites[s].e_dn;
     }
   }
   for (const BondTerm& b : m.bonds) {
@@ -353,23 +361,43 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1) {
     L.bond_anti.push_back(b.e_anti);
     L.bond_t.push_back(b.t);
     bool lo1 = (int)p1 < B, lo2 = (int)p2 < B;
-    const double anti = b.e_anti, par = b.e_par;
+    // A bond's lower bit is set first in the recurrence (the higher one is
+    // still down): par -> anti, a constant of that bit.  Setting the higher
+    // bit then changes par -> anti when the lower bit is down and
+    // anti -> par when it is up.
     if (lo1 && lo2) {
-      double* dl = L.diag_lo.data();
-      for (uint32_t v = 0; v < nlo; ++v) dl[v] += ((v >> p1 ^ v >> p2) & 1u) ? anti : par;
+      base_lo += b.e_par;
+      su_lo[std::min(p1, p2)] += b.e_anti - b.e_par;
+      bl_lo[std::max(p1, p2)].push_back({1u << std::min(p1, p2), {b.e_anti - b.e_par, b.e_par - b.e_anti}});
       if (b.t != 0.0) L.ll.push_back({(1u << p1) | (1u << p2), 0u, b.t});
     } else if (!lo1 && !lo2) {
       uint32_t q1 = p1 - B, q2 = p2 - B;
-      double* dh = L.diag_hi.data();
-      for (uint32_t h = 0; h < nhi; ++h) dh[h] += ((h >> q1 ^ h >> q2) & 1u) ? anti : par;
+      base_hi += b.e_par;
+      su_hi[std::min(q1, q2)] += b.e_anti - b.e_par;
+      bl_hi[std::max(q1, q2)].push_back({1u << std::min(q1, q2), {b.e_anti - b.e_par, b.e_par - b.e_anti}});
       if (b.t != 0.0) L.hh.push_back({(1u << q1) | (1u << q2), 0u, b.t});
     } else {
       uint32_t ph = lo1 ? p2 : p1, pl = lo1 ? p1 : p2;
       L.xb.push_back({1u << (ph - B), 1u << pl, b.e_par, b.e_anti, b.t});
     }
   }
+  auto fill = [](std::vector<double>& d, uint32_t n, double base, const std::vector<double>& su,
+                 const std::vector<std::vector<std::pair<uint32_t, std::pair<double, double>>>>& bl) {
+    d.assign(n, 0.0);
+    if (n == 0) return;
+    d[0] = base;
+    for (uint32_t w = 1; w < n; ++w) {
+      const int t = 31 - __builtin_clz(w);
+      const uint32_t u = w ^ (1u << t);
+      double v = d[u] + su[t];
+      for (const auto& e : bl[t]) v += (u & e.first) ? e.second.second : e.second.first;
+      d[w] = v;
+    }
+  };
+  fill(L.diag_lo, nlo, base_lo, su_lo, bl_lo);
+  fill(L.diag_hi, nhi, base_hi, su_hi, bl_hi);
   L.diag_lo_cls.resize(nlo);
-  for (uint32_t i = 0; i < nlo; ++i) L.diag_lo_cls[i] = L.diag_lo[L.lo_state[i]];
+  for (uint32_t i = 0; i < nlo; ++i) L.diag_lo_cls[i] = L.diag_lo[L.low->lo_state[i]];
   // padded H.v tables
   L.n_ll_p = round_up((int)L.ll.size(), kPadLL);
   L.n_hh_p = round_up((int)L.hh.size(), kPadHH);
@@ -382,8 +410,10 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1) {
     L.flip = F;
     L.nbp_off = F->nbp_off;
     L.xn_off = F->xn_off;
-    L.nbp16 = F->nbp16;
-    L.xn16 = F->xn16;
+    if (copy_shared) {
+      L.nbp16 = F->nbp16;
+      L.xn16 = F->xn16;
+    }
   }
   L.ll_t.assign(L.n_ll_p, 0.0);
   for (size_t b = 0; b < L.ll.size(); ++b) L.ll_t[b] = L.ll[b].t;

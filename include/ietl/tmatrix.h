@@ -30,6 +30,7 @@ namespace ietl {
 // the coupling to the next Lanczos vector), independent of any vector space.
 struct tmatrix_analysis {
   std::vector<double> eigval_distinct, eigval_noghost, err, err_noghost;
+  std::vector<double> eigval_all;  // every eigenvalue of T, ascending
   std::vector<int> mult, mult_noghost;
   // position of each distinct value's first member among the sorted
   // eigenvalues of T (selects its eigenvector, e.g. for Ritz vectors)
@@ -51,6 +52,7 @@ struct tmatrix_analysis {
     std::vector<double> zlast(do_err ? n : 0);
     if (tridiag::eig(n, eval.data(), beta, do_err ? zlast.data() : nullptr) > 0)
       throw std::runtime_error("LAPACK error, stev function failed.");
+    eigval_all = eval;
     const double eps = std::numeric_limits<double>::epsilon();
     const double error_tol = std::pow(eps, 2. / 3.);
     multol = std::max(alpha_max, beta_max) * 2 * eps * (10000 + n);

@@ -1,0 +1,115 @@
+// Host checks of the engine's layout and momentum-sector builders (no device):
+//  - the diagonal tables (diag_lo, diag_hi and the cross bonds) against a
+//    direct sum over sites and bonds for every state of random models;
+//  - the lattice translations and the Bloch basis (representatives and
+//    normalizations) against the oracle's restatement of
+//    blochbasisstates.h for several lattices, sectors and momenta.
+// Prints "diag_err <max abs error>" and "bloch_sectors <count>", or
+// "bloch_mismatch ..." and exits 1.
+#include <cmath>
+#include <cstdio>
+#include <random>
+
+#include "../../oracle/ed_oracle.hpp"
+#include "ed_bloch.hpp"
+#include "ed_layout.hpp"
+
+using namespace alps_ed;
+
+static double diag_error() {
+  std::mt19937 rng(5);
+  std::uniform_real_distribution<double> U(-1, 1);
+  double worst = 0;
+  for (int trial = 0; trial < 48; ++trial) {
+    const int N = 4 + trial % 18;
+    ModelSpec s;
+    s.n_sites = N;
+    s.n_up = N / 2;
+    const int nb = N + trial % 7;
+    for (int b = 0; b < nb; ++b) {
+      const int a = (int)(rng() % N), c = (int)((a + 1 + rng() % (N - 1)) % N);
+      s.bonds.push_back(make_bond(a, c, U(rng), U(rng), U(rng)));
+    }
+    for (int i = 0; i < N; ++i) s.sites.push_back(make_site(U(rng), U(rng)));
+    const SectorLayout L = build_layout(s);
+    for (uint32_t st = 0; st < (1u << N); st += 1 + (uint32_t)(rng() % 5)) {
+      double d = 0;
+      for (int i = 0; i < N; ++i) d += ((st >> (N - 1 - i)) & 1u) ? s.sites[i].e_up : s.sites[i].e_dn;
+      for (const BondTerm& b : s.bonds) {
+        const uint32_t b1 = (st >> (N - 1 - b.s1)) & 1u, b2 = (st >> (N - 1 - b.s2)) & 1u;
+        d += b1 == b2 ? b.e_par : b.e_anti;
+      }
+      const uint32_t lo = st & ((1u << L.B) - 1u), hi = L.NH ? st >> L.B : 0u;
+      double got = L.diag_lo[lo] + L.diag_hi[hi];
+      for (const CrossBond& x : L.xb) got += ((hi & x.hbit) != 0) == ((lo & x.lbit) != 0) ? x.e_par : x.e_anti;
+      worst = std::max(worst, std::fabs(got - d));
+    }
+  }
+  return worst;
+}
+
+static int bloch_sectors() {
+  struct Case {
+    const char* name;
+    int L, W;
+  };
+  const Case cases[] = {{"chain lattice", 10, 0}, {"square lattice", 4, 2}, {"ladder", 5, 0},
+                        {"triangular lattice", 3, 3}, {"frustrated square lattice", 3, 3}};
+  int checked = 0;
+  for (const Case& c : cases) {
+    Parameters p;
+    p["L"] = std::to_string(c.L);
+    if (c.W) p["W"] = std::to_string(c.W);
+    const LatticeGraph g = named_lattice_graph(c.name, Evaluator(p));
+    const edo::Graph og = edo::named_lattice(c.name, c.L, c.W, 0);
+    const std::vector<std::vector<double>> ks = translation_momenta(g);
+    if (ks != edo::translation_momenta(og)) {
+      std::printf("bloch_mismatch momenta %s\n", c.name);
+      return -1;
+    }
+    for (int n_up = 0; n_up <= g.n; ++n_up) {
+      ModelSpec s;
+      s.n_sites = g.n;
+      s.n_up = n_up;
+      for (const LatticeBond& b : g.bonds) s.bonds.push_back(make_bond(b.s, b.t, 1.0, 1.0, 0.0));
+      for (int i = 0; i < g.n; ++i) s.sites.push_back(make_site(0.0, 0.0));
+      const SectorLayout L = build_layout(s);
+      BlochOrbits orb;
+      for (size_t q = 0; q < ks.size(); ++q) {
+        const Translations t = lattice_translations(g, ks[q]);
+        const std::vector<edo::Translation> ot = edo::translations(og, ks[q]);
+        if (t.perm.size() != ot.size()) {
+          std::printf("bloch_mismatch translation count %s\n", c.name);
+          return -1;
+        }
+        for (size_t i = 0; i < ot.size(); ++i)
+          for (int site = 0; site < g.n; ++site)
+            if ((size_t)t.perm[i][site] != ot[i].perm[site] || std::abs(t.phase[i] - ot[i].phase) > 1e-14) {
+              std::printf("bloch_mismatch translation %s t=%zu\n", c.name, i);
+              return -1;
+            }
+        if (q == 0) orb = bloch_orbits(L, t);
+        const BlochSector b = bloch_sector(orb, t, g.n);
+        const edo::BlochBasis ob = edo::build_bloch(g.n, n_up, ot);
+        bool same = b.rep_state.size() == ob.reps.size();
+        for (size_t i = 0; same && i < ob.reps.size(); ++i)
+          same = b.rep_state[i] == edo::state_bits(ob.reps[i]) && std::fabs(b.norm[i] - ob.norm[i]) < 1e-14;
+        if (!same) {
+          std::printf("bloch_mismatch %s n_up=%d k=%zu: %zu vs %zu representatives\n", c.name, n_up, q,
+                      b.rep_state.size(), ob.reps.size());
+          return -1;
+        }
+        ++checked;
+      }
+    }
+  }
+  return checked;
+}
+
+int main() {
+  std::printf("diag_err %.3e\n", diag_error());
+  const int n = bloch_sectors();
+  if (n < 0) return 1;
+  std::printf("bloch_sectors %d\n", n);
+  return 0;
+}
