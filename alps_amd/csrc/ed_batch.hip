@@ -538,7 +538,7 @@ void prepare_refine(Problem& p) {
   }
 }
 
-void rebuild_lists(alps_ed_batch b, const std::vector<Problem*>& run, std::vector<BatchItem>& hv32,
+void rebuild_lists(const std::vector<Problem*>& run, std::vector<BatchItem>& hv32,
                    std::vector<BatchItem>& hv64, std::vector<BatchItem>& up32, std::vector<BatchItem>& up64,
                    uint32_t& seg32, uint32_t& seg64) {
   hv32.clear();
@@ -590,7 +590,7 @@ Group upload_group(alps_ed_batch b, const std::vector<Problem*>& run) {
   std::vector<BatchItem> hv32, hv64, up32, up64, st32, st64;
   std::vector<uint32_t> st_list;
   Group g;
-  rebuild_lists(b, run, hv32, hv64, up32, up64, g.seg32, g.seg64);
+  rebuild_lists(run, hv32, hv64, up32, up64, g.seg32, g.seg64);
   for (uint32_t i = 0; i < run.size(); ++i)
     if (run[i]->j == 0) {
       st_list.push_back(i);

@@ -1421,7 +1421,7 @@ extern "C" int alps_ed_run_parameter_file(const char* text, char* out, size_t ou
         size_t free_b = 0, total_b = 0;
         hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
         if (models.size() > 1 && need < 0.8 * (double)free_b) {
-          alps_ed_batch_config cfg{sp.dtype, need, 0, 8, 0, 0};
+          alps_ed_batch_config cfg{sp.dtype, need, 0, 8, 0, 0, 0};
           alps_ed_batch bh = nullptr;
           check_status(alps_ed_batch_create(&cfg, &bh));
           std::unique_ptr<alps_ed_batch_s, int (*)(alps_ed_batch)> hold(bh, alps_ed_batch_destroy);

@@ -2,8 +2,9 @@
 
 Bars: bit-exact for basis states, indices and matrix elements; FP64 H.v to
 1e-13 relative; FP32-storage H.v to 2e-6 relative; eigenvalues to 1e-10
-relative (the reference's own ED comparisons use
-max(1e-11, 1e-11|ref|), lib/pyalps/apptest.py:388)."""
+relative (the accuracy target of FP32-stored Lanczos vectors against FP64;
+the reference's own ED comparisons use max(1e-11, 1e-11|ref|),
+lib/pyalps/apptest.py:388)."""
 import math
 import os
 import subprocess
