@@ -1314,7 +1314,7 @@ std::string momentum_string(const std::vector<double>& k) {
 extern "C" int alps_ed_describe_parameter_file(const char* text, char* out, size_t out_len, size_t* needed) {
   return guarded([&] {
     require(text != nullptr, "parameter text is null");
-    const std::vector<Parameters> tasks = parse_parameter_file(text);
+    const std::vector<Parameters> tasks = parse_input(text);
     std::string js = "{\"tasks\": [";
     for (size_t ti = 0; ti < tasks.size(); ++ti) {
       const Parameters& p = tasks[ti];
@@ -1350,7 +1350,7 @@ extern "C" int alps_ed_describe_parameter_file(const char* text, char* out, size
 extern "C" int alps_ed_run_parameter_file(const char* text, char* out, size_t out_len, size_t* needed) {
   return guarded([&] {
     require(text != nullptr, "parameter text is null");
-    const std::vector<Parameters> tasks = parse_parameter_file(text);
+    const std::vector<Parameters> tasks = parse_input(text);
     std::string js = "{\"tasks\": [";
     for (size_t ti = 0; ti < tasks.size(); ++ti) {
       const Parameters& p = tasks[ti];

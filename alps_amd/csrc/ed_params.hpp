@@ -17,6 +17,7 @@
 #include <cctype>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <utility>
 #include <stdexcept>
 #include <string>
@@ -177,6 +178,163 @@ inline std::vector<Parameters> parse_parameter_file(const std::string& text) {
   }
   if (in_block) throw std::invalid_argument("unterminated '{' in parameter file");
   return tasks;
+}
+
+// Parameters in the reference's XML forms.  A PARAMETER element sets one
+// value (its text, trimmed; the predefined entities decoded) with the
+// keep-place assignment of Parameters::operator[]
+// (src/alps/parameter/parameters.C:168-177, parameter.C:84-101):
+//   - a document whose root is <PARAMETERLIST> (ParameterListXMLHandler,
+//     parameterlist.C:69-96): PARAMETER children set defaults, and each
+//     <PARAMETERS> child is one task: the defaults so far, then its own
+//     values;
+//   - otherwise the first <PARAMETERS> element is one task: a task file
+//     (<SIMULATION> holding <PARAMETERS>, as parameter2xml writes it,
+//     tool/parameter2xml.C:120-130) or a task result file, which repeats
+//     its parameters in the same form.
+// Comments, processing instructions and other elements are skipped.
+namespace detail {
+struct XmlTag {
+  std::string name;  // "/NAME" for an end tag
+  std::string attrs;
+  bool empty = false;
+  size_t end = 0;  // position after '>'
+};
+
+// The next tag at or after `pos` (name empty at the end of the text).
+inline XmlTag next_xml_tag(const std::string& t, size_t pos) {
+  XmlTag tag;
+  for (size_t p = t.find('<', pos); p != std::string::npos; p = t.find('<', p + 1)) {
+    if (t.compare(p, 4, "<!--") == 0) {
+      const size_t e = t.find("-->", p + 4);
+      if (e == std::string::npos) throw std::invalid_argument("XML: unterminated comment");
+      p = e + 2;
+      continue;
+    }
+    const size_t e = t.find('>', p);
+    if (e == std::string::npos) throw std::invalid_argument("XML: unterminated tag");
+    if (t[p + 1] == '?' || t[p + 1] == '!') {
+      p = e;
+      continue;
+    }
+    std::string body = t.substr(p + 1, e - p - 1);
+    if (!body.empty() && body.back() == '/') {
+      tag.empty = true;
+      body.pop_back();
+    }
+    size_t n = 0;
+    while (n < body.size() && !std::isspace((unsigned char)body[n])) ++n;
+    tag.name = body.substr(0, n);
+    tag.attrs = body.substr(n);
+    tag.end = e + 1;
+    return tag;
+  }
+  tag.end = t.size();
+  return tag;
+}
+
+inline std::string xml_decode(std::string v) {
+  static const char* ent[][2] = {{"&lt;", "<"}, {"&gt;", ">"}, {"&quot;", "\""}, {"&apos;", "'"}, {"&amp;", "&"}};
+  std::string o;
+  for (size_t i = 0; i < v.size();) {
+    bool hit = false;
+    if (v[i] == '&')
+      for (auto& e : ent)
+        if (v.compare(i, std::strlen(e[0]), e[0]) == 0) {
+          o += e[1];
+          i += std::strlen(e[0]);
+          hit = true;
+          break;
+        }
+    if (!hit) o += v[i++];
+  }
+  return o;
+}
+
+inline std::string xml_attribute(const std::string& attrs, const std::string& name) {
+  for (size_t p = attrs.find(name); p != std::string::npos; p = attrs.find(name, p + 1)) {
+    if (p > 0 && !std::isspace((unsigned char)attrs[p - 1])) continue;
+    size_t q = p + name.size();
+    while (q < attrs.size() && std::isspace((unsigned char)attrs[q])) ++q;
+    if (q >= attrs.size() || attrs[q] != '=') continue;
+    ++q;
+    while (q < attrs.size() && std::isspace((unsigned char)attrs[q])) ++q;
+    if (q >= attrs.size() || (attrs[q] != '"' && attrs[q] != '\'')) continue;
+    const size_t e = attrs.find(attrs[q], q + 1);
+    if (e == std::string::npos) break;
+    return xml_decode(attrs.substr(q + 1, e - q - 1));
+  }
+  throw std::invalid_argument("XML: name attribute not found in PARAMETER tag");
+}
+
+// One PARAMETER element starting with `tag`: sets it in `p`; returns the
+// position after it.
+inline size_t read_xml_parameter(const std::string& t, const XmlTag& tag, Parameters& p) {
+  const std::string name = xml_attribute(tag.attrs, "name");
+  if (tag.empty) {
+    p[name] = "";
+    return tag.end;
+  }
+  const size_t close = t.find("</PARAMETER>", tag.end);
+  if (close == std::string::npos) throw std::invalid_argument("XML: unterminated <PARAMETER>");
+  p[name] = xml_decode(trim(t.substr(tag.end, close - tag.end)));
+  return close + 12;
+}
+
+// The PARAMETER children of the PARAMETERS element opened by `open`, into
+// `p`; returns the position after </PARAMETERS>.
+inline size_t read_xml_parameters(const std::string& t, const XmlTag& open, Parameters& p) {
+  if (open.empty) return open.end;
+  size_t pos = open.end;
+  while (true) {
+    const XmlTag tag = next_xml_tag(t, pos);
+    if (tag.name.empty()) throw std::invalid_argument("XML: unterminated <PARAMETERS>");
+    if (tag.name == "/PARAMETERS") return tag.end;
+    if (tag.name == "PARAMETER") pos = read_xml_parameter(t, tag, p);
+    else throw std::invalid_argument("XML: unknown tag in <PARAMETERS>: " + tag.name);
+  }
+}
+}  // namespace detail
+
+inline std::vector<Parameters> parse_parameter_xml(const std::string& text) {
+  using namespace detail;
+  XmlTag root = next_xml_tag(text, 0);
+  if (root.name == "PARAMETERLIST") {
+    std::vector<Parameters> list;
+    Parameters defaults;
+    size_t pos = root.end;
+    while (!root.empty) {
+      const XmlTag tag = next_xml_tag(text, pos);
+      if (tag.name.empty()) throw std::invalid_argument("XML: unterminated <PARAMETERLIST>");
+      if (tag.name == "/PARAMETERLIST") break;
+      if (tag.name == "PARAMETER") {
+        pos = read_xml_parameter(text, tag, defaults);
+      } else if (tag.name == "PARAMETERS") {
+        Parameters cur = defaults;
+        pos = read_xml_parameters(text, tag, cur);
+        list.push_back(cur);
+      } else {
+        throw std::invalid_argument("XML: unknown tag in <PARAMETERLIST>: " + tag.name);
+      }
+    }
+    return list;
+  }
+  for (XmlTag tag = root; !tag.name.empty(); tag = next_xml_tag(text, tag.end))
+    if (tag.name == "PARAMETERS") {
+      Parameters p;
+      read_xml_parameters(text, tag, p);
+      return {p};
+    }
+  throw std::invalid_argument("XML input without <PARAMETERS> or <PARAMETERLIST>");
+}
+
+// The tasks of an input text: an XML document (its first non-blank
+// character is '<') as above, anything else a parameter file.
+inline std::vector<Parameters> parse_input(const std::string& text) {
+  size_t i = 0;
+  while (i < text.size() && std::isspace((unsigned char)text[i])) ++i;
+  if (i < text.size() && text[i] == '<') return parse_parameter_xml(text);
+  return parse_parameter_file(text);
 }
 
 // Numeric value of an expression in the context of a parameter set.

@@ -226,3 +226,29 @@ def test_cli_writes_task_xml(tmp_path):
         doc = (tmp_path / f"parm.task{i + 1}.out.xml").read_text()
         evs, _ = _xml_check(doc, t)
         assert evs[0].findall("QUANTUMNUMBER") == []  # Sz_total given: no loop
+
+
+def test_task_result_file_reruns(tmp_path):
+    # a task result file holds the task's parameters in the task file form,
+    # so it is itself an input; a job runs it and writes to its <OUTPUT>
+    import os
+    import subprocess
+
+    say("xml round trip")
+    text = ('LATTICE="chain lattice"; L=8; J=1; Jz=0.7; Sz_total=0; NUMBER_EIGENVALUES=2;\n'
+            'TRANSLATION_SYMMETRY=false; NOTE="a<b & c"\n{}\n')
+    (t,) = A.run_parameter_file(text)
+    (again,) = A.run_parameter_file(t["xml"])
+    assert again["parameters"] == t["parameters"] and again["parameters"]["NOTE"] == "a<b & c"
+    assert [s["eigenvalues"] for s in again["sectors"]] == [s["eigenvalues"] for s in t["sectors"]]
+    assert again["xml"] == t["xml"]
+
+    (tmp_path / "t1.in.xml").write_text(t["xml"])
+    job = tmp_path / "job.in.xml"
+    job.write_text('<JOB><OUTPUT file="job.out.xml"/>'
+                   '<TASK status="new"><INPUT file="t1.in.xml"/><OUTPUT file="t1.out.xml"/></TASK></JOB>')
+    cli = os.path.join(os.path.dirname(A.LIB_PATH), "alps_ed_sparsediag")
+    r = subprocess.run([cli, "--write-xml", str(job)], check=True, capture_output=True, text=True, timeout=60)
+    assert '"ground_state_energy"' in r.stdout
+    assert (tmp_path / "t1.out.xml").read_text() == t["xml"]
+

@@ -171,6 +171,108 @@ def test_cli_describe(tmp_path):
 
 
 # ---------------------------------------------------------------------------
+# XML inputs: <PARAMETERS>, <PARAMETERLIST>, task files (<SIMULATION>) and
+# job files (<JOB>, through the command-line driver)
+# ---------------------------------------------------------------------------
+def test_parameters_xml_golden():
+    # test/parameter/parameters_xml: one <PARAMETERS> document
+    (t,) = A.describe_parameter_file(_read("parameters_xml.input"))
+    want = [ln.rstrip(";").split(" = ") for ln in _read("parameters_xml.output").splitlines()[:2]]
+    assert list(t["parameters"].items()) == [tuple(w) for w in want]
+
+
+def test_parameterlist_xml_golden():
+    # test/parameter/parameterlist_xml: PARAMETER children are defaults for
+    # the <PARAMETERS> blocks after them; block values keep their place
+    tasks = A.describe_parameter_file(_read("parameterlist_xml.input"))
+    want = _blocks(_read("parameterlist_xml.output").split("[Output to alps::oxstream]")[0])
+    assert len(want) == len(tasks) == 3
+    for t, w in zip(tasks, want):
+        assert list(t["parameters"].items()) == w
+
+
+_TASK_XML = """<?xml version="1.0" encoding="UTF-8"?>
+<?xml-stylesheet type="text/xsl" href="ALPS.xsl"?>
+<SIMULATION xmlns:xsi="http://www.w3.org/2001/XMLSchema-instance">
+  <!-- a comment with <PARAMETER name="X">1</PARAMETER> inside -->
+  <PARAMETERS>
+    <PARAMETER name="LATTICE">open chain lattice</PARAMETER>
+    <PARAMETER name="L">5</PARAMETER>
+    <PARAMETER name="a">2</PARAMETER>
+    <PARAMETER name="J">a/2</PARAMETER>
+    <PARAMETER name="Jz">   0.5  </PARAMETER>
+    <PARAMETER name="NOTE">x &lt; y &amp;&amp; &quot;z&quot; &amp;lt;</PARAMETER>
+    <PARAMETER name="EMPTY"/>
+    <PARAMETER name='L'>6</PARAMETER>
+  </PARAMETERS>
+  <AVERAGES><SCALAR_AVERAGE name="Ground State Energy"><MEAN>-1</MEAN></SCALAR_AVERAGE></AVERAGES>
+</SIMULATION>
+"""
+
+
+def test_task_xml_matches_parameter_file():
+    (x,) = A.describe_parameter_file(_TASK_XML)
+    assert list(x["parameters"].items()) == [
+        ("LATTICE", "open chain lattice"), ("L", "6"), ("a", "2"), ("J", "a/2"), ("Jz", "0.5"),
+        ("NOTE", 'x < y && "z" &lt;'), ("EMPTY", "")]
+    (p,) = A.describe_parameter_file('LATTICE="open chain lattice"; L=6; a=2; J="a/2"; Jz=0.5; {}')
+    for key in ("n_sites", "bond_s", "bond_t", "jz", "jxy", "k", "h", "d", "n_up", "whole_space"):
+        assert x[key] == p[key], key
+    assert x["jz"] == [0.5] * 5 and x["jxy"] == [1.0] * 5
+
+
+def test_xml_errors():
+    for text, msg in [("<SIMULATION><AVERAGES/></SIMULATION>", "without <PARAMETERS>"),
+                      ("<PARAMETERS><PARAMETER>1</PARAMETER></PARAMETERS>", "name attribute"),
+                      ("<PARAMETERS><PARAMETER name='L'>1</PARAMETERS>", "unterminated <PARAMETER>"),
+                      ("<PARAMETERS><PARAMETER name='L'>1</PARAMETER>", "unterminated <PARAMETERS>"),
+                      ("<PARAMETERLIST><FOO/></PARAMETERLIST>", "unknown tag")]:
+        with pytest.raises(A.EDError) as ei:
+            A.describe_parameter_file(text)
+        assert msg in str(ei.value), text
+
+
+def _cli():
+    return os.path.join(os.path.dirname(A.LIB_PATH), "alps_ed_sparsediag")
+
+
+def test_cli_job_file(tmp_path):
+    import json
+    import subprocess
+
+    # a job as parameter2xml writes it (tool/parameter2xml.C:95-138), task
+    # files in a subdirectory named relative to the job file
+    (tmp_path / "sub").mkdir()
+    for i, L in enumerate((3, 4, 5), 1):
+        (tmp_path / "sub" / f"j.task{i}.in.xml").write_text(
+            _TASK_XML.replace(">6<", f">{L}<"))
+    job = tmp_path / "j.in.xml"
+    job.write_text("""<?xml version="1.0" encoding="UTF-8"?>
+<JOB xmlns:xsi="http://www.w3.org/2001/XMLSchema-instance" name="j">
+  <OUTPUT file="j.out.xml"/>
+  <TASK status="new"><INPUT file="sub/j.task1.in.xml"/><OUTPUT file="sub/j.task1.out.xml"/></TASK>
+  <TASK status="finished"><INPUT file="sub/j.task2.in.xml"/><OUTPUT file="sub/j.task2.out.xml"/></TASK>
+  <TASK><INPUT file="sub/j.task3.in.xml"></INPUT></TASK>
+</JOB>
+""")
+    out = subprocess.run([_cli(), "--describe", str(job)], check=True, capture_output=True, text=True).stdout
+    tasks = json.loads(out)["tasks"]
+    # the finished task is not run
+    assert [t["parameters"]["L"] for t in tasks] == ["3", "5"]
+    assert [len(t["bond_s"]) for t in tasks] == [2, 4]
+
+    for body, msg in [("<TASK><OUTPUT file='a'/></TASK>", "missing <INPUT>"),
+                      ("<TASK status='done'><INPUT file='a'/></TASK>", "illegal status"),
+                      ("<TASK><INPUT file='sub/j.task1.in.xml'/>", "missing </TASK>")]:
+        job.write_text("<JOB>" + body + "</JOB>")
+        r = subprocess.run([_cli(), "--describe", str(job)], capture_output=True, text=True)
+        assert r.returncode == 1 and msg in r.stderr, (body, r.stderr)
+    job.write_text("<JOB><TASK><INPUT file='sub/none.xml'/></TASK></JOB>")
+    r = subprocess.run([_cli(), "--describe", str(job)], capture_output=True, text=True)
+    assert r.returncode == 2 and "cannot read" in r.stderr
+
+
+# ---------------------------------------------------------------------------
 # measurement requests (MeasurementOperators, measurement_operators.C:25-66)
 # and the lattice labels they are reported with
 # ---------------------------------------------------------------------------
