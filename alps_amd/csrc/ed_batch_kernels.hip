@@ -121,9 +121,7 @@ __global__ __launch_bounds__(kBlock) void batch_finish_kernel(const BatchJob* __
   const bool hv = mode == kBatchAlpha;
   const double* part = hv ? ldc(&J->part_hv) : ldc(&J->part_up);
   const uint32_t n = hv ? ldc(&J->L.n_seg) : ldc(&J->n_chunks);
-  double v = 0.0;
-  for (uint32_t i = threadIdx.x; i < n; i += kBlock) v += part[i];
-  const double s = block_sum_all(v);
+  const double s = sum_partials(Partials{part, n});
   if (threadIdx.x != 0) return;
   double* scale = ldc(&J->scale);
   if (mode == kBatchStart) {

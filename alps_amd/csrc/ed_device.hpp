@@ -30,11 +30,23 @@ __device__ __forceinline__ double block_sum_all(double v) {
   return s;
 }
 
-__device__ __forceinline__ double sum_partials(const Partials& P) {
+// A thread's share of a sum of partials: entries tid, tid + kBlock, ... in
+// index order.  The loads of eight terms are issued together before their
+// additions (a loop with one load per addition waits for each in turn).
+__device__ __forceinline__ double thread_partials(const Partials& P) {
   double v = 0.0;
-  for (uint32_t i = threadIdx.x; i < P.n; i += kBlock) v += P.p[i];
-  return block_sum_all(v);
+  for (uint32_t i = threadIdx.x; i < P.n; i += 8u * kBlock) {
+    double t[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t[k] = i + k * kBlock < P.n ? P.p[i + k * kBlock] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (i + k * kBlock < P.n) v += t[k];
+  }
+  return v;
 }
+
+__device__ __forceinline__ double sum_partials(const Partials& P) { return block_sum_all(thread_partials(P)); }
 
 __device__ __forceinline__ void store_partial(double* out, double v) {
   const double s = block_sum_all(v);
@@ -135,6 +147,17 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
     d[q] = dh + dlp[rc[q]];
     off[q] = 0.0;
   }
+  // Row records in the class flip tables (flip_index, ed_layout.hpp): bond
+  // chunk k of row r starts at rec(r) + k * 64 * chunk.
+  static_assert(CLL % kLLChunk == 0 && CX % kXChunk == 0, "schedule chunks are multiples of the table chunks");
+  typedef uint16_t u16x4 __attribute__((ext_vector_type(4)));
+  typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+  uint32_t lrec[NQ], xrec[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    lrec[q] = (rc[q] >> 6) * 64u * (uint32_t)L.n_ll_p + (rc[q] & 63u) * kLLChunk;
+    xrec[q] = (rc[q] >> 6) * 64u * (uint32_t)L.n_xb_p + (rc[q] & 63u) * kXChunk;
+  }
   // low-low flips: target rows from the class table, values from LDS
   const int n_ll = (L.n_ll + CLL - 1) / CLL * CLL;
   for (int b0 = 0; b0 < n_ll; b0 += CLL) {
@@ -145,34 +168,53 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
-      for (int c = 0; c < CLL; ++c) j[q][c] = (nbp + (b0 + c) * len)[rc[q]];
+      for (int k = 0; k < CLL / kLLChunk; ++k) {
+        const u16x4 e4 = *reinterpret_cast<const u16x4*>(nbp + lrec[q] + (uint32_t)(b0 + k * kLLChunk) * 64u);
+#pragma unroll
+        for (int c = 0; c < kLLChunk; ++c) j[q][k * kLLChunk + c] = e4[c];
+      }
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
       for (int c = 0; c < CLL; ++c) off[q] += t[c] * (double)xs[j[q][c]];
   }
-  // high-high flips: uniform decision, contiguous target run
-  const int n_hh = (L.n_hh + CHH - 1) / CHH * CHH;
-  for (int b0 = 0; b0 < n_hh; b0 += CHH) {
-    const V* xt[CHH];
-    double tt[CHH];
+  // high-high flips: a bond flips every row of the segment or none (its two
+  // bits lie in h), with a contiguous target run.  The bonds that flip are
+  // found 64 at a time (one per lane, then a ballot), and only those are
+  // visited, in bond order; the unfilled slots of the last chunk read this
+  // segment with weight 0.  Skipped bonds would add exact zeros, so the
+  // sums are unchanged.
+  const uint32_t lane = threadIdx.x & 63u;
+  for (int w0 = 0; w0 < L.n_hh; w0 += 64) {
+    const int bl = w0 + (int)lane;
+    const uint32_t ml = bl < L.n_hh ? L.hh_mask[bl] : 0u;
+    const uint32_t bits = h & ml;
+    uint64_t act = __ballot(bits != 0u && bits != ml);
+    while (act != 0u) {
+      const V* xt[CHH];
+      double tt[CHH];
 #pragma unroll
-    for (int c = 0; c < CHH; ++c) {
-      const uint32_t m = ldc(L.hh_mask + b0 + c);
-      const uint32_t bits = h & m;
-      const bool act = bits != 0u && bits != m;
-      xt[c] = act ? x + ldc(L.hi_base + (h ^ m)) : xb;
-      tt[c] = act ? ldc(L.hh_t + b0 + c) : 0.0;
+      for (int c = 0; c < CHH; ++c) {
+        if (act != 0u) {
+          const int b = w0 + __builtin_ctzll(act);
+          act &= act - 1u;
+          xt[c] = x + ldc(L.hi_base + (h ^ ldc(L.hh_mask + b)));
+          tt[c] = ldc(L.hh_t + b);
+        } else {
+          xt[c] = xb;
+          tt[c] = 0.0;
+        }
+      }
+      V v[NQ][CHH];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int c = 0; c < CHH; ++c) v[q][c] = xt[c][rc[q]];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int c = 0; c < CHH; ++c) off[q] += tt[c] * (double)v[q][c];
     }
-    V v[NQ][CHH];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q)
-#pragma unroll
-      for (int c = 0; c < CHH; ++c) v[q][c] = xt[c][rc[q]];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q)
-#pragma unroll
-      for (int c = 0; c < CHH; ++c) off[q] += tt[c] * (double)v[q][c];
   }
   // cross bonds: diagonal always, flip when the high and low bits differ.
   // If the neighbouring high word has no segment no row can flip (its low
@@ -197,7 +239,11 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
-      for (int c = 0; c < CX; ++c) e[q][c] = (xnp + (b0 + c) * len)[rc[q]];
+      for (int k = 0; k < CX / kXChunk; ++k) {
+        const u16x2 e2 = *reinterpret_cast<const u16x2*>(xnp + xrec[q] + (uint32_t)(b0 + k * kXChunk) * 64u);
+#pragma unroll
+        for (int c = 0; c < kXChunk; ++c) e[q][k * kXChunk + c] = e2[c];
+      }
     V v[NQ][CX];
     bool f[NQ][CX];
 #pragma unroll
@@ -235,8 +281,11 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
 // y is rounded to the storage type).
 //
 // The segment's slice of x is staged in LDS.  Each thread owns up to kRows
-// rows (r = tid + q*kBlock); the number of row slots is uniform over the
-// workgroup and selects a fixed-size instance of the bond loops.  CLL, CHH,
+// rows (r = tid + q*kBlock); the number of row slots that hold rows is
+// uniform over each wave and selects a fixed-size instance of the bond
+// loops, so a wave does no work for slots past the segment (the waves of a
+// short segment's last slot skip it, those of a segment of at most 64 rows
+// after the first do nothing).  CLL, CHH,
 // CX: chunk sizes of the flip loops (the schedule, ed_kernels.hpp).
 template <class V, bool DOT, int CLL = 4, int CHH = 4, int CX = 2>
 __device__ __forceinline__ void hv_segment(const DevLayout& L, uint32_t h, const V* __restrict__ x,
@@ -256,10 +305,14 @@ __device__ __forceinline__ void hv_segment(const DevLayout& L, uint32_t h, const
   const uint16_t* xnp = L.xn16 + ldc(L.xn_off + kl);
   const double* dlp = L.diag_lo_cls + cls0;
   __syncthreads();
-  const uint32_t nq = (len + kBlock - 1) / kBlock;
+  // row slots of this wave: slot q holds rows wave*64 + q*kBlock + lane
+  const uint32_t w0 = tid & ~63u;
+  const uint32_t nq = __builtin_amdgcn_readfirstlane(len > w0 ? (len - w0 + kBlock - 1) / kBlock : 0u);
   static_assert(kRows == 4, "hv_segment dispatches one to four row slots");
-  double dotp;
-  if (nq <= 1)
+  double dotp = 0.0;
+  if (nq == 0)
+    ;  // no rows in this wave
+  else if (nq == 1)
     dotp = hv_rows<V, DOT, 1, CLL, CHH, CX>(L, h, x, y, xs, len, base, dh, nbp, xnp, dlp);
   else if (nq == 2)
     dotp = hv_rows<V, DOT, 2, CLL, CHH, CX>(L, h, x, y, xs, len, base, dh, nbp, xnp, dlp);

@@ -97,7 +97,11 @@ __global__ __launch_bounds__(kBlock) void update_kernel(uint64_t n, const V* w, 
 // order, scale_j = 1/sqrt(sum(norm_prev)) (the norm of u_j), beta_{j-1} =
 // sqrt(sum(norm_prev)) when j > 0, and alpha_j = scale_j^2 * sum(hv_part);
 // block 0 stores them.  Then as update_kernel.  This replaces the two
-// one-block reduction launches of a step.
+// one-block reduction launches of a step.  The vector elements of a
+// thread's first kPrefetch iterations are loaded before the partial sums,
+// so that their latency overlaps the reduction.
+constexpr int kPrefetch = 3;
+
 template <class V>
 __global__ __launch_bounds__(kBlock) void update_fused_kernel(uint64_t n, const V* w, const V* __restrict__ u_cur,
                                                               const V* u_prev, V* out, Partials hv_part,
@@ -106,6 +110,22 @@ __global__ __launch_bounds__(kBlock) void update_fused_kernel(uint64_t n, const 
                                                               double* __restrict__ scale_j,
                                                               const double* __restrict__ scale_jm1,
                                                               double* __restrict__ norm_out) {
+  // four consecutive elements per thread and iteration (vector loads)
+  typedef V v4 __attribute__((ext_vector_type(4), aligned(sizeof(V))));
+  const bool has_prev = beta_jm1 != nullptr;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t n4 = n / 4;
+  v4 wp[kPrefetch], up[kPrefetch], pp[kPrefetch];
+#pragma unroll
+  for (int k = 0; k < kPrefetch; ++k) {
+    const uint64_t q = gid + k * stride;
+    if (q < n4) {
+      wp[k] = *reinterpret_cast<const v4*>(w + 4 * q);
+      up[k] = *reinterpret_cast<const v4*>(u_cur + 4 * q);
+      pp[k] = has_prev ? *reinterpret_cast<const v4*>(u_prev + 4 * q) : up[k];
+    }
+  }
   const double nrm = sqrt(sum_partials(norm_prev));
   const double sc = 1.0 / nrm;
   const double a = sc * sc * sum_partials(hv_part);
@@ -115,18 +135,9 @@ __global__ __launch_bounds__(kBlock) void update_fused_kernel(uint64_t n, const 
     if (beta_jm1) *beta_jm1 = nrm;
   }
   const double ca = a * sc;
-  const bool has_prev = beta_jm1 != nullptr;
   const double cb = has_prev ? nrm * (*scale_jm1) : 0.0;
   double acc = 0.0;
-  // four consecutive elements per thread and iteration (vector loads)
-  typedef V v4 __attribute__((ext_vector_type(4), aligned(sizeof(V))));
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t n4 = n / 4;
-  for (uint64_t q = gid; q < n4; q += stride) {
-    const v4 wv = *reinterpret_cast<const v4*>(w + 4 * q);
-    const v4 uv = *reinterpret_cast<const v4*>(u_cur + 4 * q);
-    const v4 pv = has_prev ? *reinterpret_cast<const v4*>(u_prev + 4 * q) : uv;
+  auto step = [&](uint64_t q, const v4& wv, const v4& uv, const v4& pv) {
     v4 o;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -137,6 +148,17 @@ __global__ __launch_bounds__(kBlock) void update_fused_kernel(uint64_t n, const 
       acc += rd * rd;
     }
     *reinterpret_cast<v4*>(out + 4 * q) = o;
+  };
+#pragma unroll
+  for (int k = 0; k < kPrefetch; ++k) {
+    const uint64_t q = gid + k * stride;
+    if (q < n4) step(q, wp[k], up[k], pp[k]);
+  }
+  for (uint64_t q = gid + kPrefetch * stride; q < n4; q += stride) {
+    const v4 wv = *reinterpret_cast<const v4*>(w + 4 * q);
+    const v4 uv = *reinterpret_cast<const v4*>(u_cur + 4 * q);
+    const v4 pv = has_prev ? *reinterpret_cast<const v4*>(u_prev + 4 * q) : uv;
+    step(q, wv, uv, pv);
   }
   for (uint64_t i = 4 * n4 + gid; i < n; i += stride) {
     double r = sc * (double)w[i] - ca * (double)u_cur[i];

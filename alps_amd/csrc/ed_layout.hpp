@@ -131,10 +131,12 @@ struct SectorLayout {
   uint32_t max_seg = 0;                // longest segment
   // H.v tables.  Bond lists are padded with null bonds (coefficient 0,
   // never active) to multiples of the kernel's chunk sizes.
-  //   nbp16[nbp_off[c] + b*C(B,c) + r]: row reached from row r of a class-c
-  //     segment by low-low bond b, or C(B,c) (the zero slot) if not allowed;
-  //   xn16[xn_off[c] + b*C(B,c) + r]: rank of lo ^ lbit_b in its class, with
-  //     bit 15 = the low bit of cross bond b in row r.
+  //   nbp16[nbp_off[c] + flip_index(r, b, n_ll_p, kLLChunk)]: row reached
+  //     from row r of a class-c segment by low-low bond b, or C(B,c) (the
+  //     zero slot) if not allowed;
+  //   xn16[xn_off[c] + flip_index(r, b, n_xb_p, kXChunk)]: rank of
+  //     lo ^ lbit_b in its class, with bit 15 = the low bit of cross bond b
+  //     in row r.
   int n_ll_p = 0, n_hh_p = 0, n_xb_p = 0;
   std::vector<uint16_t> nbp16, xn16;
   std::vector<uint32_t> nbp_off, xn_off;  // [B+2]
@@ -159,6 +161,18 @@ struct SectorLayout {
 constexpr int kPadLL = 8, kPadHH = 8, kPadX = 4;
 
 inline int round_up(int n, int m) { return (n + m - 1) / m * m; }
+
+// Flip tables hold each class in groups of 64 rows (one wave's rows).
+// Within a group, the entries of bonds [k*ch, (k+1)*ch) of its 64 rows are
+// contiguous, row after row: one load brings a row's ch entries, and the
+// loads of a wave's 64 consecutive rows cover one contiguous run.  A class
+// of len rows takes round_up(len, 64) * n_pad entries.
+constexpr int kLLChunk = 4, kXChunk = 2;  // u16 entries per load (8 and 4 bytes)
+static_assert(kPadLL % kLLChunk == 0 && kPadX % kXChunk == 0, "chunks divide the padding");
+
+inline size_t flip_index(uint32_t r, int b, int n_pad, int ch) {
+  return (size_t)(r >> 6) * 64u * n_pad + (size_t)(b / ch) * 64u * ch + (size_t)(r & 63u) * ch + b % ch;
+}
 
 inline int choose_low_bits(int N) {
   // A 12-bit low word gives row segments of up to C(12,6) = 924 rows: long
@@ -225,32 +239,29 @@ inline std::shared_ptr<const FlipTables> flip_tables(int B, const std::vector<ui
   F->nbp_off.assign(B + 2, 0);
   F->xn_off.assign(B + 2, 0);
   for (int c = 0; c <= B; ++c) {
-    F->nbp_off[c + 1] = F->nbp_off[c] + (uint32_t)(binom(B, c) * n_ll_p);
-    F->xn_off[c + 1] = F->xn_off[c] + (uint32_t)(binom(B, c) * n_xb_p);
+    const uint64_t len64 = (uint64_t)round_up((int)binom(B, c), 64);
+    F->nbp_off[c + 1] = F->nbp_off[c] + (uint32_t)(len64 * n_ll_p);
+    F->xn_off[c + 1] = F->xn_off[c] + (uint32_t)(len64 * n_xb_p);
   }
   F->nbp16.assign(F->nbp_off[B + 1], 0);
   F->xn16.assign(F->xn_off[B + 1], 0);
   for (int c = 0; c <= B; ++c) {
     const uint32_t len = (uint32_t)binom(B, c);
     const uint32_t* ls = T.lo_state.data() + T.lo_class_off[c];
+    uint16_t* nb = F->nbp16.data() + F->nbp_off[c];
     for (int b = 0; b < n_ll_p; ++b) {
-      uint16_t* out = F->nbp16.data() + F->nbp_off[c] + (size_t)b * len;
-      if (b >= (int)ll_masks.size()) {
-        for (uint32_t r = 0; r < len; ++r) out[r] = (uint16_t)len;
-        continue;
-      }
-      const uint32_t m = ll_masks[b];
+      const uint32_t m = b < (int)ll_masks.size() ? ll_masks[b] : 0u;
       for (uint32_t r = 0; r < len; ++r) {
         const uint32_t lo = ls[r], bits = lo & m;
-        out[r] = (uint16_t)((bits != 0u && bits != m) ? T.lo_rank[lo ^ m] : len);
+        nb[flip_index(r, b, n_ll_p, kLLChunk)] = (uint16_t)((bits != 0u && bits != m) ? T.lo_rank[lo ^ m] : len);
       }
     }
+    uint16_t* xn = F->xn16.data() + F->xn_off[c];
     for (int b = 0; b < (int)x_lbits.size(); ++b) {
-      uint16_t* out = F->xn16.data() + F->xn_off[c] + (size_t)b * len;
       const uint32_t lb = x_lbits[b];
       for (uint32_t r = 0; r < len; ++r) {
         const uint32_t lo = ls[r];
-        out[r] = (uint16_t)(T.lo_rank[lo ^ lb] | (((lo & lb) != 0u ? 1u : 0u) << 15));
+        xn[flip_index(r, b, n_xb_p, kXChunk)] = (uint16_t)(T.lo_rank[lo ^ lb] | (((lo & lb) != 0u ? 1u : 0u) << 15));
       }
     }
   }

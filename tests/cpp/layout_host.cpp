@@ -3,9 +3,14 @@
 //    direct sum over sites and bonds for every state of random models;
 //  - the lattice translations and the Bloch basis (representatives and
 //    normalizations) against the oracle's restatement of
-//    blochbasisstates.h for several lattices, sectors and momenta.
-// Prints "diag_err <max abs error>" and "bloch_sectors <count>", or
-// "bloch_mismatch ..." and exits 1.
+//    blochbasisstates.h for several lattices, sectors and momenta;
+//  - the flip tables read with the H.v kernel's addressing (row record
+//    (r/64)*64*n_pad + (r%64)*chunk, bond chunk k at k*64*chunk) against the
+//    flipped words' ranks, for every class, row and bond of random models,
+//    and every such read inside its class.
+// Prints "diag_err <max abs error>", "bloch_sectors <count>" and
+// "flip_bad <mismatches> flip_checked <entries>", or "bloch_mismatch ..."
+// and exits 1.
 #include <cmath>
 #include <cstdio>
 #include <random>
@@ -106,8 +111,51 @@ static int bloch_sectors() {
   return checked;
 }
 
+static void flip_check(long& bad, long& checked) {
+  std::mt19937 rng(11);
+  for (int trial = 0; trial < 24; ++trial) {
+    const int N = 6 + trial % 20;
+    ModelSpec s;
+    s.n_sites = N;
+    s.n_up = N / 2;
+    for (int b = 0; b < N + trial % 9; ++b) {
+      const int a = (int)(rng() % N), c = (int)((a + 1 + rng() % (N - 1)) % N);
+      s.bonds.push_back(make_bond(a, c, 0.5, 1.0, 0.0));
+    }
+    for (int i = 0; i < N; ++i) s.sites.push_back(make_site(0.0, 0.0));
+    const SectorLayout L = build_layout(s);
+    const int B = L.B;
+    for (int c = 0; c <= B; ++c) {
+      const uint32_t len = L.lo_class_off[c + 1] - L.lo_class_off[c];
+      const size_t nll = L.nbp_off[c + 1] - L.nbp_off[c], nx = L.xn_off[c + 1] - L.xn_off[c];
+      for (uint32_t r = 0; r < len; ++r) {
+        const uint32_t lo = L.lo_state[L.lo_class_off[c] + r];
+        const uint32_t lrec = (r >> 6) * 64u * L.n_ll_p + (r & 63u) * kLLChunk;
+        const uint32_t xrec = (r >> 6) * 64u * L.n_xb_p + (r & 63u) * kXChunk;
+        for (int b = 0; b < L.n_ll_p; ++b) {
+          const size_t at = lrec + (size_t)(b / kLLChunk * kLLChunk) * 64u + b % kLLChunk;
+          const uint32_t m = b < (int)L.ll.size() ? L.ll[b].mask : 0u, bits = lo & m;
+          const uint32_t want = (bits != 0u && bits != m) ? L.lo_rank[lo ^ m] : len;
+          ++checked;
+          if (at >= nll || L.nbp16[L.nbp_off[c] + at] != want) ++bad;
+        }
+        for (int b = 0; b < (int)L.xb.size(); ++b) {
+          const size_t at = xrec + (size_t)(b / kXChunk * kXChunk) * 64u + b % kXChunk;
+          const uint32_t lb = L.xb[b].lbit;
+          const uint32_t want = L.lo_rank[lo ^ lb] | ((lo & lb) != 0u ? 0x8000u : 0u);
+          ++checked;
+          if (at >= nx || L.xn16[L.xn_off[c] + at] != want) ++bad;
+        }
+      }
+    }
+  }
+}
+
 int main() {
   std::printf("diag_err %.3e\n", diag_error());
+  long bad = 0, checked = 0;
+  flip_check(bad, checked);
+  std::printf("flip_bad %ld flip_checked %ld\n", bad, checked);
   const int n = bloch_sectors();
   if (n < 0) return 1;
   std::printf("bloch_sectors %d\n", n);
