@@ -115,35 +115,29 @@ __device__ __forceinline__ T ldc_struct(const T* p) {
   return out;
 }
 
-// H.v over the row segment of high word h (one workgroup of kBlock threads;
-// dynamic LDS of at least (L.max_seg + 1) elements of V):
-//   y[r] = sum_c H[r][c] x[c] for the segment's rows r;
-// with DOT, *dot_slot receives the segment's sum of x_r (H x)_r (FP64, before
-// y is rounded to the storage type).
-//
-// The segment's slice of x is staged in LDS.  Each thread owns up to kRows
-// rows (r = tid + q*kBlock); row slots past the segment are skipped with a
-// uniform test, so every chunk's scalar data is loaded once per thread.
-// The bond loops of hv_segment for NQ row slots per thread (rows tid,
+// The rows of hv_segment for NQ row slots per thread (rows tid,
 // tid + kBlock, ...), NQ fixed at compile time so that the loads of all
 // slots are issued together; rows past the segment are clamped to row 0 and
-// not stored.  The flip bonds of each class are processed in chunks of
-// CLL / CHH / CX (the table padding is a multiple of each).  Returns the
+// not stored.  Stages the thread's rows of x in LDS and passes the
+// workgroup barrier.  The flip bonds of each class are processed in chunks
+// of CLL / CHH / CX (the table padding is a multiple of each).  Returns the
 // thread's share of x.Hx when DOT.
 template <class V, bool DOT, int NQ, int CLL, int CHH, int CX>
 __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const V* __restrict__ x,
-                                          V* __restrict__ y, const V* xs, uint32_t len, uint32_t base, double dh,
+                                          V* __restrict__ y, V* xs, uint32_t len, uint32_t base, double dh,
                                           const uint16_t* nbp, const uint16_t* xnp, const double* dlp) {
   const uint32_t tid = threadIdx.x;
   const V* xb = x + base;
   uint32_t rc[NQ];
   bool ok[NQ];
+  V xo[NQ];
   double d[NQ], off[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     const uint32_t r = tid + q * kBlock;
     ok[q] = r < len;
     rc[q] = ok[q] ? r : 0u;  // clamped row: every read stays in bounds
+    xo[q] = xb[rc[q]];
     d[q] = dh + dlp[rc[q]];
     off[q] = 0.0;
   }
@@ -158,21 +152,59 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
     lrec[q] = (rc[q] >> 6) * 64u * (uint32_t)L.n_ll_p + (rc[q] & 63u) * kLLChunk;
     xrec[q] = (rc[q] >> 6) * 64u * (uint32_t)L.n_xb_p + (rc[q] & 63u) * kXChunk;
   }
-  // low-low flips: target rows from the class table, values from LDS
-  const int n_ll = (L.n_ll + CLL - 1) / CLL * CLL;
-  for (int b0 = 0; b0 < n_ll; b0 += CLL) {
-    double t[CLL];
+  // The cross-bond table entries of the first chunk are loaded now, so that
+  // they arrive while the other bonds are processed.
+  const int n_xb = (L.n_xb + CX - 1) / CX * CX;
+  constexpr int KX = CX / kXChunk;
+  u16x2 xr[NQ][KX];
+  if (n_xb > 0) {
 #pragma unroll
-    for (int c = 0; c < CLL; ++c) t[c] = ldc(L.ll_t + b0 + c);
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int k = 0; k < KX; ++k) xr[q][k] = *reinterpret_cast<const u16x2*>(xnp + xrec[q] + k * kXChunk * 64u);
+  }
+  // low-low flips: target rows from the class table, values from LDS.  The
+  // table entries of the next chunk are loaded while the current one is
+  // processed.
+  const int n_ll = (L.n_ll + CLL - 1) / CLL * CLL;
+  constexpr int KL = CLL / kLLChunk;
+  u16x4 tb[NQ][KL];
+  if (n_ll > 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int k = 0; k < KL; ++k) tb[q][k] = *reinterpret_cast<const u16x4*>(nbp + lrec[q] + k * kLLChunk * 64u);
+  }
+  // masks of the first 64 high-high bonds (one per lane, for the ballot below)
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t ml0 = (int)lane < L.n_hh ? L.hh_mask[lane] : 0u;
+  // The segment's slice of x into LDS: each thread stores its own rows (the
+  // rows of the segment are exactly the rows of the threads' slots), with
+  // the table loads above already in flight; then the barrier that every
+  // wave of the workgroup passes once (hv_segment).
+#pragma unroll
+  for (int q = 0; q < NQ; ++q)
+    if (ok[q]) xs[rc[q]] = xo[q];
+  if (tid == 0) xs[len] = (V)0;
+  __syncthreads();
+  for (int b0 = 0; b0 < n_ll; b0 += CLL) {
     uint32_t j[NQ][CLL];
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
-      for (int k = 0; k < CLL / kLLChunk; ++k) {
-        const u16x4 e4 = *reinterpret_cast<const u16x4*>(nbp + lrec[q] + (uint32_t)(b0 + k * kLLChunk) * 64u);
+      for (int k = 0; k < KL; ++k)
 #pragma unroll
-        for (int c = 0; c < kLLChunk; ++c) j[q][k * kLLChunk + c] = e4[c];
-      }
+        for (int c = 0; c < kLLChunk; ++c) j[q][k * kLLChunk + c] = tb[q][k][c];
+    if (b0 + CLL < n_ll) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int k = 0; k < KL; ++k)
+          tb[q][k] = *reinterpret_cast<const u16x4*>(nbp + lrec[q] + (uint32_t)(b0 + CLL + k * kLLChunk) * 64u);
+    }
+    double t[CLL];
+#pragma unroll
+    for (int c = 0; c < CLL; ++c) t[c] = ldc(L.ll_t + b0 + c);
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
@@ -184,10 +216,9 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
   // visited, in bond order; the unfilled slots of the last chunk read this
   // segment with weight 0.  Skipped bonds would add exact zeros, so the
   // sums are unchanged.
-  const uint32_t lane = threadIdx.x & 63u;
   for (int w0 = 0; w0 < L.n_hh; w0 += 64) {
     const int bl = w0 + (int)lane;
-    const uint32_t ml = bl < L.n_hh ? L.hh_mask[bl] : 0u;
+    const uint32_t ml = w0 == 0 ? ml0 : (bl < L.n_hh ? L.hh_mask[bl] : 0u);
     const uint32_t bits = h & ml;
     uint64_t act = __ballot(bits != 0u && bits != ml);
     while (act != 0u) {
@@ -220,7 +251,6 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
   // If the neighbouring high word has no segment no row can flip (its low
   // word would need a set bit more or fewer than it has), and the read
   // falls back to this segment.
-  const int n_xb = (L.n_xb + CX - 1) / CX * CX;
   for (int b0 = 0; b0 < n_xb; b0 += CX) {
     const V* xn[CX];
     bool hb[CX];
@@ -239,11 +269,16 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
-      for (int k = 0; k < CX / kXChunk; ++k) {
-        const u16x2 e2 = *reinterpret_cast<const u16x2*>(xnp + xrec[q] + (uint32_t)(b0 + k * kXChunk) * 64u);
+      for (int k = 0; k < KX; ++k)
 #pragma unroll
-        for (int c = 0; c < kXChunk; ++c) e[q][k * kXChunk + c] = e2[c];
-      }
+        for (int c = 0; c < kXChunk; ++c) e[q][k * kXChunk + c] = xr[q][k][c];
+    if (b0 + CX < n_xb) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int k = 0; k < KX; ++k)
+          xr[q][k] = *reinterpret_cast<const u16x2*>(xnp + xrec[q] + (uint32_t)(b0 + CX + k * kXChunk) * 64u);
+    }
     V v[NQ][CX];
     bool f[NQ][CX];
 #pragma unroll
@@ -280,6 +315,12 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
 // with DOT, *dot_slot receives the segment's sum of x_r (H x)_r (FP64, before
 // y is rounded to the storage type).
 //
+// H.v over the row segment of high word h (one workgroup of kBlock threads;
+// dynamic LDS of at least (L.max_seg + 1) elements of V):
+//   y[r] = sum_c H[r][c] x[c] for the segment's rows r;
+// with DOT, *dot_slot receives the segment's sum of x_r (H x)_r (FP64, before
+// y is rounded to the storage type).
+//
 // The segment's slice of x is staged in LDS.  Each thread owns up to kRows
 // rows (r = tid + q*kBlock); the number of row slots that hold rows is
 // uniform over each wave and selects a fixed-size instance of the bond
@@ -298,20 +339,19 @@ __device__ __forceinline__ void hv_segment(const DevLayout& L, uint32_t h, const
   const uint32_t len = ldc(L.lo_class_off + kl + 1) - cls0;
   const uint32_t base = ldc(L.hi_base + h);
   const double dh = ldc(L.diag_hi + h);
-  const V* xb = x + base;
-  for (uint32_t r = tid; r < len; r += kBlock) xs[r] = xb[r];
-  if (tid == 0) xs[len] = (V)0;
   const uint16_t* nbp = L.nbp16 + ldc(L.nbp_off + kl);
   const uint16_t* xnp = L.xn16 + ldc(L.xn_off + kl);
   const double* dlp = L.diag_lo_cls + cls0;
-  __syncthreads();
   // row slots of this wave: slot q holds rows wave*64 + q*kBlock + lane
   const uint32_t w0 = tid & ~63u;
   const uint32_t nq = __builtin_amdgcn_readfirstlane(len > w0 ? (len - w0 + kBlock - 1) / kBlock : 0u);
   static_assert(kRows == 4, "hv_segment dispatches one to four row slots");
+  // Every wave passes one workgroup barrier between the LDS stores of the
+  // x slice and the first LDS reads: inside hv_rows, or here for a wave
+  // without rows (the branch is uniform over each wave).
   double dotp = 0.0;
   if (nq == 0)
-    ;  // no rows in this wave
+    __syncthreads();
   else if (nq == 1)
     dotp = hv_rows<V, DOT, 1, CLL, CHH, CX>(L, h, x, y, xs, len, base, dh, nbp, xnp, dlp);
   else if (nq == 2)
