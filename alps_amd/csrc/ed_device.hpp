@@ -30,6 +30,28 @@ __device__ __forceinline__ double block_sum_all(double v) {
   return s;
 }
 
+// Two block sums through one pair of barriers (each equal to its
+// block_sum_all).
+__device__ __forceinline__ void block_sum_all2(double& a, double& b) {
+  __shared__ double red2_smem[2][kBlock / 64];
+  a = wave_sum(a);
+  b = wave_sum(b);
+  const int wave = threadIdx.x >> 6;
+  __syncthreads();  // red2_smem may still be read by a previous call
+  if ((threadIdx.x & 63) == 0) {
+    red2_smem[0][wave] = a;
+    red2_smem[1][wave] = b;
+  }
+  __syncthreads();
+  a = 0.0;
+  b = 0.0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) {
+    a += red2_smem[0][w];
+    b += red2_smem[1][w];
+  }
+}
+
 // A thread's share of a sum of partials: entries tid, tid + kBlock, ... in
 // index order.  The loads of eight terms are issued together before their
 // additions (a loop with one load per addition waits for each in turn).

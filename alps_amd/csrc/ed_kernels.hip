@@ -95,13 +95,10 @@ __global__ __launch_bounds__(kBlock) void update_kernel(uint64_t n, const V* w, 
 
 // One Lanczos update with its scalars: every block first forms, in the same
 // order, scale_j = 1/sqrt(sum(norm_prev)) (the norm of u_j), beta_{j-1} =
-// sqrt(sum(norm_prev)) when j > 0, and alpha_j = scale_j^2 * sum(hv_part);
-// block 0 stores them.  Then as update_kernel.  This replaces the two
-// one-block reduction launches of a step.  The vector elements of a
-// thread's first kPrefetch iterations are loaded before the partial sums,
-// so that their latency overlaps the reduction.
-constexpr int kPrefetch = 3;
-
+// sqrt(sum(norm_prev)) when j > 0, and alpha_j = scale_j^2 * sum(hv_part)
+// (both sums through one block reduction); block 0 stores them.  Then as
+// update_kernel.  This replaces the two one-block reduction launches of a
+// step.
 template <class V>
 __global__ __launch_bounds__(kBlock) void update_fused_kernel(uint64_t n, const V* w, const V* __restrict__ u_cur,
                                                               const V* u_prev, V* out, Partials hv_part,
@@ -110,34 +107,30 @@ __global__ __launch_bounds__(kBlock) void update_fused_kernel(uint64_t n, const 
                                                               double* __restrict__ scale_j,
                                                               const double* __restrict__ scale_jm1,
                                                               double* __restrict__ norm_out) {
-  // four consecutive elements per thread and iteration (vector loads)
-  typedef V v4 __attribute__((ext_vector_type(4), aligned(sizeof(V))));
   const bool has_prev = beta_jm1 != nullptr;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t n4 = n / 4;
-  v4 wp[kPrefetch], up[kPrefetch], pp[kPrefetch];
-#pragma unroll
-  for (int k = 0; k < kPrefetch; ++k) {
-    const uint64_t q = gid + k * stride;
-    if (q < n4) {
-      wp[k] = *reinterpret_cast<const v4*>(w + 4 * q);
-      up[k] = *reinterpret_cast<const v4*>(u_cur + 4 * q);
-      pp[k] = has_prev ? *reinterpret_cast<const v4*>(u_prev + 4 * q) : up[k];
-    }
-  }
-  const double nrm = sqrt(sum_partials(norm_prev));
+  const double s_prev = has_prev ? *scale_jm1 : 0.0;
+  double nsum = thread_partials(norm_prev), hsum = thread_partials(hv_part);
+  block_sum_all2(nsum, hsum);
+  const double nrm = sqrt(nsum);
   const double sc = 1.0 / nrm;
-  const double a = sc * sc * sum_partials(hv_part);
+  const double a = sc * sc * hsum;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     *alpha_j = a;
     *scale_j = sc;
     if (beta_jm1) *beta_jm1 = nrm;
   }
   const double ca = a * sc;
-  const double cb = has_prev ? nrm * (*scale_jm1) : 0.0;
+  const double cb = has_prev ? nrm * s_prev : 0.0;
   double acc = 0.0;
-  auto step = [&](uint64_t q, const v4& wv, const v4& uv, const v4& pv) {
+  // four consecutive elements per thread and iteration (vector loads)
+  typedef V v4 __attribute__((ext_vector_type(4), aligned(sizeof(V))));
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t n4 = n / 4;
+  for (uint64_t q = gid; q < n4; q += stride) {
+    const v4 wv = *reinterpret_cast<const v4*>(w + 4 * q);
+    const v4 uv = *reinterpret_cast<const v4*>(u_cur + 4 * q);
+    const v4 pv = has_prev ? *reinterpret_cast<const v4*>(u_prev + 4 * q) : uv;
     v4 o;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -148,17 +141,6 @@ __global__ __launch_bounds__(kBlock) void update_fused_kernel(uint64_t n, const 
       acc += rd * rd;
     }
     *reinterpret_cast<v4*>(out + 4 * q) = o;
-  };
-#pragma unroll
-  for (int k = 0; k < kPrefetch; ++k) {
-    const uint64_t q = gid + k * stride;
-    if (q < n4) step(q, wp[k], up[k], pp[k]);
-  }
-  for (uint64_t q = gid + kPrefetch * stride; q < n4; q += stride) {
-    const v4 wv = *reinterpret_cast<const v4*>(w + 4 * q);
-    const v4 uv = *reinterpret_cast<const v4*>(u_cur + 4 * q);
-    const v4 pv = has_prev ? *reinterpret_cast<const v4*>(u_prev + 4 * q) : uv;
-    step(q, wv, uv, pv);
   }
   for (uint64_t i = 4 * n4 + gid; i < n; i += stride) {
     double r = sc * (double)w[i] - ca * (double)u_cur[i];
