@@ -137,6 +137,13 @@ __device__ __forceinline__ T ldc_struct(const T* p) {
   return out;
 }
 
+// Element of p at the offset a flip table stores: 4 * (row index), the byte
+// offset of an FP32 element (twice that for FP64).
+template <class V>
+__device__ __forceinline__ V elem_at(const V* p, uint32_t off4) {
+  return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(p) + off4 * (uint32_t)(sizeof(V) / 4));
+}
+
 // The rows of hv_segment for NQ row slots per thread (rows tid,
 // tid + kBlock, ...), NQ fixed at compile time so that the loads of all
 // slots are issued together; rows past the segment are clamped to row 0 and
@@ -230,7 +237,7 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
-      for (int c = 0; c < CLL; ++c) off[q] += t[c] * (double)xs[j[q][c]];
+      for (int c = 0; c < CLL; ++c) off[q] += t[c] * (double)elem_at(xs, j[q][c]);
   }
   // high-high flips: a bond flips every row of the segment or none (its two
   // bits lie in h), with a contiguous target run.  The bonds that flip are
@@ -308,7 +315,7 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
 #pragma unroll
       for (int c = 0; c < CX; ++c) {
         f[q][c] = ((e[q][c] >> 15) != 0u) != hb[c];
-        v[q][c] = xn[c][f[q][c] ? (e[q][c] & 0x7fffu) : 0u];
+        v[q][c] = elem_at(xn[c], f[q][c] ? (e[q][c] & 0x7fffu) : 0u);
       }
 #pragma unroll
     for (int q = 0; q < NQ; ++q)

@@ -131,12 +131,14 @@ struct SectorLayout {
   uint32_t max_seg = 0;                // longest segment
   // H.v tables.  Bond lists are padded with null bonds (coefficient 0,
   // never active) to multiples of the kernel's chunk sizes.
-  //   nbp16[nbp_off[c] + flip_index(r, b, n_ll_p, kLLChunk)]: row reached
-  //     from row r of a class-c segment by low-low bond b, or C(B,c) (the
-  //     zero slot) if not allowed;
-  //   xn16[xn_off[c] + flip_index(r, b, n_xb_p, kXChunk)]: rank of
+  //   nbp16[nbp_off[c] + flip_index(r, b, n_ll_p, kLLChunk)]: 4 * the row
+  //     reached from row r of a class-c segment by low-low bond b, or
+  //     4 * C(B,c) (the zero slot) if not allowed;
+  //   xn16[xn_off[c] + flip_index(r, b, n_xb_p, kXChunk)]: 4 * the rank of
   //     lo ^ lbit_b in its class, with bit 15 = the low bit of cross bond b
   //     in row r.
+  // Rows are stored times 4, the byte offset of an FP32 element, which the
+  // kernel adds to its base without a shift (C(12,6) * 4 < 2^15).
   int n_ll_p = 0, n_hh_p = 0, n_xb_p = 0;
   std::vector<uint16_t> nbp16, xn16;
   std::vector<uint32_t> nbp_off, xn_off;  // [B+2]
@@ -234,6 +236,7 @@ inline std::shared_ptr<const FlipTables> flip_tables(int B, const std::vector<ui
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
   }
+  if (4 * binom(B, B / 2) >= 32768) throw std::invalid_argument("low word too wide for the flip tables");
   const LowTables& T = low_tables(B);
   std::shared_ptr<FlipTables> F(new FlipTables);
   F->nbp_off.assign(B + 2, 0);
@@ -253,7 +256,7 @@ inline std::shared_ptr<const FlipTables> flip_tables(int B, const std::vector<ui
       const uint32_t m = b < (int)ll_masks.size() ? ll_masks[b] : 0u;
       for (uint32_t r = 0; r < len; ++r) {
         const uint32_t lo = ls[r], bits = lo & m;
-        nb[flip_index(r, b, n_ll_p, kLLChunk)] = (uint16_t)((bits != 0u && bits != m) ? T.lo_rank[lo ^ m] : len);
+        nb[flip_index(r, b, n_ll_p, kLLChunk)] = (uint16_t)(4u * ((bits != 0u && bits != m) ? T.lo_rank[lo ^ m] : len));
       }
     }
     uint16_t* xn = F->xn16.data() + F->xn_off[c];
@@ -261,7 +264,8 @@ inline std::shared_ptr<const FlipTables> flip_tables(int B, const std::vector<ui
       const uint32_t lb = x_lbits[b];
       for (uint32_t r = 0; r < len; ++r) {
         const uint32_t lo = ls[r];
-        xn[flip_index(r, b, n_xb_p, kXChunk)] = (uint16_t)(T.lo_rank[lo ^ lb] | (((lo & lb) != 0u ? 1u : 0u) << 15));
+        xn[flip_index(r, b, n_xb_p, kXChunk)] =
+            (uint16_t)(4u * T.lo_rank[lo ^ lb] | (((lo & lb) != 0u ? 1u : 0u) << 15));
       }
     }
   }

@@ -5,9 +5,9 @@
 //    normalizations) against the oracle's restatement of
 //    blochbasisstates.h for several lattices, sectors and momenta;
 //  - the flip tables read with the H.v kernel's addressing (row record
-//    (r/64)*64*n_pad + (r%64)*chunk, bond chunk k at k*64*chunk) against the
-//    flipped words' ranks, for every class, row and bond of random models,
-//    and every such read inside its class.
+//    (r/64)*64*n_pad + (r%64)*chunk, bond chunk k at k*64*chunk) against 4 *
+//    the flipped words' ranks, for every class, row and bond of random
+//    models, and every such read inside its class.
 // Prints "diag_err <max abs error>", "bloch_sectors <count>" and
 // "flip_bad <mismatches> flip_checked <entries>", or "bloch_mismatch ..."
 // and exits 1.
@@ -135,14 +135,14 @@ static void flip_check(long& bad, long& checked) {
         for (int b = 0; b < L.n_ll_p; ++b) {
           const size_t at = lrec + (size_t)(b / kLLChunk * kLLChunk) * 64u + b % kLLChunk;
           const uint32_t m = b < (int)L.ll.size() ? L.ll[b].mask : 0u, bits = lo & m;
-          const uint32_t want = (bits != 0u && bits != m) ? L.lo_rank[lo ^ m] : len;
+          const uint32_t want = 4u * ((bits != 0u && bits != m) ? L.lo_rank[lo ^ m] : len);
           ++checked;
           if (at >= nll || L.nbp16[L.nbp_off[c] + at] != want) ++bad;
         }
         for (int b = 0; b < (int)L.xb.size(); ++b) {
           const size_t at = xrec + (size_t)(b / kXChunk * kXChunk) * 64u + b % kXChunk;
           const uint32_t lb = L.xb[b].lbit;
-          const uint32_t want = L.lo_rank[lo ^ lb] | ((lo & lb) != 0u ? 0x8000u : 0u);
+          const uint32_t want = 4u * L.lo_rank[lo ^ lb] | ((lo & lb) != 0u ? 0x8000u : 0u);
           ++checked;
           if (at >= nx || L.xn16[L.xn_off[c] + at] != want) ++bad;
         }
