@@ -24,11 +24,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of hv_kernel<float,false> on this workload (L=24),
-# from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE scaled by a copy kernel of
-# known size (profiles/hv_kernel_heisenberg_L24.md); valid for that kernel
-# version and L only.
-PMC_TRAFFIC_BYTES = {24: 51.94e6}
+# HBM bytes per launch of hv_kernel<float,false> on this workload (L=24), per
+# schedule (the order of SCHEDULES below), from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE scaled by a copy kernel of known size
+# (profiles/hv_kernel_heisenberg_L24.md); valid for that kernel version and L
+# only.  The schedules that spill write their scratch too.
+PMC_TRAFFIC_BYTES = {24: [56.46e6, 71.22e6, 51.91e6, 51.82e6, 108.25e6, 74.78e6]}
 # H.v schedules (waves per SIMD, low-low / high-high / cross chunk sizes), in
 # the order of the engine's schedule table
 SCHEDULES = [(8, 4, 4, 2), (8, 4, 8, 2), (6, 4, 8, 2), (6, 4, 4, 2), (8, 8, 4, 2), (6, 8, 8, 4)]
@@ -354,7 +355,7 @@ def main():
         "config": {"workload": f"Heisenberg ring L={L}, J=1, Sz=0, D={D}, matrix-free H.v, FP32 vectors",
                    "L": L, "dim": D, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": PMC_TRAFFIC_BYTES.get(L),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": PMC_TRAFFIC_BYTES[L][sched] if L in PMC_TRAFFIC_BYTES else None,
                      "kernel": "hv_kernel<float,false,%d,%d,%d,%d>" % SCHEDULES[sched],
                      "bytes_per_launch": bytes_per_hv},
         "timed_wall_s": round(wall, 4),
