@@ -85,12 +85,14 @@ def lib():
             "alps_ed_batch_create": ([P, P], i32),
             "alps_ed_batch_destroy": ([P], i32),
             "alps_ed_batch_submit": ([P, P, P, P], i32),
+            "alps_ed_batch_submit_momentum": ([P, P, i32, P, P, P, P, P], i32),
             "alps_ed_batch_progress": ([P, P, P], i32),
             "alps_ed_batch_run": ([P], i32),
             "alps_ed_batch_completed": ([P, P, i32, P], i32),
             "alps_ed_batch_result": ([P, u64, P, P], i32),
             "alps_ed_batch_stats": ([P, P], i32),
             "alps_ed_batch_footprint": ([P, P, i32, P], i32),
+            "alps_ed_batch_footprint_momentum": ([P, i32, P, P, P, P, i32, P], i32),
             "alps_ed_describe_parameter_file": ([ctypes.c_char_p, P, ctypes.c_size_t, P], i32),
         }
         for name, (args, res) in sig.items():
@@ -487,11 +489,18 @@ class Batch:
         except Exception:
             pass
 
-    def submit(self, model: SpinModel, max_iter=0, nev=1, seed=0, fixed=False):
+    def submit(self, model: SpinModel, max_iter=0, nev=1, seed=0, fixed=False, translations=None):
+        """Queues one solve (in the momentum sector of translations=(phases,
+        site maps) when given); returns its ticket."""
         m = model.cstruct()
         p = _SolveParams(max_iter, nev, int(fixed), seed, F64)
         t = ctypes.c_uint64()
-        _check(lib().alps_ed_batch_submit(self.handle, ctypes.byref(m), ctypes.byref(p), ctypes.byref(t)))
+        if translations is None:
+            _check(lib().alps_ed_batch_submit(self.handle, ctypes.byref(m), ctypes.byref(p), ctypes.byref(t)))
+        else:
+            nt, perms, re, im = _translation_args(translations, model.n_sites)
+            _check(lib().alps_ed_batch_submit_momentum(self.handle, ctypes.byref(m), nt, _p(perms), _p(re), _p(im),
+                                                       ctypes.byref(p), ctypes.byref(t)))
         return t.value
 
     def progress(self):
@@ -520,10 +529,16 @@ class Batch:
         return {k: getattr(s, k) for k, _ in _BatchStatus._fields_}
 
 
-def batch_footprint(model: SpinModel, dtype="f32", max_iter=0, nev=1):
-    """HBM bytes a batched request would hold while admitted."""
+def batch_footprint(model: SpinModel, dtype="f32", max_iter=0, nev=1, translations=None):
+    """HBM bytes a batched request would hold while admitted (in the
+    momentum sector of translations when given)."""
     m = model.cstruct()
     p = _SolveParams(max_iter, nev, 0, 0, F64)
     b = ctypes.c_uint64()
-    _check(lib().alps_ed_batch_footprint(ctypes.byref(m), ctypes.byref(p), _DTYPES[dtype], ctypes.byref(b)))
+    if translations is None:
+        _check(lib().alps_ed_batch_footprint(ctypes.byref(m), ctypes.byref(p), _DTYPES[dtype], ctypes.byref(b)))
+    else:
+        nt, perms, re, im = _translation_args(translations, model.n_sites)
+        _check(lib().alps_ed_batch_footprint_momentum(ctypes.byref(m), nt, _p(perms), _p(re), _p(im),
+                                                      ctypes.byref(p), _DTYPES[dtype], ctypes.byref(b)))
     return b.value

@@ -35,6 +35,7 @@
 
 #include "alps_ed.h"
 #include "ed_batch.hpp"
+#include "ed_bloch.hpp"
 #include "ed_host.hpp"
 #include "ed_kernels.hpp"
 
@@ -184,7 +185,11 @@ struct Problem {
   uint64_t ticket = 0;
   ModelSpec spec;
   SectorLayout layout;
-  uint64_t D = 0;
+  std::shared_ptr<const Translations> trans;  // set for a momentum sector
+  BlochSector bloch;                          // its basis (after sizing)
+  uint64_t D = 0;                             // dimension (representatives in a momentum sector)
+  uint64_t n = 0;                             // Lanczos vector length: D, or 2 D realified
+  uint32_t n_hv = 0;                          // H.v work items per step
   bool fp32 = false;  // storage lane
   int max_iter = 0, nev = 1;
   bool fixed = false;
@@ -296,6 +301,15 @@ size_t table_bytes(const SectorLayout& L) {
 }
 
 
+// Bytes of a momentum sector's basis arrays (none for a fixed-Sz sector).
+size_t bloch_bytes(const BlochSector& b) {
+  if (b.rep_state.empty()) return 0;
+  return align_up(b.full_rep.size() * 4) + align_up(b.full_tr.size()) + align_up(b.rep_state.size() * 4) +
+         align_up(b.norm.size() * 8) + 2 * align_up(b.ph_re.size() * 8);
+}
+
+size_t problem_table_bytes(const Problem& p) { return table_bytes(p.layout) + bloch_bytes(p.bloch); }
+
 // Lays the tables out at [base, base + table_bytes) of the device, writing
 // their bytes to the same offsets of the host staging area `stage`, and
 // returns the device layout; one copy of the staging area uploads them.
@@ -371,18 +385,18 @@ struct Footprint {
 Footprint footprint(const Problem& p) {
   Footprint f;
   const size_t es = p.fp32 ? 4 : 8;
-  const uint64_t stride = (p.D + 63) & ~uint64_t(63);
-  f.dev += table_bytes(p.layout);
+  const uint64_t stride = (p.n + 63) & ~uint64_t(63);
+  f.dev += problem_table_bytes(p);
   f.dev += align_up((size_t)p.cap * stride * es);
   f.dev += 3 * align_up((size_t)(p.max_iter + 2) * 8);  // alpha, beta, scale
-  f.dev += align_up((size_t)std::max<uint32_t>(p.layout.seg_hi.size(), 1) * 8);
+  f.dev += align_up((size_t)std::max<uint32_t>(p.n_hv, 1) * 8);
   f.dev += align_up((size_t)p.n_chunks * 8);
   if (p.fp32) {
     f.dev += align_up((size_t)p.nvr * (p.max_iter + 1) * 8);  // coef
-    f.dev += align_up((size_t)p.nvr * p.D * 8);              // Ritz vectors
-    f.dev += align_up((size_t)p.D * 8);                      // H y
+    f.dev += align_up((size_t)p.nvr * p.n * 8);              // Ritz vectors
+    f.dev += align_up((size_t)p.n * 8);                      // H y
     f.host += align_up((size_t)p.nvr * p.n_chunks * 8);
-    f.host += align_up((size_t)std::max<uint32_t>(p.layout.seg_hi.size(), 1) * 8);
+    f.host += align_up((size_t)std::max<uint32_t>(p.n_hv, 1) * 8);
     f.host += align_up((size_t)p.nvr * (p.max_iter + 1) * 8);
   }
   f.host += 2 * align_up((size_t)(p.max_iter + 2) * 8);  // alpha, beta mirrors
@@ -401,9 +415,28 @@ void place(alps_ed_batch b, Problem& p, char* stage, const SharedTables& sh) {
   BatchJob& J = p.job;
   J = BatchJob{};
   J.L = place_tables(p.layout, base, stage, sh);
-  char* q = base + table_bytes(p.layout);
+  if (!p.bloch.rep_state.empty()) {  // the momentum sector's basis after the layout tables
+    size_t o = table_bytes(p.layout);
+    auto put = [&](const auto& v) {
+      using T = typename std::decay_t<decltype(v)>::value_type;
+      std::memcpy(stage + o, v.data(), v.size() * sizeof(T));
+      const T* dst = reinterpret_cast<const T*>(base + o);
+      o += align_up(v.size() * sizeof(T));
+      return dst;
+    };
+    J.Bl.dk = (uint32_t)p.bloch.rep_state.size();
+    J.Bl.full_rep = put(p.bloch.full_rep);
+    J.Bl.full_tr = put(p.bloch.full_tr);
+    J.Bl.rep_state = put(p.bloch.rep_state);
+    J.Bl.norm = put(p.bloch.norm);
+    J.Bl.ph_re = put(p.bloch.ph_re);
+    J.Bl.ph_im = put(p.bloch.ph_im);
+  }
+  J.n = (uint32_t)p.n;
+  J.n_hv = p.n_hv;
+  char* q = base + problem_table_bytes(p);
   const size_t es = p.fp32 ? 4 : 8;
-  const uint64_t stride = (p.D + 63) & ~uint64_t(63);
+  const uint64_t stride = (p.n + 63) & ~uint64_t(63);
   J.vec = q;
   J.stride = stride;
   J.cap = p.cap;
@@ -418,7 +451,7 @@ void place(alps_ed_batch b, Problem& p, char* stage, const SharedTables& sh) {
   J.scale = reinterpret_cast<double*>(q + 2 * sc);
   q += 3 * sc;
   J.part_hv = reinterpret_cast<double*>(q);
-  q += align_up((size_t)std::max<uint32_t>(p.layout.seg_hi.size(), 1) * 8);
+  q += align_up((size_t)std::max<uint32_t>(p.n_hv, 1) * 8);
   J.part_up = reinterpret_cast<double*>(q);
   q += align_up((size_t)p.n_chunks * 8);
   char* h = b->host_base + p.host_off;
@@ -432,13 +465,13 @@ void place(alps_ed_batch b, Problem& p, char* stage, const SharedTables& sh) {
     p.coef = reinterpret_cast<double*>(q);
     q += align_up((size_t)p.nvr * (p.max_iter + 1) * 8);
     p.ritz = reinterpret_cast<double*>(q);
-    q += align_up((size_t)p.nvr * p.D * 8);
+    q += align_up((size_t)p.nvr * p.n * 8);
     p.hy = reinterpret_cast<double*>(q);
-    q += align_up((size_t)p.D * 8);
+    q += align_up((size_t)p.n * 8);
     p.ritz_part = reinterpret_cast<double*>(h + ho);
     ho += align_up((size_t)p.nvr * p.n_chunks * 8);
     p.hv_part = reinterpret_cast<double*>(h + ho);
-    ho += align_up((size_t)std::max<uint32_t>(p.layout.seg_hi.size(), 1) * 8);
+    ho += align_up((size_t)std::max<uint32_t>(p.n_hv, 1) * 8);
     p.coef_host = reinterpret_cast<double*>(h + ho);
     ho += align_up((size_t)p.nvr * (p.max_iter + 1) * 8);
   }
@@ -550,9 +583,9 @@ void rebuild_lists(const std::vector<Problem*>& run, std::vector<BatchItem>& hv3
     const Problem& p = *run[i];
     auto& hv = p.fp32 ? hv32 : hv64;
     auto& up = p.fp32 ? up32 : up64;
-    for (uint32_t s = 0; s < p.layout.seg_hi.size(); ++s) hv.push_back({i, s});
+    for (uint32_t s = 0; s < p.n_hv; ++s) hv.push_back({i, s});
     for (uint32_t c = 0; c < p.n_chunks; ++c) up.push_back({i, c});
-    (p.fp32 ? seg32 : seg64) = std::max(p.fp32 ? seg32 : seg64, p.layout.max_seg);
+    if (!p.trans) (p.fp32 ? seg32 : seg64) = std::max(p.fp32 ? seg32 : seg64, p.layout.max_seg);
   }
 }
 
@@ -634,16 +667,29 @@ Group upload_group(alps_ed_batch b, const std::vector<Problem*>& run) {
 }
 
 // Per-problem sizes that follow from its layout.
-void size_problem(alps_ed_batch b, Problem& p) {
+void size_problem(Problem& p, alps_ed_dtype dtype) {
   p.layout = build_layout(p.spec, -1, false);  // shared tables stay in L.low / L.flip
-  p.D = p.layout.D;
-  p.fp32 = use_fp32(b->cfg.dtype == ALPS_ED_F32, p.D, p.nev);
+  if (p.trans) {
+    // the Bloch basis at this momentum; the orbits are shared by the
+    // momenta of the same Sz sector
+    if (p.layout.D > 0) p.bloch = bloch_sector(*shared_bloch_orbits(p.layout, *p.trans), *p.trans, p.layout.N);
+    p.D = p.bloch.rep_state.size();
+    p.n = 2 * p.D;
+    p.n_hv = (uint32_t)std::max<uint64_t>(1, (p.D + kBlock - 1) / kBlock);
+  } else {
+    p.D = p.layout.D;
+    p.n = p.D;
+    p.n_hv = (uint32_t)std::max<size_t>(1, p.layout.seg_hi.size());
+  }
+  p.fp32 = use_fp32(dtype == ALPS_ED_F32, p.D, p.nev);
   if (p.max_iter <= 0) p.max_iter = (int)std::min<uint64_t>(10 * std::max<uint64_t>(p.D, 1), 1000);
-  p.n_chunks = (uint32_t)std::max<uint64_t>(1, (p.D + kBatchChunk - 1) / kBatchChunk);
+  p.n_chunks = (uint32_t)std::max<uint64_t>(1, (p.n + kBatchChunk - 1) / kBatchChunk);
   p.cap = p.fp32 ? (uint32_t)(p.max_iter + 2) : 3u;
   p.nvr = p.fp32 ? std::min(p.nev, kMaxRitz) : 0;
   p.sized = true;
 }
+
+void size_problem(alps_ed_batch b, Problem& p) { size_problem(p, b->cfg.dtype); }
 
 // The device copy of a layout's shared tables, uploaded on first use.
 const SharedTables& shared_tables(alps_ed_batch b, const SectorLayout& L) {
@@ -742,7 +788,7 @@ void admit(alps_ed_batch b, std::vector<Problem*>& started) {
   std::vector<size_t> at;
   for (Problem* p : placed) {
     at.push_back(total);
-    total += align_up(table_bytes(p->layout));
+    total += align_up(problem_table_bytes(*p));
   }
   ensure_tstage(b, total);
   char* st = static_cast<char*>(b->tstage);
@@ -751,7 +797,7 @@ void admit(alps_ed_batch b, std::vector<Problem*>& started) {
   b->pool->parallel_for(placed.size(), [&](size_t i) { place(b, *placed[i], st + at[i], *sh[i]); });
   for (size_t i = 0; i < placed.size(); ++i)
     hip_check(hipMemcpyAsync(static_cast<char*>(b->dev_base) + placed[i]->dev_off, st + at[i],
-                             table_bytes(placed[i]->layout), hipMemcpyHostToDevice, b->stream),
+                             problem_table_bytes(*placed[i]), hipMemcpyHostToDevice, b->stream),
               "upload tables");
 }
 
@@ -799,7 +845,7 @@ void refine(alps_ed_batch b) {
     RitzJob& r = rj[i];
     r.vec = p.job.vec;
     r.stride = p.job.stride;
-    r.D = (uint32_t)p.D;
+    r.D = (uint32_t)p.n;
     r.n_chunks = p.n_chunks;
     r.m = m;
     r.nv = (int)p.cand.size();
@@ -827,14 +873,15 @@ void refine(alps_ed_batch b) {
       if (v >= p->cand.size()) continue;
       PlainHvJob j;
       j.L = p->job.L;
-      j.x = p->ritz + v * p->D;
+      j.Bl = p->job.Bl;
+      j.x = p->ritz + v * p->n;
       j.y = p->hy;
       j.part = mapped_dev(b, p->hv_part);
       const uint32_t ji = (uint32_t)hj.size();
       hj.push_back(j);
       who.push_back(p);
-      for (uint32_t s = 0; s < p->layout.seg_hi.size(); ++s) hi.push_back({ji, s});
-      max_seg = std::max(max_seg, p->layout.max_seg);
+      for (uint32_t s = 0; s < p->n_hv; ++s) hi.push_back({ji, s});
+      if (!p->trans) max_seg = std::max(max_seg, p->layout.max_seg);
     }
     DevBuf hd(align_up(hj.size() * sizeof(PlainHvJob)) + align_up(hi.size() * sizeof(BatchItem)));
     hip_check(hipMemcpyAsync(hd.p, hj.data(), hj.size() * sizeof(PlainHvJob), hipMemcpyHostToDevice, b->stream),
@@ -847,7 +894,7 @@ void refine(alps_ed_batch b) {
     for (size_t k = 0; k < who.size(); ++k) {
       Problem& p = *who[k];
       double s = 0.0;
-      for (size_t i = 0; i < p.layout.seg_hi.size(); ++i) s += p.hv_part[i];
+      for (size_t i = 0; i < p.n_hv; ++i) s += p.hv_part[i];
       const size_t wi = std::find(work.begin(), work.end(), &p) - work.begin();
       yhy[wi].push_back(s);
     }
@@ -870,15 +917,17 @@ void refine(alps_ed_batch b) {
 }
 
 // D = 0 and D = 1 sectors need no Lanczos (sparsediag.h:101-102,120-124).
+// A momentum sector's host basis is on the device by now and is released.
 void settle_trivial(const std::vector<Problem*>& started) {
   for (Problem* p : started) {
     if (p->D == 0) {
       p->state = kDone;
     } else if (p->D == 1) {
-      p->ev.assign(1, single_state_energy(p->spec));
+      p->ev.assign(1, p->trans ? bloch_single_element(p->layout, p->bloch) : single_state_energy(p->spec));
       p->final_n = 0;
       p->state = kDone;
     }
+    if (p->trans) p->bloch = BlochSector();
   }
 }
 
@@ -996,10 +1045,27 @@ void round(alps_ed_batch b) {
   hip_check(hipStreamSynchronize(b->stream), "sync");  // staging buffers are reused next round
 }
 
-uint64_t submit_one(alps_ed_batch b, const alps_ed_model* m, const alps_ed_solve_params* prm) {
+// Translations of a momentum request, every one a permutation of the sites.
+std::shared_ptr<const Translations> checked_translations(const alps_ed_model* model, int n_t, const int* perm,
+                                                         const double* re, const double* im) {
+  require(model->n_sites >= 1 && model->n_sites <= 32, "momentum sectors support 1 to 32 sites");
+  auto t = std::make_shared<Translations>(read_translations(model->n_sites, n_t, perm, re, im));
+  for (const auto& q : t->perm) {
+    std::vector<int> seen(model->n_sites, 0);
+    for (int s : q) {
+      require(s >= 0 && s < model->n_sites && !seen[s], "translation is not a permutation of the sites");
+      seen[s] = 1;
+    }
+  }
+  return t;
+}
+
+uint64_t submit_one(alps_ed_batch b, const alps_ed_model* m, const alps_ed_solve_params* prm,
+                    std::shared_ptr<const Translations> trans = nullptr) {
   require(prm != nullptr, "params is null");
   std::unique_ptr<Problem> p(new Problem);
   p->spec = model_spec(m);
+  p->trans = std::move(trans);
   if (p->spec.n_up >= 0 && p->spec.n_up <= p->spec.n_sites)
     require(binom(p->spec.n_sites, p->spec.n_up) < 0xffffffffULL, "sector dimension exceeds 2^32-1");
   p->max_iter = prm->max_iter > 0 ? prm->max_iter : 0;  // default set when sized
@@ -1060,6 +1126,15 @@ int alps_ed_batch_submit(alps_ed_batch b, const alps_ed_model* model, const alps
     require(b != nullptr, "batch is null");
     const uint64_t t = submit_one(b, model, params);
     if (ticket) *ticket = t;
+  });
+}
+
+int alps_ed_batch_submit_momentum(alps_ed_batch b, const alps_ed_model* model, int n_translations, const int* perm,
+                                  const double* phase_re, const double* phase_im, const alps_ed_solve_params* params,
+                                  uint64_t* ticket) {
+  return guarded([&] {
+    require(b != nullptr && model != nullptr && ticket != nullptr, "null argument");
+    *ticket = submit_one(b, model, params, checked_translations(model, n_translations, perm, phase_re, phase_im));
   });
 }
 
@@ -1129,14 +1204,24 @@ int alps_ed_batch_footprint(const alps_ed_model* model, const alps_ed_solve_para
     require(params && bytes, "null argument");
     Problem p;
     p.spec = model_spec(model);
-    p.layout = build_layout(p.spec, -1, false);  // shared tables stay in L.low / L.flip
-    p.D = p.layout.D;
     p.nev = params->nev > 0 ? params->nev : 1;
-    p.fp32 = use_fp32(dtype == ALPS_ED_F32, p.D, p.nev);
-    p.max_iter = params->max_iter > 0 ? params->max_iter : (int)std::min<uint64_t>(10 * std::max<uint64_t>(p.D, 1), 1000);
-    p.n_chunks = (uint32_t)std::max<uint64_t>(1, (p.D + kBatchChunk - 1) / kBatchChunk);
-    p.cap = p.fp32 ? (uint32_t)(p.max_iter + 2) : 3u;
-    p.nvr = p.fp32 ? std::min(p.nev, kMaxRitz) : 0;
+    p.max_iter = params->max_iter > 0 ? params->max_iter : 0;
+    size_problem(p, dtype);
+    *bytes = footprint(p).dev;
+  });
+}
+
+int alps_ed_batch_footprint_momentum(const alps_ed_model* model, int n_translations, const int* perm,
+                                     const double* phase_re, const double* phase_im,
+                                     const alps_ed_solve_params* params, alps_ed_dtype dtype, uint64_t* bytes) {
+  return guarded([&] {
+    require(model && params && bytes, "null argument");
+    Problem p;
+    p.spec = model_spec(model);
+    p.trans = checked_translations(model, n_translations, perm, phase_re, phase_im);
+    p.nev = params->nev > 0 ? params->nev : 1;
+    p.max_iter = params->max_iter > 0 ? params->max_iter : 0;
+    size_problem(p, dtype);
     *bytes = footprint(p).dev;
   });
 }

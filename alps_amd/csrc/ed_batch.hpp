@@ -1,9 +1,11 @@
 // ed_batch.hpp -- many independent Lanczos solves advanced together.
 //
-// Each problem (one fixed-Sz sector of its own Hamiltonian) is a BatchJob in
-// device memory.  One launch of each batched kernel serves every problem in
-// the batch: its grid is a flat list of work items (problem, segment) for
-// H.v and (problem, row chunk) for the vector kernels, so a batch of small
+// Each problem (one fixed-Sz sector of its own Hamiltonian, or one momentum
+// sector of it) is a BatchJob in device memory.  One launch of each batched
+// kernel serves every problem in the batch: its grid is a flat list of work
+// items (problem, segment) for H.v -- (problem, block of kBlock
+// representatives) in a momentum sector -- and (problem, row chunk) for the
+// vector kernels, so a batch of small
 // sectors fills the whole chip in one launch instead of leaving it idle
 // between many tiny ones.  A Lanczos step of the batch is four launches
 // (H.v with the alpha partials, alpha, update with the beta partials,
@@ -29,6 +31,9 @@ namespace alps_ed {
 // `stride` elements; with cap > jmax the whole basis is kept.
 struct BatchJob {
   DevLayout L;
+  DevBloch Bl;          // momentum sector (Bl.dk = 0: the whole fixed-Sz sector)
+  uint32_t n;           // Lanczos vector length: L.D, or 2 * Bl.dk (realified)
+  uint32_t n_hv;        // H.v work items and partials: segments or representative blocks
   void* vec;            // slot 0 (float or double, as the batch lane)
   uint64_t stride;      // elements between slots
   uint32_t cap;         // slots in the ring
@@ -41,7 +46,7 @@ struct BatchJob {
   double* scale;        // [jmax + 2]
   double* host_alpha;   // device view of mapped host mirrors
   double* host_beta;
-  double* part_hv;      // [L.n_seg]   partials of the H.v dot
+  double* part_hv;      // [n_hv]      partials of the H.v dot
   double* part_up;      // [n_chunks]  partials of |r|^2
 };
 
@@ -50,7 +55,7 @@ struct BatchJob {
 struct RitzJob {
   const void* vec;      // slot 0 of the kept basis
   uint64_t stride;
-  uint32_t D;
+  uint32_t D;           // vector length
   uint32_t n_chunks;
   int32_t m, nv;
   const double* scale;
@@ -59,17 +64,19 @@ struct RitzJob {
   double* part;         // [nv * n_chunks]  partials of |out_i|^2 (mapped host)
 };
 
-// y = H x over FP64 vectors, with per-segment partials of x.Hx.
+// y = H x over FP64 vectors, with per-item partials of x.Hx (items as the
+// H.v of BatchJob).
 struct PlainHvJob {
   DevLayout L;
+  DevBloch Bl;
   const double* x;
   double* y;
-  double* part;         // [L.n_seg] (mapped host)
+  double* part;         // [n_hv] (mapped host)
 };
 
 struct BatchItem {
   uint32_t job;
-  uint32_t idx;         // segment (processing order) or row chunk
+  uint32_t idx;         // segment (processing order), representative block or row chunk
 };
 
 constexpr uint32_t kBatchChunk = (uint32_t)kBlock * 4;  // rows per vector work item

@@ -23,7 +23,7 @@ __global__ __launch_bounds__(kBlock) void batch_start_kernel(const BatchJob* __r
                                                              const BatchItem* __restrict__ items) {
   const BatchItem it = ldc_struct(items + blockIdx.x);
   const BatchJob* J = jobs + it.job;
-  const uint32_t D = ldc(&J->L.D);
+  const uint32_t D = ldc(&J->n);
   const uint64_t seed = ldc(&J->seed);
   V* u = static_cast<V*>(ldc(&J->vec));
   double* part = ldc(&J->part_up);
@@ -53,6 +53,13 @@ __global__ __launch_bounds__(kBlock, HvWaves<V>::value) void batch_hv_kernel(con
   const V* x = slot_ptr<V>(vec, stride, cap, j);
   V* y = slot_ptr<V>(vec, stride, cap, j + 1);
   double* part = ldc(&J->part_hv);
+  if (ldc(&J->Bl.dk) != 0u) {  // momentum sector: uniform over the workgroup
+    const DevBloch Bl = ldc_struct(&J->Bl);
+    const double p = bloch_row<V, true>(L, Bl, x, y, it.idx * kBlock + threadIdx.x);
+    const double s = block_sum_all(p);
+    if (threadIdx.x == 0) part[it.idx] = s;
+    return;
+  }
   const uint32_t h = ldc(L.proc_hi + it.idx);
   hv_segment<V, true>(L, h, x, y, part + it.idx);
 }
@@ -64,7 +71,7 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(const BatchJob* __
   const BatchJob* J = jobs + it.job;
   const int j = ldc(&J->j0) + t;
   if (j >= ldc(&J->jmax)) return;
-  const uint32_t D = ldc(&J->L.D);
+  const uint32_t D = ldc(&J->n);
   void* vec = ldc(&J->vec);
   const uint64_t stride = ldc(&J->stride);
   const uint32_t cap = ldc(&J->cap);
@@ -120,7 +127,7 @@ __global__ __launch_bounds__(kBlock) void batch_finish_kernel(const BatchJob* __
   if (mode != kBatchStart && j >= ldc(&J->jmax)) return;
   const bool hv = mode == kBatchAlpha;
   const double* part = hv ? ldc(&J->part_hv) : ldc(&J->part_up);
-  const uint32_t n = hv ? ldc(&J->L.n_seg) : ldc(&J->n_chunks);
+  const uint32_t n = hv ? ldc(&J->n_hv) : ldc(&J->n_chunks);
   const double s = sum_partials(Partials{part, n});
   if (threadIdx.x != 0) return;
   double* scale = ldc(&J->scale);
@@ -190,6 +197,13 @@ __global__ __launch_bounds__(kBlock) void batch_plain_hv_kernel(const PlainHvJob
   const BatchItem it = ldc_struct(items + blockIdx.x);
   const PlainHvJob* J = jobs + it.job;
   const DevLayout L = ldc_struct(&J->L);
+  if (ldc(&J->Bl.dk) != 0u) {  // momentum sector: uniform over the workgroup
+    const DevBloch Bl = ldc_struct(&J->Bl);
+    const double p = bloch_row<double, true>(L, Bl, ldc(&J->x), ldc(&J->y), it.idx * kBlock + threadIdx.x);
+    const double s = block_sum_all(p);
+    if (threadIdx.x == 0) ldc(&J->part)[it.idx] = s;
+    return;
+  }
   const uint32_t h = ldc(L.proc_hi + it.idx);
   hv_segment<double, true>(L, h, ldc(&J->x), ldc(&J->y), ldc(&J->part) + it.idx);
 }

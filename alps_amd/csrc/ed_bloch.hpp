@@ -26,6 +26,9 @@
 #include <cmath>
 #include <complex>
 #include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -114,9 +117,27 @@ struct BitPermutation {
   }
 };
 
+// Translations given as arrays (perm[t*N + i] = image of site i under
+// translation t, phases re[t] + i im[t]); the first must be the identity.
+inline Translations read_translations(int N, int n_t, const int* perm, const double* re, const double* im) {
+  if (n_t < 1 || n_t > 64) throw std::invalid_argument("between 1 and 64 translations");
+  if (!perm || !re || !im) throw std::invalid_argument("null argument");
+  Translations t;
+  for (int i = 0; i < n_t; ++i) {
+    t.perm.emplace_back(perm + (size_t)i * N, perm + (size_t)(i + 1) * N);
+    t.phase.emplace_back(re[i], im[i]);
+  }
+  for (int s = 0; s < N; ++s)
+    if (t.perm[0][s] != s) throw std::invalid_argument("the first translation must be the identity");
+  return t;
+}
+
+// Rank of state s (split ranking; the low-word ranks may live in the
+// layout's shared tables).
 inline uint32_t split_rank(const SectorLayout& L, uint32_t s) {
   const uint32_t h = L.NH > 0 ? (s >> L.B) : 0u;
-  return L.hi_base[h] + L.lo_rank[s & ((1u << L.B) - 1u)];
+  const std::vector<uint32_t>& lo = L.lo_rank.empty() ? L.low->lo_rank : L.lo_rank;
+  return L.hi_base[h] + lo[s & ((1u << L.B) - 1u)];
 }
 
 // Momentum-independent orbit data of a fixed-Sz sector.
@@ -216,6 +237,63 @@ inline BlochSector bloch_sector(const BlochOrbits& o, const Translations& t, int
     b.ph_im.push_back(p.imag());
   }
   return b;
+}
+
+// The orbits of a sector under a set of translations, built once and shared
+// by every caller with the same sector shape and translations (the momenta
+// of one Sz sector): one build even when several threads ask at once.
+inline std::shared_ptr<const BlochOrbits> shared_bloch_orbits(const SectorLayout& L, const Translations& t) {
+  struct Slot {
+    std::once_flag once;
+    std::shared_ptr<const BlochOrbits> orbits;
+  };
+  std::vector<int> key{L.N, L.k, L.B};
+  for (const auto& p : t.perm) key.insert(key.end(), p.begin(), p.end());
+  static std::mutex mu;
+  static std::map<std::vector<int>, std::shared_ptr<Slot>> cache;
+  std::shared_ptr<Slot> slot;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    std::shared_ptr<Slot>& e = cache[key];
+    if (!e) {
+      if (cache.size() > 64) {  // bounded: a task has at most N + 1 Sz sectors
+        cache.clear();
+        std::shared_ptr<Slot>& f = cache[key];
+        f = std::make_shared<Slot>();
+        slot = f;
+      } else {
+        e = std::make_shared<Slot>();
+        slot = e;
+      }
+    } else {
+      slot = e;
+    }
+  }
+  std::call_once(slot->once, [&] { slot->orbits = std::make_shared<const BlochOrbits>(bloch_orbits(L, t)); });
+  return slot->orbits;
+}
+
+// The single matrix element of a momentum sector with one representative
+// (the Bloch H.v row of ed_bloch.hip at row 0 applied to (1, 0)): the
+// diagonal plus the flips that return to the representative's orbit, with
+// their phases; the imaginary part cancels for a Hermitian H.
+inline double bloch_single_element(const SectorLayout& L, const BlochSector& b) {
+  const uint32_t s = b.rep_state.at(0);
+  double d = 0.0, re = 0.0;
+  for (int site = 0; site < L.N; ++site) d += ((s >> (L.N - 1 - site)) & 1u) ? L.site_up[site] : L.site_dn[site];
+  for (size_t q = 0; q < L.bond_p1.size(); ++q) {
+    const uint32_t p1 = L.bond_p1[q], p2 = L.bond_p2[q];
+    if (((s >> p1) & 1u) == ((s >> p2) & 1u)) {
+      d += L.bond_par[q];
+      continue;
+    }
+    d += L.bond_anti[q];
+    if (L.bond_t[q] == 0.0) continue;
+    const uint32_t rk = split_rank(L, s ^ ((1u << p1) | (1u << p2)));
+    if (b.full_rep[rk] != 0u) continue;  // another orbit, or one not in the sector
+    re += L.bond_t[q] * b.ph_re[b.full_tr[rk]];
+  }
+  return d + re;
 }
 
 }  // namespace alps_ed

@@ -376,25 +376,6 @@ int alps_ed_operator_create(const alps_ed_model* model, alps_ed_operator* out) {
   });
 }
 
-}  // extern "C"
-
-namespace {
-
-Translations read_translations(int N, int n_t, const int* perm, const double* re, const double* im) {
-  require(n_t >= 1 && n_t <= 64, "between 1 and 64 translations");
-  require(perm && re && im, "null argument");
-  Translations t;
-  for (int i = 0; i < n_t; ++i) {
-    t.perm.emplace_back(perm + (size_t)i * N, perm + (size_t)(i + 1) * N);
-    t.phase.emplace_back(re[i], im[i]);
-  }
-  for (int s = 0; s < N; ++s) require(t.perm[0][s] == s, "the first translation must be the identity");
-  return t;
-}
-
-}  // namespace
-
-extern "C" {
 
 int alps_ed_operator_create_momentum(const alps_ed_model* model, int n_translations, const int* perm,
                                      const double* phase_re, const double* phase_im, alps_ed_operator* out) {
@@ -1389,12 +1370,55 @@ extern "C" int alps_ed_run_parameter_file(const char* text, char* out, size_t ou
       if (momentum) {
         std::vector<Translations> trans;
         for (const auto& k : ks) trans.push_back(lattice_translations(m.graph, k));
-        for (size_t i = 0; i < models.size(); ++i) {
-          const SectorLayout L = build_layout(model_spec(&models[i]));
-          const BlochOrbits orb = bloch_orbits(L, trans[0]);
-          for (size_t q = 0; q < nk; ++q) {
-            const MomentumInput mi{&L, &orb, &trans[q]};
-            res[i * nk + q] = solve_sector(&models[i], &sp, false, &mi).r;
+        // Every (Sz, k) sector as one batch when the momentum sectors' bases
+        // are small (about 5 bytes per state of each Sz sector and momentum
+        // on the host) and the whole batch fits the free device memory,
+        // else one after the other.
+        double basis_bytes = 0;
+        for (const alps_ed_model& am : models) basis_bytes += 5.0 * (double)binom(am.n_sites, am.n_up) * (double)nk;
+        std::vector<std::vector<int>> perm(nk);
+        std::vector<std::vector<double>> re(nk), im(nk);
+        for (size_t q = 0; q < nk; ++q)
+          for (size_t t = 0; t < trans[q].perm.size(); ++t) {
+            perm[q].insert(perm[q].end(), trans[q].perm[t].begin(), trans[q].perm[t].end());
+            re[q].push_back(trans[q].phase[t].real());
+            im[q].push_back(trans[q].phase[t].imag());
+          }
+        uint64_t need = 0;
+        if (res.size() > 1 && basis_bytes < 4e9)
+          for (const alps_ed_model& am : models)
+            for (size_t q = 0; q < nk; ++q) {
+              uint64_t f = 0;
+              check_status(alps_ed_batch_footprint_momentum(&am, (int)trans[q].perm.size(), perm[q].data(),
+                                                            re[q].data(), im[q].data(), &sp, sp.dtype, &f));
+              need += f + (1u << 20);
+            }
+        size_t free_b = 0, total_b = 0;
+        hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        if (need > 0 && need < 0.8 * (double)free_b) {
+          alps_ed_batch_config cfg{sp.dtype, need, 0, 8, 0, 0, 0};
+          alps_ed_batch bh = nullptr;
+          check_status(alps_ed_batch_create(&cfg, &bh));
+          std::unique_ptr<alps_ed_batch_s, int (*)(alps_ed_batch)> hold(bh, alps_ed_batch_destroy);
+          std::vector<uint64_t> tickets(res.size());
+          for (size_t i = 0; i < models.size(); ++i)
+            for (size_t q = 0; q < nk; ++q)
+              check_status(alps_ed_batch_submit_momentum(bh, &models[i], (int)trans[q].perm.size(), perm[q].data(),
+                                                         re[q].data(), im[q].data(), &sp, &tickets[i * nk + q]));
+          check_status(alps_ed_batch_run(bh));
+          for (size_t i = 0; i < res.size(); ++i) {
+            int ready = 0;
+            check_status(alps_ed_batch_result(bh, tickets[i], &res[i], &ready));
+            if (!ready || res[i].error_code < 0) throw DeviceError("sector solve did not complete");
+          }
+        } else {
+          for (size_t i = 0; i < models.size(); ++i) {
+            const SectorLayout L = build_layout(model_spec(&models[i]));
+            const BlochOrbits orb = bloch_orbits(L, trans[0]);
+            for (size_t q = 0; q < nk; ++q) {
+              const MomentumInput mi{&L, &orb, &trans[q]};
+              res[i * nk + q] = solve_sector(&models[i], &sp, false, &mi).r;
+            }
           }
         }
       } else if (measure) {

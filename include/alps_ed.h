@@ -305,6 +305,15 @@ int alps_ed_batch_destroy(alps_ed_batch b);
 /* Queues one solve; the model arrays are copied.  *ticket identifies it. */
 int alps_ed_batch_submit(alps_ed_batch b, const alps_ed_model* model, const alps_ed_solve_params* params,
                          uint64_t* ticket);
+/* Queues one solve in a momentum sector (translations as for
+ * alps_ed_operator_create_momentum; the result's dim is the sector
+ * dimension).  The same request as alps_ed_sparsediag_momentum, advanced
+ * with the rest of the batch: DiagMatrix::dostep's momentum loop inside the
+ * Sz loop (applications/diag/diag.h:85-138) as one batch.  Momentum sectors
+ * of one Sz sector share the orbits, built once. */
+int alps_ed_batch_submit_momentum(alps_ed_batch b, const alps_ed_model* model, int n_translations, const int* perm,
+                                  const double* phase_re, const double* phase_im, const alps_ed_solve_params* params,
+                                  uint64_t* ticket);
 /* One round: admit, start, steps_per_round steps, stopping-rule tests,
  * refinement, retirement.  Returns the counts after the round. */
 int alps_ed_batch_progress(alps_ed_batch b, int* n_active, int* n_pending);
@@ -318,6 +327,10 @@ int alps_ed_batch_stats(alps_ed_batch b, alps_ed_batch_status* stats);
 /* HBM bytes a request would hold while admitted. */
 int alps_ed_batch_footprint(const alps_ed_model* model, const alps_ed_solve_params* params, alps_ed_dtype dtype,
                             uint64_t* bytes);
+/* The same for a momentum-sector request (alps_ed_batch_submit_momentum). */
+int alps_ed_batch_footprint_momentum(const alps_ed_model* model, int n_translations, const int* perm,
+                                     const double* phase_re, const double* phase_im,
+                                     const alps_ed_solve_params* params, alps_ed_dtype dtype, uint64_t* bytes);
 
 /* The sparsediag task for the spin-1/2 path: runs every task of a parameter
  * file (the grammar of src/alps/parameter/parameterlist.h:20-36; LATTICE,

@@ -2,8 +2,9 @@
 vectors against the oracle's dense Bloch Hamiltonian (the restatement of
 blochbasisstates.h and hamiltonian_matrix.hpp pinned by the reference's
 example13/15 fixtures and by the Sz-sector spectrum), Lanczos eigenvalues per
-momentum, the parameter-file momentum loop of DiagMatrix::dostep, and at
-L=24 size-independent checks (sector dimensions add up to the Sz sector,
+momentum, the parameter-file momentum loop of DiagMatrix::dostep (run as
+one batch), batched momentum sectors against single solves, and at L=24
+size-independent checks (sector dimensions add up to the Sz sector,
 the lowest level over all momenta is the Sz ground state).
 
 Bars: H.v 1e-12 relative (FP64), 2e-6 (FP32 storage); eigenvalues 1e-10."""
@@ -144,3 +145,45 @@ def test_momentum_ring24_sectors():
     assert sum(dims) == math.comb(24, 12)
     assert min(lows) == pytest.approx(-10.67001451653723, rel=1e-10)
     assert int(np.argmin(lows)) == 0
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_batch_momentum_sectors_match_single_solves(dtype):
+    # every (Sz, k) sector of a 4x3 square lattice through one batch, mixed
+    # with fixed-Sz problems, against the single-solve path; a momentum
+    # problem's result is the same alone and in the mix
+    name, L, W = "square lattice", 4, 3
+    m = typed_model(name, L, W, 7)
+    ks = O.translation_momenta(name, L, W)
+    b = A.Batch(dtype, hbm_bytes=2 << 30, steps_per_round=6)
+    jobs = []
+    for n_up in (3, 5, 6):
+        b.submit(to_product(m, n_up), nev=2)  # fixed-Sz neighbours in the mix
+        for k in ks:
+            tr = O.translations(name, L, W, k=k)
+            jobs.append((m, n_up, tr, b.submit(to_product(m, n_up), nev=2, translations=tr)))
+    # sectors of 2,700 states, above the FP32 storage threshold
+    ring = typed_model("chain lattice", 18, 0, 9)
+    for k in O.translation_momenta("chain lattice", 18)[:4]:
+        tr = O.translations("chain lattice", 18, k=k)
+        jobs.append((ring, 9, tr, b.submit(to_product(ring, 9), nev=2, translations=tr)))
+    say("batch momentum", dtype, len(jobs))
+    b.run()
+    dims = {}
+    for mm, n_up, tr, t in jobs:
+        r = b.result(t)
+        ref = A.sparsediag(to_product(mm, n_up), nev=2, dtype=dtype, translations=tr)
+        assert r["dim"] == ref["dim"] and r["error_code"] == 0
+        if mm is m:
+            dims[n_up] = dims.get(n_up, 0) + r["dim"]
+        if r["dim"]:
+            np.testing.assert_allclose(r["ev"], ref["ev"], rtol=1e-10, atol=1e-12)
+    for n_up, d in dims.items():
+        assert d == math.comb(12, n_up)
+    # momentum problems alone give bit-identical eigenvalues
+    for mm, n_up, tr, t in (jobs[1], jobs[-1]):
+        alone = A.Batch(dtype, hbm_bytes=1 << 30, steps_per_round=6)
+        ta = alone.submit(to_product(mm, n_up), nev=2, translations=tr)
+        alone.run()
+        assert np.array_equal(alone.result(ta)["ev"], b.result(t)["ev"])
+
