@@ -260,6 +260,42 @@ def j1j2_leg(ev, iterations=300):
     }
 
 
+def sweep_leg(L=20, sample=42):
+    """The parameter-file task of a periodic XXZ ring without a fixed Sz:
+    every (Sz, momentum) sector (DiagMatrix::dostep's loops), run as one
+    batch, against a sample of the same sectors solved one at a time through
+    the single-sector path, scaled to all of them."""
+    import numpy as np
+
+    import alps_amd as A
+
+    text = f'LATTICE="chain lattice"; L={L}; J=1; Jz=0.7; NUMBER_EIGENVALUES=2; {{}}'
+    A.run_parameter_file(text)  # first sweep of the process: caches and arena
+    t0 = time.perf_counter()
+    (t,) = A.run_parameter_file(text)
+    batched = time.perf_counter() - t0
+    (d,) = A.describe_parameter_file(text)
+    ks = d["momenta"]
+    n_sectors = (L + 1) * len(ks)
+    s1, s2 = np.array(d["bond_s"]), np.array(d["bond_t"])
+    jz, jxy = np.array(d["jz"]), np.array(d["jxy"])
+    # translations of the ring: shift by j sites, phase exp(i k j)
+    perms = np.array([[(i + j) % L for i in range(L)] for j in range(L)])
+    rng = np.random.default_rng(5)
+    picks = rng.choice(n_sectors, size=min(sample, n_sectors), replace=False)
+    t1 = time.perf_counter()
+    for p in picks:
+        n_up, q = divmod(int(p), len(ks))
+        k = ks[q][0]
+        phases = np.exp(1j * k * np.arange(L))
+        A.sparsediag(A.SpinModel(L, s1, s2, jz, jxy, n_up), nev=2, translations=(phases, perms))
+    one = (time.perf_counter() - t1) / len(picks) * n_sectors
+    return {"workload": f"XXZ ring L={L}, Jz=0.7, every Sz and momentum sector, lowest two levels, FP64",
+            "sectors": n_sectors, "ground_state_energy": t["ground_state_energy"],
+            "batched_s": round(batched, 4), "one_at_a_time_s": round(one, 3), "one_at_a_time_sample": len(picks),
+            "speedup": round(one / batched, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -274,6 +310,7 @@ def main():
     ap.add_argument("--batch-size", type=int, default=1024)
     ap.add_argument("--seq-sample", type=int, default=64)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--graphs", action="store_true", help="batch and serve legs replay repeated rounds from HIP graphs")
@@ -385,6 +422,8 @@ def main():
         out["batch"] = batch_leg(args.batch_size, args.seq_sample, graphs=args.graphs)
     if not args.no_serve:
         out["serve"] = serve_leg(args.serve_requests, args.serve_rate, graphs=args.graphs)
+    if not args.no_sweep:
+        out["sweep"] = sweep_leg()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(L, min(args.cpu_rows, D))
     print(json.dumps(out), flush=True)
