@@ -8,6 +8,7 @@ GPU is present, the calls raise.
 """
 import ctypes
 import json
+import math
 import os
 from dataclasses import dataclass
 
@@ -71,6 +72,7 @@ def lib():
             "alps_ed_matvec": ([P, P, P], i32),
             "alps_ed_vector_spin_moments": ([P, P, P, P, P], i32),
             "alps_ed_vector_flip_pairs": ([P, P, i32, P, P, P], i32),
+            "alps_ed_vector_bloch_expand": ([P, P, P, P], i32),
             "alps_ed_lanczos_step_vectors": ([P, P, P, P, f64, P, P], i32),
             "alps_ed_lanczos_create": ([P, i32, i32, i32, P], i32),
             "alps_ed_lanczos_ritz": ([P, i32, i32, P, P], i32),
@@ -299,6 +301,14 @@ class Operator:
         out = np.zeros(max(len(a), 1))
         _check(lib().alps_ed_vector_flip_pairs(self.handle, v.handle, len(a), _p(a), _p(b), _p(out)))
         return out[: len(a)]
+
+    def bloch_expand(self, w: "Vector"):
+        """(re, im): a momentum-sector vector in the configuration basis of
+        its fixed-Sz sector, as two FP64 device vectors."""
+        n = math.comb(self.model.n_sites, self.model.n_up)
+        re, im = Vector(n, "f64"), Vector(n, "f64")
+        _check(lib().alps_ed_vector_bloch_expand(self.handle, w.handle, re.handle, im.handle))
+        return re, im
 
 
 class Vector:

@@ -29,7 +29,42 @@ __global__ __launch_bounds__(kBlock) void bloch_hv_kernel(DevLayout L, DevBloch 
   if (DOT) store_partial(dot_out, part);
 }
 
+// Configuration-basis form of a momentum-sector vector w (realified): on
+// every state r of the fixed-Sz sector whose orbit is kept,
+// phase_t(r) * norm_a * w_a with a its representative and t(r) the
+// translation taking it there; 0 elsewhere.
+template <class V>
+__global__ __launch_bounds__(kBlock) void bloch_expand_kernel(DevBloch Bl, uint32_t D, const V* __restrict__ w,
+                                                              double* __restrict__ re, double* __restrict__ im) {
+  const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= D) return;
+  const uint32_t a = Bl.full_rep[r];
+  double vr = 0.0, vi = 0.0;
+  if (a != 0xffffffffu) {
+    const uint32_t t = Bl.full_tr[r];
+    const double pr = Bl.ph_re[t], pi = Bl.ph_im[t], nrm = Bl.norm[a];
+    const double wr = (double)w[a], wi = (double)w[Bl.dk + a];
+    vr = nrm * (pr * wr - pi * wi);
+    vi = nrm * (pr * wi + pi * wr);
+  }
+  re[r] = vr;
+  im[r] = vi;
+}
+
 }  // namespace
+
+template <class V>
+hipError_t launch_bloch_expand(const DevBloch& B, uint32_t D, const V* w, double* re, double* im, hipStream_t st) {
+  if (D == 0) return hipSuccess;
+  hipLaunchKernelGGL((bloch_expand_kernel<V>), dim3((D + kBlock - 1) / kBlock), dim3(kBlock), 0, st, B, D, w, re,
+                     im);
+  return hipGetLastError();
+}
+
+template hipError_t launch_bloch_expand<float>(const DevBloch&, uint32_t, const float*, double*, double*,
+                                               hipStream_t);
+template hipError_t launch_bloch_expand<double>(const DevBloch&, uint32_t, const double*, double*, double*,
+                                                hipStream_t);
 
 template <class V>
 hipError_t launch_bloch_hv(const DevLayout& L, const DevBloch& B, const V* x, V* y, double* dot_out,

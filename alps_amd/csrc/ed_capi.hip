@@ -662,6 +662,26 @@ int alps_ed_vector_spin_moments(alps_ed_operator op, alps_ed_vector v, double* n
   });
 }
 
+int alps_ed_vector_bloch_expand(alps_ed_operator op, alps_ed_vector w, alps_ed_vector re, alps_ed_vector im) {
+  return guarded([&] {
+    require(op != nullptr, "operator is null");
+    check_vec(w);
+    check_vec(re);
+    check_vec(im);
+    require(op->bloch, "the operator is not a momentum sector");
+    require(w->n == op->n, "vector size != operator dimension");
+    require(re->n == op->layout.D && im->n == op->layout.D, "outputs need the fixed-Sz sector's dimension");
+    require(re->dtype == ALPS_ED_F64 && im->dtype == ALPS_ED_F64, "outputs must be FP64 vectors");
+    require(re != im, "re and im must be different vectors");
+    with_dtype(w->dtype, [&](auto z) {
+      using V = decltype(z);
+      hip_check(launch_bloch_expand<V>(op->bdev, (uint32_t)op->layout.D, w->buf.as<V>(), re->buf.as<double>(),
+                                       im->buf.as<double>(), current_stream()),
+                "bloch expand");
+    });
+  });
+}
+
 int alps_ed_vector_flip_pairs(alps_ed_operator op, alps_ed_vector v, int n_pairs, const int* site_i,
                               const int* site_j, double* out) {
   return guarded([&] {
@@ -1362,12 +1382,55 @@ extern "C" int alps_ed_run_parameter_file(const char* text, char* out, size_t ou
       std::vector<alps_ed_solve_result> res(models.size() * nk);
       // Eigenvector measurements (DiagMatrix::perform_measurements,
       // diag.h:191-314) need the eigenvectors of every sector: those tasks
-      // solve their sectors one by one with the Lanczos basis kept.
+      // solve their sectors one by one with the Lanczos basis kept.  In a
+      // momentum sector the eigenvectors are expanded into the fixed-Sz
+      // configuration basis and measured there; local values are not
+      // reported there, as in the reference (diag.h:212).
       const MeasureSpec ms = measure_spec(p);
-      const bool measure = ms.any() && !ts.whole_space && !momentum;
+      const bool measure = ms.any() && !ts.whole_space;
       std::vector<std::string> meas_json(res.size());
       std::vector<MeasureValues> meas_val(res.size());
-      if (momentum) {
+      if (momentum && measure) {
+        std::vector<Translations> trans;
+        for (const auto& k : ks) trans.push_back(lattice_translations(m.graph, k));
+        const MeasurePlan plan = measure_plan(ms, m.graph);
+        for (size_t i = 0; i < models.size(); ++i) {
+          const SectorLayout L = build_layout(model_spec(&models[i]));
+          const BlochOrbits orb = bloch_orbits(L, trans[0]);
+          std::unique_ptr<alps_ed_operator_s> plain;  // the fixed-Sz sector, for the spin sums
+          for (size_t q = 0; q < nk; ++q) {
+            const MomentumInput mi{&L, &orb, &trans[q]};
+            SectorSolve sv = solve_sector(&models[i], &sp, true, &mi);
+            res[i * nk + q] = sv.r;
+            if (sv.r.dim == 0) continue;
+            if (!plain) {
+              alps_ed_operator h = nullptr;
+              check_status(alps_ed_operator_create(&models[i], &h));
+              plain.reset(h);
+            }
+            std::vector<SpinSums> sums;
+            for (auto& v : sv.vectors) {
+              alps_ed_vector re = nullptr, im = nullptr;
+              check_status(alps_ed_vector_create(L.D, ALPS_ED_F64, &re));
+              std::unique_ptr<alps_ed_vector_s> hre(re);
+              check_status(alps_ed_vector_create(L.D, ALPS_ED_F64, &im));
+              std::unique_ptr<alps_ed_vector_s> him(im);
+              check_status(alps_ed_vector_bloch_expand(sv.op.get(), v.get(), re, im));
+              // <v|O|v> of a complex v: the sums of its real and imaginary
+              // parts (the real part of each flip amplitude)
+              SpinSums a = spin_sums(plain.get(), re, plan.flips), b = spin_sums(plain.get(), im, plan.flips);
+              a.n2 += b.n2;
+              for (size_t x = 0; x < a.sz.size(); ++x) a.sz[x] += b.sz[x];
+              for (size_t x = 0; x < a.szsz.size(); ++x) a.szsz[x] += b.szsz[x];
+              for (auto& f : a.flip) f.second += b.flip[f.first];
+              sums.push_back(std::move(a));
+            }
+            meas_val[i * nk + q] = evaluate_measurements(ms, m.graph, sums);
+            meas_val[i * nk + q].local.clear();
+            meas_json[i * nk + q] = measurements_json(meas_val[i * nk + q], plan);
+          }
+        }
+      } else if (momentum) {
         std::vector<Translations> trans;
         for (const auto& k : ks) trans.push_back(lattice_translations(m.graph, k));
         // Every (Sz, k) sector as one batch when the momentum sectors' bases
@@ -1524,7 +1587,7 @@ extern "C" int alps_ed_run_parameter_file(const char* text, char* out, size_t ou
       js += "], \"ground_state_energy\": " + json_number(g) + ", \"energy_gap\": " + json_number(gap);
       if (ms.any() && !measure)
         js += std::string(", \"measurement_error\": \"eigenvector measurements need Sz-conserved sectors") +
-              " (CONSERVED_QUANTUMNUMBERS=\\\"Sz\\\" or Sz_total) without momenta (TRANSLATION_SYMMETRY=false)\"";
+              " (CONSERVED_QUANTUMNUMBERS=\\\"Sz\\\" or Sz_total)\"";
       const int nev_xml = ev.defined("NUMBER_EIGENVALUES") ? sp.nev : 0;
       js += ", \"xml\": " +
             json_string(task_xml(p, xsec, nev_xml, !std::isnan(g), g, !std::isnan(gap), gap));

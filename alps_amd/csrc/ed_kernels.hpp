@@ -192,6 +192,11 @@ template <class V>
 hipError_t launch_bloch_hv(const DevLayout& L, const DevBloch& B, const V* x, V* y, double* dot_out,
                            hipStream_t st);
 
+// The configuration-basis form (re + i im, D = the fixed-Sz sector's
+// dimension) of a realified momentum-sector vector w (ed_bloch.hip).
+template <class V>
+hipError_t launch_bloch_expand(const DevBloch& B, uint32_t D, const V* w, double* re, double* im, hipStream_t st);
+
 hipError_t launch_unrank(const DevLayout& L, uint64_t first, uint64_t count, uint32_t* out, hipStream_t st);
 hipError_t launch_rank(const DevLayout& L, const uint32_t* states, uint64_t count, uint64_t* out,
                        hipStream_t st);

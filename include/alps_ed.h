@@ -155,6 +155,18 @@ int alps_ed_vector_nrm2(alps_ed_vector x, double* result);
 /* ietl::mult(m, x, y) -> y = prod(m, x) (interface/ublas.h:57-63). */
 int alps_ed_matvec(alps_ed_operator op, alps_ed_vector x, alps_ed_vector y);
 
+/* A momentum-sector vector w (realified, of an operator from
+ * alps_ed_operator_create_momentum) in the configuration basis of its
+ * fixed-Sz sector: re + i im, FP64 vectors of C(N, n_up) elements.  State r
+ * of a kept orbit gets phase_t * norm_a * w_a, with a its representative and
+ * t the first translation taking r to it (the Bloch states of
+ * src/alps/model/blochbasisstates.h:97-271); other states get 0.  The
+ * expanded vector is an eigenvector of the sector's Hamiltonian when w is
+ * one of the momentum sector's; eigenvector measurements
+ * (DiagMatrix::perform_measurements in a momentum sector,
+ * applications/diag/diag.h:191-314) run on it. */
+int alps_ed_vector_bloch_expand(alps_ed_operator op, alps_ed_vector w, alps_ed_vector re, alps_ed_vector im);
+
 /* Spin expectation sums of a vector v of the operator's sector, the parts
  * from which SparseDiagMatrix::calculate (sparsediag.h:151-159) evaluates
  * <v|O|v> for the measurements of DiagMatrix::perform_measurements

@@ -187,3 +187,90 @@ def test_batch_momentum_sectors_match_single_solves(dtype):
         alone.run()
         assert np.array_equal(alone.result(ta)["ev"], b.result(t)["ev"])
 
+
+def _bloch_expand_reference(N, n_up, phases, perms, w):
+    """Configuration-basis coefficients of a momentum-sector vector from the
+    oracle's Bloch basis: on each state r of a kept orbit, phase_t * norm_a *
+    w_a (a its representative, the smallest state of its orbit; t the first
+    translation reaching it)."""
+    reps, norm = O.bloch_basis(N, n_up, phases, perms)
+    idx = {int(r): i for i, r in enumerate(reps)}
+    states = [s for s in range(1 << N) if bin(s).count("1") == n_up]
+
+    def image(s, perm):
+        out = 0
+        for i in range(N):
+            if (s >> (N - 1 - i)) & 1:
+                out |= 1 << (N - 1 - perm[i])
+        return out
+
+    v = np.zeros(len(states), dtype=complex)
+    for r, s in enumerate(states):
+        best, bt = s, 0
+        for t in range(1, len(perms)):
+            y = image(s, perms[t])
+            if y < best:
+                best, bt = y, t
+        if best in idx:
+            a = idx[best]
+            v[r] = phases[bt] * norm[a] * w[a]
+    return v
+
+
+@pytest.mark.parametrize("case", ["chain10", "square4x2", "triangular3x3"])
+def test_bloch_expand(case):
+    # the device expansion against the oracle's basis, and an expanded
+    # momentum eigenvector is an eigenvector of the fixed-Sz Hamiltonian
+    name, L, W, ups = CASES[case]
+    m = typed_model(name, L, W, 3)
+    N = m.n_sites
+    n_up = ups[0]
+    rng = np.random.default_rng(2)
+    plain = A.Operator(to_product(m, n_up))
+    for k in O.translation_momenta(name, L, W)[:3]:
+        tr = O.translations(name, L, W, k=k)
+        op = A.Operator(to_product(m, n_up), translations=tr)
+        dk = op.dim // 2
+        if dk == 0:
+            continue
+        w = rng.uniform(-1, 1, 2 * dk)
+        re, im = op.bloch_expand(A.Vector.from_numpy(w, "f64"))
+        want = _bloch_expand_reference(N, n_up, *tr, w[:dk] + 1j * w[dk:])
+        np.testing.assert_allclose(re.numpy(), want.real, rtol=0, atol=1e-14)
+        np.testing.assert_allclose(im.numpy(), want.imag, rtol=0, atol=1e-14)
+        # eigenvector of the dense Bloch matrix -> eigenvector of H
+        H = O.bloch_dense(m, n_up, *tr)
+        lam, V = np.linalg.eigh(H)
+        ev = np.concatenate([V[:, 0].real, V[:, 0].imag])
+        re, im = op.bloch_expand(A.Vector.from_numpy(ev, "f64"))
+        for part in (re, im):
+            hp = plain.matvec(part, A.Vector(plain.dim, "f64")).numpy()
+            assert np.linalg.norm(hp - lam[0] * part.numpy()) < 1e-10
+        assert abs(np.linalg.norm(re.numpy()) ** 2 + np.linalg.norm(im.numpy()) ** 2 - 1) < 1e-12
+
+
+def test_momentum_measurements_match_sz_sector():
+    # measurements in the momentum sectors (the reference's default on a
+    # ring) against the same task without momenta: the non-degenerate
+    # ground state of the L=10 Heisenberg ring (Sz=0, k=pi) gives the same
+    # averages, correlations and structure factors; local values are not
+    # reported in momentum sectors (diag.h:212)
+    meas = ("MEASURE_AVERAGE[Magnetization]=Sz\n"
+            "MEASURE_LOCAL[Local]=Sz\nMEASURE_CORRELATIONS[SzSz]=Sz\n"
+            'MEASURE_CORRELATIONS[SpSm]="Splus:Sminus"\nMEASURE_STRUCTURE_FACTOR[Szk]=Sz\n')
+    base = 'LATTICE="chain lattice"; L=10; J=1; Jz=0.8; Sz_total=0;\n' + meas
+    say("momentum measurements")
+    (tk,) = A.run_parameter_file(base + "{}\n")
+    (t0,) = A.run_parameter_file(base + "TRANSLATION_SYMMETRY=false;\n{}\n")
+    assert "measurement_error" not in tk
+    low = min(tk["sectors"], key=lambda s: s["eigenvalues"][0])
+    (ref,) = t0["sectors"]
+    assert low["eigenvalues"][0] == pytest.approx(ref["eigenvalues"][0], rel=1e-10)
+    mk, m0 = low["measurements"], ref["measurements"]
+    assert mk["local"] == {} and "Local" in m0["local"]
+    assert mk["average"]["Magnetization"][0] == pytest.approx(m0["average"]["Magnetization"][0], abs=1e-10)
+    for group in ("correlations", "structure_factor"):
+        for name, series in m0[group].items():
+            assert mk[group][name]["labels"] == series["labels"]
+            np.testing.assert_allclose(mk[group][name]["values"][0], series["values"][0], rtol=1e-8, atol=1e-10)
+
