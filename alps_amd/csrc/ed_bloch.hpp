@@ -251,22 +251,23 @@ inline std::shared_ptr<const BlochOrbits> shared_bloch_orbits(const SectorLayout
   for (const auto& p : t.perm) key.insert(key.end(), p.begin(), p.end());
   static std::mutex mu;
   static std::map<std::vector<int>, std::shared_ptr<Slot>> cache;
+  static uint64_t cached_states = 0;
   std::shared_ptr<Slot> slot;
   {
     std::lock_guard<std::mutex> g(mu);
-    std::shared_ptr<Slot>& e = cache[key];
-    if (!e) {
-      if (cache.size() > 64) {  // bounded: a task has at most N + 1 Sz sectors
-        cache.clear();
-        std::shared_ptr<Slot>& f = cache[key];
-        f = std::make_shared<Slot>();
-        slot = f;
-      } else {
-        e = std::make_shared<Slot>();
-        slot = e;
-      }
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      slot = it->second;
     } else {
-      slot = e;
+      // bounded: at most 64 sectors and 2^28 states (about 2.4 GB of orbit
+      // data); a task has at most N + 1 Sz sectors
+      if (cache.size() >= 64 || cached_states + L.D > (uint64_t(1) << 28)) {
+        cache.clear();
+        cached_states = 0;
+      }
+      slot = std::make_shared<Slot>();
+      cache[key] = slot;
+      cached_states += L.D;
     }
   }
   std::call_once(slot->once, [&] { slot->orbits = std::make_shared<const BlochOrbits>(bloch_orbits(L, t)); });

@@ -319,3 +319,25 @@ def test_measure_requests_and_labels():
 def test_measure_absent_without_requests():
     (t,) = A.describe_parameter_file('LATTICE="chain lattice"; L=4; {}')
     assert "measure" not in t
+
+
+def test_batch_footprint_of_momentum_sectors():
+    # host-only sizing: a momentum sector of the L=12 ring holds its Bloch
+    # basis (5 bytes per state of the Sz sector) and vectors of 2 * d_k
+    # elements; it needs less than the whole Sz sector's solve at FP64, and
+    # bad translations are rejected
+    bs = np.arange(12)
+    m = A.SpinModel(12, bs, (bs + 1) % 12, 1.0, 1.0, 6)
+    full = A.batch_footprint(m, "f64", nev=1)
+    for k in O.translation_momenta("chain lattice", 12)[:3]:
+        tr = O.translations("chain lattice", 12, k=k)
+        f = A.batch_footprint(m, "f64", nev=1, translations=tr)
+        assert 5 * 924 < f < full
+    ph, perms = O.translations("chain lattice", 12, k=[0.0])
+    bad = perms.copy()
+    bad[1, 0] = bad[1, 1]
+    with pytest.raises(A.EDError):
+        A.batch_footprint(m, "f64", translations=(ph, bad))
+    with pytest.raises(A.EDError):
+        A.batch_footprint(m, "f64", translations=(ph[1:], perms[1:]))  # first must be the identity
+
