@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cctype>
 #include <chrono>
 #include <cmath>
@@ -11,6 +12,7 @@
 #include <mutex>
 #include <map>
 #include <new>
+#include <thread>
 #include <stdexcept>
 #include <sstream>
 #include <string>
@@ -1447,15 +1449,33 @@ extern "C" int alps_ed_run_parameter_file(const char* text, char* out, size_t ou
             re[q].push_back(trans[q].phase[t].real());
             im[q].push_back(trans[q].phase[t].imag());
           }
+        // Sizing builds each Sz sector's orbits (kept for the batch): the
+        // Sz sectors are sized in parallel threads.
         uint64_t need = 0;
-        if (res.size() > 1 && basis_bytes < 4e9)
-          for (const alps_ed_model& am : models)
-            for (size_t q = 0; q < nk; ++q) {
-              uint64_t f = 0;
-              check_status(alps_ed_batch_footprint_momentum(&am, (int)trans[q].perm.size(), perm[q].data(),
-                                                            re[q].data(), im[q].data(), &sp, sp.dtype, &f));
-              need += f + (1u << 20);
-            }
+        if (res.size() > 1 && basis_bytes < 4e9) {
+          std::vector<uint64_t> fp(res.size(), 0);
+          std::vector<int> st(res.size(), ALPS_ED_OK);
+          std::atomic<size_t> next{0};
+          auto work = [&] {
+            for (size_t i = next++; i < models.size(); i = next++)
+              for (size_t q = 0; q < nk; ++q)
+                st[i * nk + q] = alps_ed_batch_footprint_momentum(&models[i], (int)trans[q].perm.size(),
+                                                                  perm[q].data(), re[q].data(), im[q].data(), &sp,
+                                                                  sp.dtype, &fp[i * nk + q]);
+          };
+          std::vector<std::thread> th;
+          const size_t nth = std::min<size_t>(models.size(), 8);
+          for (size_t t = 1; t < nth; ++t) th.emplace_back(work);
+          work();
+          for (auto& t : th) t.join();
+          for (size_t i = 0; i < res.size(); ++i) {
+            if (st[i] != ALPS_ED_OK)
+              check_status(alps_ed_batch_footprint_momentum(&models[i / nk], (int)trans[i % nk].perm.size(),
+                                                            perm[i % nk].data(), re[i % nk].data(),
+                                                            im[i % nk].data(), &sp, sp.dtype, &fp[i]));
+            need += fp[i] + (1u << 20);
+          }
+        }
         size_t free_b = 0, total_b = 0;
         hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
         if (need > 0 && need < 0.8 * (double)free_b) {
