@@ -304,7 +304,7 @@ size_t table_bytes(const SectorLayout& L) {
 // Bytes of a momentum sector's basis arrays (none for a fixed-Sz sector).
 size_t bloch_bytes(const BlochSector& b) {
   if (b.rep_state.empty()) return 0;
-  return align_up(b.full_rep.size() * 4) + align_up(b.full_tr.size()) + align_up(b.rep_state.size() * 4) +
+  return align_up(b.full_pk.size() * 8) + align_up(b.norm_tab.size() * 8) + align_up(b.rep_state.size() * 4) +
          align_up(b.norm.size() * 8) + 2 * align_up(b.ph_re.size() * 8);
 }
 
@@ -425,8 +425,8 @@ void place(alps_ed_batch b, Problem& p, char* stage, const SharedTables& sh) {
       return dst;
     };
     J.Bl.dk = (uint32_t)p.bloch.rep_state.size();
-    J.Bl.full_rep = put(p.bloch.full_rep);
-    J.Bl.full_tr = put(p.bloch.full_tr);
+    J.Bl.full_pk = put(p.bloch.full_pk);
+    J.Bl.norm_tab = put(p.bloch.norm_tab);
     J.Bl.rep_state = put(p.bloch.rep_state);
     J.Bl.norm = put(p.bloch.norm);
     J.Bl.ph_re = put(p.bloch.ph_re);

@@ -38,10 +38,11 @@ __global__ __launch_bounds__(kBlock) void bloch_expand_kernel(DevBloch Bl, uint3
                                                               double* __restrict__ re, double* __restrict__ im) {
   const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
   if (r >= D) return;
-  const uint32_t a = Bl.full_rep[r];
+  const uint64_t pk = Bl.full_pk[r];
+  const uint32_t a = (uint32_t)pk;
   double vr = 0.0, vi = 0.0;
   if (a != 0xffffffffu) {
-    const uint32_t t = Bl.full_tr[r];
+    const uint32_t t = (uint32_t)(pk >> 32) & 0xffu;
     const double pr = Bl.ph_re[t], pi = Bl.ph_im[t], nrm = Bl.norm[a];
     const double wr = (double)w[a], wi = (double)w[Bl.dk + a];
     vr = nrm * (pr * wr - pi * wi);

@@ -206,12 +206,18 @@ inline BlochOrbits bloch_orbits(const SectorLayout& L, const Translations& t) {
 
 constexpr uint32_t kNoState = 0xffffffffu;
 
-// The Bloch basis at one momentum.
+// The Bloch basis at one momentum.  The device reads a flipped state's
+// representative, translation and normalization with one gather of its
+// packed entry: kept representative index (bits 0-31, kNoState when not
+// kept), translation index (bits 32-39) and index into the table of
+// distinct normalizations (bits 40-55).
 struct BlochSector {
   std::vector<uint32_t> rep_state;  // kept representatives, ascending
   std::vector<double> norm;
   std::vector<uint32_t> full_rep;   // per rank: kept representative index, or kNoState
   std::vector<uint8_t> full_tr;     // per rank: translation index of its phase
+  std::vector<uint64_t> full_pk;    // per rank: the packed entry
+  std::vector<double> norm_tab;     // distinct normalizations
   std::vector<double> ph_re, ph_im; // per translation
 };
 
@@ -232,6 +238,23 @@ inline BlochSector bloch_sector(const BlochOrbits& o, const Translations& t, int
   b.full_rep.resize(o.state.size());
   b.full_tr = o.tr;
   for (size_t r = 0; r < o.state.size(); ++r) b.full_rep[r] = kept[o.rep_rank[r]];
+  std::map<double, uint32_t> tab;
+  std::vector<uint32_t> norm_idx(b.norm.size());
+  for (size_t q = 0; q < b.norm.size(); ++q) {
+    auto it = tab.find(b.norm[q]);
+    if (it == tab.end()) {
+      if (b.norm_tab.size() >= 65535) throw std::invalid_argument("too many distinct Bloch normalizations");
+      it = tab.emplace(b.norm[q], (uint32_t)b.norm_tab.size()).first;
+      b.norm_tab.push_back(b.norm[q]);
+    }
+    norm_idx[q] = it->second;
+  }
+  b.full_pk.resize(o.state.size());
+  for (size_t r = 0; r < o.state.size(); ++r) {
+    const uint32_t a = b.full_rep[r];
+    b.full_pk[r] = a == kNoState ? (uint64_t)kNoState
+                                 : (uint64_t)a | ((uint64_t)b.full_tr[r] << 32) | ((uint64_t)norm_idx[a] << 40);
+  }
   for (const auto& p : t.phase) {
     b.ph_re.push_back(p.real());
     b.ph_im.push_back(p.imag());

@@ -84,7 +84,7 @@ struct alps_ed_operator_s {
   // momentum sector (ed_bloch.hpp): rows are the kept representatives
   bool bloch = false;
   DevBloch bdev{};
-  DevBuf b_rep_state, b_norm, b_full_rep, b_full_tr, b_ph_re, b_ph_im;
+  DevBuf b_rep_state, b_norm, b_full_pk, b_norm_tab, b_ph_re, b_ph_im;
 };
 
 struct alps_ed_vector_s {
@@ -257,16 +257,16 @@ void operator_upload(alps_ed_operator op) {
 void bloch_upload(alps_ed_operator op, const BlochSector& b) {
   upload(op->b_rep_state, b.rep_state);
   upload(op->b_norm, b.norm);
-  upload(op->b_full_rep, b.full_rep);
-  upload(op->b_full_tr, b.full_tr);
+  upload(op->b_full_pk, b.full_pk);
+  upload(op->b_norm_tab, b.norm_tab);
   upload(op->b_ph_re, b.ph_re);
   upload(op->b_ph_im, b.ph_im);
   op->bdev = DevBloch{(uint32_t)b.rep_state.size(), op->b_rep_state.as<uint32_t>(), op->b_norm.as<double>(),
-                      op->b_full_rep.as<uint32_t>(), op->b_full_tr.as<uint8_t>(), op->b_ph_re.as<double>(),
+                      op->b_full_pk.as<uint64_t>(), op->b_norm_tab.as<double>(), op->b_ph_re.as<double>(),
                       op->b_ph_im.as<double>()};
   op->bloch = true;
   op->n = 2 * (uint64_t)b.rep_state.size();
-  for (DevBuf* x : {&op->b_rep_state, &op->b_norm, &op->b_full_rep, &op->b_full_tr, &op->b_ph_re, &op->b_ph_im})
+  for (DevBuf* x : {&op->b_rep_state, &op->b_norm, &op->b_full_pk, &op->b_norm_tab, &op->b_ph_re, &op->b_ph_im})
     op->bytes += x->bytes;
 }
 
@@ -1436,11 +1436,11 @@ extern "C" int alps_ed_run_parameter_file(const char* text, char* out, size_t ou
         std::vector<Translations> trans;
         for (const auto& k : ks) trans.push_back(lattice_translations(m.graph, k));
         // Every (Sz, k) sector as one batch when the momentum sectors' bases
-        // are small (about 5 bytes per state of each Sz sector and momentum
-        // on the host) and the whole batch fits the free device memory,
-        // else one after the other.
+        // are small (about 13 bytes per state of each Sz sector and momentum
+        // on the host while sizing) and the whole batch fits the free device
+        // memory, else one after the other.
         double basis_bytes = 0;
-        for (const alps_ed_model& am : models) basis_bytes += 5.0 * (double)binom(am.n_sites, am.n_up) * (double)nk;
+        for (const alps_ed_model& am : models) basis_bytes += 13.0 * (double)binom(am.n_sites, am.n_up) * (double)nk;
         std::vector<std::vector<int>> perm(nk);
         std::vector<std::vector<double>> re(nk), im(nk);
         for (size_t q = 0; q < nk; ++q)

@@ -426,10 +426,11 @@ __device__ __forceinline__ double bloch_row(const DevLayout& L, const DevBloch& 
     if (tb == 0.0) continue;
     const uint32_t t = s ^ ((1u << p1) | (1u << p2));
     const uint32_t rk = L.hi_base[L.NH > 0 ? (t >> L.B) : 0u] + L.lo_rank[t & lomask];
-    const uint32_t j = Bl.full_rep[rk];
+    const uint64_t pk = Bl.full_pk[rk];
+    const uint32_t j = (uint32_t)pk;
     if (j == 0xffffffffu) continue;  // an orbit not in this sector
-    const uint32_t tr = Bl.full_tr[rk];
-    const double f = tb * (Bl.norm[j] / ni);
+    const uint32_t tr = (uint32_t)(pk >> 32) & 0xffu;
+    const double f = tb * (Bl.norm_tab[(uint32_t)(pk >> 40)] / ni);
     const double cr = f * Bl.ph_re[tr], ci = f * Bl.ph_im[tr];
     const double yr = (double)x[j], yi = (double)x[dk + j];
     ar += cr * yr - ci * yi;

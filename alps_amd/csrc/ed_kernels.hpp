@@ -52,8 +52,9 @@ struct DevBloch {
   uint32_t dk;                // kept representatives
   const uint32_t* rep_state;  // [dk]
   const double* norm;         // [dk]
-  const uint32_t* full_rep;   // [D] per rank of the fixed-Sz sector
-  const uint8_t* full_tr;     // [D]
+  const uint64_t* full_pk;    // [D] per rank of the fixed-Sz sector: packed
+                              // representative, translation, normalization index
+  const double* norm_tab;     // distinct normalizations
   const double* ph_re;        // [translations]
   const double* ph_im;
 };

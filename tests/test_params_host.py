@@ -323,7 +323,7 @@ def test_measure_absent_without_requests():
 
 def test_batch_footprint_of_momentum_sectors():
     # host-only sizing: a momentum sector of the L=12 ring holds its Bloch
-    # basis (5 bytes per state of the Sz sector) and vectors of 2 * d_k
+    # basis (8 bytes per state of the Sz sector) and vectors of 2 * d_k
     # elements; it needs less than the whole Sz sector's solve at FP64, and
     # bad translations are rejected
     bs = np.arange(12)
@@ -332,7 +332,7 @@ def test_batch_footprint_of_momentum_sectors():
     for k in O.translation_momenta("chain lattice", 12)[:3]:
         tr = O.translations("chain lattice", 12, k=k)
         f = A.batch_footprint(m, "f64", nev=1, translations=tr)
-        assert 5 * 924 < f < full
+        assert 8 * 924 < f < full
     ph, perms = O.translations("chain lattice", 12, k=[0.0])
     bad = perms.copy()
     bad[1, 0] = bad[1, 1]
