@@ -61,11 +61,16 @@ static bool read_file(const std::string& path, std::string& text) {
   return true;
 }
 
-// The value of attribute `name` in the tag text `tag` ("" when absent).
+// The value of attribute `name` in the tag text `tag` ("" when absent);
+// white space may surround the '='.
 static std::string attribute(const std::string& tag, const std::string& name) {
-  for (size_t p = tag.find(name + "="); p != std::string::npos; p = tag.find(name + "=", p + 1)) {
+  for (size_t p = tag.find(name); p != std::string::npos; p = tag.find(name, p + 1)) {
     if (p > 0 && !std::isspace((unsigned char)tag[p - 1])) continue;
-    const size_t q = p + name.size() + 1;
+    size_t q = p + name.size();
+    while (q < tag.size() && std::isspace((unsigned char)tag[q])) ++q;
+    if (q >= tag.size() || tag[q] != '=') continue;
+    ++q;
+    while (q < tag.size() && std::isspace((unsigned char)tag[q])) ++q;
     if (q >= tag.size() || (tag[q] != '"' && tag[q] != '\'')) continue;
     const size_t e = tag.find(tag[q], q + 1);
     if (e == std::string::npos) return "";

@@ -252,7 +252,7 @@ def test_cli_job_file(tmp_path):
   <OUTPUT file="j.out.xml"/>
   <TASK status="new"><INPUT file="sub/j.task1.in.xml"/><OUTPUT file="sub/j.task1.out.xml"/></TASK>
   <TASK status="finished"><INPUT file="sub/j.task2.in.xml"/><OUTPUT file="sub/j.task2.out.xml"/></TASK>
-  <TASK><INPUT file="sub/j.task3.in.xml"></INPUT></TASK>
+  <TASK><INPUT file = "sub/j.task3.in.xml"></INPUT></TASK>
 </JOB>
 """)
     out = subprocess.run([_cli(), "--describe", str(job)], check=True, capture_output=True, text=True).stdout
