@@ -353,13 +353,17 @@ int alps_ed_batch_footprint_momentum(const alps_ed_model* model, int n_translati
  * DiagMatrix::dostep does (applications/diag/diag.h:85-175): one Lanczos
  * solve per Sz sector and, on lattices with periodic directions unless
  * TRANSLATION_SYMMETRY is false, per momentum inside it (the Bloch sectors
- * of alps_ed_operator_create_momentum; TOTAL_MOMENTUM selects one).  Sz
- * sectors without momenta advance together as one batch (alps_ed_batch_*)
- * when their footprint fits the free device memory, else one after the
- * other.  Then the ground-state energy and gap of
+ * of alps_ed_operator_create_momentum; TOTAL_MOMENTUM selects one).  The
+ * sectors of a task, with or without momenta, advance together as one batch
+ * (alps_ed_batch_*) when their footprint fits the free device memory, else
+ * one after the other.  Then the ground-state energy and gap of
  * SparseDiagMatrix::write_xml_body (sparsediag.h:59-90), the eigenvector
- * measurements (diag.h:191-314; Sz sectors without momenta) and the task's
- * result XML.  The results are written to `out` as a JSON document (at most
+ * measurements (diag.h:191-314; in momentum sectors on the eigenvectors
+ * expanded by alps_ed_vector_bloch_expand, without local values, :212) and
+ * the task's result XML.  The input may also be a task or parameter XML
+ * document (<SIMULATION>/<PARAMETERS>, <PARAMETERLIST>).  Tasks that measure
+ * solve their sectors one at a time.  The results are written to `out` as a
+ * JSON document (at most
  * out_len bytes including the terminating zero; *needed receives the full
  * length). */
 int alps_ed_run_parameter_file(const char* text, char* out, size_t out_len, size_t* needed);
