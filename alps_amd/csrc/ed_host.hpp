@@ -70,6 +70,13 @@ int guarded(F&& f) {
   } catch (const std::runtime_error& e) {
     last_error() = e.what();
     return ALPS_ED_ERR_SOLVER;
+  } catch (const std::logic_error& e) {
+    // a broken internal invariant of the host code (not a device fault)
+    last_error() = std::string("internal error: ") + e.what();
+    return ALPS_ED_ERR_SOLVER;
+  } catch (const std::exception& e) {
+    last_error() = e.what();
+    return ALPS_ED_ERR_SOLVER;
   } catch (...) {
     last_error() = "unknown error";
     return ALPS_ED_ERR_DEVICE;

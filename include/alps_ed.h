@@ -38,7 +38,8 @@ extern "C" {
 #define ALPS_ED_ERR_ARGUMENT -1   /* invalid argument (std::invalid_argument in C++) */
 #define ALPS_ED_ERR_DEVICE -2     /* HIP runtime error */
 #define ALPS_ED_ERR_MEMORY -3     /* device allocation failed or over budget */
-#define ALPS_ED_ERR_SOLVER -4     /* tridiagonal solve failed (tmatrix.h:144-145) */
+#define ALPS_ED_ERR_SOLVER -4     /* tridiagonal solve failed (tmatrix.h:144-145), Lanczos breakdown or
+                                     non-finite coefficients, or a broken internal invariant */
 #define ALPS_ED_ERR_STATE -5      /* call out of order (lanczos.h:362-363) */
 
 typedef enum { ALPS_ED_F32 = 0, ALPS_ED_F64 = 1 } alps_ed_dtype;

@@ -287,3 +287,16 @@ def bloch_dense(m: Model, n_up, phases, perms):
                                  _p(out_re), _p(out_im))
     assert D >= 0, D
     return (out_re[: D * D] + 1j * out_im[: D * D]).reshape(D, D)
+
+
+def ring24_fixture(n_up, nev):
+    """The oracle's FP64 sparsediag of the L=24 Heisenberg ring for one
+    (n_up, nev), stored by tests/golden/make_ring24_fixture.py."""
+    import json
+
+    with open(os.path.join(ROOT, "tests", "golden", "ring24_oracle.json")) as f:
+        runs = json.load(f)["runs"]
+    for r in runs:
+        if r["n_up"] == n_up and r["nev"] == nev:
+            return r
+    raise KeyError((n_up, nev))

@@ -143,7 +143,8 @@ def test_momentum_ring24_sectors():
         dims.append(r["dim"])
         lows.append(r["ev"][0])
     assert sum(dims) == math.comb(24, 12)
-    assert min(lows) == pytest.approx(-10.67001451653723, rel=1e-10)
+    # the oracle's FP64 sparsediag of the Sz=0 sector
+    assert min(lows) == pytest.approx(O.ring24_fixture(12, 1)["eigenvalues"][0], rel=1e-10)
     assert int(np.argmin(lows)) == 0
 
 

@@ -310,6 +310,41 @@ def test_ground_and_gap_cases():
         assert g == g2 and gap == gap2, sec
 
 
+def bethe_ground_energy(L, M):
+    """Lowest energy of the spin-1/2 Heisenberg ring (J=1) with M down spins
+    from the Bethe ansatz: 2 L arctan(2 l_j) = 2 pi I_j + 2 sum_k arctan(l_j -
+    l_k) with I_j = -(M-1)/2, ..., (M-1)/2, and E = L/4 - sum_j 2 / (4 l_j^2 + 1)
+    (magnon energy cos k - 1 with e^{ik} = (l + i/2) / (l - i/2))."""
+    from scipy.optimize import fsolve
+
+    I = np.arange(M) - (M - 1) / 2.0
+
+    def f(lam):
+        return 2 * L * np.arctan(2 * lam) - 2 * np.pi * I - 2 * np.arctan(lam[:, None] - lam[None, :]).sum(axis=1)
+
+    lam = fsolve(f, 0.5 * np.tan(np.pi * I / L), xtol=1e-15)
+    assert np.abs(f(lam)).max() < 1e-12
+    return L / 4 - np.sum(2 / (4 * lam ** 2 + 1))
+
+
+def test_ring24_fixture_matches_bethe_ansatz():
+    """The stored oracle solves of the L=24 ring (the reference values of the
+    GPU tests at that size) against the Bethe ansatz: the Sz=0 ground state
+    (12 down spins), the lowest Sz=1 level (11 down spins), which SU(2) makes
+    the second Sz=0 level; the Bethe form itself gives the ring of four's
+    E0 = -2 exactly."""
+    assert abs(bethe_ground_energy(4, 2) + 2.0) < 1e-14
+    e0, e1 = bethe_ground_energy(24, 12), bethe_ground_energy(24, 11)
+    for nev in (1, 2, 4):
+        ev = O.ring24_fixture(12, nev)["eigenvalues"]
+        assert len(ev) == nev
+        assert abs(ev[0] - e0) < 1e-12 * abs(e0)
+        if nev > 1:
+            assert abs(ev[1] - e1) < 1e-12 * abs(e1)
+            assert np.all(np.diff(ev) > 1e-3)
+    assert abs(O.ring24_fixture(13, 1)["eigenvalues"][0] - e1) < 1e-12 * abs(e1)
+
+
 def test_lanczos_three_term_relation():
     # alpha/beta from the recurrence reproduce H's spectrum extremes and the
     # first step's alpha is the Rayleigh quotient of the start vector
