@@ -74,6 +74,7 @@ def lib():
             "alps_ed_vector_flip_pairs": ([P, P, i32, P, P, P], i32),
             "alps_ed_vector_bloch_expand": ([P, P, P, P], i32),
             "alps_ed_lanczos_step_vectors": ([P, P, P, P, f64, P, P], i32),
+            "alps_ed_lanczos_step_vectors_noalloc": ([P, P, P, P, f64, P, P], i32),
             "alps_ed_lanczos_create": ([P, i32, i32, i32, P], i32),
             "alps_ed_lanczos_ritz": ([P, i32, i32, P, P], i32),
             "alps_ed_lanczos_destroy": ([P], i32),
@@ -374,10 +375,12 @@ class Vector:
         return p.value
 
 
-def lanczos_step_vectors(op, v_prev, v_cur, w, beta_prev):
+def lanczos_step_vectors(op, v_prev, v_cur, w, beta_prev, noalloc=False):
+    """One make_step on caller vectors; noalloc=True uses the entry point that
+    allocates nothing (static device scratch)."""
     a, b = ctypes.c_double(), ctypes.c_double()
-    _check(lib().alps_ed_lanczos_step_vectors(op.handle, v_prev.handle, v_cur.handle, w.handle,
-                                              beta_prev, ctypes.byref(a), ctypes.byref(b)))
+    fn = lib().alps_ed_lanczos_step_vectors_noalloc if noalloc else lib().alps_ed_lanczos_step_vectors
+    _check(fn(op.handle, v_prev.handle, v_cur.handle, w.handle, beta_prev, ctypes.byref(a), ctypes.byref(b)))
     return a.value, b.value
 
 

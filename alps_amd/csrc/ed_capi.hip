@@ -770,6 +770,81 @@ int alps_ed_lanczos_step_vectors(alps_ed_operator op, alps_ed_vector v_prev, alp
   });
 }
 
+}  // extern "C"
+
+// Scratch of the allocation-free step: the H.v dot partials (one per H.v
+// block), the norm partials of the update (one per streaming block) and the
+// scalar results, 2 MiB + 8 KiB + 128 B.  Each GPU holds its own copy for the
+// life of the process.
+constexpr size_t kStaticHvPartials = size_t(1) << 18;
+constexpr size_t kStaticNormPartials = 1024;
+constexpr size_t kStaticResults = 16;
+static_assert(kStaticNormPartials >= 1024, "stream_blocks() is at most 1024");
+static __device__ __align__(256) double scratch[kStaticHvPartials + kStaticNormPartials + kStaticResults];
+
+namespace {
+
+// The device address of `scratch` on the current device, looked up once per
+// device (the array's host-side name is not a device address).
+double* static_scratch() {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static double* addr[kMaxDevices] = {};
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  require(dev >= 0 && dev < kMaxDevices, "device ordinal beyond the static scratch table");
+  std::lock_guard<std::mutex> g(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    addr[dev] = static_cast<double*>(p);
+  }
+  return addr[dev];
+}
+
+}  // namespace
+
+extern "C" {
+
+// alps_ed_lanczos_step_vectors with the reductions in the library's static
+// device scratch: the same kernels in the same order, no device or host
+// allocation.
+int alps_ed_lanczos_step_vectors_noalloc(alps_ed_operator op, alps_ed_vector v_prev, alps_ed_vector v_cur,
+                                         alps_ed_vector w, double beta_prev, double* alpha, double* beta) {
+  return guarded([&] {
+    require(op && alpha && beta, "null argument");
+    require(op_hv_blocks(op) <= kStaticHvPartials,
+            "operator too large for the static scratch (more than 262144 H.v blocks)");
+    check_vec(v_prev);
+    check_vec(v_cur);
+    check_vec(w);
+    const uint64_t n = op->n;
+    require(v_prev->n == n && v_cur->n == n && w->n == n, "vector size != operator dimension");
+    require(v_prev->dtype == v_cur->dtype && v_cur->dtype == w->dtype, "dtype mismatch");
+    double* hv_part = static_scratch();
+    double* norm_part = hv_part + kStaticHvPartials;
+    double* res = norm_part + kStaticNormPartials;
+    hipStream_t st = current_stream();
+    // beta_prev enters through a one-element fill kernel, not a host copy
+    hip_check(launch_fill<double>(1, res + 2, beta_prev, st), "fill");
+    with_dtype(w->dtype, [&](auto z) {
+      using V = decltype(z);
+      V* wv = w->buf.as<V>();
+      hip_check(op_hv<V>(op, v_cur->buf.as<V>(), wv, hv_part, st), "hv");
+      hip_check(launch_finish(Partials{hv_part, op_hv_blocks(op)}, nullptr, res, nullptr, kAlpha, st), "finish");
+      hip_check(launch_update<V>(n, wv, v_cur->buf.as<V>(), v_prev->buf.as<V>(), wv, res, nullptr, res + 2, nullptr,
+                                 norm_part, st),
+                "update");
+      hip_check(launch_normalize<V>(n, wv, Partials{norm_part, stream_blocks(n)}, res + 1, st), "normalize");
+    });
+    double h[2];
+    hip_check(hipMemcpyAsync(h, res, 2 * sizeof(double), hipMemcpyDeviceToHost, st), "copy");
+    hip_check(hipStreamSynchronize(st), "sync");
+    *alpha = h[0];
+    *beta = h[1];
+  });
+}
+
 
 // ---------------------------------------------------------------------------
 // device-resident Lanczos

@@ -190,6 +190,18 @@ int alps_ed_vector_flip_pairs(alps_ed_operator op, alps_ed_vector v, int n_pairs
 int alps_ed_lanczos_step_vectors(alps_ed_operator op, alps_ed_vector v_prev, alps_ed_vector v_cur,
                                  alps_ed_vector w, double beta_prev, double* alpha, double* beta);
 
+/* alps_ed_lanczos_step_vectors without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same kernels, same
+ * results bit for bit; the reduction partials and scalars live in a static
+ * device array of 2 MiB + 8 KiB + 128 B per GPU, held until the process ends.
+ * Operators whose H.v has more than 262144 blocks (segments, or blocks of 256
+ * momentum-sector rows) are rejected with ALPS_ED_ERR_ARGUMENT.  Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU (several
+ * host threads or streams) race and must be serialized by the caller. */
+int alps_ed_lanczos_step_vectors_noalloc(alps_ed_operator op, alps_ed_vector v_prev, alps_ed_vector v_cur,
+                                         alps_ed_vector w, double beta_prev, double* alpha, double* beta);
+
 /* Lanczos recurrence with the Krylov state resident on the device
  * (ietl::lanczos::make_first_step / make_step / generate_tmatrix,
  * src/ietl/lanczos.h:341-406).  Storage dtype applies to the three Lanczos
