@@ -78,6 +78,8 @@ def lib():
             "alps_ed_lanczos_create": ([P, i32, i32, i32, P], i32),
             "alps_ed_lanczos_ritz": ([P, i32, i32, P, P], i32),
             "alps_ed_lanczos_destroy": ([P], i32),
+            "alps_ed_lanczos_set_reorthogonalization": ([P, i32], i32),
+            "alps_ed_lanczos_reorthogonalizations": ([P, P], i32),
             "alps_ed_lanczos_start": ([P, u64, P], i32),
             "alps_ed_lanczos_steps": ([P, i32], i32),
             "alps_ed_lanczos_coefficients": ([P, P, P, i32, P], i32),
@@ -130,7 +132,8 @@ class _SolveParams(ctypes.Structure):
 class _SolveResult(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_uint64), ("n_eigenvalues", ctypes.c_int),
                 ("eigenvalues", ctypes.c_double * 64), ("iterations", ctypes.c_int),
-                ("error_code", ctypes.c_int), ("seconds", ctypes.c_double)]
+                ("error_code", ctypes.c_int), ("seconds", ctypes.c_double), ("storage", ctypes.c_int),
+                ("reorthogonalizations", ctypes.c_int)]
 
 
 class _BatchConfig(ctypes.Structure):
@@ -149,9 +152,13 @@ class _BatchStatus(ctypes.Structure):
                 ("graph_captures", ctypes.c_uint64)]
 
 
+_STORAGE = {0: "f32", 1: "f64"}
+
+
 def _result_dict(r):
     return dict(dim=r.dim, ev=np.array(r.eigenvalues[: r.n_eigenvalues]), iterations=r.iterations,
-                error_code=r.error_code, seconds=r.seconds)
+                error_code=r.error_code, seconds=r.seconds, storage=_STORAGE.get(r.storage),
+                reorthogonalizations=r.reorthogonalizations)
 
 
 @dataclass
@@ -387,12 +394,20 @@ def lanczos_step_vectors(op, v_prev, v_cur, w, beta_prev, noalloc=False):
 class Lanczos:
     """Device-resident Lanczos recurrence (alps_ed_lanczos)."""
 
-    def __init__(self, op: Operator, dtype="f32", max_steps=1000, keep_basis=False):
+    def __init__(self, op: Operator, dtype="f32", max_steps=1000, keep_basis=False, reorth=False):
+        """reorth=True: partial reorthogonalization (needs keep_basis)."""
         self.op = op
         self.max_steps = max_steps
         h = ctypes.c_void_p()
         _check(lib().alps_ed_lanczos_create(op.handle, _DTYPES[dtype], max_steps, int(keep_basis), ctypes.byref(h)))
         self.handle = h
+        if reorth:
+            _check(lib().alps_ed_lanczos_set_reorthogonalization(self.handle, 1))
+
+    def reorthogonalizations(self):
+        c = ctypes.c_int()
+        _check(lib().alps_ed_lanczos_reorthogonalizations(self.handle, ctypes.byref(c)))
+        return c.value
 
     def close(self):
         if getattr(self, "handle", None):
@@ -452,8 +467,7 @@ def sparsediag(model: SpinModel, max_iter=0, nev=1, seed=0, dtype="f64", fixed=F
         nt, perms, re, im = _translation_args(translations, model.n_sites)
         _check(lib().alps_ed_sparsediag_momentum(ctypes.byref(m), nt, _p(perms), _p(re), _p(im), ctypes.byref(p),
                                                  ctypes.byref(r)))
-    return dict(dim=r.dim, ev=np.array(r.eigenvalues[: r.n_eigenvalues]), iterations=r.iterations,
-                error_code=r.error_code, seconds=r.seconds)
+    return _result_dict(r)
 
 
 def _json_call(fn, text):

@@ -493,6 +493,8 @@ void finish_problem(alps_ed_batch b, Problem& p) {
   r.iterations = p.final_n < 0 ? 0 : p.final_n;
   r.error_code = p.error_code;
   r.seconds = p.t_done - p.t_admit;
+  r.storage = p.fp32 ? ALPS_ED_F32 : ALPS_ED_F64;
+  r.reorthogonalizations = 0;
   b->results[p.ticket] = r;
   b->completed.push_back(p.ticket);
   ++b->solves_done;

@@ -108,14 +108,36 @@ struct alps_ed_lanczos_s {
   DevBuf alpha, beta, scale;  // FP64 on device
   Reducer red;
   hipStream_t stream = nullptr;
+  // Partial reorthogonalization (ReorthDev, ed_kernels.hpp) of a kept
+  // basis: the omega rows, step flags, Gram-Schmidt coefficients and
+  // partials, and the device table of the kept vectors' addresses (mirrored
+  // on the host; the mirror is reserved once, so queued copies from it stay
+  // valid).
+  bool reorth = false;
+  DevBuf r_omega, r_flag, r_gamma, r_dot, r_hnorm, r_slots;
+  std::vector<void*> slot_host;
+  ReorthDev rdev{};
 
   // slots are padded to 64 elements, so every slot is 256-byte aligned
   size_t vec_bytes() const { return ((op->n + 63) & ~uint64_t(63)) * (dtype == ALPS_ED_F32 ? 4 : 8); }
   void* slot(int j) {
     if (!keep_basis) return u[j & 1].p;
     const size_t c = (size_t)j / kChunk;
-    while (chunks.size() <= c) chunks.emplace_back(new DevBuf(vec_bytes() * kChunk));
+    while (chunks.size() <= c) {
+      chunks.emplace_back(new DevBuf(vec_bytes() * kChunk));
+      if (reorth) publish_chunk(chunks.size() - 1);
+    }
     return static_cast<char*>(chunks[c]->p) + (size_t)(j % kChunk) * vec_bytes();
+  }
+  // the addresses of chunk c's vectors into the device slot table
+  void publish_chunk(size_t c) {
+    const size_t first = c * kChunk;
+    if (first >= slot_host.size()) return;
+    const size_t cnt = std::min<size_t>(kChunk, slot_host.size() - first);
+    for (size_t i = 0; i < cnt; ++i) slot_host[first + i] = static_cast<char*>(chunks[c]->p) + i * vec_bytes();
+    hip_check(hipMemcpyAsync(static_cast<void**>(r_slots.p) + first, slot_host.data() + first, cnt * sizeof(void*),
+                             hipMemcpyHostToDevice, stream),
+              "slot table");
   }
 };
 
@@ -318,6 +340,14 @@ void lanczos_step_dev(alps_ed_lanczos lz, int j) {
                                    norm_buf(lz, j + 1), lz->stream),
             "update");
   lz->norm_count[(j + 1) & 1] = fused_blocks(lz->op->n);
+  if (lz->reorth) {
+    // omega estimates of step j and, when they call for it, the Gram-Schmidt
+    // pass on u_{j+1}, which replaces the norm partials of the update
+    hip_check(launch_omega(j, alpha, beta, Partials{norm_buf(lz, j + 1), lz->norm_count[(j + 1) & 1]}, lz->rdev,
+                           lz->stream),
+              "omega");
+    hip_check(launch_reorth<V>(lz->op->n, j, next, scale, lz->rdev, norm_buf(lz, j + 1), lz->stream), "reorth");
+  }
 }
 
 // beta_{n-1} and scale_n after n steps (the host reads beta).
@@ -881,6 +911,51 @@ int alps_ed_lanczos_destroy(alps_ed_lanczos lz) {
   return ALPS_ED_OK;
 }
 
+int alps_ed_lanczos_set_reorthogonalization(alps_ed_lanczos lz, int mode) {
+  return guarded([&] {
+    require(lz != nullptr, "null argument");
+    require(mode == 0 || mode == 1, "reorthogonalization mode must be 0 (none) or 1 (partial)");
+    if (lz->steps != 0) throw StateError("the reorthogonalization mode is set before lanczos_start");
+    if (mode == 0) {
+      lz->reorth = false;
+      return;
+    }
+    if (!lz->keep_basis) throw StateError("partial reorthogonalization needs a Lanczos object created with keep_basis");
+    // the Gram-Schmidt update stages one coefficient per kept vector in LDS
+    require(lz->max_steps <= 4096, "partial reorthogonalization supports at most 4096 steps");
+    const uint32_t stride = (uint32_t)lz->max_steps + 2;
+    lz->r_omega.alloc(3 * (size_t)stride * sizeof(double));
+    lz->r_flag.alloc(stride * sizeof(int));
+    lz->r_gamma.alloc(stride * sizeof(double));
+    lz->r_dot.alloc((size_t)kReorthDotBlocks * stride * sizeof(double));
+    lz->r_hnorm.alloc(sizeof(double));
+    lz->r_slots.alloc(stride * sizeof(void*));
+    hip_check(hipMemsetAsync(lz->r_flag.p, 0, stride * sizeof(int), current_stream()), "memset");
+    lz->slot_host.assign(stride, nullptr);
+    lz->rdev = ReorthDev{lz->r_omega.as<double>(), stride, lz->r_flag.as<int>(), lz->r_gamma.as<double>(),
+                         lz->r_dot.as<double>(), lz->r_hnorm.as<double>(),
+                         static_cast<const void* const*>(lz->r_slots.p),
+                         lz->dtype == ALPS_ED_F32 ? 0.5 * std::numeric_limits<float>::epsilon()
+                                                  : 0.5 * std::numeric_limits<double>::epsilon()};
+    lz->reorth = true;
+    lz->stream = current_stream();
+    for (size_t c = 0; c < lz->chunks.size(); ++c) lz->publish_chunk(c);
+  });
+}
+
+int alps_ed_lanczos_reorthogonalizations(alps_ed_lanczos lz, int* count) {
+  return guarded([&] {
+    require(lz && count, "null argument");
+    *count = 0;
+    if (!lz->reorth || lz->steps == 0) return;
+    std::vector<int> f(lz->steps);
+    hip_check(hipMemcpyAsync(f.data(), lz->r_flag.p, f.size() * sizeof(int), hipMemcpyDeviceToHost, lz->stream),
+              "copy");
+    hip_check(hipStreamSynchronize(lz->stream), "sync");
+    for (int v : f) *count += v != 0;
+  });
+}
+
 int alps_ed_lanczos_start(alps_ed_lanczos lz, uint64_t seed, alps_ed_vector start) {
   return guarded([&] {
     require(lz != nullptr, "null argument");
@@ -1160,7 +1235,8 @@ SectorSolve solve_sector(const alps_ed_model* model, const alps_ed_solve_params*
     result->seconds = seconds();
     return out;
   }
-  const bool fp32 = use_fp32(params->dtype == ALPS_ED_F32, D, params->nev > 0 ? params->nev : 1);
+  // FP32 storage with partial reorthogonalization for any number of levels
+  const bool fp32 = use_fp32_reorth(params->dtype == ALPS_ED_F32, D);
   if (fp32) tune_schedule(op);
   const int max_iter = params->max_iter > 0 ? params->max_iter : (int)std::min<uint64_t>(10 * D, 1000);
   const int nev = params->nev > 0 ? params->nev : 1;
@@ -1169,6 +1245,7 @@ SectorSolve solve_sector(const alps_ed_model* model, const alps_ed_solve_params*
   check_status(
       alps_ed_lanczos_create(op, fp32 ? ALPS_ED_F32 : ALPS_ED_F64, max_iter, (fp32 || want_vectors) ? 1 : 0, &lzh));
   std::unique_ptr<alps_ed_lanczos_s> lz(lzh);
+  if (fp32) check_status(alps_ed_lanczos_set_reorthogonalization(lz.get(), 1));
   check_status(alps_ed_lanczos_start(lz.get(), params->seed, nullptr));
   std::vector<double> a(max_iter + 1), b(max_iter + 1);
   int n = 1, checked = 1, final_n = -1, error_code = 0;
@@ -1232,6 +1309,8 @@ SectorSolve solve_sector(const alps_ed_model* model, const alps_ed_solve_params*
   for (size_t i = 0; i < ev.size(); ++i) result->eigenvalues[i] = ev[i];
   result->iterations = final_n;
   result->error_code = error_code;
+  result->storage = fp32 ? ALPS_ED_F32 : ALPS_ED_F64;
+  check_status(alps_ed_lanczos_reorthogonalizations(lz.get(), &result->reorthogonalizations));
   result->seconds = seconds();
   return out;
 }

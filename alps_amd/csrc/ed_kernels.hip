@@ -153,6 +153,193 @@ __global__ __launch_bounds__(kBlock) void update_fused_kernel(uint64_t n, const 
   store_partial(norm_out, acc);
 }
 
+// ---------------------------------------------------------------------------
+// Partial reorthogonalization (ReorthDev, ed_kernels.hpp)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double block_max_all(double v) {
+  __shared__ double max_smem[kBlock / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) max_smem[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double m = max_smem[0];
+#pragma unroll
+  for (int w = 1; w < kBlock / 64; ++w) m = fmax(m, max_smem[w]);
+  return m;
+}
+
+// One block.  With the update of step j done (alpha_j, beta_{j-1} stored,
+// r = u_{j+1} with norm partials norm_r), the omega recurrence gives row j+1
+// from rows j and j-1: for k < j
+//   beta_j w_{j+1,k} = beta_k w_{j,k+1} + (alpha_k - alpha_j) w_{j,k}
+//                      + beta_{k-1} w_{j,k-1} - beta_{j-1} w_{j-1,k} + theta,
+// which follows from the Lanczos relation of steps j and k and the symmetry
+// of H; theta bounds the rounding terms u_k.f_j - u_j.f_k by 2 eps |H| with
+// the sign of the rest (a deterministic worst case), |H| estimated by the
+// largest |alpha_i| + beta_i + beta_{i-1} so far.  w_{j+1,j} = 2 eps |H| /
+// beta_j (the local loss), w_{j+1,j+1} = 1.  If the largest |w_{j+1,k}|
+// exceeds sqrt(eps), or the previous step crossed it, the step's flag is set
+// and the row is reset to eps (the Gram-Schmidt pass that follows makes
+// u_{j+1} orthogonal to working accuracy).
+__global__ __launch_bounds__(kBlock) void omega_kernel(int j, const double* __restrict__ alpha,
+                                                       const double* __restrict__ beta, Partials norm_r,
+                                                       ReorthDev R) {
+  const double bj = sqrt(sum_partials(norm_r));
+  const double aj = alpha[j];
+  const double bjm1 = j > 0 ? beta[j - 1] : 0.0;
+  const double hn = fmax(j > 0 ? *R.hnorm : 0.0, fabs(aj) + bj + bjm1);
+  const double eps = R.eps;
+  const double noise = 2.0 * eps * hn;
+  double* w1 = R.omega + (size_t)((j + 1) % 3) * R.stride;
+  const double* w0 = R.omega + (size_t)(j % 3) * R.stride;
+  const double* wm = R.omega + (size_t)((j + 2) % 3) * R.stride;  // row j-1 (unused at j = 0)
+  const bool ok = bj > 0.0 && isfinite(bj);
+  double mx = 0.0;
+  if (ok)
+    for (int k = threadIdx.x; k < j; k += kBlock) {
+      double t = beta[k] * w0[k + 1] + (alpha[k] - aj) * w0[k] - bjm1 * wm[k];
+      if (k > 0) t += beta[k - 1] * w0[k - 1];
+      const double v = (t + copysign(noise, t)) / bj;
+      w1[k] = v;
+      mx = fmax(mx, fabs(v));
+    }
+  mx = block_max_all(mx);
+  const int prev = j > 0 ? R.flag[j - 1] : 0;
+  const int flag = !ok ? 0 : (prev == 1 ? 2 : (mx > sqrt(eps) ? 1 : 0));
+  if (flag)
+    for (int k = threadIdx.x; k < j; k += kBlock) w1[k] = eps;
+  if (threadIdx.x == 0) {
+    if (j == 0) R.omega[0] = 1.0;  // row 0: v_0.v_0
+    w1[j] = ok ? noise / bj : eps;
+    w1[j + 1] = 1.0;
+    R.flag[j] = flag;
+    *R.hnorm = hn;
+  }
+}
+
+// Partials of u_k . r for k = 0..j: block b owns a contiguous range of the
+// four-element groups of the vectors and writes dot_part[k * G + b]; the
+// vectors are read eight at a time with r (one read of r per eight vectors).
+template <class V>
+__global__ __launch_bounds__(kBlock) void reorth_dot_kernel(uint64_t n, int j, const V* __restrict__ r,
+                                                            ReorthDev R) {
+  if (R.flag[j] == 0) return;
+  typedef V v4 __attribute__((ext_vector_type(4), aligned(sizeof(V))));
+  const V* const* slots = reinterpret_cast<const V* const*>(R.slots);
+  const uint64_t n4 = n / 4;
+  const uint64_t per = (n4 + gridDim.x - 1) / gridDim.x;
+  const uint64_t q0 = (uint64_t)blockIdx.x * per;
+  const uint64_t q1 = q0 + per < n4 ? q0 + per : n4;
+  const int m = j + 1;
+  constexpr int KG = 8;
+  for (int k0 = 0; k0 < m; k0 += KG) {
+    const V* u[KG];
+#pragma unroll
+    for (int c = 0; c < KG; ++c) u[c] = slots[k0 + c < m ? k0 + c : k0];
+    double acc[KG];
+#pragma unroll
+    for (int c = 0; c < KG; ++c) acc[c] = 0.0;
+    for (uint64_t q = q0 + threadIdx.x; q < q1; q += kBlock) {
+      const v4 rv = *reinterpret_cast<const v4*>(r + 4 * q);
+#pragma unroll
+      for (int c = 0; c < KG; ++c) {
+        if (k0 + c < m) {
+          const v4 uv = *reinterpret_cast<const v4*>(u[c] + 4 * q);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) acc[c] += (double)uv[e] * (double)rv[e];
+        }
+      }
+    }
+    // the last n mod 4 elements belong to block 0
+    if (blockIdx.x == 0 && threadIdx.x < n - 4 * n4) {
+      const uint64_t i = 4 * n4 + threadIdx.x;
+#pragma unroll
+      for (int c = 0; c < KG; ++c)
+        if (k0 + c < m) acc[c] += (double)u[c][i] * (double)r[i];
+    }
+#pragma unroll
+    for (int c = 0; c < KG; c += 2) {
+      double a = acc[c], b = acc[c + 1];
+      block_sum_all2(a, b);
+      if (threadIdx.x == 0) {
+        if (k0 + c < m) R.dot_part[(size_t)(k0 + c) * gridDim.x + blockIdx.x] = a;
+        if (k0 + c + 1 < m) R.dot_part[(size_t)(k0 + c + 1) * gridDim.x + blockIdx.x] = b;
+      }
+    }
+  }
+}
+
+// gamma_k = scale_k^2 * sum_b dot_part[k * G + b] (in block order), one wave
+// per k.
+__global__ __launch_bounds__(kBlock) void reorth_coef_kernel(int j, int G, const double* __restrict__ scale,
+                                                             ReorthDev R) {
+  if (R.flag[j] == 0) return;
+  const int k = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (k > j) return;
+  double s = 0.0;
+  for (int b = lane; b < G; b += 64) s += R.dot_part[(size_t)k * G + b];
+  s = wave_sum(s);
+  if (lane == 0) R.gamma[k] = scale[k] * scale[k] * s;
+}
+
+// r -= sum_{k<=j} gamma_k u_k (FP64 per element, k ascending), stored in
+// place; per-block partials of |r|^2 into norm_out (one per block of the
+// grid, the same count as the update's).
+template <class V>
+__global__ __launch_bounds__(kBlock) void reorth_update_kernel(uint64_t n, int j, V* __restrict__ r, ReorthDev R,
+                                                               double* __restrict__ norm_out) {
+  if (R.flag[j] == 0) return;
+  extern __shared__ double gam[];
+  typedef V v4 __attribute__((ext_vector_type(4), aligned(sizeof(V))));
+  const V* const* slots = reinterpret_cast<const V* const*>(R.slots);
+  const int m = j + 1;
+  for (int k = threadIdx.x; k < m; k += kBlock) gam[k] = R.gamma[k];
+  __syncthreads();
+  double acc = 0.0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t n4 = n / 4;
+  for (uint64_t q = gid; q < n4; q += stride) {
+    const v4 rv = *reinterpret_cast<const v4*>(r + 4 * q);
+    double o[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) o[e] = (double)rv[e];
+    int k = 0;
+    for (; k + 4 <= m; k += 4) {
+      v4 uv[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) uv[c] = *reinterpret_cast<const v4*>(slots[k + c] + 4 * q);
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) o[e] -= gam[k + c] * (double)uv[c][e];
+    }
+    for (; k < m; ++k) {
+      const v4 uv = *reinterpret_cast<const v4*>(slots[k] + 4 * q);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) o[e] -= gam[k] * (double)uv[e];
+    }
+    v4 ov;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      ov[e] = (V)o[e];
+      const double rd = (double)ov[e];
+      acc += rd * rd;
+    }
+    *reinterpret_cast<v4*>(r + 4 * q) = ov;
+  }
+  for (uint64_t i = 4 * n4 + gid; i < n; i += stride) {
+    double o = (double)r[i];
+    for (int k = 0; k < m; ++k) o -= gam[k] * (double)slots[k][i];
+    const V ov = (V)o;
+    r[i] = ov;
+    acc += (double)ov * (double)ov;
+  }
+  store_partial(norm_out, acc);
+}
+
 template <class V>
 __global__ __launch_bounds__(kBlock) void dot_kernel(uint64_t n, const V* __restrict__ x,
                                                      const V* __restrict__ y, double* partials) {
@@ -347,6 +534,26 @@ hipError_t launch_update_fused(uint64_t n, const V* w, const V* u_cur, const V* 
   return hipGetLastError();
 }
 
+hipError_t launch_omega(int j, const double* alpha, const double* beta, Partials norm_r, const ReorthDev& R,
+                        hipStream_t st) {
+  if (j < 0 || (uint32_t)j + 2 > R.stride) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(omega_kernel, dim3(1), dim3(kBlock), 0, st, j, alpha, beta, norm_r, R);
+  return hipGetLastError();
+}
+
+template <class V>
+hipError_t launch_reorth(uint64_t n, int j, V* r, const double* scale, const ReorthDev& R, double* norm_out,
+                         hipStream_t st) {
+  if (j < 0 || (uint32_t)j + 2 > R.stride) return hipErrorInvalidValue;
+  const int G = kReorthDotBlocks;
+  hipLaunchKernelGGL((reorth_dot_kernel<V>), dim3(G), dim3(kBlock), 0, st, n, j, r, R);
+  const unsigned cb = (unsigned)((j + 1 + kBlock / 64 - 1) / (kBlock / 64));
+  hipLaunchKernelGGL(reorth_coef_kernel, dim3(cb), dim3(kBlock), 0, st, j, G, scale, R);
+  hipLaunchKernelGGL((reorth_update_kernel<V>), dim3(fused_blocks(n)), dim3(kBlock), (size_t)(j + 1) * sizeof(double),
+                     st, n, j, r, R, norm_out);
+  return hipGetLastError();
+}
+
 template <class V>
 hipError_t launch_dot(uint64_t n, const V* x, const V* y, double* partials, hipStream_t st) {
   hipLaunchKernelGGL((dot_kernel<V>), dim3(stream_blocks(n)), dim3(kBlock), 0, st, n, x, y, partials);
@@ -437,7 +644,8 @@ hipError_t launch_elements(const DevLayout& L, uint64_t row0, uint64_t nrows, in
   template hipError_t launch_fill<V>(uint64_t, V*, double, hipStream_t);                                    \
   template hipError_t launch_axpby<V>(uint64_t, V*, double, const V*, double, hipStream_t);                 \
   template hipError_t launch_ritz<V>(uint64_t, const V* const*, const double*, int, int, const double*,    \
-                                     double* const*, hipStream_t);
+                                     double* const*, hipStream_t);                                          \
+  template hipError_t launch_reorth<V>(uint64_t, int, V*, const double*, const ReorthDev&, double*, hipStream_t);
 
 ALPS_ED_INSTANTIATE(float)
 ALPS_ED_INSTANTIATE(double)

@@ -77,15 +77,18 @@ struct HvWaves {
   static constexpr int value = sizeof(V) == 4 ? 8 : 1;
 };
 // FP32 storage of the Lanczos vectors is used, when asked for, only where it
-// pays: sectors of at least kFp32MinDim states and at most kFp32MaxNev
-// requested eigenvalues.  In small sectors the Krylov space is a large part
-// of the sector; for more eigenvalues, the copies of converged levels that
-// FP32 rounding creates early delay the higher levels (about twice the FP64
-// iteration count for four levels), which costs more than the halved bytes
-// save.  Elsewhere the vectors are stored in FP64.
+// pays: sectors of at least kFp32MinDim states (in small sectors the Krylov
+// space is a large part of the sector).  Without reorthogonalization (the
+// batch) also at most kFp32MaxNev requested eigenvalues: for more, the copies
+// of converged levels that FP32 rounding creates early delay the higher
+// levels (about twice the FP64 iteration count for four levels), which costs
+// more than the halved bytes save.  Single solves reorthogonalize FP32
+// recurrences partially (ReorthDev), which keeps the copies away, so they
+// store any number of levels in FP32.  Elsewhere the vectors are FP64.
 constexpr uint64_t kFp32MinDim = 1000;
 constexpr int kFp32MaxNev = 2;
 inline bool use_fp32(bool asked, uint64_t dim, int nev) { return asked && dim >= kFp32MinDim && nev <= kFp32MaxNev; }
+inline bool use_fp32_reorth(bool asked, uint64_t dim) { return asked && dim >= kFp32MinDim; }
 constexpr int kMaxRitz = 8;     // Ritz vectors per pass of launch_ritz
 // H.v rows per thread: a segment of up to kRows*kBlock rows is one workgroup
 constexpr int kRows = 4;
@@ -150,6 +153,37 @@ template <class V>
 hipError_t launch_update_fused(uint64_t n, const V* w, const V* u_cur, const V* u_prev, V* out, Partials hv_part,
                                Partials norm_prev, double* alpha_j, double* beta_jm1, double* scale_j,
                                const double* scale_jm1, double* norm_out, hipStream_t st);
+
+// Partial reorthogonalization (Simon, "The Lanczos algorithm with partial
+// reorthogonalization", Math. Comp. 42 (1984) 115-142) of a recurrence whose
+// basis u_0..u_j is kept in HBM (v_k = scale_k u_k).  After the update of
+// step j has stored r = beta_j v_{j+1} as u_{j+1}, omega_kernel extends the
+// estimates omega_{j+1,k} ~ v_{j+1}.v_k by the omega recurrence in FP64 and
+// decides: reorthogonalize when max_k |omega_{j+1,k}| > sqrt(eps) (eps = unit
+// roundoff of the vector storage), and again at the next step (the pair
+// rule).  The three Gram-Schmidt kernels run only when the step's flag is
+// set (uniform early exit otherwise): per-block partials of u_k.r, the
+// coefficients gamma_k = scale_k^2 u_k.r, and r -= sum_k gamma_k u_k with new
+// norm partials of r in place of the update's.  Every sum is in a fixed
+// order, so the recurrence stays deterministic.
+struct ReorthDev {
+  double* omega;            // three rows of `stride`: row of step j at (j % 3) * stride
+  uint32_t stride;          // max_steps + 2
+  int* flag;                // per step: 0 none, 1 threshold crossed, 2 second of a pair
+  double* gamma;            // per kept vector: the Gram-Schmidt coefficient
+  double* dot_part;         // kReorthDotBlocks partials per kept vector
+  double* hnorm;            // running estimate of |T| (one double)
+  const void* const* slots; // device addresses of u_0, u_1, ...
+  double eps;               // unit roundoff of the vector storage
+};
+constexpr int kReorthDotBlocks = 256;
+hipError_t launch_omega(int j, const double* alpha, const double* beta, Partials norm_r, const ReorthDev& R,
+                        hipStream_t st);
+// The Gram-Schmidt pass of step j on r = u_{j+1} (three launches, each a
+// no-op unless R.flag[j] is set); norm_out receives fused_blocks(n) partials.
+template <class V>
+hipError_t launch_reorth(uint64_t n, int j, V* r, const double* scale, const ReorthDev& R, double* norm_out,
+                         hipStream_t st);
 
 // per-block partials of sum x_i y_i
 template <class V>

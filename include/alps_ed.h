@@ -212,6 +212,17 @@ int alps_ed_lanczos_step_vectors_noalloc(alps_ed_operator op, alps_ed_vector v_p
 int alps_ed_lanczos_create(alps_ed_operator op, alps_ed_dtype dtype, int max_steps, int keep_basis,
                            alps_ed_lanczos* lz);
 int alps_ed_lanczos_destroy(alps_ed_lanczos lz);
+/* Reorthogonalization of the recurrence, set before alps_ed_lanczos_start.
+ * mode 0: none, the reference's recurrence (make_step, lanczos.h:392-406,
+ * reorthogonalizes nothing).  mode 1: partial reorthogonalization (Simon
+ * 1984) for FP32 storage, which needs keep_basis: each step extends the
+ * omega estimates of v_{j+1}.v_k in FP64 on the device, and when the largest
+ * exceeds sqrt(eps) (eps the storage's unit roundoff) the new vector and the
+ * next one are orthogonalized against the kept basis by one Gram-Schmidt
+ * pass.  Steps stay free of host synchronization. */
+int alps_ed_lanczos_set_reorthogonalization(alps_ed_lanczos lz, int mode);
+/* Number of steps whose new vector was reorthogonalized (synchronizes). */
+int alps_ed_lanczos_reorthogonalizations(alps_ed_lanczos lz, int* count);
 /* make_first_step with the start vector given by fill_random(seed, -1, 1),
  * or by `start` when it is not NULL (must have the operator's dimension). */
 int alps_ed_lanczos_start(alps_ed_lanczos lz, uint64_t seed, alps_ed_vector start);
@@ -270,6 +281,11 @@ typedef struct {
   int iterations;
   int error_code;         /* 1 when max_iter was reached (iteration.h:80) */
   double seconds;         /* device wall time of the solve */
+  alps_ed_dtype storage;  /* storage of the Lanczos vectors actually used: an FP32 request is served
+                           * in FP64 for sectors of fewer than 1000 states (and, in a batch, for
+                           * more than two requested levels) */
+  int reorthogonalizations; /* Lanczos steps whose new vector was reorthogonalized (partial
+                             * reorthogonalization, used with FP32 storage in single solves) */
 } alps_ed_solve_result;
 
 int alps_ed_sparsediag(const alps_ed_model* model, const alps_ed_solve_params* params,

@@ -232,17 +232,19 @@ def test_sparsediag_l24_fp32_and_fp64_vs_oracle(nev):
 
 
 def test_sparsediag_l24_four_levels():
-    """The lowest four levels of the L=24 ring against the oracle's FP64
+    """The lowest four levels of the L=24 ring, FP64 and FP32 storage (the
+    latter with partial reorthogonalization), against the oracle's FP64
     sparsediag (1e-10), and SU(2): the lowest Sz=1 level (also from the
     oracle) is one of them."""
     ref = np.array(O.ring24_fixture(12, 4)["eigenvalues"])
     trip = O.ring24_fixture(13, 1)["eigenvalues"][0]
     assert np.min(np.abs(ref - trip)) < 1e-9 * abs(trip)
     m = to_product(O.lattice_model("chain lattice", 24), 12)
-    r64 = A.sparsediag(m, nev=4, seed=1, dtype="f64")
-    say("f64", r64["ev"].tolist(), r64["iterations"], r64["seconds"])
-    assert len(r64["ev"]) == 4
-    assert (np.abs(r64["ev"] - ref) / np.abs(ref)).max() <= 1e-10
+    for dtype in ("f64", "f32"):
+        r = A.sparsediag(m, nev=4, seed=1, dtype=dtype)
+        say(dtype, r["ev"].tolist(), r["iterations"], r["storage"], r["reorthogonalizations"], r["seconds"])
+        assert len(r["ev"]) == 4 and r["storage"] == dtype
+        assert (np.abs(r["ev"] - ref) / np.abs(ref)).max() <= 1e-10
     t = A.sparsediag(to_product(O.lattice_model("chain lattice", 24), 13), nev=1, seed=2, dtype="f64")
     assert abs(t["ev"][0] - trip) <= 1e-10 * abs(trip)
 
