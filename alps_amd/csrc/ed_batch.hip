@@ -509,22 +509,35 @@ void finish_problem(alps_ed_batch b, Problem& p) {
 // the lowest value gates the full T-matrix analysis.
 void decide(Problem& p) {
   const int n = p.j;
+  // A zero beta_j ends the recurrence: T[0..j+1) is exact and later
+  // coefficients would divide by it.  Non-finite coefficients (NaN or Inf in
+  // the couplings) stop the problem with error code -2 and no eigenvalues.
+  for (int j = p.tested_j; j < n; ++j) {
+    if (!std::isfinite(p.host_alpha[j]) || !std::isfinite(p.host_beta[j])) {
+      p.final_n = 0;
+      p.error_code = -2;
+      return;
+    }
+    if (p.host_beta[j] == 0.0) {
+      p.final_n = j + 1;
+      return;
+    }
+  }
   if (p.fixed) {
     if (n >= p.max_iter) p.final_n = p.max_iter;
     return;
   }
   if (n >= 2) {
-    const double tol = stop_tolerance(p.fp32);
     double lam = 0.0, lo = 0.0, hi = -1.0;
     if (p.lam_prev == p.lam_prev) {  // bracket from the previous test
       const double drop = std::max(p.lam_drop, 64 * std::numeric_limits<double>::epsilon() * std::fabs(p.lam_prev));
       hi = p.lam_prev + 4 * std::numeric_limits<double>::epsilon() * std::fabs(p.lam_prev);
       lo = p.lam_prev - 4 * drop;
     }
-    const double e = lowest_ritz_error(p.host_alpha, p.host_beta, n, &lam, lo, hi);
+    const bool may = may_be_converged(p.host_alpha, p.host_beta, n, p.fp32, &lam, lo, hi);
     if (p.lam_prev == p.lam_prev) p.lam_drop = std::max(0.0, p.lam_prev - lam);
     p.lam_prev = lam;
-    if (e <= 4.0 * std::max(tol, tol * std::fabs(lam))) {
+    if (may) {
       std::vector<double> a(p.host_alpha, p.host_alpha + n), bb(p.host_beta, p.host_beta + n);
       TValues t = analyze_prefix(a, bb, n, p.fp32);
       if (nlowest_converged(t, p.nev, p.fp32)) {
@@ -544,6 +557,11 @@ void decide(Problem& p) {
 // After the stopping rule: eigenvalues for FP64 lanes; for FP32 lanes the
 // candidates of the first refinement pass.
 void conclude(Problem& p) {
+  if (p.error_code == -2) {  // non-finite coefficients: nothing to report
+    p.ev.clear();
+    p.state = kDone;
+    return;
+  }
   if (p.tv_n != p.final_n) {
     std::vector<double> a(p.host_alpha, p.host_alpha + p.final_n), bb(p.host_beta, p.host_beta + p.final_n);
     p.tv = analyze_prefix(a, bb, p.final_n, p.fp32);

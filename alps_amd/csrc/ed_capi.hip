@@ -33,12 +33,6 @@ using namespace alps_ed::host;
 
 namespace {
 
-template <class T>
-void upload(DevBuf& b, const std::vector<T>& v) {
-  b.alloc(std::max<size_t>(v.size(), 1) * sizeof(T));
-  if (!v.empty()) hip_check(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
-}
-
 // Scratch for reductions: two partial-sum arrays (one per producer kind,
 // so a consumer never reads an array that is being rewritten) and result
 // slots.  Partial counts follow the producing kernel's grid.
@@ -74,9 +68,7 @@ Reducer& thread_reducer() {
 struct alps_ed_operator_s {
   SectorLayout layout;
   DevLayout dev{};
-  DevBuf lo_rank, lo_state, lo_class_off, hi_base, seg_hi, seg_start, proc_hi, diag_lo_cls, diag_hi, nbp16,
-      nbp_off, xn16, xn_off, ll_t, hh_mask, hh_t, xb_hbit, xb_par, xb_anti, xb_t, site_dn, site_up, bond_p1,
-      bond_p2, bond_par, bond_anti, bond_t;
+  DevBuf tables;  // every layout table, packed (operator_upload)
   uint64_t bytes = 0;
   uint64_t n = 0;  // vector length: D, or 2 * dk for a momentum sector (realified)
   int schedule = 0;  // H.v schedule for FP32 vectors (kHvSchedule)
@@ -84,7 +76,7 @@ struct alps_ed_operator_s {
   // momentum sector (ed_bloch.hpp): rows are the kept representatives
   bool bloch = false;
   DevBloch bdev{};
-  DevBuf b_rep_state, b_norm, b_full_pk, b_norm_tab, b_ph_re, b_ph_im;
+  DevBuf btables;  // the Bloch basis tables, packed (bloch_upload)
 };
 
 struct alps_ed_vector_s {
@@ -104,8 +96,15 @@ struct alps_ed_lanczos_s {
   static constexpr int kChunk = 32;
   DevBuf u[2], w;
   unsigned norm_count[2] = {0, 0};  // partials in each norm buffer
-  std::vector<std::unique_ptr<DevBuf>> chunks;
+  std::vector<std::unique_ptr<PooledBuf>> chunks;
   DevBuf alpha, beta, scale;  // FP64 on device
+  // host-mapped copies of alpha (first max_steps + 2) and beta (next
+  // max_steps + 2), written by the kernels that form them
+  MappedBuf mirror;
+  double* alpha_h(int j) const { return static_cast<double*>(mirror.d) + j; }
+  double* beta_h(int j) const { return static_cast<double*>(mirror.d) + max_steps + 2 + j; }
+  const double* host_alpha() const { return static_cast<const double*>(mirror.h); }
+  const double* host_beta() const { return static_cast<const double*>(mirror.h) + max_steps + 2; }
   Reducer red;
   hipStream_t stream = nullptr;
   // Partial reorthogonalization (ReorthDev, ed_kernels.hpp) of a kept
@@ -118,13 +117,20 @@ struct alps_ed_lanczos_s {
   std::vector<void*> slot_host;
   ReorthDev rdev{};
 
+  // Steps queued past what a caller reads (a solve's speculative chunk) may
+  // still run: they write the mapped mirror and the basis chunks, which go
+  // back to the host heap and the buffer pool, so wait for them first.
+  ~alps_ed_lanczos_s() {
+    if (steps > 0) (void)hipStreamSynchronize(stream);
+  }
+
   // slots are padded to 64 elements, so every slot is 256-byte aligned
   size_t vec_bytes() const { return ((op->n + 63) & ~uint64_t(63)) * (dtype == ALPS_ED_F32 ? 4 : 8); }
   void* slot(int j) {
     if (!keep_basis) return u[j & 1].p;
     const size_t c = (size_t)j / kChunk;
     while (chunks.size() <= c) {
-      chunks.emplace_back(new DevBuf(vec_bytes() * kChunk));
+      chunks.emplace_back(new PooledBuf(vec_bytes() * kChunk));
       if (reorth) publish_chunk(chunks.size() - 1);
     }
     return static_cast<char*>(chunks[c]->p) + (size_t)(j % kChunk) * vec_bytes();
@@ -216,6 +222,39 @@ void tune_schedule(alps_ed_operator op) {
   cache[key] = {best, sec};
 }
 
+// Host arrays packed into one device allocation with one copy: each array
+// starts at a 256-byte boundary; `add` records an array and returns where it
+// will live on the device once `upload` has run.
+class PackedUpload {
+ public:
+  template <class T>
+  const T** add(const std::vector<T>& v, const T** dst) {
+    items_.push_back({v.data(), v.size() * sizeof(T), off_, reinterpret_cast<const void**>(dst)});
+    off_ += (std::max<size_t>(v.size() * sizeof(T), 1) + 255) & ~size_t(255);
+    return dst;
+  }
+  // one allocation into `buf`, one host-to-device copy; fills the pointers
+  void upload(DevBuf& buf) {
+    buf.alloc(std::max<size_t>(off_, 256));
+    std::vector<char> stage(off_);
+    for (const Item& it : items_) {
+      if (it.bytes) std::memcpy(stage.data() + it.off, it.src, it.bytes);
+      *it.dst = static_cast<char*>(buf.p) + it.off;
+    }
+    if (off_) hip_check(hipMemcpy(buf.p, stage.data(), off_, hipMemcpyHostToDevice), "upload");
+  }
+  size_t bytes() const { return off_; }
+
+ private:
+  struct Item {
+    const void* src;
+    size_t bytes, off;
+    const void** dst;
+  };
+  std::vector<Item> items_;
+  size_t off_ = 0;
+};
+
 void operator_upload(alps_ed_operator op) {
   SectorLayout& L = op->layout;
   DevLayout& d = op->dev;
@@ -234,62 +273,57 @@ void operator_upload(alps_ed_operator op) {
   d.n_hh = (int)L.hh.size();
   d.n_xb = (int)L.xb.size();
   d.n_bonds = (int)L.bond_p1.size();
-#define ALPS_ED_UP(field, T) \
-  upload(op->field, L.field); \
-  d.field = op->field.as<T>();
-  ALPS_ED_UP(lo_rank, uint32_t)
-  ALPS_ED_UP(lo_state, uint32_t)
-  ALPS_ED_UP(lo_class_off, uint32_t)
-  ALPS_ED_UP(hi_base, uint32_t)
-  ALPS_ED_UP(seg_hi, uint32_t)
-  ALPS_ED_UP(seg_start, uint32_t)
-  ALPS_ED_UP(proc_hi, uint32_t)
-  ALPS_ED_UP(diag_lo_cls, double)
-  ALPS_ED_UP(diag_hi, double)
-  ALPS_ED_UP(nbp16, uint16_t)
-  ALPS_ED_UP(nbp_off, uint32_t)
-  ALPS_ED_UP(xn16, uint16_t)
-  ALPS_ED_UP(xn_off, uint32_t)
-  ALPS_ED_UP(ll_t, double)
-  ALPS_ED_UP(hh_mask, uint32_t)
-  ALPS_ED_UP(hh_t, double)
-  ALPS_ED_UP(xb_hbit, uint32_t)
-  ALPS_ED_UP(xb_par, double)
-  ALPS_ED_UP(xb_anti, double)
-  ALPS_ED_UP(xb_t, double)
-  ALPS_ED_UP(site_dn, double)
-  ALPS_ED_UP(site_up, double)
-  ALPS_ED_UP(bond_p1, uint32_t)
-  ALPS_ED_UP(bond_p2, uint32_t)
-  ALPS_ED_UP(bond_par, double)
-  ALPS_ED_UP(bond_anti, double)
-  ALPS_ED_UP(bond_t, double)
+  PackedUpload pk;
+#define ALPS_ED_UP(field) pk.add(L.field, &d.field);
+  ALPS_ED_UP(lo_rank)
+  ALPS_ED_UP(lo_state)
+  ALPS_ED_UP(lo_class_off)
+  ALPS_ED_UP(hi_base)
+  ALPS_ED_UP(seg_hi)
+  ALPS_ED_UP(seg_start)
+  ALPS_ED_UP(proc_hi)
+  ALPS_ED_UP(diag_lo_cls)
+  ALPS_ED_UP(diag_hi)
+  ALPS_ED_UP(nbp16)
+  ALPS_ED_UP(nbp_off)
+  ALPS_ED_UP(xn16)
+  ALPS_ED_UP(xn_off)
+  ALPS_ED_UP(ll_t)
+  ALPS_ED_UP(hh_mask)
+  ALPS_ED_UP(hh_t)
+  ALPS_ED_UP(xb_hbit)
+  ALPS_ED_UP(xb_par)
+  ALPS_ED_UP(xb_anti)
+  ALPS_ED_UP(xb_t)
+  ALPS_ED_UP(site_dn)
+  ALPS_ED_UP(site_up)
+  ALPS_ED_UP(bond_p1)
+  ALPS_ED_UP(bond_p2)
+  ALPS_ED_UP(bond_par)
+  ALPS_ED_UP(bond_anti)
+  ALPS_ED_UP(bond_t)
 #undef ALPS_ED_UP
-  op->bytes = 0;
-  for (DevBuf* b : {&op->lo_rank, &op->lo_state, &op->lo_class_off, &op->hi_base, &op->seg_hi, &op->seg_start,
-                    &op->proc_hi, &op->diag_lo_cls, &op->diag_hi, &op->nbp16, &op->nbp_off, &op->xn16, &op->xn_off,
-                    &op->ll_t, &op->hh_mask, &op->hh_t, &op->xb_hbit, &op->xb_par, &op->xb_anti, &op->xb_t,
-                    &op->site_dn, &op->site_up, &op->bond_p1, &op->bond_p2, &op->bond_par, &op->bond_anti,
-                    &op->bond_t})
-    op->bytes += b->bytes;
+  pk.upload(op->tables);
+  op->bytes = pk.bytes();
   op->n = L.D;
 }
 
 // Turns an uploaded operator into its momentum sector.
 void bloch_upload(alps_ed_operator op, const BlochSector& b) {
-  upload(op->b_rep_state, b.rep_state);
-  upload(op->b_norm, b.norm);
-  upload(op->b_full_pk, b.full_pk);
-  upload(op->b_norm_tab, b.norm_tab);
-  upload(op->b_ph_re, b.ph_re);
-  upload(op->b_ph_im, b.ph_im);
-  op->bdev = DevBloch{(uint32_t)b.rep_state.size(), op->b_rep_state.as<uint32_t>(), op->b_norm.as<double>(),
-                      op->b_full_pk.as<uint64_t>(), op->b_norm_tab.as<double>(), op->b_ph_re.as<double>(),
-                      op->b_ph_im.as<double>()};
+  DevBloch& d = op->bdev;
+  d = DevBloch{};
+  d.dk = (uint32_t)b.rep_state.size();
+  PackedUpload pk;
+  pk.add(b.rep_state, &d.rep_state);
+  pk.add(b.norm, &d.norm);
+  pk.add(b.full_pk, &d.full_pk);
+  pk.add(b.norm_tab, &d.norm_tab);
+  pk.add(b.ph_re, &d.ph_re);
+  pk.add(b.ph_im, &d.ph_im);
+  pk.upload(op->btables);
   op->bloch = true;
   op->n = 2 * (uint64_t)b.rep_state.size();
-  for (DevBuf* x : {&op->b_rep_state, &op->b_norm, &op->b_full_pk, &op->b_norm_tab, &op->b_ph_re, &op->b_ph_im})
-    op->bytes += x->bytes;
+  op->bytes += pk.bytes();
 }
 
 size_t dtype_size(alps_ed_dtype t) { return t == ALPS_ED_F32 ? 4 : 8; }
@@ -337,23 +371,19 @@ void lanczos_step_dev(alps_ed_lanczos lz, int j) {
   hip_check(launch_update_fused<V>(lz->op->n, w, cur, prev, next, Partials{lz->red.a(), op_hv_blocks(lz->op)},
                                    Partials{norm_buf(lz, j), lz->norm_count[j & 1]}, alpha + j,
                                    j > 0 ? beta + j - 1 : nullptr, scale + j, j > 0 ? scale + j - 1 : nullptr,
-                                   norm_buf(lz, j + 1), lz->stream),
+                                   norm_buf(lz, j + 1), lz->stream, lz->alpha_h(j), j > 0 ? lz->beta_h(j - 1) : nullptr),
             "update");
   lz->norm_count[(j + 1) & 1] = fused_blocks(lz->op->n);
-  if (lz->reorth) {
-    // omega estimates of step j and, when they call for it, the Gram-Schmidt
-    // pass on u_{j+1}, which replaces the norm partials of the update
-    hip_check(launch_omega(j, alpha, beta, Partials{norm_buf(lz, j + 1), lz->norm_count[(j + 1) & 1]}, lz->rdev,
-                           lz->stream),
-              "omega");
+  // partial reorthogonalization: the omega step of step j and, when it calls
+  // for it, the Gram-Schmidt pass on u_{j+1} (replacing its norm partials)
+  if (lz->reorth)
     hip_check(launch_reorth<V>(lz->op->n, j, next, scale, lz->rdev, norm_buf(lz, j + 1), lz->stream), "reorth");
-  }
 }
 
 // beta_{n-1} and scale_n after n steps (the host reads beta).
 void finish_norm(alps_ed_lanczos lz, int n) {
   hip_check(launch_finish(Partials{norm_buf(lz, n), lz->norm_count[n & 1]}, nullptr, lz->beta.as<double>() + n - 1,
-                          lz->scale.as<double>() + n, kNorm, lz->stream),
+                          lz->scale.as<double>() + n, kNorm, lz->stream, lz->beta_h(n - 1)),
             "finish");
 }
 
@@ -899,6 +929,7 @@ int alps_ed_lanczos_create(alps_ed_operator op, alps_ed_dtype dtype, int max_ste
     lz->w.alloc(lz->vec_bytes());
     lz->alpha.alloc((max_steps + 2) * sizeof(double));
     lz->beta.alloc((max_steps + 2) * sizeof(double));
+    lz->mirror.alloc(2 * (size_t)(max_steps + 2) * sizeof(double));
     lz->scale.alloc((max_steps + 3) * sizeof(double));
     lz->red.ensure(std::max<size_t>(op_hv_blocks(op), stream_blocks(n)));
     lz->stream = current_stream();
@@ -928,12 +959,13 @@ int alps_ed_lanczos_set_reorthogonalization(alps_ed_lanczos lz, int mode) {
     lz->r_flag.alloc(stride * sizeof(int));
     lz->r_gamma.alloc(stride * sizeof(double));
     lz->r_dot.alloc((size_t)kReorthDotBlocks * stride * sizeof(double));
-    lz->r_hnorm.alloc(sizeof(double));
+    lz->r_hnorm.alloc(stride * sizeof(double));
     lz->r_slots.alloc(stride * sizeof(void*));
     hip_check(hipMemsetAsync(lz->r_flag.p, 0, stride * sizeof(int), current_stream()), "memset");
     lz->slot_host.assign(stride, nullptr);
     lz->rdev = ReorthDev{lz->r_omega.as<double>(), stride, lz->r_flag.as<int>(), lz->r_gamma.as<double>(),
-                         lz->r_dot.as<double>(), lz->r_hnorm.as<double>(),
+                         lz->r_dot.as<double>(), lz->r_hnorm.as<double>(), lz->alpha.as<double>(),
+                         lz->beta.as<double>(),
                          static_cast<const void* const*>(lz->r_slots.p),
                          lz->dtype == ALPS_ED_F32 ? 0.5 * std::numeric_limits<float>::epsilon()
                                                   : 0.5 * std::numeric_limits<double>::epsilon()};
@@ -1108,21 +1140,37 @@ void check_status(int st) {
   if (st != ALPS_ED_OK) throw DeviceError(last_error());
 }
 
+// Steps j < upto whose new vector u_{j+1} was reorthogonalized.
+int reorthogonalized_steps(alps_ed_lanczos lz, int upto) {
+  upto = std::min(upto, lz->steps);
+  if (!lz->reorth || upto <= 0) return 0;
+  std::vector<int> f(upto);
+  hip_check(hipMemcpyAsync(f.data(), lz->r_flag.p, f.size() * sizeof(int), hipMemcpyDeviceToHost, lz->stream), "copy");
+  hip_check(hipStreamSynchronize(lz->stream), "sync");
+  int c = 0;
+  for (int v : f) c += v != 0;
+  return c;
+}
+
 // Ritz vectors (FP64, device) of the T[0..m) eigenvalues at the given
 // positions, from the stored basis.
+// `evals` are the eigenvalues of T[0..m) in ascending order, as
+// analyze_prefix returns them; each eigenvector of T comes from inverse
+// iteration at its eigenvalue (O(m) per vector instead of the full O(m^3)
+// eigendecomposition).
 std::vector<std::unique_ptr<alps_ed_vector_s>> ritz_vectors(alps_ed_lanczos lz, const std::vector<double>& a,
                                                             const std::vector<double>& b, int m,
-                                                            const std::vector<int>& index) {
-  std::vector<double> d(a.begin(), a.begin() + m), z((size_t)m * m);
-  if (ietl::tridiag::eig_full(m, d.data(), b.data(), z.data()) > 0)
-    throw std::runtime_error("LAPACK error, stev function failed.");
+                                                            const std::vector<int>& index,
+                                                            const std::vector<double>& evals) {
   const uint64_t n = lz->op->n;
   std::vector<std::unique_ptr<alps_ed_vector_s>> out;
   for (size_t i0 = 0; i0 < index.size(); i0 += kMaxRitz) {
     const int nv = (int)std::min<size_t>(kMaxRitz, index.size() - i0);
     std::vector<double> coef((size_t)nv * m);
-    for (int i = 0; i < nv; ++i)
-      for (int j = 0; j < m; ++j) coef[(size_t)i * m + j] = z[j + (size_t)index[i0 + i] * m];
+    for (int i = 0; i < nv; ++i) {
+      require(index[i0 + i] >= 0 && index[i0 + i] < (int)evals.size(), "Ritz index beyond the T spectrum");
+      tridiag_eigenvector(a.data(), b.data(), m, evals[index[i0 + i]], coef.data() + (size_t)i * m);
+    }
     std::vector<alps_ed_vector> ys(nv, nullptr);
     for (int i = 0; i < nv; ++i) {
       check_status(alps_ed_vector_create(n, ALPS_ED_F64, &ys[i]));
@@ -1153,9 +1201,9 @@ double rayleigh_quotient(alps_ed_lanczos lz, alps_ed_vector y, alps_ed_vector hy
 // given positions, from the stored FP32 basis; the vectors are handed to
 // `keep` (in the same order) when it is set.
 std::vector<double> refine_ritz(alps_ed_lanczos lz, const std::vector<double>& a, const std::vector<double>& b,
-                                int m, const std::vector<int>& index,
+                                int m, const std::vector<int>& index, const std::vector<double>& evals,
                                 std::vector<std::unique_ptr<alps_ed_vector_s>>* keep) {
-  std::vector<std::unique_ptr<alps_ed_vector_s>> ys = ritz_vectors(lz, a, b, m, index);
+  std::vector<std::unique_ptr<alps_ed_vector_s>> ys = ritz_vectors(lz, a, b, m, index, evals);
   alps_ed_vector hy = nullptr;
   check_status(alps_ed_vector_create(lz->op->n, ALPS_ED_F64, &hy));
   std::unique_ptr<alps_ed_vector_s> hold_hy(hy);
@@ -1235,7 +1283,8 @@ SectorSolve solve_sector(const alps_ed_model* model, const alps_ed_solve_params*
     result->seconds = seconds();
     return out;
   }
-  // FP32 storage with partial reorthogonalization for any number of levels
+  // FP32 storage for any number of levels; partial reorthogonalization where
+  // copies of converged levels would delay higher ones (kFp32MaxNev)
   const bool fp32 = use_fp32_reorth(params->dtype == ALPS_ED_F32, D);
   if (fp32) tune_schedule(op);
   const int max_iter = params->max_iter > 0 ? params->max_iter : (int)std::min<uint64_t>(10 * D, 1000);
@@ -1245,40 +1294,116 @@ SectorSolve solve_sector(const alps_ed_model* model, const alps_ed_solve_params*
   check_status(
       alps_ed_lanczos_create(op, fp32 ? ALPS_ED_F32 : ALPS_ED_F64, max_iter, (fp32 || want_vectors) ? 1 : 0, &lzh));
   std::unique_ptr<alps_ed_lanczos_s> lz(lzh);
-  if (fp32) check_status(alps_ed_lanczos_set_reorthogonalization(lz.get(), 1));
+  if (fp32 && nev > kFp32MaxNev) check_status(alps_ed_lanczos_set_reorthogonalization(lz.get(), 1));
   check_status(alps_ed_lanczos_start(lz.get(), params->seed, nullptr));
   std::vector<double> a(max_iter + 1), b(max_iter + 1);
-  int n = 1, checked = 1, final_n = -1, error_code = 0;
-  // The stopping rule is tested on chunk boundaries; when it holds, the
-  // first prefix length at which it holds is found by scanning forward, so
-  // the T matrix is the one the per-step test of the reference would stop at.
-  while (final_n < 0) {
-    const int target =
-        params->fixed_iterations ? max_iter : std::min(max_iter, n + std::max(1, std::min(n / 8, 64)));
-    if (target > n) {
-      check_status(alps_ed_lanczos_steps(lz.get(), target - n));
-      n = target;
-    }
-    int got = 0;
-    check_status(alps_ed_lanczos_coefficients(lz.get(), a.data(), b.data(), n, &got));
-    if (!params->fixed_iterations && n >= 2 && nlowest_converged(analyze_prefix(a, b, n, fp32), nev, fp32)) {
-      for (int m = std::max(2, checked + 1); m <= n; ++m)
-        if (m == n || nlowest_converged(analyze_prefix(a, b, m, fp32), nev, fp32)) {
-          final_n = m;
+  int final_n = -1, error_code = 0;
+  {
+    // alpha and beta reach the host through the Lanczos object's mapped
+    // mirror, which the kernels write as they form them: after each chunk of
+    // steps an event is queued, and the next chunk is queued before the host
+    // waits for the previous one, so the stopping test of prefix n runs while
+    // the device computes the next steps.  Steps past the stopping point are
+    // discarded (the reported T matrix and the Ritz vectors use prefix
+    // final_n only).
+    // Prefixes queued and not yet tested, each with the event that marks
+    // its completion (at most two: the one being waited for and the next).
+    Event evs[3];
+    struct Pending {
+      int n;
+      hipEvent_t e;
+    };
+    std::vector<Pending> pending;
+    int queued = 1, next_ev = 0;
+    auto queue_to = [&](int to) {
+      if (to > queued) check_status(alps_ed_lanczos_steps(lz.get(), to - queued));
+      queued = std::max(queued, to);
+      hipEvent_t e = evs[next_ev].e;
+      next_ev = (next_ev + 1) % 3;
+      hip_check(hipEventRecord(e, lz->stream), "event");
+      pending.push_back({to, e});
+    };
+    // The rule is tested at least eight steps apart, at most 64 (an eighth
+    // of the length in between); when it holds, the first prefix at which
+    // it holds is found by scanning forward, so the T matrix is the one the
+    // per-step test of the reference stops at.  may_be_converged skips
+    // prefixes where the rule cannot hold without the full analysis.  For
+    // one level, the stopping step is predicted from the decay of the lowest
+    // Ritz value's error bound between the last two tests, and the chunk
+    // queued ahead ends just past it (fewer steps computed and discarded).
+    int predicted = -1;  // predicted stopping step, -1 when unknown
+    auto next_target = [&](int from) {
+      if (params->fixed_iterations) return max_iter;
+      int to = from + std::max(8, std::min(from / 8, 64));
+      if (predicted > 0) to = std::min(to, std::max(from + 2, predicted + 1));
+      return std::min(max_iter, to);
+    };
+    auto holds = [&](int m) {
+      return may_be_converged(a.data(), b.data(), m, fp32) && nlowest_converged(analyze_prefix(a, b, m, fp32), nev, fp32);
+    };
+    int checked = 1, err_n = -1;
+    double err_prev = 0.0;
+    queue_to(1);
+    while (final_n < 0) {
+      const int n = pending.front().n;
+      // speculation: the next chunk is queued before the host waits for
+      // prefix n, unless the prediction puts the stop at or before n
+      if (pending.size() == 1 && queued < max_iter && !(predicted > 0 && predicted <= n)) queue_to(next_target(queued));
+      hip_check(hipEventSynchronize(pending.front().e), "event");
+      pending.erase(pending.begin());
+      std::copy(lz->host_alpha(), lz->host_alpha() + n, a.begin());
+      std::copy(lz->host_beta(), lz->host_beta() + n, b.begin());
+      // A zero beta_j ends the recurrence: T[0..j+1) is then exact (an
+      // invariant subspace); later coefficients would divide by it.  Other
+      // non-finite coefficients (NaN or Inf in the start vector or the
+      // couplings) are an error, reported instead of analysed.
+      int good = n;
+      bool exact = false;
+      for (int j = 0; j < n; ++j) {
+        if (!std::isfinite(a[j]) || !std::isfinite(b[j]))
+          throw std::runtime_error("non-finite Lanczos coefficient at step " + std::to_string(j));
+        if (b[j] == 0.0) {
+          good = j + 1;
+          exact = true;
           break;
         }
-    } else if (n >= max_iter) {
-      final_n = n;
-      error_code = 1;  // iteration.h:80
+      }
+      if (!params->fixed_iterations && good >= 2 && holds(good))
+        for (int m = std::max(2, checked + 1); m <= good; ++m)
+          if (m == good || holds(m)) {
+            final_n = m;
+            break;
+          }
+      if (final_n < 0 && exact) {
+        final_n = good;
+      } else if (final_n < 0 && n >= max_iter) {
+        final_n = n;
+        error_code = 1;  // iteration.h:80
+      }
+      if (final_n >= 0) break;
+      checked = n;
+      if (nev == 1 && !params->fixed_iterations && n >= 2) {
+        double lam = 0.0;
+        const double err = lowest_ritz_error(a.data(), b.data(), n, &lam);
+        const double tol = stop_tolerance(fp32) * std::max(1.0, std::fabs(lam));
+        predicted = -1;
+        if (err_n > 0 && err > 0.0 && err_prev > err && err > tol) {
+          const double rate = (std::log(err) - std::log(err_prev)) / (n - err_n);  // < 0
+          predicted = n + (int)std::ceil((std::log(tol) - std::log(err)) / rate);
+        }
+        err_prev = err;
+        err_n = n;
+      }
+      if (pending.empty()) queue_to(next_target(queued));
     }
-    checked = n;
   }
   TValues t = analyze_prefix(a, b, final_n, fp32);
   std::vector<double> ev;
   if (!fp32) {
     ev.assign(t.ev.begin(), t.ev.begin() + std::min<size_t>(nev, t.ev.size()));
     if (want_vectors)
-      out.vectors = ritz_vectors(lz.get(), a, b, final_n, std::vector<int>(t.index.begin(), t.index.begin() + ev.size()));
+      out.vectors = ritz_vectors(lz.get(), a, b, final_n, std::vector<int>(t.index.begin(), t.index.begin() + ev.size()),
+                                 t.all);
   } else {
     // Rayleigh quotients of the Ritz vectors; values that agree to 1e-9
     // relative are the same level reached twice and are reported once.
@@ -1288,7 +1413,7 @@ SectorSolve solve_sector(const alps_ed_model* model, const alps_ed_solve_params*
       const size_t cnt = std::min<size_t>(t.ev.size() - next, (size_t)(nev - (int)kept.size()));
       std::vector<int> idx(t.index.begin() + next, t.index.begin() + next + cnt);
       std::vector<std::unique_ptr<alps_ed_vector_s>> vs;
-      std::vector<double> r = refine_ritz(lz.get(), a, b, final_n, idx, want_vectors ? &vs : nullptr);
+      std::vector<double> r = refine_ritz(lz.get(), a, b, final_n, idx, t.all, want_vectors ? &vs : nullptr);
       next += cnt;
       for (size_t i = 0; i < r.size(); ++i) {
         bool dup = false;
@@ -1310,7 +1435,7 @@ SectorSolve solve_sector(const alps_ed_model* model, const alps_ed_solve_params*
   result->iterations = final_n;
   result->error_code = error_code;
   result->storage = fp32 ? ALPS_ED_F32 : ALPS_ED_F64;
-  check_status(alps_ed_lanczos_reorthogonalizations(lz.get(), &result->reorthogonalizations));
+  result->reorthogonalizations = reorthogonalized_steps(lz.get(), final_n - 1);
   result->seconds = seconds();
   return out;
 }

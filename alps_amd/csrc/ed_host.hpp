@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -111,6 +112,97 @@ struct DevBuf {
   T* as() const {
     return static_cast<T*>(p);
   }
+};
+
+// Device buffers of the kept Lanczos bases, recycled between solves of the
+// process instead of freed: a basis chunk of an L=24 sector is 346 MB, and
+// its hipMalloc would otherwise stall the host (and with it the queue of
+// steps) once per 32 steps of every solve.  At most kMaxBytes stay cached;
+// buffers are reused only at their exact size.
+class BufferPool {
+ public:
+  static BufferPool& get() {
+    static BufferPool p;
+    return p;
+  }
+  void* acquire(size_t bytes) {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      int dev = 0;
+      (void)hipGetDevice(&dev);
+      for (size_t i = 0; i < free_.size(); ++i)
+        if (free_[i].bytes == bytes && free_[i].dev == dev) {
+          void* p = free_[i].p;
+          cached_ -= bytes;
+          free_.erase(free_.begin() + i);
+          return p;
+        }
+    }
+    void* p = nullptr;
+    hip_check(hipMalloc(&p, bytes), "hipMalloc");
+    return p;
+  }
+  void release(void* p, size_t bytes) {
+    if (!p) return;
+    std::lock_guard<std::mutex> g(mu_);
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (cached_ + bytes <= kMaxBytes) {
+      free_.push_back({p, bytes, dev});
+      cached_ += bytes;
+    } else {
+      (void)hipFree(p);
+    }
+  }
+
+ private:
+  static constexpr size_t kMaxBytes = size_t(4) << 30;
+  struct Entry {
+    void* p;
+    size_t bytes;
+    int dev;
+  };
+  std::mutex mu_;
+  std::vector<Entry> free_;
+  size_t cached_ = 0;
+};
+
+// A device buffer from the pool (returned to it when destroyed).
+struct PooledBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  explicit PooledBuf(size_t b) : p(BufferPool::get().acquire(b)), bytes(b) {}
+  ~PooledBuf() { BufferPool::get().release(p, bytes); }
+  PooledBuf(const PooledBuf&) = delete;
+  PooledBuf& operator=(const PooledBuf&) = delete;
+};
+
+// RAII mapped pinned host buffer: `h` for the host, `d` for kernels, which
+// write it directly (small scalars read by the host after an event)
+struct MappedBuf {
+  void* h = nullptr;
+  void* d = nullptr;
+  MappedBuf() = default;
+  void alloc(size_t b) {
+    hip_check(hipHostMalloc(&h, b ? b : 1, hipHostMallocMapped), "hipHostMalloc");
+    hip_check(hipHostGetDevicePointer(&d, h, 0), "hipHostGetDevicePointer");
+  }
+  ~MappedBuf() {
+    if (h) (void)hipHostFree(h);
+  }
+  MappedBuf(const MappedBuf&) = delete;
+  MappedBuf& operator=(const MappedBuf&) = delete;
+};
+
+// RAII HIP event without timing
+struct Event {
+  hipEvent_t e = nullptr;
+  Event() { hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate"); }
+  ~Event() {
+    if (e) (void)hipEventDestroy(e);
+  }
+  Event(const Event&) = delete;
+  Event& operator=(const Event&) = delete;
 };
 
 inline ModelSpec model_spec(const alps_ed_model* m) {
@@ -246,6 +338,46 @@ inline double lowest_ritz_error(const double* a, const double* b, int m, double*
     }
   }
   return std::fabs(b[m - 1]) * std::fabs(zlast) / std::sqrt(nrm2);
+}
+
+// Number of eigenvalues of T[0..m) below x (Sturm sequence of the LDL^T
+// pivots).
+inline int sturm_count_below(const double* a, const double* b, int m, double x) {
+  const double tiny = std::numeric_limits<double>::min();
+  int c = 0;
+  double q = a[0] - x;
+  if (q < 0) ++c;
+  for (int i = 1; i < m; ++i) {
+    if (std::fabs(q) < tiny) q = q < 0 ? -tiny : tiny;
+    q = a[i] - x - b[i - 1] * b[i - 1] / q;
+    if (q < 0) ++c;
+  }
+  return c;
+}
+
+// A necessary condition for nlowest_converged(analyze_prefix(a, b, m, fp32),
+// nev, fp32), in O(m): the lowest eigenvalue of T[0..m) has an error bound
+// within 4 times the stopping tolerance (the slack covers the difference
+// between this bound's backward recurrence and the full analysis'
+// eigenvectors), or another eigenvalue of T lies within
+// 1e-5 max(|T|, 1) of it.  In the second case the full analysis may merge
+// the two (the FP64 rule's thold, tmatrix.h:149-150, is about 1e-10 |T|; the
+// FP32 near-copy window 64 eps_f32 max(|lambda|, 1) is below 1e-5 max(|T|,
+// 1)) and give the group
+// a smaller bound, so the full analysis has to decide.  When the test fails
+// the stopping rule cannot hold, because the lowest distinct value of the
+// analysis is then this eigenvalue with this bound.
+inline bool may_be_converged(const double* a, const double* b, int m, bool fp32, double* lambda_out = nullptr,
+                             double hint_lo = 0.0, double hint_hi = -1.0) {
+  double lam = 0.0;
+  const double err = lowest_ritz_error(a, b, m, &lam, hint_lo, hint_hi);
+  if (lambda_out) *lambda_out = lam;
+  const double tol = stop_tolerance(fp32);
+  if (!(err > 4.0 * std::max(tol, tol * std::fabs(lam)))) return true;  // also NaN: the full analysis reports it
+  double tn = 0.0;
+  for (int i = 0; i < m; ++i)
+    tn = std::max(tn, std::fabs(a[i]) + (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < m ? std::fabs(b[i]) : 0.0));
+  return sturm_count_below(a, b, m, lam + 1e-5 * std::max({tn, std::fabs(lam), 1.0})) >= 2;
 }
 
 // Eigenvector of T[0..m) for the (already accurate) eigenvalue lambda by

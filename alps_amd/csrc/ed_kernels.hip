@@ -93,66 +93,6 @@ __global__ __launch_bounds__(kBlock) void update_kernel(uint64_t n, const V* w, 
   store_partial(norm_out, acc);
 }
 
-// One Lanczos update with its scalars: every block first forms, in the same
-// order, scale_j = 1/sqrt(sum(norm_prev)) (the norm of u_j), beta_{j-1} =
-// sqrt(sum(norm_prev)) when j > 0, and alpha_j = scale_j^2 * sum(hv_part)
-// (both sums through one block reduction); block 0 stores them.  Then as
-// update_kernel.  This replaces the two one-block reduction launches of a
-// step.
-template <class V>
-__global__ __launch_bounds__(kBlock) void update_fused_kernel(uint64_t n, const V* w, const V* __restrict__ u_cur,
-                                                              const V* u_prev, V* out, Partials hv_part,
-                                                              Partials norm_prev, double* __restrict__ alpha_j,
-                                                              double* __restrict__ beta_jm1,
-                                                              double* __restrict__ scale_j,
-                                                              const double* __restrict__ scale_jm1,
-                                                              double* __restrict__ norm_out) {
-  const bool has_prev = beta_jm1 != nullptr;
-  const double s_prev = has_prev ? *scale_jm1 : 0.0;
-  double nsum = thread_partials(norm_prev), hsum = thread_partials(hv_part);
-  block_sum_all2(nsum, hsum);
-  const double nrm = sqrt(nsum);
-  const double sc = 1.0 / nrm;
-  const double a = sc * sc * hsum;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    *alpha_j = a;
-    *scale_j = sc;
-    if (beta_jm1) *beta_jm1 = nrm;
-  }
-  const double ca = a * sc;
-  const double cb = has_prev ? nrm * s_prev : 0.0;
-  double acc = 0.0;
-  // four consecutive elements per thread and iteration (vector loads)
-  typedef V v4 __attribute__((ext_vector_type(4), aligned(sizeof(V))));
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t n4 = n / 4;
-  for (uint64_t q = gid; q < n4; q += stride) {
-    const v4 wv = *reinterpret_cast<const v4*>(w + 4 * q);
-    const v4 uv = *reinterpret_cast<const v4*>(u_cur + 4 * q);
-    const v4 pv = has_prev ? *reinterpret_cast<const v4*>(u_prev + 4 * q) : uv;
-    v4 o;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      double r = sc * (double)wv[c] - ca * (double)uv[c];
-      if (has_prev) r -= cb * (double)pv[c];
-      o[c] = (V)r;
-      const double rd = (double)o[c];
-      acc += rd * rd;
-    }
-    *reinterpret_cast<v4*>(out + 4 * q) = o;
-  }
-  for (uint64_t i = 4 * n4 + gid; i < n; i += stride) {
-    double r = sc * (double)w[i] - ca * (double)u_cur[i];
-    if (has_prev) r -= cb * (double)u_prev[i];
-    const V rs = (V)r;
-    out[i] = rs;
-    const double rd = (double)rs;
-    acc += rd * rd;
-  }
-  store_partial(norm_out, acc);
-}
-
 // ---------------------------------------------------------------------------
 // Partial reorthogonalization (ReorthDev, ed_kernels.hpp)
 // ---------------------------------------------------------------------------
@@ -169,9 +109,12 @@ __device__ __forceinline__ double block_max_all(double v) {
   return m;
 }
 
-// One block.  With the update of step j done (alpha_j, beta_{j-1} stored,
-// r = u_{j+1} with norm partials norm_r), the omega recurrence gives row j+1
-// from rows j and j-1: for k < j
+// The omega step of partial reorthogonalization for step j, computed by
+// every block of reorth_dot_kernel with the same FP64 arithmetic on the same
+// data (so every block reaches the same decision); block 0 stores the new
+// row, the flag and the |T| estimate.  With the update of step j done
+// (alpha_j, beta_{j-1} stored, r = u_{j+1} with norm partials norm_r), the
+// omega recurrence gives row j+1 from rows j and j-1: for k < j
 //   beta_j w_{j+1,k} = beta_k w_{j,k+1} + (alpha_k - alpha_j) w_{j,k}
 //                      + beta_{k-1} w_{j,k-1} - beta_{j-1} w_{j-1,k} + theta,
 // which follows from the Lanczos relation of steps j and k and the symmetry
@@ -179,18 +122,19 @@ __device__ __forceinline__ double block_max_all(double v) {
 // the sign of the rest (a deterministic worst case), |H| estimated by the
 // largest |alpha_i| + beta_i + beta_{i-1} so far.  w_{j+1,j} = 2 eps |H| /
 // beta_j (the local loss), w_{j+1,j+1} = 1.  If the largest |w_{j+1,k}|
-// exceeds sqrt(eps), or the previous step crossed it, the step's flag is set
-// and the row is reset to eps (the Gram-Schmidt pass that follows makes
-// u_{j+1} orthogonal to working accuracy).
-__global__ __launch_bounds__(kBlock) void omega_kernel(int j, const double* __restrict__ alpha,
-                                                       const double* __restrict__ beta, Partials norm_r,
-                                                       ReorthDev R) {
-  const double bj = sqrt(sum_partials(norm_r));
+// exceeds sqrt(eps), or the previous step crossed it, the step is flagged
+// and the row reset to eps (the Gram-Schmidt pass that follows makes
+// u_{j+1} orthogonal to working accuracy).  Returns the flag.
+__device__ int omega_step(int j, Partials norm_r, const ReorthDev& R) {
+  const double* __restrict__ alpha = R.alpha;
+  const double* __restrict__ beta = R.beta;
   const double aj = alpha[j];
   const double bjm1 = j > 0 ? beta[j - 1] : 0.0;
-  const double hn = fmax(j > 0 ? *R.hnorm : 0.0, fabs(aj) + bj + bjm1);
+  const double bj = sqrt(sum_partials(norm_r));
+  const double hn = fmax(j > 0 ? R.hnorm[j - 1] : 0.0, fabs(aj) + bj + bjm1);
   const double eps = R.eps;
   const double noise = 2.0 * eps * hn;
+  const bool store = blockIdx.x == 0;
   double* w1 = R.omega + (size_t)((j + 1) % 3) * R.stride;
   const double* w0 = R.omega + (size_t)(j % 3) * R.stride;
   const double* wm = R.omega + (size_t)((j + 2) % 3) * R.stride;  // row j-1 (unused at j = 0)
@@ -201,21 +145,24 @@ __global__ __launch_bounds__(kBlock) void omega_kernel(int j, const double* __re
       double t = beta[k] * w0[k + 1] + (alpha[k] - aj) * w0[k] - bjm1 * wm[k];
       if (k > 0) t += beta[k - 1] * w0[k - 1];
       const double v = (t + copysign(noise, t)) / bj;
-      w1[k] = v;
+      if (store) w1[k] = v;
       mx = fmax(mx, fabs(v));
     }
   mx = block_max_all(mx);
   const int prev = j > 0 ? R.flag[j - 1] : 0;
   const int flag = !ok ? 0 : (prev == 1 ? 2 : (mx > sqrt(eps) ? 1 : 0));
-  if (flag)
-    for (int k = threadIdx.x; k < j; k += kBlock) w1[k] = eps;
-  if (threadIdx.x == 0) {
-    if (j == 0) R.omega[0] = 1.0;  // row 0: v_0.v_0
-    w1[j] = ok ? noise / bj : eps;
-    w1[j + 1] = 1.0;
-    R.flag[j] = flag;
-    *R.hnorm = hn;
+  if (store) {
+    if (flag)
+      for (int k = threadIdx.x; k < j; k += kBlock) w1[k] = eps;
+    if (threadIdx.x == 0) {
+      if (j == 0) R.omega[0] = 1.0;  // row 0: v_0.v_0
+      w1[j] = ok ? noise / bj : eps;
+      w1[j + 1] = 1.0;
+      R.flag[j] = flag;
+      R.hnorm[j] = hn;
+    }
   }
+  return flag;
 }
 
 // Partials of u_k . r for k = 0..j: block b owns a contiguous range of the
@@ -223,8 +170,8 @@ __global__ __launch_bounds__(kBlock) void omega_kernel(int j, const double* __re
 // vectors are read eight at a time with r (one read of r per eight vectors).
 template <class V>
 __global__ __launch_bounds__(kBlock) void reorth_dot_kernel(uint64_t n, int j, const V* __restrict__ r,
-                                                            ReorthDev R) {
-  if (R.flag[j] == 0) return;
+                                                            Partials norm_r, ReorthDev R) {
+  if (omega_step(j, norm_r, R) == 0) return;
   typedef V v4 __attribute__((ext_vector_type(4), aligned(sizeof(V))));
   const V* const* slots = reinterpret_cast<const V* const*>(R.slots);
   const uint64_t n4 = n / 4;
@@ -340,6 +287,70 @@ __global__ __launch_bounds__(kBlock) void reorth_update_kernel(uint64_t n, int j
   store_partial(norm_out, acc);
 }
 
+// One Lanczos update with its scalars: every block first forms, in the same
+// order, scale_j = 1/sqrt(sum(norm_prev)) (the norm of u_j), beta_{j-1} =
+// sqrt(sum(norm_prev)) when j > 0, and alpha_j = scale_j^2 * sum(hv_part)
+// (both sums through one block reduction); block 0 stores them.  Then as
+// update_kernel.  This replaces the two one-block reduction launches of a
+// step.
+template <class V>
+__global__ __launch_bounds__(kBlock) void update_fused_kernel(uint64_t n, const V* w, const V* __restrict__ u_cur,
+                                                              const V* u_prev, V* out, Partials hv_part,
+                                                              Partials norm_prev, double* __restrict__ alpha_j,
+                                                              double* __restrict__ beta_jm1,
+                                                              double* __restrict__ scale_j,
+                                                              const double* __restrict__ scale_jm1,
+                                                              double* __restrict__ norm_out, double* alpha_h,
+                                                              double* beta_h) {
+  const bool has_prev = beta_jm1 != nullptr;
+  const double s_prev = has_prev ? *scale_jm1 : 0.0;
+  double nsum = thread_partials(norm_prev), hsum = thread_partials(hv_part);
+  block_sum_all2(nsum, hsum);
+  const double nrm = sqrt(nsum);
+  const double sc = 1.0 / nrm;
+  const double a = sc * sc * hsum;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *alpha_j = a;
+    *scale_j = sc;
+    if (beta_jm1) *beta_jm1 = nrm;
+    // host-mapped copies for the stopping test (no copy launch needed)
+    if (alpha_h) *alpha_h = a;
+    if (beta_h && beta_jm1) *beta_h = nrm;
+  }
+  const double ca = a * sc;
+  const double cb = has_prev ? nrm * s_prev : 0.0;
+  double acc = 0.0;
+  // four consecutive elements per thread and iteration (vector loads)
+  typedef V v4 __attribute__((ext_vector_type(4), aligned(sizeof(V))));
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t n4 = n / 4;
+  for (uint64_t q = gid; q < n4; q += stride) {
+    const v4 wv = *reinterpret_cast<const v4*>(w + 4 * q);
+    const v4 uv = *reinterpret_cast<const v4*>(u_cur + 4 * q);
+    const v4 pv = has_prev ? *reinterpret_cast<const v4*>(u_prev + 4 * q) : uv;
+    v4 o;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      double r = sc * (double)wv[c] - ca * (double)uv[c];
+      if (has_prev) r -= cb * (double)pv[c];
+      o[c] = (V)r;
+      const double rd = (double)o[c];
+      acc += rd * rd;
+    }
+    *reinterpret_cast<v4*>(out + 4 * q) = o;
+  }
+  for (uint64_t i = 4 * n4 + gid; i < n; i += stride) {
+    double r = sc * (double)w[i] - ca * (double)u_cur[i];
+    if (has_prev) r -= cb * (double)u_prev[i];
+    const V rs = (V)r;
+    out[i] = rs;
+    const double rd = (double)rs;
+    acc += rd * rd;
+  }
+  store_partial(norm_out, acc);
+}
+
 template <class V>
 __global__ __launch_bounds__(kBlock) void dot_kernel(uint64_t n, const V* __restrict__ x,
                                                      const V* __restrict__ y, double* partials) {
@@ -351,18 +362,21 @@ __global__ __launch_bounds__(kBlock) void dot_kernel(uint64_t n, const V* __rest
 }
 
 __global__ __launch_bounds__(kBlock) void finish_kernel(Partials in, const double* scale, double* out, double* out2,
-                                                        int mode) {
+                                                        int mode, double* out_h) {
   const double s = sum_partials(in);
   if (threadIdx.x == 0) {
+    double v;
     if (mode == kNorm) {
-      out[0] = sqrt(s);
+      v = sqrt(s);
       if (out2) out2[0] = 1.0 / sqrt(s);
     } else if (mode == kAlpha) {
       const double sc = scale ? *scale : 1.0;
-      out[0] = sc * sc * s;
+      v = sc * sc * s;
     } else {
-      out[0] = s;
+      v = s;
     }
+    out[0] = v;
+    if (out_h) out_h[0] = v;
   }
 }
 
@@ -411,25 +425,73 @@ __global__ __launch_bounds__(kBlock) void convert_kernel(uint64_t n, VD* __restr
 }
 
 // Ritz vectors over a stored Lanczos basis:
-//   out_i[r] = sum_{j<m} coef[i*m + j] * scale[j] * u_j[r],  i < nvec <= 8.
-template <class V>
+//   out_i[r] = sum_{j<m} (coef[i*m + j] * scale[j]) * u_j[r],  i < nvec <= NV,
+// summed over j in ascending order.  Each thread takes four consecutive rows
+// (vector loads) and four basis vectors at a time, so sixteen elements per
+// thread are in flight.
+template <class V, int NV>
 __global__ __launch_bounds__(kBlock) void ritz_kernel(uint64_t n, const V* const* __restrict__ slots,
                                                       const double* __restrict__ scale, int m, int nvec,
                                                       const double* __restrict__ coef,
                                                       double* const* __restrict__ outs) {
+  typedef V v4 __attribute__((ext_vector_type(4), aligned(sizeof(V))));
+  typedef double d4 __attribute__((ext_vector_type(4), aligned(8)));
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
-    double acc[kMaxRitz];
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t n4 = n / 4;
+  for (uint64_t q = gid; q < n4; q += stride) {
+    double acc[NV][4];
 #pragma unroll
-    for (int i = 0; i < kMaxRitz; ++i) acc[i] = 0.0;
-    for (int j = 0; j < m; ++j) {
-      const double u = scale[j] * (double)slots[j][r];
+    for (int i = 0; i < NV; ++i)
 #pragma unroll
-      for (int i = 0; i < kMaxRitz; ++i)
-        if (i < nvec) acc[i] += coef[i * m + j] * u;
+      for (int e = 0; e < 4; ++e) acc[i][e] = 0.0;
+    int j = 0;
+    for (; j + 4 <= m; j += 4) {
+      v4 u[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) u[c] = *reinterpret_cast<const v4*>(slots[j + c] + 4 * q);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const double sj = scale[j + c];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+          if (i < nvec) {
+            const double w = coef[i * m + j + c] * sj;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc[i][e] += w * (double)u[c][e];
+          }
+        }
+      }
+    }
+    for (; j < m; ++j) {
+      const v4 u = *reinterpret_cast<const v4*>(slots[j] + 4 * q);
+      const double sj = scale[j];
+#pragma unroll
+      for (int i = 0; i < NV; ++i) {
+        if (i < nvec) {
+          const double w = coef[i * m + j] * sj;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) acc[i][e] += w * (double)u[e];
+        }
+      }
     }
 #pragma unroll
-    for (int i = 0; i < kMaxRitz; ++i)
+    for (int i = 0; i < NV; ++i)
+      if (i < nvec) *reinterpret_cast<d4*>(outs[i] + 4 * q) = d4{acc[i][0], acc[i][1], acc[i][2], acc[i][3]};
+  }
+  for (uint64_t r = 4 * n4 + gid; r < n; r += stride) {
+    double acc[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+    for (int j = 0; j < m; ++j) {
+      const double u = (double)slots[j][r];
+      const double sj = scale[j];
+#pragma unroll
+      for (int i = 0; i < NV; ++i)
+        if (i < nvec) acc[i] += coef[i * m + j] * sj * u;
+    }
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
       if (i < nvec) outs[i][r] = acc[i];
   }
 }
@@ -528,16 +590,10 @@ hipError_t launch_update(uint64_t n, const V* w, const V* u_cur, const V* u_prev
 template <class V>
 hipError_t launch_update_fused(uint64_t n, const V* w, const V* u_cur, const V* u_prev, V* out, Partials hv_part,
                                Partials norm_prev, double* alpha_j, double* beta_jm1, double* scale_j,
-                               const double* scale_jm1, double* norm_out, hipStream_t st) {
+                               const double* scale_jm1, double* norm_out, hipStream_t st, double* alpha_h,
+                               double* beta_h) {
   hipLaunchKernelGGL((update_fused_kernel<V>), dim3(fused_blocks(n)), dim3(kBlock), 0, st, n, w, u_cur, u_prev, out,
-                     hv_part, norm_prev, alpha_j, beta_jm1, scale_j, scale_jm1, norm_out);
-  return hipGetLastError();
-}
-
-hipError_t launch_omega(int j, const double* alpha, const double* beta, Partials norm_r, const ReorthDev& R,
-                        hipStream_t st) {
-  if (j < 0 || (uint32_t)j + 2 > R.stride) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(omega_kernel, dim3(1), dim3(kBlock), 0, st, j, alpha, beta, norm_r, R);
+                     hv_part, norm_prev, alpha_j, beta_jm1, scale_j, scale_jm1, norm_out, alpha_h, beta_h);
   return hipGetLastError();
 }
 
@@ -546,7 +602,8 @@ hipError_t launch_reorth(uint64_t n, int j, V* r, const double* scale, const Reo
                          hipStream_t st) {
   if (j < 0 || (uint32_t)j + 2 > R.stride) return hipErrorInvalidValue;
   const int G = kReorthDotBlocks;
-  hipLaunchKernelGGL((reorth_dot_kernel<V>), dim3(G), dim3(kBlock), 0, st, n, j, r, R);
+  hipLaunchKernelGGL((reorth_dot_kernel<V>), dim3(G), dim3(kBlock), 0, st, n, j, r,
+                     Partials{norm_out, fused_blocks(n)}, R);
   const unsigned cb = (unsigned)((j + 1 + kBlock / 64 - 1) / (kBlock / 64));
   hipLaunchKernelGGL(reorth_coef_kernel, dim3(cb), dim3(kBlock), 0, st, j, G, scale, R);
   hipLaunchKernelGGL((reorth_update_kernel<V>), dim3(fused_blocks(n)), dim3(kBlock), (size_t)(j + 1) * sizeof(double),
@@ -560,8 +617,9 @@ hipError_t launch_dot(uint64_t n, const V* x, const V* y, double* partials, hipS
   return hipGetLastError();
 }
 
-hipError_t launch_finish(Partials in, const double* scale, double* out, double* out2, int mode, hipStream_t st) {
-  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(kBlock), 0, st, in, scale, out, out2, mode);
+hipError_t launch_finish(Partials in, const double* scale, double* out, double* out2, int mode, hipStream_t st,
+                         double* out_h) {
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(kBlock), 0, st, in, scale, out, out2, mode, out_h);
   return hipGetLastError();
 }
 
@@ -603,8 +661,16 @@ template <class V>
 hipError_t launch_ritz(uint64_t n, const V* const* slots, const double* scale, int m, int nvec,
                        const double* coef, double* const* outs, hipStream_t st) {
   if (n == 0 || nvec == 0) return hipSuccess;
-  hipLaunchKernelGGL((ritz_kernel<V>), dim3(stream_blocks(n)), dim3(kBlock), 0, st, n, slots, scale, m, nvec,
-                     coef, outs);
+  if (nvec < 0 || nvec > kMaxRitz) return hipErrorInvalidValue;
+  const dim3 g(stream_blocks(n)), b(kBlock);
+  if (nvec == 1)
+    hipLaunchKernelGGL((ritz_kernel<V, 1>), g, b, 0, st, n, slots, scale, m, nvec, coef, outs);
+  else if (nvec == 2)
+    hipLaunchKernelGGL((ritz_kernel<V, 2>), g, b, 0, st, n, slots, scale, m, nvec, coef, outs);
+  else if (nvec <= 4)
+    hipLaunchKernelGGL((ritz_kernel<V, 4>), g, b, 0, st, n, slots, scale, m, nvec, coef, outs);
+  else
+    hipLaunchKernelGGL((ritz_kernel<V, kMaxRitz>), g, b, 0, st, n, slots, scale, m, nvec, coef, outs);
   return hipGetLastError();
 }
 
@@ -638,7 +704,8 @@ hipError_t launch_elements(const DevLayout& L, uint64_t row0, uint64_t nrows, in
                                        const double*, const double*, const double*, double*, hipStream_t); \
   template hipError_t launch_dot<V>(uint64_t, const V*, const V*, double*, hipStream_t);                    \
   template hipError_t launch_update_fused<V>(uint64_t, const V*, const V*, const V*, V*, Partials, Partials,   \
-                                             double*, double*, double*, const double*, double*, hipStream_t);  \
+                                             double*, double*, double*, const double*, double*, hipStream_t,   \
+                                             double*, double*);                                                \
   template hipError_t launch_normalize<V>(uint64_t, V*, Partials, double*, hipStream_t);                    \
   template hipError_t launch_fill_random<V>(uint64_t, V*, uint64_t, double, double, hipStream_t);           \
   template hipError_t launch_fill<V>(uint64_t, V*, double, hipStream_t);                                    \

@@ -83,8 +83,10 @@ struct HvWaves {
 // of converged levels that FP32 rounding creates early delay the higher
 // levels (about twice the FP64 iteration count for four levels), which costs
 // more than the halved bytes save.  Single solves reorthogonalize FP32
-// recurrences partially (ReorthDev), which keeps the copies away, so they
-// store any number of levels in FP32.  Elsewhere the vectors are FP64.
+// recurrences of more than kFp32MaxNev levels partially (ReorthDev), which
+// keeps the copies away, so they store any number of levels in FP32 (for
+// one or two levels the copies do not delay the stopping test and the
+// reorthogonalization would only cost time).  Elsewhere the vectors are FP64.
 constexpr uint64_t kFp32MinDim = 1000;
 constexpr int kFp32MaxNev = 2;
 inline bool use_fp32(bool asked, uint64_t dim, int nev) { return asked && dim >= kFp32MinDim && nev <= kFp32MaxNev; }
@@ -132,7 +134,9 @@ hipError_t launch_hv(const DevLayout& L, const V* x, V* y, double* dot_out, hipS
 //   mode kAlpha: alpha = scale^2 * sum(in)            -> out[0]
 //   mode kBeta:  beta = sqrt(sum(in)), scale = 1/beta -> out[0], out2[0]
 enum { kSum = 0, kNorm = 1, kAlpha = 2 };
-hipError_t launch_finish(Partials in, const double* scale, double* out, double* out2, int mode, hipStream_t st);
+// out_h, when set (host-mapped memory), receives a copy of out[0].
+hipError_t launch_finish(Partials in, const double* scale, double* out, double* out2, int mode, hipStream_t st,
+                         double* out_h = nullptr);
 
 // Lanczos update: r = s_cur*w - alpha*s_cur*u_cur - beta_prev*s_prev*u_prev
 // stored into `out` (may alias w or u_prev); per-block partials of |r|^2
@@ -143,26 +147,16 @@ hipError_t launch_update(uint64_t n, const V* w, const V* u_cur, const V* u_prev
                          const double* s_cur, const double* beta_prev, const double* s_prev, double* norm_out,
                          hipStream_t st);
 
-// The update with its scalars (one launch instead of three): scale_j =
-// 1/sqrt(sum(norm_prev)), beta_{j-1} = sqrt(sum(norm_prev)) (when beta_jm1 is
-// set; it is null at j = 0), alpha_j = scale_j^2 * sum(hv_part); then
-// r = scale_j w - alpha_j scale_j u_cur - beta_{j-1} scale_{j-1} u_prev into
-// `out` with per-block partials of |r|^2 into norm_out (fused_blocks(n) of
-// them).  The scalars are bit-identical to launch_finish + launch_update.
-template <class V>
-hipError_t launch_update_fused(uint64_t n, const V* w, const V* u_cur, const V* u_prev, V* out, Partials hv_part,
-                               Partials norm_prev, double* alpha_j, double* beta_jm1, double* scale_j,
-                               const double* scale_jm1, double* norm_out, hipStream_t st);
-
 // Partial reorthogonalization (Simon, "The Lanczos algorithm with partial
 // reorthogonalization", Math. Comp. 42 (1984) 115-142) of a recurrence whose
 // basis u_0..u_j is kept in HBM (v_k = scale_k u_k).  After the update of
-// step j has stored r = beta_j v_{j+1} as u_{j+1}, omega_kernel extends the
-// estimates omega_{j+1,k} ~ v_{j+1}.v_k by the omega recurrence in FP64 and
-// decides: reorthogonalize when max_k |omega_{j+1,k}| > sqrt(eps) (eps = unit
-// roundoff of the vector storage), and again at the next step (the pair
-// rule).  The three Gram-Schmidt kernels run only when the step's flag is
-// set (uniform early exit otherwise): per-block partials of u_k.r, the
+// step j has stored r = beta_j v_{j+1} as u_{j+1}, the first Gram-Schmidt
+// kernel extends the estimates omega_{j+1,k} ~ v_{j+1}.v_k by the omega
+// recurrence in FP64 (every block the same way; block 0 stores them) and
+// decides: reorthogonalize when max_k |omega_{j+1,k}| > sqrt(eps) (eps =
+// unit roundoff of the vector storage), and again at the next step (the pair
+// rule).  The Gram-Schmidt kernels do their work only when the step is
+// flagged (uniform early exit otherwise): per-block partials of u_k.r, the
 // coefficients gamma_k = scale_k^2 u_k.r, and r -= sum_k gamma_k u_k with new
 // norm partials of r in place of the update's.  Every sum is in a fixed
 // order, so the recurrence stays deterministic.
@@ -172,18 +166,33 @@ struct ReorthDev {
   int* flag;                // per step: 0 none, 1 threshold crossed, 2 second of a pair
   double* gamma;            // per kept vector: the Gram-Schmidt coefficient
   double* dot_part;         // kReorthDotBlocks partials per kept vector
-  double* hnorm;            // running estimate of |T| (one double)
+  double* hnorm;            // per step: running estimate of |T|
+  const double* alpha;      // the recurrence's alpha and beta
+  const double* beta;
   const void* const* slots; // device addresses of u_0, u_1, ...
   double eps;               // unit roundoff of the vector storage
 };
-constexpr int kReorthDotBlocks = 256;
-hipError_t launch_omega(int j, const double* alpha, const double* beta, Partials norm_r, const ReorthDev& R,
-                        hipStream_t st);
-// The Gram-Schmidt pass of step j on r = u_{j+1} (three launches, each a
-// no-op unless R.flag[j] is set); norm_out receives fused_blocks(n) partials.
+constexpr int kReorthDotBlocks = 1024;
+// The omega step and Gram-Schmidt pass of step j on r = u_{j+1} (three
+// launches; norm_out holds the update's fused_blocks(n) norm partials of r
+// and receives those of the reorthogonalized r when the step is flagged).
 template <class V>
 hipError_t launch_reorth(uint64_t n, int j, V* r, const double* scale, const ReorthDev& R, double* norm_out,
                          hipStream_t st);
+
+// The update with its scalars (one launch instead of three): scale_j =
+// 1/sqrt(sum(norm_prev)), beta_{j-1} = sqrt(sum(norm_prev)) (when beta_jm1 is
+// set; it is null at j = 0), alpha_j = scale_j^2 * sum(hv_part); then
+// r = scale_j w - alpha_j scale_j u_cur - beta_{j-1} scale_{j-1} u_prev into
+// `out` with per-block partials of |r|^2 into norm_out (fused_blocks(n) of
+// them).  The scalars are bit-identical to launch_finish + launch_update.
+// alpha_h / beta_h, when set (host-mapped memory), receive copies of alpha_j
+// and beta_{j-1}.
+template <class V>
+hipError_t launch_update_fused(uint64_t n, const V* w, const V* u_cur, const V* u_prev, V* out, Partials hv_part,
+                               Partials norm_prev, double* alpha_j, double* beta_jm1, double* scale_j,
+                               const double* scale_jm1, double* norm_out, hipStream_t st, double* alpha_h = nullptr,
+                               double* beta_h = nullptr);
 
 // per-block partials of sum x_i y_i
 template <class V>

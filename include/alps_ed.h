@@ -279,7 +279,10 @@ typedef struct {
   int n_eigenvalues;
   double eigenvalues[64];
   int iterations;
-  int error_code;         /* 1 when max_iter was reached (iteration.h:80) */
+  int error_code;         /* 1 when max_iter was reached (iteration.h:80); in a batch also -1 (can
+                           * never fit the arena) and -2 (non-finite Lanczos coefficients, no
+                           * eigenvalues), which a single solve reports as ALPS_ED_ERR_SOLVER.
+                           * A zero beta ends the recurrence early with the exact T (error 0). */
   double seconds;         /* device wall time of the solve */
   alps_ed_dtype storage;  /* storage of the Lanczos vectors actually used: an FP32 request is served
                            * in FP64 for sectors of fewer than 1000 states (and, in a batch, for
