@@ -144,6 +144,9 @@ __device__ __forceinline__ V elem_at(const V* p, uint32_t off4) {
   return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(p) + off4 * (uint32_t)(sizeof(V) / 4));
 }
 
+// Every multiply-add of a row is an explicit fma, so that rows and dot
+// partials do not depend on what the compiler contracts.
+//
 // The rows of hv_segment for NQ row slots per thread (rows tid,
 // tid + kBlock, ...), NQ fixed at compile time so that the loads of all
 // slots are issued together; rows past the segment are clamped to row 0 and
@@ -237,7 +240,7 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
-      for (int c = 0; c < CLL; ++c) off[q] += t[c] * (double)elem_at(xs, j[q][c]);
+      for (int c = 0; c < CLL; ++c) off[q] = fma(t[c], (double)elem_at(xs, j[q][c]), off[q]);
   }
   // high-high flips: a bond flips every row of the segment or none (its two
   // bits lie in h), with a contiguous target run.  The bonds that flip are
@@ -273,7 +276,7 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
 #pragma unroll
       for (int q = 0; q < NQ; ++q)
 #pragma unroll
-        for (int c = 0; c < CHH; ++c) off[q] += tt[c] * (double)v[q][c];
+        for (int c = 0; c < CHH; ++c) off[q] = fma(tt[c], (double)v[q][c], off[q]);
     }
   }
   // cross bonds: diagonal always, flip when the high and low bits differ.
@@ -322,7 +325,7 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
 #pragma unroll
       for (int c = 0; c < CX; ++c) {
         d[q] += f[q][c] ? ea[c] : ep[c];
-        off[q] += (f[q][c] ? tx[c] : 0.0) * (double)v[q][c];
+        off[q] = fma(f[q][c] ? tx[c] : 0.0, (double)v[q][c], off[q]);
       }
   }
   double dotp = 0.0;
@@ -330,9 +333,9 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
   for (int q = 0; q < NQ; ++q) {
     if (ok[q]) {
       const double xi = (double)xs[rc[q]];
-      const double yv = d[q] * xi + off[q];
+      const double yv = fma(d[q], xi, off[q]);
       y[base + rc[q]] = (V)yv;
-      if (DOT) dotp += xi * yv;  // FP64 value before the storage rounding
+      if (DOT) dotp = fma(xi, yv, dotp);  // FP64 value before the storage rounding
     }
   }
   return dotp;

@@ -116,8 +116,8 @@ inline unsigned hv_blocks(const DevLayout& L) { return L.n_seg ? L.n_seg : 1; }
 // for and the chunk sizes of its low-low, high-high and cross flip loops.
 // Schedule 0 is the default (FP64 vectors always use it); an operator of a
 // large FP32 sector picks its schedule by timing them all (the autotuner in
-// ed_capi.hip).  Every schedule adds the bonds in the same order, so all
-// give identical results.
+// ed_capi.hip).  Every schedule adds the bonds in the same order and forms
+// the dot partials the same way, so all give identical results.
 struct HvSchedule {
   int waves, cll, chh, cx;
 };

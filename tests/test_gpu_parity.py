@@ -418,3 +418,23 @@ def test_hv_autotuner_full_size():
         op.matvec(x, y)
         ys.append(y.numpy())
     assert np.array_equal(ys[0], ys[1]) and np.array_equal(ys[0], ys[2])
+
+
+@pytest.mark.parametrize("name", ["heis_chain16", "j1j2_4x4", "general_4x3", "xxz_open13"])
+def test_hv_schedules_give_identical_lanczos_coefficients(name):
+    # the dot partials of H.v (alpha) are formed per segment with the same
+    # reduction tree by every schedule, so the whole FP32 recurrence is
+    # bit-identical whichever schedule the autotuner picks
+    mk, sectors = CASES[name]
+    m = mk()
+    op = A.Operator(to_product(m, sectors[0]))
+    runs = []
+    for s in range(A.Operator.HV_SCHEDULES):
+        op.set_schedule(s)
+        lz = A.Lanczos(op, "f32", max_steps=40)
+        lz.start(seed=7)
+        lz.steps(39)
+        runs.append(lz.coefficients())
+    for a, b in runs[1:]:
+        assert np.array_equal(a.view(np.int64), runs[0][0].view(np.int64))
+        assert np.array_equal(b.view(np.int64), runs[0][1].view(np.int64))
