@@ -36,6 +36,7 @@
 #pragma once
 
 #include <algorithm>
+#include <chrono>
 #include <complex>
 #include <cmath>
 #include <cstdint>
@@ -888,12 +889,30 @@ struct Iteration {
 // Lanczos (lanczos.h:341-406), FP64 vectors
 // ---------------------------------------------------------------------------
 
+// Wall seconds per phase of a solve (for the CPU baseline timing): basis
+// build, matrix build, the H.v products, the vector operations of the
+// recurrence, and the T-matrix analysis of the stopping test.
+struct Phases {
+  double basis = 0, matrix = 0, hv = 0, vector = 0, tmatrix = 0;
+};
+typedef std::chrono::steady_clock phase_clock;
+inline double since(phase_clock::time_point t0) {
+  return std::chrono::duration<double>(phase_clock::now() - t0).count();
+}
+
 struct Lanczos {
   const MapMatrix* H;
   size_t D;
   std::vector<double> startvector, vec2, vec3;
   Tmatrix T;
   unsigned n = 0;
+  Phases* ph = nullptr;  // optional phase timers
+
+  void timed_spmv(const double* x, double* y) {
+    const auto t0 = phase_clock::now();
+    spmv(*H, x, y, D);
+    if (ph) ph->hv += since(t0);
+  }
 
   Lanczos(const MapMatrix& h, size_t dim) : H(&h), D(dim), startvector(dim, 0.0), vec2(dim, 0.0), vec3(dim, 0.0) {}
 
@@ -913,7 +932,7 @@ struct Lanczos {
     startvector = v0;
     double nrm = two_norm(startvector);
     for (auto& v : startvector) v /= nrm;
-    spmv(*H, startvector.data(), vec2.data(), D);
+    timed_spmv(startvector.data(), vec2.data());
     double a = dot(startvector, vec2);
     for (size_t i = 0; i < D; ++i) vec2[i] -= a * startvector[i];
     double b = two_norm(vec2);
@@ -924,7 +943,7 @@ struct Lanczos {
   // make_step (:392-406)
   std::pair<double, double> make_step(unsigned j) {
     double b = T.beta[j - 1];
-    spmv(*H, vec2.data(), vec3.data(), D);
+    timed_spmv(vec2.data(), vec3.data());
     double a = dot(vec2, vec3);
     for (size_t i = 0; i < D; ++i) vec3[i] -= a * vec2[i];
     for (size_t i = 0; i < D; ++i) vec3[i] -= b * startvector[i];
@@ -937,17 +956,27 @@ struct Lanczos {
 
   // generate_tmatrix(iter, gen) then generate_tmatrix(iter) (:341-372)
   void calculate_eigenvalues(Iteration& it, const std::vector<double>& v0) {
+    const auto t_all = phase_clock::now();
+    double t_test = 0.0, t_hv0 = ph ? ph->hv : 0.0;
     if (T.alpha.empty()) {
       auto ab = make_first_step(v0);
       T.push_back(ab.first, ab.second);
       n = 1;
     }
     for (unsigned j = 0; j < n; ++j) ++it.i;
-    while (!it.finished(T)) {
+    for (;;) {
+      const auto t0 = phase_clock::now();
+      const bool done = it.finished(T);
+      t_test += since(t0);
+      if (done) break;
       auto ab = make_step(n);
       if (n == T.alpha.size()) T.push_back(ab.first, ab.second);
       ++it.i;
       ++n;
+    }
+    if (ph) {
+      ph->tmatrix += t_test;
+      ph->vector += since(t_all) - t_test - (ph->hv - t_hv0);
     }
   }
 };
@@ -967,12 +996,16 @@ struct SubspaceResult {
 // do_subspace: D==0 -> nothing, D==1 -> H(0,0), else nlowest Lanczos with
 // max_iter default min(10*D, 1000) and nev default 1.
 inline SubspaceResult do_subspace(const SpinModel& m, int n_up, int max_iter, int nev,
-                                  uint64_t seed, bool fixed_iterations = false) {
+                                  uint64_t seed, bool fixed_iterations = false, Phases* ph = nullptr) {
   SubspaceResult r;
+  auto t0 = phase_clock::now();
   std::vector<State> states = build_basis(m.n_sites, n_up);
+  if (ph) ph->basis += since(t0);
   r.dim = states.size();
   if (r.dim == 0) return r;
+  t0 = phase_clock::now();
   MapMatrix H = build_matrix(m, states);
+  if (ph) ph->matrix += since(t0);
   if (r.dim == 1) {
     r.ev.push_back(H[0][0]);
     return r;
@@ -984,6 +1017,7 @@ inline SubspaceResult do_subspace(const SpinModel& m, int n_up, int max_iter, in
   it.max_iter = (unsigned)max_iter;
   it.nev = (unsigned)nev;
   Lanczos lz(H, r.dim);
+  lz.ph = ph;
   std::vector<double> v0(r.dim);
   for (size_t i = 0; i < r.dim; ++i) v0[i] = start_value(seed, i, -1.0, 1.0);
   lz.calculate_eigenvalues(it, v0);

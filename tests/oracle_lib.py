@@ -41,8 +41,10 @@ def lib():
         L.oracle_start_value.restype = f64
         L.oracle_sparsediag.argtypes = [i32, i32] + model + [i32, i32, u64, i32, P, P, P, P, i32, P, P, P, P]
         L.oracle_sparsediag.restype = i32
-        L.oracle_bench_hv.argtypes = [i32, i32] + model + [i64, i32, P, P]
+        L.oracle_bench_hv.argtypes = [i32, i32] + model + [i64, i32, i32, P, P]
         L.oracle_bench_hv.restype = f64
+        L.oracle_sparsediag_timed.argtypes = [i32, i32] + model + [i32, i32, u64, i32, P, P, P, P, P]
+        L.oracle_sparsediag_timed.restype = i32
         L.oracle_ground_and_gap.argtypes = [i32, P, P, P, P]
         L.oracle_ground_and_gap.restype = i32
         L.oracle_expect_site_term.argtypes = [i32, i32, P, i32, i32, i32, i32, P]
@@ -216,11 +218,29 @@ def sparsediag(m: Model, n_up, max_iter=0, nev=1, seed=0, fixed=False, max_ab=40
                 iterations=its.value, error_code=err.value, dim=dim.value)
 
 
-def bench_hv(m: Model, n_up, rows, reps=3):
+def bench_hv(m: Model, n_up, rows, reps=3, threads=1):
+    """Median seconds of `reps` map-of-maps H.v over rows [0, rows), on
+    `threads` threads, and the build time of those rows."""
     bs = ctypes.c_double()
     nnz = ctypes.c_long()
-    t = lib().oracle_bench_hv(m.n_sites, n_up, *m.cargs(), rows, reps, ctypes.byref(bs), ctypes.byref(nnz))
+    t = lib().oracle_bench_hv(m.n_sites, n_up, *m.cargs(), rows, reps, threads, ctypes.byref(bs),
+                              ctypes.byref(nnz))
     return dict(seconds=t, build_seconds=bs.value, nnz=nnz.value)
+
+
+PHASES = ("basis", "matrix", "hv", "vector", "tmatrix")
+
+
+def sparsediag_timed(m: Model, n_up, max_iter=0, nev=1, seed=0, fixed=False):
+    """The oracle's do_subspace with wall seconds per phase."""
+    ev = np.zeros(max(nev, 1))
+    ph = np.zeros(len(PHASES))
+    n_ev, its, dim = ctypes.c_int(), ctypes.c_int(), ctypes.c_long()
+    rc = lib().oracle_sparsediag_timed(m.n_sites, n_up, *m.cargs(), max_iter, nev, seed, int(fixed), _p(ev),
+                                       ctypes.byref(n_ev), ctypes.byref(its), ctypes.byref(dim), _p(ph))
+    assert rc == 0
+    return dict(ev=ev[: n_ev.value].copy(), iterations=its.value, dim=dim.value,
+                phases=dict(zip(PHASES, ph.tolist())))
 
 
 SITE_OPS = {"Sz": 0, "Splus": 1, "Sminus": 2, "Sx": 3}
