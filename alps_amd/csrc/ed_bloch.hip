@@ -8,9 +8,11 @@
 // recurrence of ietl::lanczos on the sector (same alpha, beta, T matrix):
 // every other kernel of the engine applies unchanged.
 //
-// One thread per representative row (dev::bloch_row, ed_device.hpp, which
-// the batched kernels share).  Gathers of x_j are scattered; the rows are
-// about D / N, so the kernel is small next to the sector's real H.v.
+// A single momentum sector's operator lists the flips of its rows once on
+// the host (bloch_flip_rows) and streams them (bloch_list_kernel); where the
+// list does not fit its packing, and in the batch, one thread per
+// representative row searches for each flipped state's representative
+// (dev::bloch_row, ed_device.hpp, which the batched kernels share).
 #include <hip/hip_runtime.h>
 
 #include "ed_device.hpp"
@@ -26,6 +28,69 @@ template <class V, bool DOT>
 __global__ __launch_bounds__(kBlock) void bloch_hv_kernel(DevLayout L, DevBloch Bl, const V* __restrict__ x,
                                                           V* __restrict__ y, double* __restrict__ dot_out) {
   const double part = bloch_row<V, DOT>(L, Bl, x, y, blockIdx.x * kBlock + threadIdx.x);
+  if (DOT) store_partial(dot_out, part);
+}
+
+// H.v over the listed flips (DevBloch::row_ptr/ent, bloch_flip_rows):
+// kBlochLanes lanes per row, each adding every kBlochLanes-th entry of the
+// row; the lanes' sums meet by shuffles and the first lane adds the
+// diagonal and stores the row.  A wave's lanes read one contiguous run of
+// entries (the rows of a wave are consecutive in the list); the gathers of
+// x are the only scattered reads.
+template <class V, bool DOT>
+__global__ __launch_bounds__(kBlock) void bloch_list_kernel(DevBloch Bl, const V* __restrict__ x,
+                                                            V* __restrict__ y, double* __restrict__ dot_out) {
+  constexpr uint32_t P = kBlochLanes;
+  const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t i = g / P, sub = g % P;
+  const uint32_t dk = Bl.dk;
+  double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+  // the row's own terms are loaded first, next to the row bounds, so that
+  // their latency overlaps the entry gathers
+  double inv = 0.0, d = 0.0, xr = 0.0, xi = 0.0;
+  if (i < dk) {
+    inv = Bl.inv_norm[i];
+    d = Bl.diag[i];
+    xr = (double)x[i];
+    xi = (double)x[dk + i];
+    const uint32_t k0 = Bl.row_ptr[i], k1 = Bl.row_ptr[i + 1];
+    // two entries per lane and iteration, summed into two pairs
+    uint32_t k = k0 + sub;
+    for (; k + P < k1; k += 2 * P) {
+      const uint32_t e0 = Bl.ent[k], e1 = Bl.ent[k + P];
+      const uint32_t a0 = e0 & Bl.mask_a, c0 = e0 >> Bl.sh_c;
+      const uint32_t a1 = e1 & Bl.mask_a, c1 = e1 >> Bl.sh_c;
+      const double cr0 = Bl.coef_re[c0], ci0 = Bl.coef_im[c0], cr1 = Bl.coef_re[c1], ci1 = Bl.coef_im[c1];
+      const double yr0 = (double)x[a0], yi0 = (double)x[dk + a0];
+      const double yr1 = (double)x[a1], yi1 = (double)x[dk + a1];
+      ar = fma(cr0, yr0, fma(-ci0, yi0, ar));
+      ai = fma(cr0, yi0, fma(ci0, yr0, ai));
+      br = fma(cr1, yr1, fma(-ci1, yi1, br));
+      bi = fma(cr1, yi1, fma(ci1, yr1, bi));
+    }
+    if (k < k1) {
+      const uint32_t e0 = Bl.ent[k];
+      const uint32_t a0 = e0 & Bl.mask_a, c0 = e0 >> Bl.sh_c;
+      const double cr0 = Bl.coef_re[c0], ci0 = Bl.coef_im[c0];
+      const double yr0 = (double)x[a0], yi0 = (double)x[dk + a0];
+      ar = fma(cr0, yr0, fma(-ci0, yi0, ar));
+      ai = fma(cr0, yi0, fma(ci0, yr0, ai));
+    }
+    ar += br;
+    ai += bi;
+  }
+#pragma unroll
+  for (uint32_t o = P / 2; o > 0; o >>= 1) {
+    ar += __shfl_xor(ar, o, 64);
+    ai += __shfl_xor(ai, o, 64);
+  }
+  double part = 0.0;
+  if (i < dk && sub == 0) {
+    const double outr = fma(d, xr, inv * ar), outi = fma(d, xi, inv * ai);
+    y[i] = (V)outr;
+    y[dk + i] = (V)outi;
+    if (DOT) part = fma(xr, outr, xi * outi);
+  }
   if (DOT) store_partial(dot_out, part);
 }
 
@@ -72,10 +137,16 @@ hipError_t launch_bloch_hv(const DevLayout& L, const DevBloch& B, const V* x, V*
                            hipStream_t st) {
   if (B.dk == 0) return hipSuccess;
   const unsigned nb = bloch_blocks(B);
-  if (dot_out)
+  if (B.row_ptr) {
+    if (dot_out)
+      hipLaunchKernelGGL((bloch_list_kernel<V, true>), dim3(nb), dim3(kBlock), 0, st, B, x, y, dot_out);
+    else
+      hipLaunchKernelGGL((bloch_list_kernel<V, false>), dim3(nb), dim3(kBlock), 0, st, B, x, y, dot_out);
+  } else if (dot_out) {
     hipLaunchKernelGGL((bloch_hv_kernel<V, true>), dim3(nb), dim3(kBlock), 0, st, L, B, x, y, dot_out);
-  else
+  } else {
     hipLaunchKernelGGL((bloch_hv_kernel<V, false>), dim3(nb), dim3(kBlock), 0, st, L, B, x, y, dot_out);
+  }
   return hipGetLastError();
 }
 

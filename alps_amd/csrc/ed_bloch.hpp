@@ -262,6 +262,95 @@ inline BlochSector bloch_sector(const BlochOrbits& o, const Translations& t, int
   return b;
 }
 
+// The flips of every kept representative of a momentum sector, listed once
+// so that the H.v streams them instead of searching for each flipped state's
+// representative: for kept representative i (state s) in order, the bonds
+// in edge order that are antiparallel in s and have a flip element, whose
+// flipped state lies in a kept orbit a, give one entry: a and the code of
+// the complex coefficient t_b * norm_a * phase_t (t the translation taking
+// the flipped state to a); the row scales it by 1 / norm_i, which gives the
+// element t_b * phase_t * norm_a / norm_i of hamiltonian_matrix.hpp:308-345
+// with bloch_basis_states::index_and_phase (blochbasisstates.h:47-55).
+// Coefficients are tabulated per (coupling, normalization, translation).
+// The diagonal of row i is summed as the device row does it: the site terms
+// in site order, then each bond's parallel or antiparallel value in bond
+// order.  Entries are packed into 32 bits; `ok` is false when they do not
+// fit (the device then searches per flip).
+struct BlochFlipRows {
+  bool ok = false;
+  std::vector<uint32_t> row_ptr, ent;
+  std::vector<double> diag, inv_norm, coef_re, coef_im;
+  uint32_t sh_c = 0, mask_a = 0;
+};
+
+inline int bits_for(uint64_t max_value) {
+  int b = 1;
+  while (b < 32 && (max_value >> b) != 0) ++b;
+  return b;
+}
+
+inline BlochFlipRows bloch_flip_rows(const SectorLayout& L, const BlochSector& b) {
+  BlochFlipRows r;
+  const uint32_t dk = (uint32_t)b.rep_state.size();
+  // coupling classes: the distinct nonzero flip elements
+  std::map<double, uint32_t> cls;
+  std::vector<uint32_t> bond_cls(L.bond_t.size(), 0);
+  for (size_t q = 0; q < L.bond_t.size(); ++q)
+    if (L.bond_t[q] != 0.0) {
+      auto it = cls.emplace(L.bond_t[q], (uint32_t)cls.size()).first;
+      bond_cls[q] = it->second;
+    }
+  std::vector<double> cls_t(cls.size(), 0.0);
+  for (const auto& kv : cls) cls_t[kv.second] = kv.first;
+  const uint64_t n_tr = b.ph_re.size(), n_ni = std::max<size_t>(b.norm_tab.size(), 1);
+  const uint64_t n_code = std::max<uint64_t>(cls.size(), 1) * n_ni * n_tr;
+  const int ba = bits_for(dk > 0 ? dk - 1 : 0), bc = bits_for(n_code - 1);
+  if (ba + bc > 32 || n_code > (1u << 20)) return r;
+  r.sh_c = (uint32_t)ba;
+  r.mask_a = ba >= 32 ? 0xffffffffu : ((1u << ba) - 1u);
+  r.coef_re.resize(n_code);
+  r.coef_im.resize(n_code);
+  for (uint64_t c = 0; c < cls.size(); ++c)
+    for (uint64_t n = 0; n < b.norm_tab.size(); ++n)
+      for (uint64_t t = 0; t < n_tr; ++t) {
+        const uint64_t code = (c * n_ni + n) * n_tr + t;
+        const double f = cls_t[c] * b.norm_tab[n];
+        r.coef_re[code] = f * b.ph_re[t];
+        r.coef_im[code] = f * b.ph_im[t];
+      }
+  r.row_ptr.reserve(dk + 1);
+  r.row_ptr.push_back(0);
+  r.diag.resize(dk);
+  r.inv_norm.resize(dk);
+  for (uint32_t i = 0; i < dk; ++i) {
+    const uint32_t s = b.rep_state[i];
+    double d = 0.0;
+    for (int site = 0; site < L.N; ++site) d += ((s >> (L.N - 1 - site)) & 1u) ? L.site_up[site] : L.site_dn[site];
+    for (size_t q = 0; q < L.bond_p1.size(); ++q) {
+      const uint32_t p1 = L.bond_p1[q], p2 = L.bond_p2[q];
+      if (((s >> p1) & 1u) == ((s >> p2) & 1u)) {
+        d += L.bond_par[q];
+        continue;
+      }
+      d += L.bond_anti[q];
+      if (L.bond_t[q] == 0.0) continue;
+      const uint32_t rk = split_rank(L, s ^ ((1u << p1) | (1u << p2)));
+      const uint64_t pk = b.full_pk[rk];
+      const uint32_t a = (uint32_t)pk;
+      if (a == kNoState) continue;
+      const uint64_t tr = (pk >> 32) & 0xffu, ni = (pk >> 40) & 0xffffu;
+      const uint64_t code = ((uint64_t)bond_cls[q] * n_ni + ni) * n_tr + tr;
+      r.ent.push_back(a | (uint32_t)(code << r.sh_c));
+    }
+    if (r.ent.size() > 0xffffffffull) return BlochFlipRows{};
+    r.row_ptr.push_back((uint32_t)r.ent.size());
+    r.diag[i] = d;
+    r.inv_norm[i] = 1.0 / b.norm[i];
+  }
+  r.ok = true;
+  return r;
+}
+
 // The orbits of a sector under a set of translations, built once and shared
 // by every caller with the same sector shape and translations (the momenta
 // of one Sz sector): one build even when several threads ask at once.

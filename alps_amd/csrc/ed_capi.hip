@@ -320,6 +320,18 @@ void bloch_upload(alps_ed_operator op, const BlochSector& b) {
   pk.add(b.norm_tab, &d.norm_tab);
   pk.add(b.ph_re, &d.ph_re);
   pk.add(b.ph_im, &d.ph_im);
+  // the flips of every row, listed once (the H.v streams them)
+  const BlochFlipRows fr = bloch_flip_rows(op->layout, b);
+  if (fr.ok) {
+    pk.add(fr.row_ptr, &d.row_ptr);
+    pk.add(fr.ent, &d.ent);
+    pk.add(fr.diag, &d.diag);
+    pk.add(fr.inv_norm, &d.inv_norm);
+    pk.add(fr.coef_re, &d.coef_re);
+    pk.add(fr.coef_im, &d.coef_im);
+    d.sh_c = fr.sh_c;
+    d.mask_a = fr.mask_a;
+  }
   pk.upload(op->btables);
   op->bloch = true;
   op->n = 2 * (uint64_t)b.rep_state.size();

@@ -57,6 +57,18 @@ struct DevBloch {
   const double* norm_tab;     // distinct normalizations
   const double* ph_re;        // [translations]
   const double* ph_im;
+  // The flips of each kept representative, listed once on the host
+  // (bloch_flip_rows, ed_bloch.hpp); null where not built (the batch, which
+  // uses bloch_row).  Entry = target representative | coefficient code <<
+  // sh_c; the code indexes the complex coefficient t_b * norm_a * phase_t
+  // (coef_re, coef_im), which the row scales by 1 / norm_i.
+  const uint32_t* row_ptr;    // [dk + 1]
+  const uint32_t* ent;
+  const double* diag;         // [dk] the diagonal element of each row
+  const double* inv_norm;     // [dk] 1 / norm
+  const double* coef_re;      // per coefficient code
+  const double* coef_im;
+  uint32_t sh_c, mask_a;
 };
 
 // Reductions without atomics: the producing kernel stores one partial sum
@@ -229,9 +241,17 @@ template <class V>
 hipError_t launch_flip_pairs(const DevLayout& L, const V* v, const uint32_t* masks, int n_pairs, double* part,
                              double* out, hipStream_t st);
 
+// Lanes per row of the listed-flips Bloch kernel (4 measured fastest on the
+// L=24 ring's k sectors: 7.3 us against 7.9 us with 2 and 9.3 us with 8).
+constexpr int kBlochLanes = 4;
+// Blocks of a momentum-sector H.v launch, and of its dot partials.
+inline unsigned bloch_blocks(const DevBloch& B) {
+  if (!B.dk) return 1u;
+  const uint64_t threads = (uint64_t)B.dk * (B.row_ptr ? kBlochLanes : 1);
+  return (unsigned)((threads + kBlock - 1) / kBlock);
+}
 // y = H x in a momentum sector (ed_bloch.hip), x and y realified; with
 // dot_out, per-block partials of Re <x, Hx> (bloch_blocks(B) of them).
-inline unsigned bloch_blocks(const DevBloch& B) { return B.dk ? (B.dk + kBlock - 1) / kBlock : 1u; }
 template <class V>
 hipError_t launch_bloch_hv(const DevLayout& L, const DevBloch& B, const V* x, V* y, double* dot_out,
                            hipStream_t st);
