@@ -1579,6 +1579,7 @@ std::vector<std::vector<double>> task_momenta(const Parameters& p, const Evaluat
   loop = false;
   std::vector<std::vector<double>> ks;
   if (ev.defined("TOTAL_MOMENTUM")) {
+    if (g.extent.empty()) throw std::invalid_argument("TOTAL_MOMENTUM needs a LATTICE, not a GRAPH");
     std::vector<double> k;
     std::istringstream is(p.at("TOTAL_MOMENTUM"));
     for (std::string tok; is >> tok;) k.push_back(ev.value(tok));
@@ -1614,7 +1615,7 @@ extern "C" int alps_ed_describe_parameter_file(const char* text, char* out, size
       const Parameters& p = tasks[ti];
       js += ti ? ", {" : "{";
       js += "\"parameters\": " + json_params(p);
-      if (p.count("LATTICE")) {
+      if (p.count("LATTICE") || p.count("GRAPH")) {
         const TaskModel m = task_model(p);
         std::vector<double> s1(m.s1.begin(), m.s1.end()), s2(m.s2.begin(), m.s2.end()), up;
         const TaskSectors ts = task_sectors(p, m.n_sites);
@@ -1623,6 +1624,16 @@ extern "C" int alps_ed_describe_parameter_file(const char* text, char* out, size
               ", \"bond_t\": " + json_list(s2) + ", \"jz\": " + json_list(m.jz) + ", \"jxy\": " + json_list(m.jxy) +
               ", \"k\": " + json_list(m.k) + ", \"h\": " + json_list(m.h) + ", \"d\": " + json_list(m.d) +
               ", \"n_up\": " + json_list(up) + ", \"whole_space\": " + (ts.whole_space ? "true" : "false");
+        // vertex types and coordinates (lattices only), as the lattice
+        // library's graphs give them (src/alps/lattice/graph.h:64-108)
+        std::vector<double> types(m.graph.site_type.begin(), m.graph.site_type.end()), bt;
+        for (const LatticeBond& b : m.graph.bonds) bt.push_back(b.type);
+        js += ", \"site_type\": " + json_list(types) + ", \"bond_type\": " + json_list(bt);
+        if (!m.graph.extent.empty()) {
+          js += ", \"coordinates\": [";
+          for (int s = 0; s < m.n_sites; ++s) js += (s ? ", " : "") + json_list(site_coordinate(m.graph, s));
+          js += "]";
+        }
         bool loop = false;
         const std::vector<std::vector<double>> ks = task_momenta(p, Evaluator(p), m.graph, loop);
         if (!ks.empty()) {
@@ -1631,6 +1642,7 @@ extern "C" int alps_ed_describe_parameter_file(const char* text, char* out, size
           js += "]";
         }
         const MeasureSpec ms = measure_spec(p);
+        require_geometry(ms, m.graph);
         if (ms.any()) js += ", \"measure\": " + json_measure_plan(ms, m.graph);
       }
       js += "}";
@@ -1680,6 +1692,7 @@ extern "C" int alps_ed_run_parameter_file(const char* text, char* out, size_t ou
       // configuration basis and measured there; local values are not
       // reported there, as in the reference (diag.h:212).
       const MeasureSpec ms = measure_spec(p);
+      require_geometry(ms, m.graph);
       const bool measure = ms.any() && !ts.whole_space;
       std::vector<std::string> meas_json(res.size());
       std::vector<MeasureValues> meas_val(res.size());

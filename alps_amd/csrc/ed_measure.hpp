@@ -41,6 +41,13 @@ struct MeasureSpec {
   bool any() const { return !average.empty() || !local.empty() || !correlation.empty() || !structure_factor.empty(); }
 };
 
+// Correlations and structure factors are labelled by lattice distances and
+// momenta, which a GRAPH (no geometry) does not have.
+inline void require_geometry(const MeasureSpec& m, const LatticeGraph& g) {
+  if ((!m.correlation.empty() || !m.structure_factor.empty()) && g.extent.empty())
+    throw std::invalid_argument("MEASURE_CORRELATIONS and MEASURE_STRUCTURE_FACTOR need a LATTICE, not a GRAPH");
+}
+
 inline MeasureSpec measure_spec(const Parameters& p) {
   MeasureSpec m;
   auto inner = [](const std::string& key, const std::string& prefix, std::string& name) {

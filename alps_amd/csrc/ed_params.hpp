@@ -436,6 +436,13 @@ class Evaluator {
         i = e;
         skip();
         if (i < s.size() && s[i] == '(') {
+          // the disorder functions draw from the DISORDER_SEED generator
+          // (src/alps/expression/evaluator.h:196-228) in the order the
+          // reference builds its matrix; couplings here are deterministic
+          if (name == "random" || name == "gaussian_random" || name == "normal_random" ||
+              name == "integer_random")
+            throw std::invalid_argument("random couplings (" + name +
+                                        "()) are not supported: give each edge of a GRAPH its own type and value");
           ++i;
           const double a = sum();
           skip();

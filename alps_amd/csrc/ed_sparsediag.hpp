@@ -4,6 +4,8 @@
 //   - lattice graphs: make_graph_from_lattice (src/alps/lattice/
 //     latticegraph.h:35-171) over the hypercubic finite lattices
 //     (hypercubic.h:110-200,253-280) and unit cells of lib/xml/lattices.xml.in;
+//   - explicit graphs (GRAPH, LATTICE_LIBRARY): graph_helper::make_graph
+//     (graph_helper.h:651-700) and read_graph_xml (graph.h:120-215);
 //   - the "spin" HAMILTONIAN's parameters and per-type couplings
 //     (lib/xml/models.xml.in:342-373, '#' substitution model/substitute.h);
 //   - DiagMatrix::dostep / build_subspaces (applications/diag/diag.h:85-175):
@@ -16,7 +18,10 @@
 #pragma once
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
+#include <cstdlib>
+#include <fstream>
 #include <limits>
 #include <sstream>
 #include <stdexcept>
@@ -141,6 +146,227 @@ inline LatticeGraph named_lattice_graph(const std::string& name, const Evaluator
   throw std::invalid_argument("unsupported LATTICE '" + name + "'");
 }
 
+inline bool has_named_lattice(const std::string& name) {
+  static const char* names[] = {"dimer", "chain lattice", "open chain lattice", "nnn chain lattice",
+                                "nnn open chain lattice", "square lattice", "open square lattice",
+                                "anisotropic square lattice", "ladder", "open ladder", "frustrated square lattice",
+                                "triangular lattice", "anisotropic triangular lattice", "simple cubic lattice",
+                                "open simple cubic lattice"};
+  for (const char* n : names)
+    if (name == n) return true;
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// GRAPH: explicit graphs of a lattice library
+// ---------------------------------------------------------------------------
+// One XML tag: name ("/NAME" for a closing tag), attributes, and whether it
+// closes itself ("<EDGE .../>").
+struct XmlTag {
+  std::string name;
+  std::vector<std::pair<std::string, std::string>> attr;
+  bool single = false;
+  std::string get(const std::string& k) const {
+    for (const auto& a : attr)
+      if (a.first == k) return a.second;
+    return "";
+  }
+};
+
+// The tags of an XML text in order; declarations, comments and text between
+// tags are skipped.
+inline std::vector<XmlTag> xml_tags(const std::string& s) {
+  std::vector<XmlTag> out;
+  size_t i = 0;
+  while ((i = s.find('<', i)) != std::string::npos) {
+    if (s.compare(i, 4, "<!--") == 0) {
+      const size_t e = s.find("-->", i + 4);
+      if (e == std::string::npos) throw std::invalid_argument("unterminated comment in lattice library");
+      i = e + 3;
+      continue;
+    }
+    if (i + 1 < s.size() && (s[i + 1] == '?' || s[i + 1] == '!')) {
+      const size_t e = s.find('>', i);
+      if (e == std::string::npos) throw std::invalid_argument("unterminated declaration in lattice library");
+      i = e + 1;
+      continue;
+    }
+    XmlTag t;
+    size_t j = i + 1;
+    auto ws = [&] {
+      while (j < s.size() && std::isspace((unsigned char)s[j])) ++j;
+    };
+    auto word = [&] {
+      const size_t b = j;
+      if (j == i + 1 && j < s.size() && s[j] == '/') ++j;  // the '/' of a closing tag
+      while (j < s.size() && !std::isspace((unsigned char)s[j]) && s[j] != '>' && s[j] != '/' && s[j] != '=') ++j;
+      return s.substr(b, j - b);
+    };
+    t.name = word();
+    if (t.name.empty()) throw std::invalid_argument("malformed tag in lattice library");
+    for (;;) {
+      ws();
+      if (j >= s.size()) throw std::invalid_argument("unterminated tag <" + t.name + "> in lattice library");
+      if (s[j] == '>') {
+        ++j;
+        break;
+      }
+      if (s[j] == '/' && j + 1 < s.size() && s[j + 1] == '>') {
+        t.single = true;
+        j += 2;
+        break;
+      }
+      const std::string k = word();
+      ws();
+      if (k.empty() || j >= s.size() || s[j] != '=')
+        throw std::invalid_argument("malformed attribute in <" + t.name + "> of lattice library");
+      ++j;
+      ws();
+      if (j >= s.size() || (s[j] != '"' && s[j] != '\''))
+        throw std::invalid_argument("unquoted attribute '" + k + "' in <" + t.name + ">");
+      const char q = s[j++];
+      const size_t e = s.find(q, j);
+      if (e == std::string::npos) throw std::invalid_argument("unterminated attribute '" + k + "'");
+      t.attr.emplace_back(k, s.substr(j, e - j));
+      j = e + 1;
+    }
+    out.push_back(std::move(t));
+    i = j;
+  }
+  return out;
+}
+
+inline int xml_int(const std::string& v, const char* what) {
+  char* end = nullptr;
+  const long x = std::strtol(v.c_str(), &end, 10);
+  if (v.empty() || *end != '\0') throw std::invalid_argument(std::string("bad ") + what + " '" + v + "' in <GRAPH>");
+  return (int)x;
+}
+
+// The <GRAPH> elements of a lattice library, read as read_graph_xml does
+// (src/alps/lattice/graph.h:120-215): vertices="N" fixes the vertex count
+// (a VERTEX id beyond it is an error); otherwise vertices are added up to
+// the largest id used.  VERTEX: id 1-based (default: the next one), type
+// (default 0), an optional COORDINATE inside.  EDGE: source and target
+// 1-based, type (default 0), in listed order; an edge's end beyond the
+// vertex count adds vertices (boost::add_edge on a vector-backed graph).
+inline std::vector<std::pair<std::string, LatticeGraph>> library_graphs(const std::string& xml) {
+  std::vector<std::pair<std::string, LatticeGraph>> out;
+  const std::vector<XmlTag> tags = xml_tags(xml);
+  for (size_t i = 0; i < tags.size(); ++i) {
+    if (tags[i].name != "GRAPH") continue;
+    const XmlTag& gt = tags[i];
+    LatticeGraph g;
+    const std::string nv = gt.get("vertices");
+    const bool fixed = !nv.empty();
+    if (fixed) g.n = xml_int(nv, "vertices");
+    if (g.n < 0) throw std::invalid_argument("negative vertex count in <GRAPH>");
+    g.site_type.assign(g.n, 0);
+    auto grow = [&](int n) {
+      if (n > g.n) {
+        g.n = n;
+        g.site_type.resize(n, 0);
+      }
+    };
+    int next_vertex = 0;
+    if (!gt.single) {
+      for (++i;; ++i) {
+        if (i >= tags.size()) throw std::invalid_argument("closing </GRAPH> tag missing");
+        const XmlTag& t = tags[i];
+        if (t.name == "/GRAPH") break;
+        if (t.name == "VERTEX") {
+          const std::string ty = t.get("type"), id = t.get("id");
+          const int v = id.empty() ? next_vertex++ : xml_int(id, "VERTEX id") - 1;
+          if (v < 0) throw std::invalid_argument("VERTEX id must be >= 1");
+          if (v >= g.n) {
+            if (fixed) throw std::invalid_argument("too many vertices in <GRAPH>");
+            grow(v + 1);
+          }
+          g.site_type[v] = ty.empty() ? 0 : xml_int(ty, "VERTEX type");
+          if (!t.single) {
+            ++i;
+            if (i < tags.size() && tags[i].name == "COORDINATE") {
+              if (!tags[i].single) {
+                ++i;
+                if (i >= tags.size() || tags[i].name != "/COORDINATE")
+                  throw std::invalid_argument("closing </COORDINATE> tag missing");
+              }
+              ++i;
+            }
+            if (i >= tags.size() || tags[i].name != "/VERTEX")
+              throw std::invalid_argument("closing </VERTEX> tag missing");
+          }
+        } else if (t.name == "EDGE") {
+          const int s = xml_int(t.get("source"), "EDGE source") - 1;
+          const int d = xml_int(t.get("target"), "EDGE target") - 1;
+          const std::string ty = t.get("type");
+          if (s < 0 || d < 0) throw std::invalid_argument("EDGE source and target must be >= 1");
+          if (s == d) throw std::invalid_argument("an EDGE from a vertex to itself is not a bond");
+          if (!t.single) {
+            ++i;
+            if (i >= tags.size() || tags[i].name != "/EDGE")
+              throw std::invalid_argument("Nonempty <EDGE> tag in <GRAPH>");
+          }
+          grow(std::max(s, d) + 1);
+          g.bonds.push_back({s, d, ty.empty() ? 0 : xml_int(ty, "EDGE type")});
+        } else {
+          throw std::invalid_argument("illegal element <" + t.name + "> in <GRAPH>");
+        }
+      }
+    }
+    out.emplace_back(gt.get("name"), std::move(g));
+  }
+  return out;
+}
+
+// The GRAPHs of lib/xml/lattices.xml.in:617-641.
+inline std::vector<std::pair<std::string, LatticeGraph>> builtin_graphs() {
+  auto make = [](int n, std::vector<int> types, std::vector<LatticeBond> bonds) {
+    LatticeGraph g;
+    g.n = n;
+    g.site_type = types.empty() ? std::vector<int>(n, 0) : types;
+    g.bonds = std::move(bonds);
+    return g;
+  };
+  return {{"5-site dimerized", make(5, {}, {{0, 1, 0}, {1, 2, 1}, {2, 3, 0}, {3, 4, 1}})},
+          {"4-site mixed",
+           make(4, {0, 1, 0, 1}, {{0, 1, 0}, {1, 2, 0}, {2, 3, 0}, {3, 0, 0}, {0, 2, 1}, {1, 3, 1}})},
+          {"triangle", make(3, {}, {{0, 1, 0}, {1, 2, 0}, {2, 0, 0}})}};
+}
+
+inline std::string read_text_file(const std::string& path, const char* what) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) throw std::invalid_argument(std::string("Could not find ") + what + " " + path);
+  std::ostringstream o;
+  o << f.rdbuf();
+  return o.str();
+}
+
+// The graph of a task (graph_helper::make_graph, src/alps/lattice/
+// graph_helper.h:651-700): GRAPH and LATTICE together are an error; a
+// LATTICE name that is one of the lattices above builds that lattice;
+// otherwise the name (of GRAPH or LATTICE) is looked up among the GRAPHs of
+// the library.  The library is the file LATTICE_LIBRARY names (read from
+// the path as given), else the GRAPHs of lattices.xml; the lattices above
+// stay available with either library.  A GRAPH has no geometry: no
+// translations or momenta, and no distance labels.
+inline LatticeGraph task_graph(const Parameters& in, const Evaluator& ev) {
+  const bool have_graph = in.count("GRAPH") != 0, have_lattice = in.count("LATTICE") != 0;
+  if (have_graph && have_lattice) throw std::invalid_argument("both GRAPH and LATTICE were specified");
+  if (!have_graph && !have_lattice) throw std::invalid_argument("LATTICE is not defined");
+  const std::string name = in.at(have_graph ? "GRAPH" : "LATTICE");
+  if (have_lattice && has_named_lattice(name)) return named_lattice_graph(name, ev);
+  const auto graphs = in.count("LATTICE_LIBRARY")
+                          ? library_graphs(read_text_file(in.at("LATTICE_LIBRARY"), "lattice library file"))
+                          : builtin_graphs();
+  for (const auto& ng : graphs)
+    if (ng.first == name) {
+      if (ng.second.n < 1) throw std::invalid_argument("GRAPH '" + name + "' has no vertices");
+      return ng.second;
+    }
+  throw std::invalid_argument("could not find graph/lattice specified in parameters: name was: '" + name + "'");
+}
+
 // Parameter defaults of the spin HAMILTONIAN (models.xml.in:342-359) and of
 // its bond and site terms for the types present ('#' replaced by the type).
 inline Parameters with_spin_defaults(Parameters p, const LatticeGraph& g) {
@@ -182,11 +408,10 @@ struct TaskModel {
 inline TaskModel task_model(const Parameters& in) {
   if (in.count("MODEL") && in.at("MODEL") != "spin")
     throw std::invalid_argument("only MODEL=\"spin\" is supported (got '" + in.at("MODEL") + "')");
-  if (!in.count("LATTICE")) throw std::invalid_argument("LATTICE is not defined");
   Evaluator ev0(in);
   if (ev0.defined("local_S") && std::fabs(ev0.param("local_S") - 0.5) > 1e-12)
     throw std::invalid_argument("only local_S=1/2 is supported");
-  const LatticeGraph g = named_lattice_graph(in.at("LATTICE"), ev0);
+  const LatticeGraph g = task_graph(in, ev0);
   const Parameters p = with_spin_defaults(in, g);
   Evaluator ev(p);
   TaskModel m;

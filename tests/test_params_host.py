@@ -141,7 +141,7 @@ def test_sectors():
     ('LATTICE="chain lattice"; L=4; MODEL="boson Hubbard"; {}', "MODEL"),
     ('LATTICE="chain lattice"; L=4; local_S=1; {}', "local_S"),
     ('LATTICE="chain lattice"; L=4; Gamma=0.5; {}', "Gamma"),
-    ('LATTICE="kagome lattice"; L=4; {}', "LATTICE"),
+    ('LATTICE="kagome lattice"; L=4; {}', "could not find graph/lattice"),
     ("L=4; { J=1 ", "unterminated"),
     ('LATTICE="chain lattice"; { J=1 }', "L"),
 ])
