@@ -335,6 +335,7 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1, bool c
   // top three bits of the high word, block b takes its segment from group
   // b % 8: most high-high neighbours (hi ^ mask with the top bits unchanged)
   // of a segment are then read on the XCD that owns them, from its L2.
+  bool grouped = false;
   if (L.NH >= 3) {
     std::vector<std::vector<uint32_t>> grp(kXcds);
     for (uint32_t h : L.proc_hi) grp[h >> (L.NH - 3)].push_back(h);
@@ -344,6 +345,33 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1, bool c
       L.proc_hi.clear();
       for (size_t i = 0; i < grp[0].size(); ++i)
         for (int g = 0; g < kXcds; ++g) L.proc_hi.push_back(grp[g][i]);
+      grouped = true;
+    }
+  }
+  // Otherwise (popcount(hi) limited to a range narrower than 0..NH, so the
+  // top bits split the segments unevenly) a large sector is dealt class by
+  // class: each class, in address order, is cut into kXcds contiguous runs
+  // of equal count, and the position p of the order takes from run p % 8,
+  // i.e. XCD g sweeps run g of every class.  The high-high and cross
+  // neighbours of a segment (hi ^ mask, near hi in address when the flipped
+  // bits are low ones) are then read mostly on the XCD that reads hi.  Equal
+  // counts of one class are equal work, so the XCDs stay balanced.
+  if (!grouped && L.seg_hi.size() >= 64u * kXcds) {
+    std::vector<int> cls_order;
+    std::vector<std::vector<uint32_t>> by_pc(L.NH + 1);
+    for (uint32_t h : L.proc_hi) {
+      const int c = __builtin_popcount(h);
+      if (by_pc[c].empty()) cls_order.push_back(c);
+      by_pc[c].push_back(h);
+    }
+    L.proc_hi.clear();
+    for (int c : cls_order) {
+      std::vector<uint32_t>& v = by_pc[c];
+      std::sort(v.begin(), v.end());  // address order: hi_base ascends with hi
+      const size_t base = v.size() / kXcds, s = L.proc_hi.size();
+      for (size_t i = 0; i < base; ++i)
+        for (int j = 0; j < kXcds; ++j) L.proc_hi.push_back(v[((s + j) % kXcds) * base + i]);
+      for (size_t r = base * kXcds; r < v.size(); ++r) L.proc_hi.push_back(v[r]);
     }
   }
   // terms in bit coordinates
