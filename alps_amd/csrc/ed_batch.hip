@@ -539,7 +539,7 @@ void decide(Problem& p) {
     p.lam_prev = lam;
     if (may) {
       std::vector<double> a(p.host_alpha, p.host_alpha + n), bb(p.host_beta, p.host_beta + n);
-      TValues t = analyze_prefix(a, bb, n, p.fp32);
+      TValues t = stop_analysis(a, bb, n, p.fp32, p.nev);
       if (nlowest_converged(t, p.nev, p.fp32)) {
         p.final_n = n;
         p.tv = std::move(t);  // conclude() uses this analysis of T[0..n)
@@ -564,7 +564,7 @@ void conclude(Problem& p) {
   }
   if (p.tv_n != p.final_n) {
     std::vector<double> a(p.host_alpha, p.host_alpha + p.final_n), bb(p.host_beta, p.host_beta + p.final_n);
-    p.tv = analyze_prefix(a, bb, p.final_n, p.fp32);
+    p.tv = stop_analysis(a, bb, p.final_n, p.fp32, p.nev);
     p.tv_n = p.final_n;
   }
   if (!p.fp32) {
@@ -932,6 +932,12 @@ void refine(alps_ed_batch b) {
     }
     std::sort(p.ev.begin(), p.ev.end());
     p.cand.clear();
+    if ((int)p.ev.size() < p.nev && p.next_cand >= p.tv.ev.size() && !p.tv.complete) {
+      // more candidates than the stopping test listed: the full analysis
+      // (whose first entries are the same) lists them all
+      std::vector<double> a(p.host_alpha, p.host_alpha + p.final_n), bb(p.host_beta, p.host_beta + p.final_n);
+      p.tv = analyze_prefix(a, bb, p.final_n, p.fp32);
+    }
     if ((int)p.ev.size() >= p.nev || p.next_cand >= p.tv.ev.size()) p.state = kDone;
   }
 }

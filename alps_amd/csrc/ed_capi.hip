@@ -1351,7 +1351,8 @@ SectorSolve solve_sector(const alps_ed_model* model, const alps_ed_solve_params*
       return std::min(max_iter, to);
     };
     auto holds = [&](int m) {
-      return may_be_converged(a.data(), b.data(), m, fp32) && nlowest_converged(analyze_prefix(a, b, m, fp32), nev, fp32);
+      return may_be_converged(a.data(), b.data(), m, fp32) &&
+             nlowest_converged(stop_analysis(a, b, m, fp32, nev), nev, fp32);
     };
     int checked = 1, err_n = -1;
     double err_prev = 0.0;
@@ -1409,7 +1410,7 @@ SectorSolve solve_sector(const alps_ed_model* model, const alps_ed_solve_params*
       if (pending.empty()) queue_to(next_target(queued));
     }
   }
-  TValues t = analyze_prefix(a, b, final_n, fp32);
+  TValues t = stop_analysis(a, b, final_n, fp32, nev);
   std::vector<double> ev;
   if (!fp32) {
     ev.assign(t.ev.begin(), t.ev.begin() + std::min<size_t>(nev, t.ev.size()));
@@ -1421,7 +1422,11 @@ SectorSolve solve_sector(const alps_ed_model* model, const alps_ed_solve_params*
     // relative are the same level reached twice and are reported once.
     size_t next = 0;
     std::vector<std::pair<double, std::unique_ptr<alps_ed_vector_s>>> kept;
-    while ((int)kept.size() < nev && next < t.ev.size()) {
+    while ((int)kept.size() < nev) {
+      // more candidates than the stopping test listed: the full analysis
+      // (whose first entries are the same) lists them all
+      if (next >= t.ev.size() && !t.complete) t = analyze_prefix(a, b, final_n, fp32);
+      if (next >= t.ev.size()) break;
       const size_t cnt = std::min<size_t>(t.ev.size() - next, (size_t)(nev - (int)kept.size()));
       std::vector<int> idx(t.index.begin() + next, t.index.begin() + next + cnt);
       std::vector<std::unique_ptr<alps_ed_vector_s>> vs;

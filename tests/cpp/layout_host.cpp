@@ -151,8 +151,87 @@ static void flip_check(long& bad, long& checked) {
   }
 }
 
+// The H.v processing order is a permutation of the segments, and where the
+// classes are dealt in runs, position p of a class reads run p % 8 of it in
+// address order.
+static long proc_check() {
+  long bad = 0;
+  const int cases[][2] = {{24, 12}, {30, 15}, {28, 13}, {26, 12}, {20, 10}, {27, 14}};
+  for (const auto& c : cases) {
+    ModelSpec s;
+    s.n_sites = c[0];
+    s.n_up = c[1];
+    for (int i = 0; i < c[0]; ++i) {
+      s.bonds.push_back(make_bond(i, (i + 1) % c[0], 1.0, 1.0, 0.0));
+      s.sites.push_back(make_site(0.0, 0.0));
+    }
+    const SectorLayout L = build_layout(s);
+    std::vector<uint32_t> a = L.seg_hi, b = L.proc_hi;
+    std::sort(a.begin(), a.end());
+    std::sort(b.begin(), b.end());
+    if (a != b) ++bad;
+    // within one class, the positions dealt to one XCD ascend in address
+    std::vector<std::vector<uint32_t>> last(L.NH + 1, std::vector<uint32_t>(kXcds, 0));
+    std::vector<std::vector<bool>> seen(L.NH + 1, std::vector<bool>(kXcds, false));
+    if (L.seg_hi.size() >= 64u * kXcds && L.NH >= 3) {
+      long order_bad = 0;
+      for (size_t p = 0; p < L.proc_hi.size(); ++p) {
+        const uint32_t h = L.proc_hi[p];
+        const int pc = __builtin_popcount(h), g = (int)(p % kXcds);
+        if (seen[pc][g] && h < last[pc][g]) ++order_bad;
+        seen[pc][g] = true;
+        last[pc][g] = h;
+      }
+      // the top-bit grouping (N=24) deals by hi >> (NH-3) instead: skip it
+      if (c[0] != 24 && order_bad > (long)(L.NH + 1) * kXcds) ++bad;
+    }
+  }
+  return bad;
+}
+
+// Parameter files and explicit graphs through the host parsers (the
+// sanitizer build runs them too): tasks, evaluated couplings, graph bonds,
+// and inputs that must be rejected with an error rather than misread.
+static long parse_check() {
+  long bad = 0;
+  const std::string text =
+      "LATTICE=\"chain lattice\"; L=8; J=1; Jz='J*0.5'; NUMBER_EIGENVALUES=2;\n"
+      "{ Sz_total=0 }\n{ Sz_total=1; h=0.25 }\n";
+  std::vector<Parameters> tasks = parse_input(text);
+  const std::vector<Parameters> more = parse_input("GRAPH=\"4-site mixed\"; { J0=1; J1=0.5; h1=0.1 }\n");
+  tasks.insert(tasks.end(), more.begin(), more.end());
+  if (tasks.size() != 3) ++bad;
+  for (const Parameters& p : tasks) {
+    const TaskModel m = task_model(p);
+    const TaskSectors ts = task_sectors(p, m.n_sites);
+    if (m.s1.size() != m.jz.size() || ts.n_up.empty()) ++bad;
+  }
+  const std::string xml =
+      "<?xml version=\"1.0\"?>\n<!-- graphs -->\n<LATTICES>\n"
+      "<GRAPH name=\"g\" vertices=\"3\"><VERTEX id=\"2\" type=\"1\"><COORDINATE>0 1</COORDINATE></VERTEX>"
+      "<EDGE source=\"1\" target=\"2\" type=\"3\"/><EDGE source=\"2\" target=\"3\"></EDGE></GRAPH>\n"
+      "<GRAPH name='h'><EDGE source='1' target='5'/></GRAPH></LATTICES>\n";
+  const auto gs = library_graphs(xml);
+  if (gs.size() != 2 || gs[0].second.n != 3 || gs[0].second.bonds.size() != 2 || gs[0].second.site_type[1] != 1 ||
+      gs[1].second.n != 5)
+    ++bad;
+  const char* rejected[] = {"<GRAPH name=\"x\"><EDGE source=\"1\"", "<GRAPH name=\"x\" vertices=\"2\"><VERTEX id=\"9\"/>",
+                            "<GRAPH name=\"x\"><EDGE source=\"0\" target=\"1\"/></GRAPH>", "<GRAPH name=x>",
+                            "<!-- open", "<GRAPH name=\"x\"><BOND/></GRAPH>"};
+  for (const char* r : rejected) {
+    try {
+      library_graphs(r);
+      ++bad;
+    } catch (const std::invalid_argument&) {
+    }
+  }
+  return bad;
+}
+
 int main() {
   std::printf("diag_err %.3e\n", diag_error());
+  std::printf("parse_bad %ld\n", parse_check());
+  std::printf("proc_bad %ld\n", proc_check());
   long bad = 0, checked = 0;
   flip_check(bad, checked);
   std::printf("flip_bad %ld flip_checked %ld\n", bad, checked);

@@ -20,3 +20,5 @@ def test_layout_diagonal_and_bloch_bases():
     # 5 lattices x every Sz sector x every momentum, minus none
     assert int(vals["bloch_sectors"]) == 417
     assert int(vals["flip_bad"]) == 0 and int(vals["flip_checked"]) > 10**6
+    assert int(vals["proc_bad"]) == 0
+    assert int(vals["parse_bad"]) == 0
