@@ -84,6 +84,9 @@ def lib():
             "alps_ed_lanczos_steps": ([P, i32], i32),
             "alps_ed_lanczos_coefficients": ([P, P, P, i32, P], i32),
             "alps_ed_lanczos_vectors": ([P, P, P], i32),
+            "alps_ed_lanczos_snapshot_size": ([P, P], i32),
+            "alps_ed_lanczos_save": ([P, P, ctypes.c_size_t], i32),
+            "alps_ed_lanczos_load": ([P, ctypes.c_char_p, ctypes.c_size_t], i32),
             "alps_ed_tmatrix_analyze": ([i32, P, P, P, P, P, P], i32),
             "alps_ed_sparsediag": ([P, P, P], i32),
             "alps_ed_run_parameter_file": ([ctypes.c_char_p, P, ctypes.c_size_t, P], i32),
@@ -188,12 +191,23 @@ class SpinModel:
         self.d = np.ascontiguousarray(np.broadcast_to(z if self.d is None else self.d, (self.n_sites,)), dtype=np.float64)
         if self.n_up < 0:
             self.n_up = self.n_sites // 2
+        self.cstruct()
 
     def cstruct(self):
+        """The alps_ed_model view of this model (pointers into its arrays),
+        kept while the fields are the same objects: building it costs more
+        than a batch submit itself."""
+        key = (self.n_sites, self.n_up, id(self.bond_site1), id(self.bond_site2), id(self.jz), id(self.jxy),
+               id(self.k), id(self.h), id(self.d))
+        cached = self.__dict__.get("_cstruct")
+        if cached is not None and cached[0] == key:
+            return cached[1]
         m = _Model()
         m.n_sites, m.n_up, m.n_bonds = self.n_sites, self.n_up, len(self.bond_site1)
-        m.bond_site1, m.bond_site2 = _p(self.bond_site1), _p(self.bond_site2)
-        m.jz, m.jxy, m.k, m.h, m.d = _p(self.jz), _p(self.jxy), _p(self.k), _p(self.h), _p(self.d)
+        addr = lambda a: a.__array_interface__["data"][0]
+        m.bond_site1, m.bond_site2 = addr(self.bond_site1), addr(self.bond_site2)
+        m.jz, m.jxy, m.k, m.h, m.d = addr(self.jz), addr(self.jxy), addr(self.k), addr(self.h), addr(self.d)
+        self.__dict__["_cstruct"] = (key, m)
         return m
 
     def with_sector(self, n_up):
@@ -437,6 +451,19 @@ class Lanczos:
         arr = (ctypes.c_void_p * max(nvec, 1))(*[o.handle for o in outs])
         _check(lib().alps_ed_lanczos_ritz(self.handle, nvec, m, _p(c), arr))
         return outs
+
+    def save(self):
+        """Snapshot of the recurrence (alps_ed_lanczos_save) as bytes."""
+        n = ctypes.c_size_t()
+        _check(lib().alps_ed_lanczos_snapshot_size(self.handle, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(max(1, n.value))
+        _check(lib().alps_ed_lanczos_save(self.handle, buf, n.value))
+        return buf.raw[: n.value]
+
+    def load(self, data: bytes):
+        """Continues a recurrence from a snapshot of save() (a fresh object
+        of the same operator model, dtype, max_steps, keep_basis, reorth)."""
+        _check(lib().alps_ed_lanczos_load(self.handle, data, len(data)))
 
     def vectors(self, dtype="f64"):
         vp, vc = Vector(self.op.dim, dtype), Vector(self.op.dim, dtype)

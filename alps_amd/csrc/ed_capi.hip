@@ -1087,6 +1087,181 @@ int alps_ed_lanczos_vectors(alps_ed_lanczos lz, alps_ed_vector v_prev, alps_ed_v
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+// Snapshot of a device recurrence (lanczos::save / load,
+// src/ietl/lanczos.h:86-98, which archive the T matrix, startvector, vec2
+// and n): everything a later make_step reads.  Layout: the header, then
+// alpha, beta and scale (whole device arrays), the norm partials of the
+// next stage, the saved Lanczos vectors (u_{n-1} and u_n, or the whole kept
+// basis u_0..u_n), and with partial reorthogonalization the omega rows,
+// step flags, Gram-Schmidt coefficients and |H| estimates.
+struct LanczosSnapshotHeader {
+  char magic[8];
+  uint64_t model_hash;  // hash_model of the operator
+  uint64_t n;           // vector length
+  int32_t dtype, keep_basis, reorth, max_steps, steps;
+  uint32_t norm_count;  // partials of the next stage
+  uint64_t vec_bytes;   // bytes of one saved vector
+  uint32_t first_vector, n_vectors;
+  uint64_t bytes[4];    // alpha, beta, scale, norm partials
+  uint64_t reorth_bytes[4];  // omega, flag, gamma, hnorm
+};
+constexpr char kSnapshotMagic[8] = {'A', 'L', 'P', 'S', 'L', 'Z', 'S', '1'};
+
+// FNV-1a over the model the operator was built from (sites, up spins, the
+// reference-order site and bond terms) and the vector length.
+uint64_t hash_model(const alps_ed_operator_s* op) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](const void* p, size_t len) {
+    const unsigned char* c = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < len; ++i) h = (h ^ c[i]) * 1099511628211ull;
+  };
+  const SectorLayout& L = op->layout;
+  mix(&L.N, sizeof L.N);
+  mix(&L.k, sizeof L.k);
+  mix(&op->n, sizeof op->n);
+  const int bl = op->bloch ? 1 : 0;
+  mix(&bl, sizeof bl);
+  auto vec = [&](const auto& v) { mix(v.data(), v.size() * sizeof(v[0])); };
+  vec(L.site_dn);
+  vec(L.site_up);
+  vec(L.bond_p1);
+  vec(L.bond_p2);
+  vec(L.bond_par);
+  vec(L.bond_anti);
+  vec(L.bond_t);
+  return h;
+}
+
+LanczosSnapshotHeader snapshot_header(const alps_ed_lanczos_s* lz) {
+  LanczosSnapshotHeader hd{};
+  std::memcpy(hd.magic, kSnapshotMagic, 8);
+  hd.model_hash = hash_model(lz->op);
+  hd.n = lz->op->n;
+  hd.dtype = lz->dtype;
+  hd.keep_basis = lz->keep_basis;
+  hd.reorth = lz->reorth;
+  hd.max_steps = lz->max_steps;
+  hd.steps = lz->steps;
+  hd.norm_count = lz->steps > 0 ? lz->norm_count[lz->steps & 1] : 0u;
+  hd.vec_bytes = lz->op->n * (lz->dtype == ALPS_ED_F32 ? 4 : 8);
+  hd.first_vector = lz->steps == 0 ? 0u : (lz->keep_basis ? 0u : (uint32_t)(lz->steps - 1));
+  hd.n_vectors = lz->steps == 0 ? 0u : (uint32_t)(lz->steps + 1) - hd.first_vector;
+  hd.bytes[0] = lz->alpha.bytes;
+  hd.bytes[1] = lz->beta.bytes;
+  hd.bytes[2] = lz->scale.bytes;
+  hd.bytes[3] = (uint64_t)hd.norm_count * sizeof(double);
+  if (lz->reorth) {
+    hd.reorth_bytes[0] = lz->r_omega.bytes;
+    hd.reorth_bytes[1] = lz->r_flag.bytes;
+    hd.reorth_bytes[2] = lz->r_gamma.bytes;
+    hd.reorth_bytes[3] = lz->r_hnorm.bytes;
+  }
+  return hd;
+}
+
+uint64_t snapshot_bytes(const LanczosSnapshotHeader& hd) {
+  uint64_t b = sizeof hd + hd.bytes[0] + hd.bytes[1] + hd.bytes[2] + hd.bytes[3] + hd.n_vectors * hd.vec_bytes;
+  for (uint64_t r : hd.reorth_bytes) b += r;
+  return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int alps_ed_lanczos_snapshot_size(alps_ed_lanczos lz, size_t* bytes) {
+  return guarded([&] {
+    require(lz && bytes, "null argument");
+    *bytes = (size_t)snapshot_bytes(snapshot_header(lz));
+  });
+}
+
+int alps_ed_lanczos_save(alps_ed_lanczos lz, void* buf, size_t len) {
+  return guarded([&] {
+    require(lz && buf, "null argument");
+    const LanczosSnapshotHeader hd = snapshot_header(lz);
+    require(len >= snapshot_bytes(hd), "snapshot buffer smaller than alps_ed_lanczos_snapshot_size");
+    char* o = static_cast<char*>(buf);
+    std::memcpy(o, &hd, sizeof hd);
+    o += sizeof hd;
+    auto get = [&](const void* src, uint64_t b) {
+      if (b) hip_check(hipMemcpyAsync(o, src, b, hipMemcpyDeviceToHost, lz->stream), "snapshot copy");
+      o += b;
+    };
+    get(lz->alpha.p, hd.bytes[0]);
+    get(lz->beta.p, hd.bytes[1]);
+    get(lz->scale.p, hd.bytes[2]);
+    if (hd.norm_count) get(norm_buf(lz, lz->steps), hd.bytes[3]);
+    for (uint32_t v = 0; v < hd.n_vectors; ++v) get(lz->slot((int)(hd.first_vector + v)), hd.vec_bytes);
+    if (hd.reorth) {
+      get(lz->r_omega.p, hd.reorth_bytes[0]);
+      get(lz->r_flag.p, hd.reorth_bytes[1]);
+      get(lz->r_gamma.p, hd.reorth_bytes[2]);
+      get(lz->r_hnorm.p, hd.reorth_bytes[3]);
+    }
+    hip_check(hipStreamSynchronize(lz->stream), "sync");
+  });
+}
+
+int alps_ed_lanczos_load(alps_ed_lanczos lz, const void* buf, size_t len) {
+  return guarded([&] {
+    require(lz && buf, "null argument");
+    require(len >= sizeof(LanczosSnapshotHeader), "snapshot too short");
+    LanczosSnapshotHeader hd;
+    std::memcpy(&hd, buf, sizeof hd);
+    require(std::memcmp(hd.magic, kSnapshotMagic, 8) == 0, "not a Lanczos snapshot");
+    require(len >= snapshot_bytes(hd), "snapshot truncated");
+    if (lz->steps != 0) throw StateError("alps_ed_lanczos_load needs a Lanczos object before lanczos_start");
+    require(hd.model_hash == hash_model(lz->op) && hd.n == lz->op->n, "snapshot of a different operator");
+    require(hd.dtype == lz->dtype && hd.keep_basis == (int)lz->keep_basis && hd.max_steps == lz->max_steps,
+            "snapshot of a Lanczos object with another dtype, keep_basis or max_steps");
+    require(hd.reorth == (int)lz->reorth, "snapshot with another reorthogonalization mode");
+    require(hd.steps >= 1 && hd.steps <= hd.max_steps, "snapshot has no steps");
+    const LanczosSnapshotHeader mine = snapshot_header(lz);  // this object's array sizes
+    require(hd.bytes[0] == mine.bytes[0] && hd.bytes[1] == mine.bytes[1] && hd.bytes[2] == mine.bytes[2] &&
+                hd.vec_bytes == mine.vec_bytes && hd.reorth_bytes[0] == mine.reorth_bytes[0] &&
+                hd.reorth_bytes[1] == mine.reorth_bytes[1] && hd.reorth_bytes[2] == mine.reorth_bytes[2] &&
+                hd.reorth_bytes[3] == mine.reorth_bytes[3],
+            "snapshot layout does not match this Lanczos object");
+    require(hd.norm_count <= lz->red.cap, "snapshot partials exceed this object's reduction buffers");
+    const uint32_t want_first = hd.keep_basis ? 0u : (uint32_t)(hd.steps - 1);
+    require(hd.first_vector == want_first && hd.n_vectors == (uint32_t)(hd.steps + 1) - want_first,
+            "snapshot vector set does not match its step count");
+    lz->stream = current_stream();
+    const char* in = static_cast<const char*>(buf) + sizeof hd;
+    auto put = [&](void* dst, uint64_t b) {
+      if (b) hip_check(hipMemcpyAsync(dst, in, b, hipMemcpyHostToDevice, lz->stream), "snapshot copy");
+      in += b;
+    };
+    const char* alpha_in = in;
+    put(lz->alpha.p, hd.bytes[0]);
+    const char* beta_in = in;
+    put(lz->beta.p, hd.bytes[1]);
+    put(lz->scale.p, hd.bytes[2]);
+    if (hd.norm_count) put(norm_buf(lz, hd.steps), hd.bytes[3]);
+    for (uint32_t v = 0; v < hd.n_vectors; ++v) put(lz->slot((int)(hd.first_vector + v)), hd.vec_bytes);
+    if (hd.reorth) {
+      put(lz->r_omega.p, hd.reorth_bytes[0]);
+      put(lz->r_flag.p, hd.reorth_bytes[1]);
+      put(lz->r_gamma.p, hd.reorth_bytes[2]);
+      put(lz->r_hnorm.p, hd.reorth_bytes[3]);
+    }
+    hip_check(hipStreamSynchronize(lz->stream), "sync");
+    // the host mirror of alpha and beta, as the kernels that formed them
+    // would have written it
+    double* mh = static_cast<double*>(lz->mirror.h);
+    const size_t cnt = (size_t)lz->max_steps + 2;
+    std::memcpy(mh, alpha_in, std::min<uint64_t>(hd.bytes[0], cnt * sizeof(double)));
+    std::memcpy(mh + cnt, beta_in, std::min<uint64_t>(hd.bytes[1], cnt * sizeof(double)));
+    lz->norm_count[hd.steps & 1] = hd.norm_count;
+    lz->steps = hd.steps;
+  });
+}
+
 int alps_ed_lanczos_ritz(alps_ed_lanczos lz, int nvec, int m, const double* coeffs, alps_ed_vector* outs) {
   return guarded([&] {
     require(lz && coeffs && outs, "null argument");

@@ -234,6 +234,23 @@ int alps_ed_lanczos_coefficients(alps_ed_lanczos lz, double* alpha, double* beta
  * into caller vectors of the same dimension. */
 int alps_ed_lanczos_vectors(alps_ed_lanczos lz, alps_ed_vector v_prev, alps_ed_vector v_cur);
 
+/* Snapshot and restore of a recurrence (lanczos::save / load,
+ * src/ietl/lanczos.h:86-98, which archive the T matrix, the last two
+ * Lanczos vectors and the step count), e.g. to free the device for other
+ * work and continue later.  The snapshot is a host byte image: alpha, beta,
+ * the normalizations, the reduction partials of the next step, the last two
+ * Lanczos vectors (the whole kept basis with keep_basis) and the partial
+ * reorthogonalization state.  alps_ed_lanczos_save synchronizes.
+ * alps_ed_lanczos_load takes a fresh object (before lanczos_start) of an
+ * operator of the same model, with the same dtype, max_steps, keep_basis and
+ * reorthogonalization mode; steps after it give bit-identical coefficients to
+ * an uninterrupted run.  Errors: ALPS_ED_ERR_ARGUMENT for a buffer that is
+ * too short, not a snapshot, or of another operator or configuration;
+ * ALPS_ED_ERR_STATE for a load into an object that has started. */
+int alps_ed_lanczos_snapshot_size(alps_ed_lanczos lz, size_t* bytes);
+int alps_ed_lanczos_save(alps_ed_lanczos lz, void* buf, size_t len);
+int alps_ed_lanczos_load(alps_ed_lanczos lz, const void* buf, size_t len);
+
 /* Ritz vectors from the kept basis (the basis transformation of
  * lanczos::eigenvectors, src/ietl/lanczos.h:242-281, without the replay):
  * outs[i] = sum_{j<m} coeffs[i*m + j] v_j for i < nvec, v_j the normalized

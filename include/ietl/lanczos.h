@@ -208,6 +208,20 @@ class lanczos : public Tmatrix<VS> {
   const vector_type& current_vector() const { return vec2; }
   unsigned int steps() const { return n; }
 
+  // Archive the recurrence (src/ietl/lanczos.h:86-98): the T matrix, the
+  // current pair of Lanczos vectors, the step count and the eigenvector
+  // bookkeeping, in the reference's order.
+  template <class Archive>
+  void save(Archive& ar) const {
+    static_cast<const super_type&>(*this).save(ar);
+    ar << startvector << vec2 << n << M1 << M2 << Ma << Tvectors;
+  }
+  template <class Archive>
+  void load(Archive& ar) {
+    static_cast<super_type&>(*this).load(ar);
+    ar >> startvector >> vec2 >> n >> M1 >> M2 >> Ma >> Tvectors;
+  }
+
  private:
   template <class GEN>
   std::pair<magnitude_type, magnitude_type> make_first_step(GEN gen) {

@@ -3,13 +3,19 @@
 // and prints alpha/beta and the distinct non-ghost eigenvalues.
 //   usage: ietl_host_lanczos <csr file> <max_iter> <nev> <seed> <fixed 0|1> [vectors]
 // With "vectors", also the Ritz vectors of the reported eigenvalues
-// (lanczos::eigenvectors) with their Info.
+// (lanczos::eigenvectors) with their Info.  With "resume" (fixed runs): the
+// run is also made in two halves, the first archived by lanczos::save into
+// a byte archive and continued by a fresh object after lanczos::load, and
+// "resume_equal 1" says the halves give the uninterrupted alpha/beta bit
+// for bit.
 // CSR file: D nnz, then D+1 row pointers, nnz columns, nnz values (text).
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
 #include <iterator>
+#include <string>
+#include <type_traits>
 #include <vector>
 
 #include <ietl/vectorspace.h>
@@ -34,6 +40,40 @@ inline void mult(const csr_matrix& m, const std::vector<double>& x, std::vector<
 // the matrix binding must be visible before the algorithm, as with the
 // reference's interface headers
 #include <ietl/lanczos.h>
+
+// byte archives with the << / >> interface lanczos::save / load use
+struct byte_oarchive {
+  std::string data;
+  template <class T>
+  typename std::enable_if<std::is_arithmetic<T>::value, byte_oarchive&>::type operator<<(const T& x) {
+    data.append(reinterpret_cast<const char*>(&x), sizeof x);
+    return *this;
+  }
+  template <class T>
+  byte_oarchive& operator<<(const std::vector<T>& v) {
+    *this << (uint64_t)v.size();
+    for (const T& e : v) *this << e;
+    return *this;
+  }
+};
+struct byte_iarchive {
+  const std::string& data;
+  size_t at = 0;
+  template <class T>
+  typename std::enable_if<std::is_arithmetic<T>::value, byte_iarchive&>::type operator>>(T& x) {
+    data.copy(reinterpret_cast<char*>(&x), sizeof x, at);
+    at += sizeof x;
+    return *this;
+  }
+  template <class T>
+  byte_iarchive& operator>>(std::vector<T>& v) {
+    uint64_t n = 0;
+    *this >> n;
+    v.resize(n);
+    for (T& e : v) *this >> e;
+    return *this;
+  }
+};
 
 // splitmix64 counter generator, uniform on [-1, 1)
 struct start_gen {
@@ -80,7 +120,22 @@ int main(int argc, char** argv) {
   for (size_t i = 0; i < lz.diagonal().size(); ++i) printf("ab %.17g %.17g\n", lz.diagonal()[i], lz.subdiagonal()[i]);
   const std::vector<double>& ev = lz.eigenvalues();
   for (size_t i = 0; i < ev.size() && i < (size_t)nev; ++i) printf("ev %.17g\n", ev[i]);
-  if (argc > 6) {
+  if (argc > 6 && std::string(argv[6]) == "resume") {
+    const int half = max_iter / 2;
+    ietl::lanczos<csr_matrix, vs_t> a(m, vs);
+    ietl::fixed_lanczos_iteration<double> it1(half);
+    a.calculate_eigenvalues(it1, start_gen{seed});
+    byte_oarchive oa;
+    a.save(oa);
+    ietl::lanczos<csr_matrix, vs_t> b(m, vs);
+    byte_iarchive ia{oa.data};
+    b.load(ia);
+    ietl::fixed_lanczos_iteration<double> it2(max_iter);
+    b.more_eigenvalues(it2);
+    const bool same = b.diagonal() == lz.diagonal() && b.subdiagonal() == lz.subdiagonal() &&
+                      ia.at == oa.data.size() && (int)a.diagonal().size() == half;
+    printf("resume_equal %d archive_bytes %zu\n", same ? 1 : 0, oa.data.size());
+  } else if (argc > 6) {
     std::vector<std::vector<double> > vecs;
     ietl::Info<double> info;
     const size_t k = std::min(ev.size(), (size_t)nev);

@@ -70,3 +70,22 @@ def test_host_ietl_eigenvectors(name, L, k, tmp_path):
         j = np.argmin(abs(w - lam))
         if np.min(abs(np.delete(w, j) - w[j])) > 1e-6:   # non-degenerate: the eigenvector itself
             assert abs(abs(v @ Z[:, j]) - 1) < 1e-8
+
+
+@pytest.mark.parametrize("name,L,k", [("chain lattice", 10, 5), ("frustrated square lattice", 4, 8)])
+def test_host_ietl_save_load_resumes_bit_for_bit(name, L, k, tmp_path):
+    # lanczos::save / load (src/ietl/lanczos.h:86-98): the first half of a
+    # fixed run archived, a fresh object loaded from the archive continues
+    # with more_eigenvalues, and alpha/beta equal the uninterrupted run's
+    m = O.lattice_model(name, L, by_type={1: (0.5, 0.5)} if "frustrated" in name else None)
+    rp, col, val = O.build_csr(m, k)
+    f = tmp_path / "csr.txt"
+    with open(f, "w") as fh:
+        fh.write(f"{len(rp) - 1} {len(col)}\n")
+        fh.write(" ".join(map(str, rp.tolist())) + "\n")
+        fh.write(" ".join(map(str, col.tolist())) + "\n")
+        fh.write(" ".join(repr(float(v)) for v in val) + "\n")
+    out = subprocess.run([BIN, str(f), "70", "2", "5", "1", "resume"], capture_output=True, text=True,
+                         check=True).stdout.split("\n")
+    line = [l for l in out if l.startswith("resume_equal")][0].split()
+    assert line[1] == "1" and int(line[3]) > 16 * (len(rp) - 1)
