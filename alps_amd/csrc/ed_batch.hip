@@ -256,6 +256,12 @@ struct alps_ed_batch_s {
   size_t stage_cap = 0;
   void* tstage = nullptr;
   size_t tstage_cap = 0;
+  DevBuf tdev;  // the device copy of tstage the scatter launch reads
+  size_t tdev_cap = 0;
+  void* rstage = nullptr;  // pinned stage of a refinement pass's uploads
+  size_t rstage_cap = 0;
+  DevBuf rdev;  // its device copy
+  size_t rdev_cap = 0;
   // shared low-word and flip tables, one device copy per lattice topology
   std::map<std::pair<const void*, const void*>, std::unique_ptr<SharedTables>> shared;
 
@@ -266,6 +272,7 @@ struct alps_ed_batch_s {
     if (host_base) (void)hipHostFree(host_base);
     if (stage) (void)hipHostFree(stage);
     if (tstage) (void)hipHostFree(tstage);
+    if (rstage) (void)hipHostFree(rstage);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -591,24 +598,6 @@ void prepare_refine(Problem& p) {
   }
 }
 
-void rebuild_lists(const std::vector<Problem*>& run, std::vector<BatchItem>& hv32,
-                   std::vector<BatchItem>& hv64, std::vector<BatchItem>& up32, std::vector<BatchItem>& up64,
-                   uint32_t& seg32, uint32_t& seg64) {
-  hv32.clear();
-  hv64.clear();
-  up32.clear();
-  up64.clear();
-  seg32 = seg64 = 0;
-  for (uint32_t i = 0; i < run.size(); ++i) {
-    const Problem& p = *run[i];
-    auto& hv = p.fp32 ? hv32 : hv64;
-    auto& up = p.fp32 ? up32 : up64;
-    for (uint32_t s = 0; s < p.n_hv; ++s) hv.push_back({i, s});
-    for (uint32_t c = 0; c < p.n_chunks; ++c) up.push_back({i, c});
-    if (!p.trans) (p.fp32 ? seg32 : seg64) = std::max(p.fp32 ? seg32 : seg64, p.layout.max_seg);
-  }
-}
-
 void ensure_buf(DevBuf& buf, size_t& cap, size_t bytes) {
   if (bytes <= cap) return;
   buf.alloc(std::max<size_t>(bytes, 2 * cap));
@@ -640,21 +629,28 @@ struct Group {
 };
 
 Group upload_group(alps_ed_batch b, const std::vector<Problem*>& run) {
-  std::vector<BatchItem> hv32, hv64, up32, up64, st32, st64;
-  std::vector<uint32_t> st_list;
+  // Work items, written straight into the pinned stage after one sizing
+  // pass (building them in growing vectors and copying cost about 1.7 ms of
+  // host time per round for a thousand problems): H.v items (job, segment
+  // or block), update items (job, 1024-row chunk), start items of the jobs
+  // new this round, then the job list and the new jobs' list.
   Group g;
-  rebuild_lists(run, hv32, hv64, up32, up64, g.seg32, g.seg64);
-  for (uint32_t i = 0; i < run.size(); ++i)
-    if (run[i]->j == 0) {
-      st_list.push_back(i);
-      for (uint32_t c = 0; c < run[i]->n_chunks; ++c) (run[i]->fp32 ? st32 : st64).push_back({i, c});
-    }
   const size_t nj = run.size();
+  size_t n_hv[2] = {0, 0}, n_up[2] = {0, 0}, n_st[2] = {0, 0}, n_new = 0;
+  for (const Problem* p : run) {
+    const int f = p->fp32 ? 1 : 0;
+    n_hv[f] += p->n_hv;
+    n_up[f] += p->n_chunks;
+    if (p->j == 0) {
+      n_st[f] += p->n_chunks;
+      ++n_new;
+    }
+    if (!p->trans) (p->fp32 ? g.seg32 : g.seg64) = std::max(p->fp32 ? g.seg32 : g.seg64, p->layout.max_seg);
+  }
   auto ib = [](size_t n) { return align_up((n + 1) * sizeof(BatchItem)); };
   const size_t bj = align_up(nj * sizeof(BatchJob));
   const size_t bl = align_up((nj + 1) * sizeof(uint32_t));
-  const size_t total = bj + ib(hv32.size()) + ib(hv64.size()) + ib(up32.size()) + ib(up64.size()) + ib(st32.size()) +
-                       ib(st64.size()) + 2 * bl;
+  const size_t total = bj + ib(n_hv[1]) + ib(n_hv[0]) + ib(n_up[1]) + ib(n_up[0]) + ib(n_st[1]) + ib(n_st[0]) + 2 * bl;
   ensure_buf(b->jobs_d, b->jobs_cap, total);
   ensure_stage(b, total);
   char* s = static_cast<char*>(b->stage);
@@ -662,26 +658,40 @@ Group upload_group(alps_ed_batch b, const std::vector<Problem*>& run) {
   for (size_t i = 0; i < nj; ++i) reinterpret_cast<BatchJob*>(s)[i] = run[i]->job;
   g.jobs = reinterpret_cast<BatchJob*>(d);
   size_t o = bj;
-  auto put_items = [&](const std::vector<BatchItem>& v, BatchItem*& dst, uint32_t& n) {
-    if (!v.empty()) std::memcpy(s + o, v.data(), v.size() * sizeof(BatchItem));
+  auto region = [&](size_t n, BatchItem*& dst, uint32_t& count) {
     dst = reinterpret_cast<BatchItem*>(d + o);
-    n = (uint32_t)v.size();
-    o += ib(v.size());
+    count = (uint32_t)n;
+    BatchItem* h = reinterpret_cast<BatchItem*>(s + o);
+    o += ib(n);
+    return h;
   };
-  put_items(hv32, g.hv32, g.n_hv32);
-  put_items(hv64, g.hv64, g.n_hv64);
-  put_items(up32, g.up32, g.n_up32);
-  put_items(up64, g.up64, g.n_up64);
-  put_items(st32, g.st32, g.n_st32);
-  put_items(st64, g.st64, g.n_st64);
+  BatchItem* hv32 = region(n_hv[1], g.hv32, g.n_hv32);
+  BatchItem* hv64 = region(n_hv[0], g.hv64, g.n_hv64);
+  BatchItem* up32 = region(n_up[1], g.up32, g.n_up32);
+  BatchItem* up64 = region(n_up[0], g.up64, g.n_up64);
+  BatchItem* st32 = region(n_st[1], g.st32, g.n_st32);
+  BatchItem* st64 = region(n_st[0], g.st64, g.n_st64);
   uint32_t* lst = reinterpret_cast<uint32_t*>(s + o);
-  for (uint32_t i = 0; i < nj; ++i) lst[i] = i;
   g.list = reinterpret_cast<uint32_t*>(d + o);
   g.n_list = (uint32_t)nj;
   o += bl;
-  if (!st_list.empty()) std::memcpy(s + o, st_list.data(), st_list.size() * sizeof(uint32_t));
+  uint32_t* new_list = reinterpret_cast<uint32_t*>(s + o);
   g.st_list = reinterpret_cast<uint32_t*>(d + o);
-  g.n_st_list = (uint32_t)st_list.size();
+  g.n_st_list = (uint32_t)n_new;
+  uint32_t k_new = 0;
+  for (uint32_t i = 0; i < nj; ++i) {
+    const Problem& p = *run[i];
+    BatchItem*& hv = p.fp32 ? hv32 : hv64;
+    BatchItem*& up = p.fp32 ? up32 : up64;
+    for (uint32_t sg = 0; sg < p.n_hv; ++sg) *hv++ = {i, sg};
+    for (uint32_t c = 0; c < p.n_chunks; ++c) *up++ = {i, c};
+    if (p.j == 0) {
+      BatchItem*& st = p.fp32 ? st32 : st64;
+      for (uint32_t c = 0; c < p.n_chunks; ++c) *st++ = {i, c};
+      new_list[k_new++] = i;
+    }
+    lst[i] = i;
+  }
   hip_check(hipMemcpyAsync(d, s, total, hipMemcpyHostToDevice, b->stream), "upload work lists");
   return g;
 }
@@ -803,22 +813,31 @@ void admit(alps_ed_batch b, std::vector<Problem*>& started) {
     b->active.push_back(std::move(q));
   }
   if (placed.empty()) return;
-  // stage every new problem's tables in pinned memory, one copy per problem
+  // stage every new problem's tables in pinned memory, with the list of
+  // their destinations after them: one upload of the whole stage and one
+  // scatter launch (a runtime copy per problem cost more host time than
+  // the copies themselves)
   size_t total = 0;
   std::vector<size_t> at;
   for (Problem* p : placed) {
     at.push_back(total);
     total += align_up(problem_table_bytes(*p));
   }
-  ensure_tstage(b, total);
+  const size_t items_at = align_up(total);
+  const size_t stage_bytes = items_at + placed.size() * sizeof(ScatterItem);
+  ensure_tstage(b, stage_bytes);
   char* st = static_cast<char*>(b->tstage);
   std::vector<const SharedTables*> sh(placed.size());
   for (size_t i = 0; i < placed.size(); ++i) sh[i] = &shared_tables(b, placed[i]->layout);
   b->pool->parallel_for(placed.size(), [&](size_t i) { place(b, *placed[i], st + at[i], *sh[i]); });
+  ScatterItem* items = reinterpret_cast<ScatterItem*>(st + items_at);
   for (size_t i = 0; i < placed.size(); ++i)
-    hip_check(hipMemcpyAsync(static_cast<char*>(b->dev_base) + placed[i]->dev_off, st + at[i],
-                             problem_table_bytes(*placed[i]), hipMemcpyHostToDevice, b->stream),
-              "upload tables");
+    items[i] = ScatterItem{at[i], problem_table_bytes(*placed[i]), static_cast<char*>(b->dev_base) + placed[i]->dev_off};
+  ensure_buf(b->tdev, b->tdev_cap, stage_bytes);
+  hip_check(hipMemcpyAsync(b->tdev.p, st, stage_bytes, hipMemcpyHostToDevice, b->stream), "upload tables");
+  hip_check(launch_scatter_copy(b->tdev.as<unsigned char>(), reinterpret_cast<const ScatterItem*>(b->tdev.as<char>() + items_at),
+                                (uint32_t)placed.size(), b->stream),
+            "scatter tables");
 }
 
 void retire(alps_ed_batch b) {
@@ -853,21 +872,17 @@ void refine(alps_ed_batch b) {
     work.push_back(p);
   }
   if (work.empty()) return;
-  // coefficient upload and Ritz vectors
+  // Ritz jobs and, per candidate rank v, the plain H.v jobs for H y
   std::vector<RitzJob> rj(work.size());
   std::vector<BatchItem> ri;
   for (uint32_t i = 0; i < work.size(); ++i) {
     Problem& p = *work[i];
-    const int m = p.final_n;
-    hip_check(hipMemcpyAsync(p.coef, p.coef_host, p.cand.size() * m * sizeof(double), hipMemcpyHostToDevice,
-                             b->stream),
-              "upload Ritz coefficients");
     RitzJob& r = rj[i];
     r.vec = p.job.vec;
     r.stride = p.job.stride;
     r.D = (uint32_t)p.n;
     r.n_chunks = p.n_chunks;
-    r.m = m;
+    r.m = p.final_n;
     r.nv = (int)p.cand.size();
     r.scale = p.job.scale;
     r.coef = p.coef;
@@ -875,21 +890,15 @@ void refine(alps_ed_batch b) {
     r.part = mapped_dev(b, p.ritz_part);
     for (uint32_t c = 0; c < p.n_chunks; ++c) ri.push_back({i, c});
   }
-  DevBuf rjd(align_up(rj.size() * sizeof(RitzJob)) + align_up(ri.size() * sizeof(BatchItem)));
-  hip_check(hipMemcpyAsync(rjd.p, rj.data(), rj.size() * sizeof(RitzJob), hipMemcpyHostToDevice, b->stream), "upload");
-  BatchItem* rid = reinterpret_cast<BatchItem*>(rjd.as<char>() + align_up(rj.size() * sizeof(RitzJob)));
-  hip_check(hipMemcpyAsync(rid, ri.data(), ri.size() * sizeof(BatchItem), hipMemcpyHostToDevice, b->stream), "upload");
-  hip_check(launch_batch_ritz<float>(rjd.as<RitzJob>(), rid, (uint32_t)ri.size(), b->stream), "batch ritz");
-  // H y for each candidate, one plain H.v group per candidate rank
   size_t max_nv = 0;
   for (Problem* p : work) max_nv = std::max(max_nv, p->cand.size());
-  std::vector<std::vector<double>> yhy(work.size());
-  for (size_t v = 0; v < max_nv; ++v) {
-    std::vector<PlainHvJob> hj;
-    std::vector<BatchItem> hi;
-    std::vector<Problem*> who;
-    uint32_t max_seg = 0;
-    for (Problem* p : work) {
+  std::vector<std::vector<PlainHvJob>> hj(max_nv);
+  std::vector<std::vector<BatchItem>> hi(max_nv);
+  std::vector<std::vector<size_t>> who(max_nv);  // positions in work
+  std::vector<uint32_t> max_seg(max_nv, 0);
+  for (size_t v = 0; v < max_nv; ++v)
+    for (size_t wi = 0; wi < work.size(); ++wi) {
+      Problem* p = work[wi];
       if (v >= p->cand.size()) continue;
       PlainHvJob j;
       j.L = p->job.L;
@@ -897,25 +906,73 @@ void refine(alps_ed_batch b) {
       j.x = p->ritz + v * p->n;
       j.y = p->hy;
       j.part = mapped_dev(b, p->hv_part);
-      const uint32_t ji = (uint32_t)hj.size();
-      hj.push_back(j);
-      who.push_back(p);
-      for (uint32_t s = 0; s < p->n_hv; ++s) hi.push_back({ji, s});
-      if (!p->trans) max_seg = std::max(max_seg, p->layout.max_seg);
+      const uint32_t ji = (uint32_t)hj[v].size();
+      hj[v].push_back(j);
+      who[v].push_back(wi);
+      for (uint32_t sg = 0; sg < p->n_hv; ++sg) hi[v].push_back({ji, sg});
+      if (!p->trans) max_seg[v] = std::max(max_seg[v], p->layout.max_seg);
     }
-    DevBuf hd(align_up(hj.size() * sizeof(PlainHvJob)) + align_up(hi.size() * sizeof(BatchItem)));
-    hip_check(hipMemcpyAsync(hd.p, hj.data(), hj.size() * sizeof(PlainHvJob), hipMemcpyHostToDevice, b->stream),
-              "upload");
-    BatchItem* hid = reinterpret_cast<BatchItem*>(hd.as<char>() + align_up(hj.size() * sizeof(PlainHvJob)));
-    hip_check(hipMemcpyAsync(hid, hi.data(), hi.size() * sizeof(BatchItem), hipMemcpyHostToDevice, b->stream),
-              "upload");
-    hip_check(launch_batch_plain_hv(hd.as<PlainHvJob>(), hid, (uint32_t)hi.size(), max_seg, b->stream), "plain hv");
+  // One pinned stage holds every upload of the pass: the Ritz coefficients
+  // (scattered to each problem's coefficient buffer), the Ritz jobs and
+  // items, and the H.v jobs and items of every rank; one copy brings it to
+  // the device (a runtime copy, allocation and free per problem and per rank
+  // cost more host time than the work).
+  size_t o = 0;
+  auto reserve = [&](size_t bytes) {
+    const size_t at = o;
+    o += align_up(bytes);
+    return at;
+  };
+  std::vector<size_t> coef_at(work.size());
+  for (size_t i = 0; i < work.size(); ++i) coef_at[i] = reserve(work[i]->cand.size() * work[i]->final_n * sizeof(double));
+  const size_t items_at = reserve(work.size() * sizeof(ScatterItem));
+  const size_t rj_at = reserve(rj.size() * sizeof(RitzJob));
+  const size_t ri_at = reserve(ri.size() * sizeof(BatchItem));
+  std::vector<size_t> hj_at(max_nv), hi_at(max_nv);
+  for (size_t v = 0; v < max_nv; ++v) {
+    hj_at[v] = reserve(hj[v].size() * sizeof(PlainHvJob));
+    hi_at[v] = reserve(hi[v].size() * sizeof(BatchItem));
+  }
+  if (o > b->rstage_cap) {
+    if (b->rstage) hip_check(hipHostFree(b->rstage), "hipHostFree");
+    b->rstage = nullptr;
+    b->rstage_cap = std::max(o, 2 * b->rstage_cap);
+    hip_check(hipHostMalloc(&b->rstage, b->rstage_cap, hipHostMallocDefault), "hipHostMalloc");
+  }
+  ensure_buf(b->rdev, b->rdev_cap, o);
+  char* st = static_cast<char*>(b->rstage);
+  char* dv = b->rdev.as<char>();
+  ScatterItem* items = reinterpret_cast<ScatterItem*>(st + items_at);
+  for (size_t i = 0; i < work.size(); ++i) {
+    const size_t bytes = work[i]->cand.size() * work[i]->final_n * sizeof(double);
+    std::memcpy(st + coef_at[i], work[i]->coef_host, bytes);
+    items[i] = ScatterItem{coef_at[i], bytes, work[i]->coef};
+  }
+  std::memcpy(st + rj_at, rj.data(), rj.size() * sizeof(RitzJob));
+  std::memcpy(st + ri_at, ri.data(), ri.size() * sizeof(BatchItem));
+  for (size_t v = 0; v < max_nv; ++v) {
+    std::memcpy(st + hj_at[v], hj[v].data(), hj[v].size() * sizeof(PlainHvJob));
+    std::memcpy(st + hi_at[v], hi[v].data(), hi[v].size() * sizeof(BatchItem));
+  }
+  hip_check(hipMemcpyAsync(dv, st, o, hipMemcpyHostToDevice, b->stream), "upload refinement jobs");
+  hip_check(launch_scatter_copy(reinterpret_cast<const unsigned char*>(dv), reinterpret_cast<const ScatterItem*>(dv + items_at),
+                                (uint32_t)work.size(), b->stream),
+            "scatter coefficients");
+  hip_check(launch_batch_ritz<float>(reinterpret_cast<RitzJob*>(dv + rj_at), reinterpret_cast<BatchItem*>(dv + ri_at),
+                                     (uint32_t)ri.size(), b->stream),
+            "batch ritz");
+  // H y for each candidate, one plain H.v group per candidate rank (each
+  // problem's y and partials are reused by the next rank)
+  std::vector<std::vector<double>> yhy(work.size());
+  for (size_t v = 0; v < max_nv; ++v) {
+    hip_check(launch_batch_plain_hv(reinterpret_cast<PlainHvJob*>(dv + hj_at[v]), reinterpret_cast<BatchItem*>(dv + hi_at[v]),
+                                    (uint32_t)hi[v].size(), max_seg[v], b->stream),
+              "plain hv");
     hip_check(hipStreamSynchronize(b->stream), "sync");
-    for (size_t k = 0; k < who.size(); ++k) {
-      Problem& p = *who[k];
+    for (size_t wi : who[v]) {
+      const Problem& p = *work[wi];
       double s = 0.0;
       for (size_t i = 0; i < p.n_hv; ++i) s += p.hv_part[i];
-      const size_t wi = std::find(work.begin(), work.end(), &p) - work.begin();
       yhy[wi].push_back(s);
     }
   }

@@ -65,6 +65,19 @@ void launch_hv_schedule(const DevLayout& L, const V* x, V* y, double* dot_out, h
   launch_hv_with<V, DOT, W, 4, 4, 2>(L, x, y, dot_out, st, lds);
 }
 
+// One workgroup per item; 16-byte moves where source and destination allow.
+__global__ __launch_bounds__(kBlock) void scatter_copy_kernel(const unsigned char* __restrict__ src,
+                                                              const ScatterItem* __restrict__ items) {
+  const ScatterItem it = items[blockIdx.x];
+  const unsigned char* s = src + it.src;
+  unsigned char* d = static_cast<unsigned char*>(it.dst);
+  const bool aligned = ((reinterpret_cast<uintptr_t>(s) | reinterpret_cast<uintptr_t>(d)) & 15u) == 0u;
+  const uint64_t n16 = aligned ? it.bytes / 16u : 0u;
+  for (uint64_t i = threadIdx.x; i < n16; i += kBlock)
+    reinterpret_cast<uint4*>(d)[i] = reinterpret_cast<const uint4*>(s)[i];
+  for (uint64_t i = n16 * 16u + threadIdx.x; i < it.bytes; i += kBlock) d[i] = s[i];
+}
+
 // ---------------------------------------------------------------------------
 // Lanczos update, reductions, vector kernels
 // ---------------------------------------------------------------------------
@@ -626,6 +639,12 @@ hipError_t launch_finish(Partials in, const double* scale, double* out, double* 
 template <class V>
 hipError_t launch_normalize(uint64_t n, V* v, Partials norm_in, double* b_out, hipStream_t st) {
   hipLaunchKernelGGL((normalize_kernel<V>), dim3(stream_blocks(n)), dim3(kBlock), 0, st, n, v, norm_in, b_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter_copy(const unsigned char* src, const ScatterItem* items, uint32_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(scatter_copy_kernel, dim3(n), dim3(kBlock), 0, st, src, items);
   return hipGetLastError();
 }
 

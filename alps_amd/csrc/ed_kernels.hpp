@@ -215,6 +215,15 @@ hipError_t launch_normalize(uint64_t n, V* v, Partials norm_in, double* b_out, h
 
 template <class V>
 hipError_t launch_fill_random(uint64_t n, V* v, uint64_t seed, double lo, double hi, hipStream_t st);
+
+// Copies of many blocks out of one staged device buffer: item i copies
+// items[i].bytes from src + items[i].src to items[i].dst (one launch
+// instead of a runtime copy per block).
+struct ScatterItem {
+  uint64_t src, bytes;
+  void* dst;
+};
+hipError_t launch_scatter_copy(const unsigned char* src, const ScatterItem* items, uint32_t n, hipStream_t st);
 template <class V>
 hipError_t launch_fill(uint64_t n, V* v, double value, hipStream_t st);
 // y = a*x + b*y
