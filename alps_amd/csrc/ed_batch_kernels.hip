@@ -221,7 +221,7 @@ template <class V>
 hipError_t launch_batch_hv(const BatchJob* jobs, const BatchItem* items, uint32_t n_items, int t,
                            uint32_t max_seg, hipStream_t st) {
   if (n_items == 0) return hipSuccess;
-  const size_t lds = (size_t)(max_seg + 1) * sizeof(V);
+  const size_t lds = (size_t)(max_seg + 1) * sizeof(double);  // the FP64 slice of hv_segment
   hipLaunchKernelGGL((batch_hv_kernel<V>), dim3(n_items), dim3(kBlock), lds, st, jobs, items, t);
   return hipGetLastError();
 }

@@ -144,6 +144,25 @@ __device__ __forceinline__ V elem_at(const V* p, uint32_t off4) {
   return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(p) + off4 * (uint32_t)(sizeof(V) / 4));
 }
 
+// A load of T at byte offset `off` (per lane, 32 bits) from a wave-uniform
+// global base: the load adds the offset to the base, held in scalar
+// registers, in the address unit (global_load with a scalar base), instead
+// of a 64-bit vector add per lane and load.  The empty asm keeps the
+// compiler from folding the offset into a 64-bit per-lane address.
+template <class T>
+__device__ __forceinline__ T ld_lane(const void* base, uint32_t off) {
+  typedef __attribute__((address_space(1))) const char gchar;
+  typedef __attribute__((address_space(1))) const T gT;
+  asm volatile("" : "+v"(off));
+  return *reinterpret_cast<gT*>(reinterpret_cast<gchar*>(reinterpret_cast<uintptr_t>(base)) + off);
+}
+
+// elem_at for a wave-uniform base p.
+template <class V>
+__device__ __forceinline__ V elem_at_uniform(const V* p, uint32_t off4) {
+  return ld_lane<V>(p, off4 * (uint32_t)(sizeof(V) / 4));
+}
+
 // Every multiply-add of a row is an explicit fma, so that rows and dot
 // partials do not depend on what the compiler contracts.
 //
@@ -156,7 +175,7 @@ __device__ __forceinline__ V elem_at(const V* p, uint32_t off4) {
 // thread's share of x.Hx when DOT.
 template <class V, bool DOT, int NQ, int CLL, int CHH, int CX>
 __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const V* __restrict__ x,
-                                          V* __restrict__ y, V* xs, uint32_t len, uint32_t base, double dh,
+                                          V* __restrict__ y, double* xs, uint32_t len, uint32_t base, double dh,
                                           const uint16_t* nbp, const uint16_t* xnp, const double* dlp) {
   const uint32_t tid = threadIdx.x;
   const V* xb = x + base;
@@ -193,7 +212,7 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
-      for (int k = 0; k < KX; ++k) xr[q][k] = *reinterpret_cast<const u16x2*>(xnp + xrec[q] + k * kXChunk * 64u);
+      for (int k = 0; k < KX; ++k) xr[q][k] = ld_lane<u16x2>(xnp + k * kXChunk * 64u, 2u * xrec[q]);
   }
   // low-low flips: target rows from the class table, values from LDS.  The
   // table entries of the next chunk are loaded while the current one is
@@ -205,7 +224,7 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
-      for (int k = 0; k < KL; ++k) tb[q][k] = *reinterpret_cast<const u16x4*>(nbp + lrec[q] + k * kLLChunk * 64u);
+      for (int k = 0; k < KL; ++k) tb[q][k] = ld_lane<u16x4>(nbp + k * kLLChunk * 64u, 2u * lrec[q]);
   }
   // masks of the first 64 high-high bonds (one per lane, for the ballot below)
   const uint32_t lane = threadIdx.x & 63u;
@@ -216,8 +235,8 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
   // wave of the workgroup passes once (hv_segment).
 #pragma unroll
   for (int q = 0; q < NQ; ++q)
-    if (ok[q]) xs[rc[q]] = xo[q];
-  if (tid == 0) xs[len] = (V)0;
+    if (ok[q]) xs[rc[q]] = (double)xo[q];
+  if (tid == 0) xs[len] = 0.0;
   __syncthreads();
   for (int b0 = 0; b0 < n_ll; b0 += CLL) {
     uint32_t j[NQ][CLL];
@@ -232,7 +251,7 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
       for (int q = 0; q < NQ; ++q)
 #pragma unroll
         for (int k = 0; k < KL; ++k)
-          tb[q][k] = *reinterpret_cast<const u16x4*>(nbp + lrec[q] + (uint32_t)(b0 + CLL + k * kLLChunk) * 64u);
+          tb[q][k] = ld_lane<u16x4>(nbp + (uint32_t)(b0 + CLL + k * kLLChunk) * 64u, 2u * lrec[q]);
     }
     double t[CLL];
 #pragma unroll
@@ -240,7 +259,8 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
-      for (int c = 0; c < CLL; ++c) off[q] = fma(t[c], (double)elem_at(xs, j[q][c]), off[q]);
+      for (int c = 0; c < CLL; ++c)
+        off[q] = fma(t[c], *reinterpret_cast<const double*>(reinterpret_cast<const char*>(xs) + j[q][c]), off[q]);
   }
   // high-high flips: a bond flips every row of the segment or none (its two
   // bits lie in h), with a contiguous target run.  The bonds that flip are
@@ -254,17 +274,19 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
     const uint32_t bits = h & ml;
     uint64_t act = __ballot(bits != 0u && bits != ml);
     while (act != 0u) {
-      const V* xt[CHH];
+      // neighbour segment starts as element offsets (a select between
+      // pointers would make the loads generic-address ones)
+      uint32_t xt[CHH];
       double tt[CHH];
 #pragma unroll
       for (int c = 0; c < CHH; ++c) {
         if (act != 0u) {
           const int b = w0 + __builtin_ctzll(act);
           act &= act - 1u;
-          xt[c] = x + ldc(L.hi_base + (h ^ ldc(L.hh_mask + b)));
+          xt[c] = ldc(L.hi_base + (h ^ ldc(L.hh_mask + b)));
           tt[c] = ldc(L.hh_t + b);
         } else {
-          xt[c] = xb;
+          xt[c] = base;
           tt[c] = 0.0;
         }
       }
@@ -272,7 +294,7 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
 #pragma unroll
       for (int q = 0; q < NQ; ++q)
 #pragma unroll
-        for (int c = 0; c < CHH; ++c) v[q][c] = xt[c][rc[q]];
+        for (int c = 0; c < CHH; ++c) v[q][c] = elem_at_uniform(x + xt[c], 4u * rc[q]);
 #pragma unroll
       for (int q = 0; q < NQ; ++q)
 #pragma unroll
@@ -282,19 +304,20 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
   // cross bonds: diagonal always, flip when the high and low bits differ.
   // If the neighbouring high word has no segment no row can flip (its low
   // word would need a set bit more or fewer than it has), and the read
-  // falls back to this segment.
+  // falls back to this segment.  The diagonal already holds every cross
+  // bond's parallel value (diag_hi); each antiparallel bond adds its
+  // anti - par difference.
   for (int b0 = 0; b0 < n_xb; b0 += CX) {
     const V* xn[CX];
     bool hb[CX];
-    double ea[CX], ep[CX], tx[CX];
+    double de[CX], tx[CX];
 #pragma unroll
     for (int c = 0; c < CX; ++c) {
       const uint32_t hbit = ldc(L.xb_hbit + b0 + c);
       hb[c] = (h & hbit) != 0u;
       const uint32_t nb = ldc(L.hi_base + (h ^ hbit));
       xn[c] = x + (nb == 0xffffffffu ? base : nb);
-      ea[c] = ldc(L.xb_anti + b0 + c);
-      ep[c] = ldc(L.xb_par + b0 + c);
+      de[c] = ldc(L.xb_anti + b0 + c) - ldc(L.xb_par + b0 + c);
       tx[c] = ldc(L.xb_t + b0 + c);
     }
     uint32_t e[NQ][CX];
@@ -309,7 +332,7 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
       for (int q = 0; q < NQ; ++q)
 #pragma unroll
         for (int k = 0; k < KX; ++k)
-          xr[q][k] = *reinterpret_cast<const u16x2*>(xnp + xrec[q] + (uint32_t)(b0 + CX + k * kXChunk) * 64u);
+          xr[q][k] = ld_lane<u16x2>(xnp + (uint32_t)(b0 + CX + k * kXChunk) * 64u, 2u * xrec[q]);
     }
     V v[NQ][CX];
     bool f[NQ][CX];
@@ -318,21 +341,23 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
 #pragma unroll
       for (int c = 0; c < CX; ++c) {
         f[q][c] = ((e[q][c] >> 15) != 0u) != hb[c];
-        v[q][c] = elem_at(xn[c], f[q][c] ? (e[q][c] & 0x7fffu) : 0u);
+        v[q][c] = elem_at_uniform(xn[c], f[q][c] ? (e[q][c] & 0x7fffu) : 0u);
       }
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
       for (int c = 0; c < CX; ++c) {
-        d[q] += f[q][c] ? ea[c] : ep[c];
-        off[q] = fma(f[q][c] ? tx[c] : 0.0, (double)v[q][c], off[q]);
+        // selects on the flag as 1.0 / 0.0 and on the FP32 value (one
+        // 32-bit select each, where selecting the FP64 terms takes two)
+        d[q] = fma(f[q][c] ? 1.0 : 0.0, de[c], d[q]);
+        off[q] = fma(tx[c], (double)(f[q][c] ? v[q][c] : (V)0), off[q]);
       }
   }
   double dotp = 0.0;
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     if (ok[q]) {
-      const double xi = (double)xs[rc[q]];
+      const double xi = xs[rc[q]];
       const double yv = fma(d[q], xi, off[q]);
       y[base + rc[q]] = (V)yv;
       if (DOT) dotp = fma(xi, yv, dotp);  // FP64 value before the storage rounding
@@ -342,18 +367,12 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
 }
 
 // H.v over the row segment of high word h (one workgroup of kBlock threads;
-// dynamic LDS of at least (L.max_seg + 1) elements of V):
+// dynamic LDS of at least (L.max_seg + 1) doubles):
 //   y[r] = sum_c H[r][c] x[c] for the segment's rows r;
 // with DOT, *dot_slot receives the segment's sum of x_r (H x)_r (FP64, before
 // y is rounded to the storage type).
 //
-// H.v over the row segment of high word h (one workgroup of kBlock threads;
-// dynamic LDS of at least (L.max_seg + 1) elements of V):
-//   y[r] = sum_c H[r][c] x[c] for the segment's rows r;
-// with DOT, *dot_slot receives the segment's sum of x_r (H x)_r (FP64, before
-// y is rounded to the storage type).
-//
-// The segment's slice of x is staged in LDS.  Each thread owns up to kRows
+// The segment's slice of x is staged in LDS in FP64.  Each thread owns up to kRows
 // rows (r = tid + q*kBlock); the number of row slots that hold rows is
 // uniform over each wave and selects a fixed-size instance of the bond
 // loops, so a wave does no work for slots past the segment (the waves of a
@@ -364,7 +383,9 @@ template <class V, bool DOT, int CLL = 4, int CHH = 4, int CX = 2>
 __device__ __forceinline__ void hv_segment(const DevLayout& L, uint32_t h, const V* __restrict__ x,
                                            V* __restrict__ y, double* __restrict__ dot_slot) {
   extern __shared__ __align__(16) unsigned char smem[];
-  V* xs = reinterpret_cast<V*>(smem);  // segment slice; entry len is the zero slot
+  // the segment's slice of x in FP64 (converted once per row, not once per
+  // low-low flip that reads it); entry len is the zero slot
+  double* xs = reinterpret_cast<double*>(smem);
   const uint32_t tid = threadIdx.x;
   const int kl = L.k - __builtin_popcount(h);
   const uint32_t cls0 = ldc(L.lo_class_off + kl);

@@ -583,7 +583,7 @@ template <class V>
 hipError_t launch_hv(const DevLayout& L, const V* x, V* y, double* dot_out, hipStream_t st, int schedule) {
   if (L.n_seg == 0) return hipSuccess;
   if (schedule < 0 || schedule >= kHvSchedules) return hipErrorInvalidValue;
-  const size_t lds = (size_t)(L.max_seg + 1) * sizeof(V);
+  const size_t lds = (size_t)(L.max_seg + 1) * sizeof(double);  // the FP64 slice of hv_segment
   if (dot_out)
     launch_hv_schedule<V, true>(L, x, y, dot_out, st, lds, schedule);
   else

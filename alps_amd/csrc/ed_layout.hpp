@@ -131,14 +131,15 @@ struct SectorLayout {
   uint32_t max_seg = 0;                // longest segment
   // H.v tables.  Bond lists are padded with null bonds (coefficient 0,
   // never active) to multiples of the kernel's chunk sizes.
-  //   nbp16[nbp_off[c] + flip_index(r, b, n_ll_p, kLLChunk)]: 4 * the row
+  //   nbp16[nbp_off[c] + flip_index(r, b, n_ll_p, kLLChunk)]: 8 * the row
   //     reached from row r of a class-c segment by low-low bond b, or
-  //     4 * C(B,c) (the zero slot) if not allowed;
+  //     8 * C(B,c) (the zero slot) if not allowed: the byte offset of the
+  //     row's FP64 element in the segment's LDS slice;
   //   xn16[xn_off[c] + flip_index(r, b, n_xb_p, kXChunk)]: 4 * the rank of
   //     lo ^ lbit_b in its class, with bit 15 = the low bit of cross bond b
-  //     in row r.
-  // Rows are stored times 4, the byte offset of an FP32 element, which the
-  // kernel adds to its base without a shift (C(12,6) * 4 < 2^15).
+  //     in row r: the byte offset of an FP32 element of x (twice it for FP64).
+  // Both are used as byte offsets without a shift (C(12,6) * 8 < 2^16,
+  // C(12,6) * 4 < 2^15).
   int n_ll_p = 0, n_hh_p = 0, n_xb_p = 0;
   std::vector<uint16_t> nbp16, xn16;
   std::vector<uint32_t> nbp_off, xn_off;  // [B+2]
@@ -148,7 +149,7 @@ struct SectorLayout {
   std::vector<uint32_t> xb_hbit;          // [n_xb_p], 0 = null
   std::vector<double> xb_par, xb_anti, xb_t;
   std::vector<double> diag_lo;         // [2^B] site + low-low bond diagonal
-  std::vector<double> diag_hi;         // [2^NH] site + high-high bond diagonal
+  std::vector<double> diag_hi;         // [2^NH] site + high-high bond diagonal + every cross bond's par value
   std::vector<FlipBond> ll, hh;        // low-low and high-high flip bonds
   std::vector<CrossBond> xb;           // cross bonds (all, for the diagonal)
   // reference-order description for the element enumeration
@@ -256,7 +257,7 @@ inline std::shared_ptr<const FlipTables> flip_tables(int B, const std::vector<ui
       const uint32_t m = b < (int)ll_masks.size() ? ll_masks[b] : 0u;
       for (uint32_t r = 0; r < len; ++r) {
         const uint32_t lo = ls[r], bits = lo & m;
-        nb[flip_index(r, b, n_ll_p, kLLChunk)] = (uint16_t)(4u * ((bits != 0u && bits != m) ? T.lo_rank[lo ^ m] : len));
+        nb[flip_index(r, b, n_ll_p, kLLChunk)] = (uint16_t)(8u * ((bits != 0u && bits != m) ? T.lo_rank[lo ^ m] : len));
       }
     }
     uint16_t* xn = F->xn16.data() + F->xn_off[c];
@@ -439,6 +440,11 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1, bool c
   };
   fill(L.diag_lo, nlo, base_lo, su_lo, bl_lo);
   fill(L.diag_hi, nhi, base_hi, su_hi, bl_hi);
+  // every cross bond's parallel value is part of every row's diagonal; the
+  // kernel adds anti - par for the antiparallel ones
+  double cross_par = 0.0;
+  for (const CrossBond& c : L.xb) cross_par += c.e_par;
+  for (double& v : L.diag_hi) v += cross_par;
   L.diag_lo_cls.resize(nlo);
   for (uint32_t i = 0; i < nlo; ++i) L.diag_lo_cls[i] = L.diag_lo[L.low->lo_state[i]];
   // padded H.v tables

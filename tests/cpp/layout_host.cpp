@@ -46,7 +46,10 @@ static double diag_error() {
       }
       const uint32_t lo = st & ((1u << L.B) - 1u), hi = L.NH ? st >> L.B : 0u;
       double got = L.diag_lo[lo] + L.diag_hi[hi];
-      for (const CrossBond& x : L.xb) got += ((hi & x.hbit) != 0) == ((lo & x.lbit) != 0) ? x.e_par : x.e_anti;
+      // diag_hi holds every cross bond's parallel value; the antiparallel
+      // ones add anti - par, as the H.v kernel does
+      for (const CrossBond& x : L.xb)
+        if (((hi & x.hbit) != 0) != ((lo & x.lbit) != 0)) got += x.e_anti - x.e_par;
       worst = std::max(worst, std::fabs(got - d));
     }
   }
@@ -135,7 +138,7 @@ static void flip_check(long& bad, long& checked) {
         for (int b = 0; b < L.n_ll_p; ++b) {
           const size_t at = lrec + (size_t)(b / kLLChunk * kLLChunk) * 64u + b % kLLChunk;
           const uint32_t m = b < (int)L.ll.size() ? L.ll[b].mask : 0u, bits = lo & m;
-          const uint32_t want = 4u * ((bits != 0u && bits != m) ? L.lo_rank[lo ^ m] : len);
+          const uint32_t want = 8u * ((bits != 0u && bits != m) ? L.lo_rank[lo ^ m] : len);
           ++checked;
           if (at >= nll || L.nbp16[L.nbp_off[c] + at] != want) ++bad;
         }
