@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE scaled by a copy kernel of known size
 # (profiles/hv_kernel_heisenberg_L24.md); valid for that kernel version and L
 # only.  The schedules that spill write their scratch too.
-PMC_TRAFFIC_BYTES = {24: [56.46e6, 71.22e6, 51.91e6, 51.82e6, 108.25e6, 74.78e6]}
+PMC_TRAFFIC_BYTES = {24: [51.89e6, 71.02e6, 51.94e6, 51.80e6, 70.99e6, 52.29e6]}
 PMC_TRAFFIC_SOURCE = ("profiles/hv_kernel_heisenberg_L24.md, table 'HBM bytes per launch' "
                       "(FETCH_SIZE + WRITE_SIZE, scaled by the copy kernel), row of the chosen schedule")
 # H.v schedules (waves per SIMD, low-low / high-high / cross chunk sizes), in
