@@ -1741,13 +1741,19 @@ namespace {
 
 // TRANSLATION_SYMMETRY as a boolean parameter (default true,
 // applications/diag/diag.h:91 and factory.C:25).
-bool translation_symmetry(const Parameters& p, const Evaluator& ev) {
-  if (!p.count("TRANSLATION_SYMMETRY")) return true;
-  std::string v = p.at("TRANSLATION_SYMMETRY");
+// A boolean parameter (true/false/yes/no or an expression), `def` when
+// absent.
+bool bool_param(const Parameters& p, const Evaluator& ev, const char* name, bool def) {
+  if (!p.count(name)) return def;
+  std::string v = p.at(name);
   for (char& c : v) c = (char)std::tolower((unsigned char)c);
   if (v == "true" || v == "yes") return true;
   if (v == "false" || v == "no") return false;
   return ev.value(v) != 0.0;
+}
+
+bool translation_symmetry(const Parameters& p, const Evaluator& ev) {
+  return bool_param(p, ev, "TRANSLATION_SYMMETRY", true);
 }
 
 // The momenta a task loops over (DiagMatrix::dostep, diag.h:90-97,117-120):
@@ -2093,8 +2099,11 @@ extern "C" int alps_ed_run_parameter_file(const char* text, char* out, size_t ou
         js += std::string(", \"measurement_error\": \"eigenvector measurements need Sz-conserved sectors") +
               " (CONSERVED_QUANTUMNUMBERS=\\\"Sz\\\" or Sz_total)\"";
       const int nev_xml = ev.defined("NUMBER_EIGENVALUES") ? sp.nev : 0;
+      // <EIGENSTATES> when averages are measured or MEASURE_ENERGY (default
+      // true) holds (DiagTask::write_xml_body, diag.hpp:141)
+      const bool states = ms.any() || bool_param(p, ev, "MEASURE_ENERGY", true);
       js += ", \"xml\": " +
-            json_string(task_xml(p, xsec, nev_xml, !std::isnan(g), g, !std::isnan(gap), gap));
+            json_string(task_xml(p, xsec, nev_xml, !std::isnan(g), g, !std::isnan(gap), gap, states));
       js += "}";
     }
     js += "]}";

@@ -308,29 +308,52 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1, bool c
     }
     L.lo_class_off = T.lo_class_off;
   }
-  // high tables and segments
+  // high tables and segments (the set bits of each word in turn, binomials
+  // from one row table: this loop runs 2^(N-B) times per layout)
   L.hi_base.assign(nhi, 0xffffffffu);
-  for (uint32_t h = 0; h < nhi; ++h) {
-    int kl = L.k - __builtin_popcount(h);
-    if (kl < 0 || kl > B || L.D == 0) continue;
-    uint64_t r = 0;
-    int m_idx = kl;
+  {
+    std::vector<std::vector<uint64_t>> cb(L.NH, std::vector<uint64_t>(N + 2, 0));
     for (int q = 0; q < L.NH; ++q)
-      if (h >> q & 1u) r += binom(B + q, ++m_idx);
-    L.hi_base[h] = (uint32_t)r;
-    L.seg_hi.push_back(h);
-    L.seg_start.push_back((uint32_t)r);
+      for (int m_idx = 0; m_idx <= N + 1; ++m_idx) cb[q][m_idx] = binom(B + q, m_idx);
+    L.seg_hi.reserve(nhi);
+    L.seg_start.reserve(nhi);
+    for (uint32_t h = 0; h < nhi; ++h) {
+      const int kl = L.k - __builtin_popcount(h);
+      if (kl < 0 || kl > B || L.D == 0) continue;
+      uint64_t r = 0;
+      int m_idx = kl;
+      for (uint32_t w = h; w != 0u; w &= w - 1u) r += cb[__builtin_ctz(w)][++m_idx];
+      L.hi_base[h] = (uint32_t)r;
+      L.seg_hi.push_back(h);
+      L.seg_start.push_back((uint32_t)r);
+    }
   }
-  // segment processing order of the H.v kernel
-  L.proc_hi = L.seg_hi;
-  for (uint32_t h : L.seg_hi) L.max_seg = std::max<uint32_t>(L.max_seg, (uint32_t)binom(B, L.k - __builtin_popcount(h)));
-  // longest segments first, so that short ones fill the tail of the launch
-  // (popcount(hi) picks the class; a longer class has kl nearer B/2)
-  std::stable_sort(L.proc_hi.begin(), L.proc_hi.end(), [&](uint32_t a, uint32_t b) {
-    const uint64_t la = L.lo_class_off[L.k - __builtin_popcount(a) + 1] - L.lo_class_off[L.k - __builtin_popcount(a)];
-    const uint64_t lb = L.lo_class_off[L.k - __builtin_popcount(b) + 1] - L.lo_class_off[L.k - __builtin_popcount(b)];
-    return la > lb;
-  });
+  // segment processing order of the H.v kernel: longest segments first, so
+  // that short ones fill the tail of the launch (popcount(hi) picks the
+  // class; a longer class has kl nearer B/2), ascending hi within a length
+  // (a stable counting sort by segment length)
+  {
+    std::vector<std::vector<uint32_t>> by_len(B + 1);  // by class kl, in ascending hi
+    for (uint32_t h : L.seg_hi) by_len[L.k - __builtin_popcount(h)].push_back(h);
+    std::vector<int> cls;
+    for (int c = 0; c <= B; ++c)
+      if (!by_len[c].empty()) cls.push_back(c);
+    std::stable_sort(cls.begin(), cls.end(), [&](int a, int b) { return binom(B, a) > binom(B, b); });
+    // classes of equal length interleave in ascending hi, as a stable sort
+    // of the whole list by length leaves them
+    L.proc_hi.clear();
+    L.proc_hi.reserve(L.seg_hi.size());
+    for (size_t i = 0; i < cls.size();) {
+      size_t j = i;
+      while (j < cls.size() && binom(B, cls[j]) == binom(B, cls[i])) ++j;
+      std::vector<uint32_t> merged;
+      for (size_t q = i; q < j; ++q) merged.insert(merged.end(), by_len[cls[q]].begin(), by_len[cls[q]].end());
+      if (j - i > 1) std::sort(merged.begin(), merged.end());
+      L.proc_hi.insert(L.proc_hi.end(), merged.begin(), merged.end());
+      i = j;
+    }
+    if (!cls.empty()) L.max_seg = (uint32_t)binom(B, cls[0]);
+  }
   // Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD
   // b % 8), each XCD with its own L2.  When the segments split evenly by the
   // top three bits of the high word, block b takes its segment from group

@@ -100,9 +100,11 @@ class XmlWriter {
 };
 
 // One task's result document.  Each sector writes min(NUMBER_EIGENVALUES,
-// its number of values) levels; number_eigenvalues <= 0 means all.
+// its number of values) levels; number_eigenvalues <= 0 means all.  The
+// <EIGENSTATES> blocks only when `eigenstates` (measurements requested or
+// MEASURE_ENERGY, diag.hpp:141).
 inline std::string task_xml(const Parameters& p, const std::vector<XmlSector>& sectors, int number_eigenvalues,
-                            bool have_ground, double ground, bool have_gap, double gap) {
+                            bool have_ground, double ground, bool have_gap, double gap, bool eigenstates = true) {
   XmlWriter w;
   w.raw("<?xml version=\"1.0\" encoding=\"UTF-8\"?>");
   w.raw("<?xml-stylesheet type=\"text/xsl\" href=\"ALPS.xsl\"?>");
@@ -126,6 +128,7 @@ inline std::string task_xml(const Parameters& p, const std::vector<XmlSector>& s
     w.close("EIGENVALUES");
   }
   for (const XmlSector& s : sectors) {
+    if (!eigenstates) break;
     const size_t n = std::stoul(count(s));
     w.open("EIGENSTATES", {{"number", count(s)}});
     for (const auto& q : s.qn) w.empty("QUANTUMNUMBER", {{"name", q.first}, {"value", q.second}});

@@ -213,6 +213,21 @@ def test_task_xml_matches_results(tmp_path):
             assert abs(sum(loc["values"][j]) - float(eval(sz))) < 1e-9
 
 
+@pytest.mark.parametrize("flag,measured,states", [("", False, True), ("MEASURE_ENERGY=false;", False, False),
+                                                   ("MEASURE_ENERGY=false; MEASURE_AVERAGE[M]=Sz;", True, True)])
+def test_task_xml_eigenstates_follow_measure_energy(flag, measured, states):
+    # DiagTask::write_xml_body (diag.hpp:141): <EIGENSTATES> only when
+    # averages are measured or MEASURE_ENERGY (default true) holds
+    import xml.etree.ElementTree as ET
+
+    text = f'LATTICE="chain lattice"; L=6; J=1; Sz_total=0; TRANSLATION_SYMMETRY=false; {flag}\n{{}}\n'
+    (t,) = A.run_parameter_file(text)
+    root = ET.fromstring(t["xml"])
+    assert len(root.findall("EIGENVALUES")) == 1
+    assert len(root.findall("EIGENSTATES")) == (1 if states else 0)
+    assert ("measurements" in t["sectors"][0]) == measured
+
+
 def test_cli_writes_task_xml(tmp_path):
     import os
     import subprocess
