@@ -41,6 +41,7 @@ namespace alps_ed {
 
 constexpr int kMaxSites = 32;
 constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
+constexpr int kClassSlices = 4;  // contiguous slices of a class swept in turn (uneven segment order)
 
 // Binomial coefficients C(n, k) for 0 <= n <= 64 (Pascal's triangle, built
 // once); 0 outside 0 <= k <= n.
@@ -379,7 +380,11 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1, bool c
   // i.e. XCD g sweeps run g of every class.  The high-high and cross
   // neighbours of a segment (hi ^ mask, near hi in address when the flipped
   // bits are low ones) are then read mostly on the XCD that reads hi.  Equal
-  // counts of one class are equal work, so the XCDs stay balanced.
+  // counts of one class are equal work, so the XCDs stay balanced.  A class
+  // is swept in kClassSlices contiguous slices one after the other, each
+  // dealt as eight runs, which keeps the x of the segments in flight (and of
+  // their neighbours) within a quarter of the class (J1-J2 6x5: 4.64 ->
+  // 4.51 ms; 2 slices 4.58, 8 slices 4.50).
   if (!grouped && L.seg_hi.size() >= 64u * kXcds) {
     std::vector<int> cls_order;
     std::vector<std::vector<uint32_t>> by_pc(L.NH + 1);
@@ -392,10 +397,15 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1, bool c
     for (int c : cls_order) {
       std::vector<uint32_t>& v = by_pc[c];
       std::sort(v.begin(), v.end());  // address order: hi_base ascends with hi
-      const size_t base = v.size() / kXcds, s = L.proc_hi.size();
-      for (size_t i = 0; i < base; ++i)
-        for (int j = 0; j < kXcds; ++j) L.proc_hi.push_back(v[((s + j) % kXcds) * base + i]);
-      for (size_t r = base * kXcds; r < v.size(); ++r) L.proc_hi.push_back(v[r]);
+      // kClassSlices contiguous slices in turn, each dealt as eight runs
+      const size_t slice = (v.size() + kClassSlices - 1) / kClassSlices;
+      for (size_t s0 = 0; s0 < v.size(); s0 += slice) {
+        const size_t n = std::min(slice, v.size() - s0);
+        const size_t base = n / kXcds, s = L.proc_hi.size();
+        for (size_t i = 0; i < base; ++i)
+          for (int j = 0; j < kXcds; ++j) L.proc_hi.push_back(v[s0 + ((s + j) % kXcds) * base + i]);
+        for (size_t r = base * kXcds; r < n; ++r) L.proc_hi.push_back(v[s0 + r]);
+      }
     }
   }
   // terms in bit coordinates
