@@ -291,9 +291,11 @@ size_t table_bytes(const SectorLayout& L) {
   add(L.diag_lo_cls.size(), 8);
   add(L.diag_hi.size(), 8);
   add(L.ll_t.size(), 8);
-  add(L.hh_mask.size(), 4);
-  add(L.hh_t.size(), 8);
+  add(L.hh_off.size(), 4);
+  add(L.hh_nb.size(), 4);
+  add(L.hh_tv.size(), 8);
   add(L.xb_hbit.size(), 4);
+  add(L.xb_nb.size(), 4);
   add(L.xb_par.size(), 8);
   add(L.xb_anti.size(), 8);
   add(L.xb_t.size(), 8);
@@ -358,9 +360,11 @@ DevLayout place_tables(const SectorLayout& L, char* base, char* stage, const Sha
   d.diag_lo_cls = put(L.diag_lo_cls);
   d.diag_hi = put(L.diag_hi);
   d.ll_t = put(L.ll_t);
-  d.hh_mask = put(L.hh_mask);
-  d.hh_t = put(L.hh_t);
+  d.hh_off = put(L.hh_off);
+  d.hh_nb = put(L.hh_nb);
+  d.hh_tv = put(L.hh_tv);
   d.xb_hbit = put(L.xb_hbit);
+  d.xb_nb = put(L.xb_nb);
   d.xb_par = put(L.xb_par);
   d.xb_anti = put(L.xb_anti);
   d.xb_t = put(L.xb_t);

@@ -137,13 +137,6 @@ __device__ __forceinline__ T ldc_struct(const T* p) {
   return out;
 }
 
-// Element of p at the offset a flip table stores: 4 * (row index), the byte
-// offset of an FP32 element (twice that for FP64).
-template <class V>
-__device__ __forceinline__ V elem_at(const V* p, uint32_t off4) {
-  return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(p) + off4 * (uint32_t)(sizeof(V) / 4));
-}
-
 // A load of T at byte offset `off` (per lane, 32 bits) from a wave-uniform
 // global base: the load adds the offset to the base, held in scalar
 // registers, in the address unit (global_load with a scalar base), instead
@@ -157,10 +150,25 @@ __device__ __forceinline__ T ld_lane(const void* base, uint32_t off) {
   return *reinterpret_cast<gT*>(reinterpret_cast<gchar*>(reinterpret_cast<uintptr_t>(base)) + off);
 }
 
-// elem_at for a wave-uniform base p.
-template <class V>
-__device__ __forceinline__ V elem_at_uniform(const V* p, uint32_t off4) {
-  return ld_lane<V>(p, off4 * (uint32_t)(sizeof(V) / 4));
+// The same for an offset that serves many loads: lane_off() makes it opaque
+// once, and ld_at / st_at use it as it is (no register copy per access).
+__device__ __forceinline__ uint32_t lane_off(uint32_t off) {
+  asm volatile("" : "+v"(off));
+  return off;
+}
+
+template <class T>
+__device__ __forceinline__ T ld_at(const void* base, uint32_t off) {
+  typedef __attribute__((address_space(1))) const char gchar;
+  typedef __attribute__((address_space(1))) const T gT;
+  return *reinterpret_cast<gT*>(reinterpret_cast<gchar*>(reinterpret_cast<uintptr_t>(base)) + off);
+}
+
+template <class T>
+__device__ __forceinline__ void st_at(void* base, uint32_t off, T v) {
+  typedef __attribute__((address_space(1))) char gchar;
+  typedef __attribute__((address_space(1))) T gT;
+  *reinterpret_cast<gT*>(reinterpret_cast<gchar*>(reinterpret_cast<uintptr_t>(base)) + off) = v;
 }
 
 // Every multiply-add of a row is an explicit fma, so that rows and dot
@@ -168,40 +176,51 @@ __device__ __forceinline__ V elem_at_uniform(const V* p, uint32_t off4) {
 //
 // The rows of hv_segment for NQ row slots per thread (rows tid,
 // tid + kBlock, ...), NQ fixed at compile time so that the loads of all
-// slots are issued together; rows past the segment are clamped to row 0 and
-// not stored.  Stages the thread's rows of x in LDS and passes the
-// workgroup barrier.  The flip bonds of each class are processed in chunks
-// of CLL / CHH / CX (the table padding is a multiple of each).  Returns the
-// thread's share of x.Hx when DOT.
+// slots are issued together.  Every slot of the wave but the last is full
+// (hv_segment gives a wave the slots up to its last row); in the last, rows
+// past the segment read row 0 and are not stored.  Stages the thread's rows
+// of x in LDS and passes the workgroup barrier.  The flip bonds of each
+// class are processed in chunks of CLL / CHH / CX (the table padding is a
+// multiple of each).  Returns the thread's share of x.Hx when DOT.
+//
+// Addressing: a slot's element of a segment of x (or y) is read at the
+// lane offset of slot 0 from a base moved by q * kBlock elements, or, for
+// the last slot, at its own clamped offset; its flip-table records at slot
+// 0's lane offset from a base moved by q * kBlock records.  Rows of the last
+// slot past the segment stay inside the class table (padded to groups of
+// 64 rows), whose padding entries are in-range offsets.
 template <class V, bool DOT, int NQ, int CLL, int CHH, int CX>
 __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const V* __restrict__ x,
                                           V* __restrict__ y, double* xs, uint32_t len, uint32_t base, double dh,
                                           const uint16_t* nbp, const uint16_t* xnp, const double* dlp) {
+  constexpr uint32_t VS = (uint32_t)sizeof(V);
+  constexpr int QL = NQ - 1;  // the last slot
   const uint32_t tid = threadIdx.x;
   const V* xb = x + base;
-  uint32_t rc[NQ];
-  bool ok[NQ];
+  static_assert(CLL % kLLChunk == 0 && CX % kXChunk == 0, "schedule chunks are multiples of the table chunks");
+  typedef uint16_t u16x4 __attribute__((ext_vector_type(4)));
+  typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+  const uint32_t rl = tid + QL * kBlock;
+  const bool okl = rl < len;
+  const uint32_t vo0 = lane_off(VS * tid);               // slot 0 (and every full slot)
+  const uint32_t vol = lane_off(okl ? VS * rl : 0u);     // the last slot, clamped
+  // flip-table records (flip_index, ed_layout.hpp): bond chunk k of row r
+  // starts at rec(r) + k * 64 * chunk
+  const uint32_t lo_ll = lane_off(2u * ((tid >> 6) * 64u * (uint32_t)L.n_ll_p + (tid & 63u) * kLLChunk));
+  const uint32_t lo_x = lane_off(2u * ((tid >> 6) * 64u * (uint32_t)L.n_xb_p + (tid & 63u) * kXChunk));
+  const uint32_t ll_slot = (uint32_t)kBlock * (uint32_t)L.n_ll_p;  // u16 entries per slot
+  const uint32_t x_slot = (uint32_t)kBlock * (uint32_t)L.n_xb_p;
+  // element of slot q of the segment starting at p
+  auto seg_ld = [&](const V* p, int q) -> V {
+    return q < QL ? ld_at<V>(p + q * kBlock, vo0) : ld_at<V>(p, vol);
+  };
   V xo[NQ];
   double d[NQ], off[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
-    const uint32_t r = tid + q * kBlock;
-    ok[q] = r < len;
-    rc[q] = ok[q] ? r : 0u;  // clamped row: every read stays in bounds
-    xo[q] = xb[rc[q]];
-    d[q] = dh + dlp[rc[q]];
+    xo[q] = seg_ld(xb, q);
+    d[q] = dh + (q < QL ? ld_at<double>(dlp + q * kBlock, vo0 * (8u / VS)) : ld_at<double>(dlp, vol * (8u / VS)));
     off[q] = 0.0;
-  }
-  // Row records in the class flip tables (flip_index, ed_layout.hpp): bond
-  // chunk k of row r starts at rec(r) + k * 64 * chunk.
-  static_assert(CLL % kLLChunk == 0 && CX % kXChunk == 0, "schedule chunks are multiples of the table chunks");
-  typedef uint16_t u16x4 __attribute__((ext_vector_type(4)));
-  typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
-  uint32_t lrec[NQ], xrec[NQ];
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    lrec[q] = (rc[q] >> 6) * 64u * (uint32_t)L.n_ll_p + (rc[q] & 63u) * kLLChunk;
-    xrec[q] = (rc[q] >> 6) * 64u * (uint32_t)L.n_xb_p + (rc[q] & 63u) * kXChunk;
   }
   // The cross-bond table entries of the first chunk are loaded now, so that
   // they arrive while the other bonds are processed.
@@ -212,79 +231,68 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
-      for (int k = 0; k < KX; ++k) xr[q][k] = ld_lane<u16x2>(xnp + k * kXChunk * 64u, 2u * xrec[q]);
+      for (int k = 0; k < KX; ++k) xr[q][k] = ld_at<u16x2>(xnp + q * x_slot + k * kXChunk * 64u, lo_x);
   }
   // low-low flips: target rows from the class table, values from LDS.  The
-  // table entries of the next chunk are loaded while the current one is
-  // processed.
+  // table entries of chunk b + CLL are loaded while chunk b is processed,
+  // into the other of two register sets (ta, tb).
   const int n_ll = (L.n_ll + CLL - 1) / CLL * CLL;
   constexpr int KL = CLL / kLLChunk;
-  u16x4 tb[NQ][KL];
-  if (n_ll > 0) {
+  u16x4 ta[NQ][KL], tb[NQ][KL];
+  auto ll_load = [&](u16x4(&t)[NQ][KL], int b) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
-      for (int k = 0; k < KL; ++k) tb[q][k] = ld_lane<u16x4>(nbp + k * kLLChunk * 64u, 2u * lrec[q]);
-  }
-  // masks of the first 64 high-high bonds (one per lane, for the ballot below)
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t ml0 = (int)lane < L.n_hh ? L.hh_mask[lane] : 0u;
+      for (int k = 0; k < KL; ++k)
+        t[q][k] = ld_at<u16x4>(nbp + q * ll_slot + (uint32_t)(b + k * kLLChunk) * 64u, lo_ll);
+  };
+  auto ll_apply = [&](const u16x4(&t)[NQ][KL], int b) {
+    double tc[CLL];
+#pragma unroll
+    for (int c = 0; c < CLL; ++c) tc[c] = ldc(L.ll_t + b + c);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int c = 0; c < CLL; ++c) {
+        const uint32_t j = t[q][c / kLLChunk][c % kLLChunk];
+        off[q] = fma(tc[c], *reinterpret_cast<const double*>(reinterpret_cast<const char*>(xs) + j), off[q]);
+      }
+  };
+  if (n_ll > 0) ll_load(ta, 0);
   // The segment's slice of x into LDS: each thread stores its own rows (the
   // rows of the segment are exactly the rows of the threads' slots), with
   // the table loads above already in flight; then the barrier that every
   // wave of the workgroup passes once (hv_segment).
 #pragma unroll
-  for (int q = 0; q < NQ; ++q)
-    if (ok[q]) xs[rc[q]] = (double)xo[q];
+  for (int q = 0; q < QL; ++q) xs[tid + q * kBlock] = (double)xo[q];
+  if (okl) xs[rl] = (double)xo[QL];
   if (tid == 0) xs[len] = 0.0;
   __syncthreads();
-  for (int b0 = 0; b0 < n_ll; b0 += CLL) {
-    uint32_t j[NQ][CLL];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q)
-#pragma unroll
-      for (int k = 0; k < KL; ++k)
-#pragma unroll
-        for (int c = 0; c < kLLChunk; ++c) j[q][k * kLLChunk + c] = tb[q][k][c];
-    if (b0 + CLL < n_ll) {
-#pragma unroll
-      for (int q = 0; q < NQ; ++q)
-#pragma unroll
-        for (int k = 0; k < KL; ++k)
-          tb[q][k] = ld_lane<u16x4>(nbp + (uint32_t)(b0 + CLL + k * kLLChunk) * 64u, 2u * lrec[q]);
-    }
-    double t[CLL];
-#pragma unroll
-    for (int c = 0; c < CLL; ++c) t[c] = ldc(L.ll_t + b0 + c);
-#pragma unroll
-    for (int q = 0; q < NQ; ++q)
-#pragma unroll
-      for (int c = 0; c < CLL; ++c)
-        off[q] = fma(t[c], *reinterpret_cast<const double*>(reinterpret_cast<const char*>(xs) + j[q][c]), off[q]);
+  for (int b0 = 0; b0 < n_ll; b0 += 2 * CLL) {
+    if (b0 + CLL < n_ll) ll_load(tb, b0 + CLL);
+    ll_apply(ta, b0);
+    if (b0 + CLL >= n_ll) break;
+    if (b0 + 2 * CLL < n_ll) ll_load(ta, b0 + 2 * CLL);
+    ll_apply(tb, b0 + CLL);
   }
   // high-high flips: a bond flips every row of the segment or none (its two
-  // bits lie in h), with a contiguous target run.  The bonds that flip are
-  // found 64 at a time (one per lane, then a ballot), and only those are
-  // visited, in bond order; the unfilled slots of the last chunk read this
-  // segment with weight 0.  Skipped bonds would add exact zeros, so the
-  // sums are unchanged.
-  for (int w0 = 0; w0 < L.n_hh; w0 += 64) {
-    const int bl = w0 + (int)lane;
-    const uint32_t ml = w0 == 0 ? ml0 : (bl < L.n_hh ? L.hh_mask[bl] : 0u);
-    const uint32_t bits = h & ml;
-    uint64_t act = __ballot(bits != 0u && bits != ml);
-    while (act != 0u) {
+  // bits lie in h), with a contiguous target run.  The bonds that flip h are
+  // listed per high word in bond order (hh_off / hh_nb / hh_tv), padded with
+  // weight-0 reads of this segment; slots of a chunk past the list do the
+  // same.  Bonds that do not flip would add exact zeros, so the sums are
+  // those over all bonds.
+  {
+    const uint32_t e0 = ldc(L.hh_off + h), e1 = ldc(L.hh_off + h + 1);
+    for (uint32_t e = e0; e < e1; e += CHH) {
       // neighbour segment starts as element offsets (a select between
       // pointers would make the loads generic-address ones)
       uint32_t xt[CHH];
       double tt[CHH];
 #pragma unroll
       for (int c = 0; c < CHH; ++c) {
-        if (act != 0u) {
-          const int b = w0 + __builtin_ctzll(act);
-          act &= act - 1u;
-          xt[c] = ldc(L.hi_base + (h ^ ldc(L.hh_mask + b)));
-          tt[c] = ldc(L.hh_t + b);
+        if (c < kHHList || e + c < e1) {
+          xt[c] = ldc(L.hh_nb + e + c);
+          tt[c] = ldc(L.hh_tv + e + c);
         } else {
           xt[c] = base;
           tt[c] = 0.0;
@@ -294,29 +302,27 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
 #pragma unroll
       for (int q = 0; q < NQ; ++q)
 #pragma unroll
-        for (int c = 0; c < CHH; ++c) v[q][c] = elem_at_uniform(x + xt[c], 4u * rc[q]);
+        for (int c = 0; c < CHH; ++c) v[q][c] = seg_ld(x + xt[c], q);
 #pragma unroll
       for (int q = 0; q < NQ; ++q)
 #pragma unroll
         for (int c = 0; c < CHH; ++c) off[q] = fma(tt[c], (double)v[q][c], off[q]);
     }
   }
-  // cross bonds: diagonal always, flip when the high and low bits differ.
-  // If the neighbouring high word has no segment no row can flip (its low
-  // word would need a set bit more or fewer than it has), and the read
-  // falls back to this segment.  The diagonal already holds every cross
-  // bond's parallel value (diag_hi); each antiparallel bond adds its
-  // anti - par difference.
+  // cross bonds: diagonal always, flip when the high and low bits differ
+  // (bit 15 of the table entry is the row's low bit).  Where the
+  // neighbouring high word has no segment no row can flip (its low word
+  // would need a set bit more or fewer than it has), and xb_nb points to this
+  // segment.  The diagonal already holds every cross bond's parallel value
+  // (diag_hi); each antiparallel bond adds its anti - par difference.
+  const uint32_t* xnb = L.xb_nb + h * (uint32_t)L.n_xb_p;
   for (int b0 = 0; b0 < n_xb; b0 += CX) {
-    const V* xn[CX];
-    bool hb[CX];
+    uint32_t xt[CX], hx[CX];
     double de[CX], tx[CX];
 #pragma unroll
     for (int c = 0; c < CX; ++c) {
-      const uint32_t hbit = ldc(L.xb_hbit + b0 + c);
-      hb[c] = (h & hbit) != 0u;
-      const uint32_t nb = ldc(L.hi_base + (h ^ hbit));
-      xn[c] = x + (nb == 0xffffffffu ? base : nb);
+      hx[c] = (h & ldc(L.xb_hbit + b0 + c)) != 0u ? 0x8000u : 0u;
+      xt[c] = ldc(xnb + b0 + c);
       de[c] = ldc(L.xb_anti + b0 + c) - ldc(L.xb_par + b0 + c);
       tx[c] = ldc(L.xb_t + b0 + c);
     }
@@ -332,7 +338,7 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
       for (int q = 0; q < NQ; ++q)
 #pragma unroll
         for (int k = 0; k < KX; ++k)
-          xr[q][k] = ld_lane<u16x2>(xnp + (uint32_t)(b0 + CX + k * kXChunk) * 64u, 2u * xrec[q]);
+          xr[q][k] = ld_at<u16x2>(xnp + q * x_slot + (uint32_t)(b0 + CX + k * kXChunk) * 64u, lo_x);
     }
     V v[NQ][CX];
     bool f[NQ][CX];
@@ -340,26 +346,31 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
       for (int c = 0; c < CX; ++c) {
-        f[q][c] = ((e[q][c] >> 15) != 0u) != hb[c];
-        v[q][c] = elem_at_uniform(xn[c], f[q][c] ? (e[q][c] & 0x7fffu) : 0u);
+        f[q][c] = ((e[q][c] ^ hx[c]) & 0x8000u) != 0u;
+        v[q][c] = ld_lane<V>(x + xt[c], f[q][c] ? (e[q][c] & 0x7fffu) * (VS / 4u) : 0u);
       }
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
       for (int c = 0; c < CX; ++c) {
-        // selects on the flag as 1.0 / 0.0 and on the FP32 value (one
-        // 32-bit select each, where selecting the FP64 terms takes two)
+        // the flag as 1.0 / 0.0 and the value selected in the storage type
+        // (one 32-bit select each; the asm keeps the select before the
+        // conversion, where selecting the FP64 value takes two)
+        V w = f[q][c] ? v[q][c] : (V)0;
+        asm volatile("" : "+v"(w));
         d[q] = fma(f[q][c] ? 1.0 : 0.0, de[c], d[q]);
-        off[q] = fma(tx[c], (double)(f[q][c] ? v[q][c] : (V)0), off[q]);
+        off[q] = fma(tx[c], (double)w, off[q]);
       }
   }
   double dotp = 0.0;
+  V* yb = y + base;
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
-    if (ok[q]) {
-      const double xi = xs[rc[q]];
+    if (q < QL || okl) {
+      const double xi = xs[q < QL ? tid + q * kBlock : rl];
       const double yv = fma(d[q], xi, off[q]);
-      y[base + rc[q]] = (V)yv;
+      if (q < QL) st_at<V>(yb + q * kBlock, vo0, (V)yv);
+      else st_at<V>(yb, vol, (V)yv);
       if (DOT) dotp = fma(xi, yv, dotp);  // FP64 value before the storage rounding
     }
   }

@@ -145,9 +145,18 @@ struct SectorLayout {
   std::vector<uint16_t> nbp16, xn16;
   std::vector<uint32_t> nbp_off, xn_off;  // [B+2]
   std::vector<double> ll_t;               // [n_ll_p]
-  std::vector<uint32_t> hh_mask;          // [n_hh_p], 0 = null
-  std::vector<double> hh_t;
+  // The high-high bonds that flip the rows of high word h (the two bits of
+  // h & mask differ), in bond order: entry e in [hh_off[h], hh_off[h+1])
+  // holds the first row of the segment of h ^ mask (hh_nb) and the bond's
+  // coefficient (hh_tv).  Each list is padded to a multiple of kHHList with
+  // coefficient-0 entries that read h's own segment.
+  std::vector<uint32_t> hh_off;           // [2^NH + 1]
+  std::vector<uint32_t> hh_nb;
+  std::vector<double> hh_tv;
   std::vector<uint32_t> xb_hbit;          // [n_xb_p], 0 = null
+  // xb_nb[h * n_xb_p + c]: the first row of the segment of h ^ hbit_c, or of
+  // h's own segment when that high word has none (no row of h flips then)
+  std::vector<uint32_t> xb_nb;
   std::vector<double> xb_par, xb_anti, xb_t;
   std::vector<double> diag_lo;         // [2^B] site + low-low bond diagonal
   std::vector<double> diag_hi;         // [2^NH] site + high-high bond diagonal + every cross bond's par value
@@ -163,6 +172,8 @@ struct SectorLayout {
 // chunks of up to 8; bond lists and flip tables are padded to these
 // multiples, which every schedule's chunk sizes divide.
 constexpr int kPadLL = 8, kPadHH = 8, kPadX = 4;
+// padding of each high word's list of flipping high-high bonds
+constexpr int kHHList = 4;
 
 inline int round_up(int n, int m) { return (n + m - 1) / m * m; }
 
@@ -499,11 +510,27 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1, bool c
   }
   L.ll_t.assign(L.n_ll_p, 0.0);
   for (size_t b = 0; b < L.ll.size(); ++b) L.ll_t[b] = L.ll[b].t;
-  L.hh_mask.assign(L.n_hh_p, 0u);
-  L.hh_t.assign(L.n_hh_p, 0.0);
-  for (size_t b = 0; b < L.hh.size(); ++b) {
-    L.hh_mask[b] = L.hh[b].mask;
-    L.hh_t[b] = L.hh[b].t;
+  // flipping high-high bonds per high word, in bond order
+  L.hh_off.assign((size_t)nhi + 1, 0u);
+  L.hh_nb.clear();
+  L.hh_tv.clear();
+  for (uint32_t h = 0; h < nhi; ++h) {
+    const uint32_t own = L.hi_base[h];
+    if (own != 0xffffffffu) {
+      const size_t e0 = L.hh_nb.size();
+      for (const FlipBond& f : L.hh) {
+        const uint32_t bits = h & f.mask;
+        if (bits == 0u || bits == f.mask) continue;
+        L.hh_nb.push_back(L.hi_base[h ^ f.mask]);
+        L.hh_tv.push_back(f.t);
+      }
+      while ((L.hh_nb.size() - e0) % kHHList != 0) {
+        L.hh_nb.push_back(own);
+        L.hh_tv.push_back(0.0);
+      }
+    }
+    if (L.hh_nb.size() >= 0xffffffffULL) throw std::invalid_argument("high-high flip lists exceed 2^32-1 entries");
+    L.hh_off[h + 1] = (uint32_t)L.hh_nb.size();
   }
   L.xb_hbit.assign(L.n_xb_p, 0u);
   L.xb_par.assign(L.n_xb_p, 0.0);
@@ -514,6 +541,15 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1, bool c
     L.xb_par[b] = L.xb[b].e_par;
     L.xb_anti[b] = L.xb[b].e_anti;
     L.xb_t[b] = L.xb[b].t;
+  }
+  L.xb_nb.assign((size_t)nhi * L.n_xb_p, 0u);
+  for (uint32_t h = 0; h < nhi; ++h) {
+    const uint32_t own = L.hi_base[h];
+    if (own == 0xffffffffu) continue;
+    for (int c = 0; c < L.n_xb_p; ++c) {
+      const uint32_t nb = L.hi_base[h ^ L.xb_hbit[c]];
+      L.xb_nb[(size_t)h * L.n_xb_p + c] = nb == 0xffffffffu ? own : nb;
+    }
   }
   return L;
 }
