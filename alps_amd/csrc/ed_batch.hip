@@ -292,13 +292,9 @@ size_t table_bytes(const SectorLayout& L) {
   add(L.diag_hi.size(), 8);
   add(L.ll_t.size(), 8);
   add(L.hh_off.size(), 4);
-  add(L.hh_nb.size(), 4);
-  add(L.hh_tv.size(), 8);
-  add(L.xb_hbit.size(), 4);
+  add(L.hh_ent.size(), sizeof(HHEntry));
+  add(L.xb_c.size(), sizeof(XBondConst));
   add(L.xb_nb.size(), 4);
-  add(L.xb_par.size(), 8);
-  add(L.xb_anti.size(), 8);
-  add(L.xb_t.size(), 8);
   add(L.site_dn.size(), 8);
   add(L.site_up.size(), 8);
   add(L.bond_p1.size(), 4);
@@ -361,13 +357,9 @@ DevLayout place_tables(const SectorLayout& L, char* base, char* stage, const Sha
   d.diag_hi = put(L.diag_hi);
   d.ll_t = put(L.ll_t);
   d.hh_off = put(L.hh_off);
-  d.hh_nb = put(L.hh_nb);
-  d.hh_tv = put(L.hh_tv);
-  d.xb_hbit = put(L.xb_hbit);
+  d.hh_ent = put(L.hh_ent);
+  d.xb_c = put(L.xb_c);
   d.xb_nb = put(L.xb_nb);
-  d.xb_par = put(L.xb_par);
-  d.xb_anti = put(L.xb_anti);
-  d.xb_t = put(L.xb_t);
   d.site_dn = put(L.site_dn);
   d.site_up = put(L.site_up);
   d.bond_p1 = put(L.bond_p1);

@@ -290,13 +290,9 @@ void operator_upload(alps_ed_operator op) {
   ALPS_ED_UP(xn_off)
   ALPS_ED_UP(ll_t)
   ALPS_ED_UP(hh_off)
-  ALPS_ED_UP(hh_nb)
-  ALPS_ED_UP(hh_tv)
-  ALPS_ED_UP(xb_hbit)
+  ALPS_ED_UP(hh_ent)
+  ALPS_ED_UP(xb_c)
   ALPS_ED_UP(xb_nb)
-  ALPS_ED_UP(xb_par)
-  ALPS_ED_UP(xb_anti)
-  ALPS_ED_UP(xb_t)
   ALPS_ED_UP(site_dn)
   ALPS_ED_UP(site_up)
   ALPS_ED_UP(bond_p1)

@@ -277,7 +277,7 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
   }
   // high-high flips: a bond flips every row of the segment or none (its two
   // bits lie in h), with a contiguous target run.  The bonds that flip h are
-  // listed per high word in bond order (hh_off / hh_nb / hh_tv), padded with
+  // listed per high word in bond order (hh_off / hh_ent), padded with
   // weight-0 reads of this segment; slots of a chunk past the list do the
   // same.  Bonds that do not flip would add exact zeros, so the sums are
   // those over all bonds.
@@ -291,8 +291,9 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
 #pragma unroll
       for (int c = 0; c < CHH; ++c) {
         if (c < kHHList || e + c < e1) {
-          xt[c] = ldc(L.hh_nb + e + c);
-          tt[c] = ldc(L.hh_tv + e + c);
+          const HHEntry en = ldc_struct(L.hh_ent + e + c);
+          xt[c] = en.nb;
+          tt[c] = en.t;
         } else {
           xt[c] = base;
           tt[c] = 0.0;
@@ -321,10 +322,11 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
     double de[CX], tx[CX];
 #pragma unroll
     for (int c = 0; c < CX; ++c) {
-      hx[c] = (h & ldc(L.xb_hbit + b0 + c)) != 0u ? 0x8000u : 0u;
+      const XBondConst k = ldc_struct(L.xb_c + b0 + c);
+      hx[c] = (h & k.hbit) != 0u ? 0x8000u : 0u;
       xt[c] = ldc(xnb + b0 + c);
-      de[c] = ldc(L.xb_anti + b0 + c) - ldc(L.xb_par + b0 + c);
-      tx[c] = ldc(L.xb_t + b0 + c);
+      de[c] = k.de;
+      tx[c] = k.t;
     }
     uint32_t e[NQ][CX];
 #pragma unroll

@@ -31,13 +31,9 @@ struct DevLayout {
   const uint32_t* xn_off;
   const double* ll_t;
   const uint32_t* hh_off;     // [2^NH + 1] per high word: its flipping high-high bonds
-  const uint32_t* hh_nb;      //   first row of the neighbour segment
-  const double* hh_tv;        //   coefficient (0 for padding)
-  const uint32_t* xb_hbit;
+  const HHEntry* hh_ent;      //   (neighbour segment, coefficient), padded with 0-coefficient entries
+  const XBondConst* xb_c;     // [n_xb_p] cross-bond constants
   const uint32_t* xb_nb;      // [2^NH * n_xb_p] first row of the cross-bond neighbour segment
-  const double* xb_par;
-  const double* xb_anti;
-  const double* xb_t;
   // reference-order description for the element enumeration
   const double* site_dn;
   const double* site_up;

@@ -113,6 +113,21 @@ struct CrossBond {
 struct LowTables;
 struct FlipTables;
 
+// A listed high-high flip (the H.v kernel reads four with one scalar load).
+struct HHEntry {
+  uint32_t nb;   // first row of the neighbour segment
+  uint32_t pad;
+  double t;      // coefficient
+};
+
+// The per-bond constants of a cross bond as the H.v kernel reads them.
+struct XBondConst {
+  double de;      // e_anti - e_par
+  double t;       // flip coefficient
+  uint32_t hbit;  // bit in the high word (0 for a null bond)
+  uint32_t pad;
+};
+
 struct SectorLayout {
   int N = 0, k = 0, B = 0, NH = 0;
   uint64_t D = 0;
@@ -146,18 +161,17 @@ struct SectorLayout {
   std::vector<uint32_t> nbp_off, xn_off;  // [B+2]
   std::vector<double> ll_t;               // [n_ll_p]
   // The high-high bonds that flip the rows of high word h (the two bits of
-  // h & mask differ), in bond order: entry e in [hh_off[h], hh_off[h+1])
-  // holds the first row of the segment of h ^ mask (hh_nb) and the bond's
-  // coefficient (hh_tv).  Each list is padded to a multiple of kHHList with
+  // h & mask differ), in bond order: entries [hh_off[h], hh_off[h+1]) of
+  // hh_ent, each the first row of the segment of h ^ mask and the bond's
+  // coefficient.  Each list is padded to a multiple of kHHList with
   // coefficient-0 entries that read h's own segment.
   std::vector<uint32_t> hh_off;           // [2^NH + 1]
-  std::vector<uint32_t> hh_nb;
-  std::vector<double> hh_tv;
+  std::vector<HHEntry> hh_ent;
   std::vector<uint32_t> xb_hbit;          // [n_xb_p], 0 = null
+  std::vector<XBondConst> xb_c;           // [n_xb_p] per cross bond, null bonds zero
   // xb_nb[h * n_xb_p + c]: the first row of the segment of h ^ hbit_c, or of
   // h's own segment when that high word has none (no row of h flips then)
   std::vector<uint32_t> xb_nb;
-  std::vector<double> xb_par, xb_anti, xb_t;
   std::vector<double> diag_lo;         // [2^B] site + low-low bond diagonal
   std::vector<double> diag_hi;         // [2^NH] site + high-high bond diagonal + every cross bond's par value
   std::vector<FlipBond> ll, hh;        // low-low and high-high flip bonds
@@ -512,36 +526,25 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1, bool c
   for (size_t b = 0; b < L.ll.size(); ++b) L.ll_t[b] = L.ll[b].t;
   // flipping high-high bonds per high word, in bond order
   L.hh_off.assign((size_t)nhi + 1, 0u);
-  L.hh_nb.clear();
-  L.hh_tv.clear();
+  L.hh_ent.clear();
   for (uint32_t h = 0; h < nhi; ++h) {
     const uint32_t own = L.hi_base[h];
     if (own != 0xffffffffu) {
-      const size_t e0 = L.hh_nb.size();
+      const size_t e0 = L.hh_ent.size();
       for (const FlipBond& f : L.hh) {
         const uint32_t bits = h & f.mask;
         if (bits == 0u || bits == f.mask) continue;
-        L.hh_nb.push_back(L.hi_base[h ^ f.mask]);
-        L.hh_tv.push_back(f.t);
+        L.hh_ent.push_back({L.hi_base[h ^ f.mask], 0u, f.t});
       }
-      while ((L.hh_nb.size() - e0) % kHHList != 0) {
-        L.hh_nb.push_back(own);
-        L.hh_tv.push_back(0.0);
-      }
+      while ((L.hh_ent.size() - e0) % kHHList != 0) L.hh_ent.push_back({own, 0u, 0.0});
     }
-    if (L.hh_nb.size() >= 0xffffffffULL) throw std::invalid_argument("high-high flip lists exceed 2^32-1 entries");
-    L.hh_off[h + 1] = (uint32_t)L.hh_nb.size();
+    if (L.hh_ent.size() >= 0xffffffffULL) throw std::invalid_argument("high-high flip lists exceed 2^32-1 entries");
+    L.hh_off[h + 1] = (uint32_t)L.hh_ent.size();
   }
   L.xb_hbit.assign(L.n_xb_p, 0u);
-  L.xb_par.assign(L.n_xb_p, 0.0);
-  L.xb_anti.assign(L.n_xb_p, 0.0);
-  L.xb_t.assign(L.n_xb_p, 0.0);
-  for (size_t b = 0; b < L.xb.size(); ++b) {
-    L.xb_hbit[b] = L.xb[b].hbit;
-    L.xb_par[b] = L.xb[b].e_par;
-    L.xb_anti[b] = L.xb[b].e_anti;
-    L.xb_t[b] = L.xb[b].t;
-  }
+  for (size_t b = 0; b < L.xb.size(); ++b) L.xb_hbit[b] = L.xb[b].hbit;
+  L.xb_c.assign(L.n_xb_p, XBondConst{0.0, 0.0, 0u, 0u});
+  for (size_t b = 0; b < L.xb.size(); ++b) L.xb_c[b] = {L.xb[b].e_anti - L.xb[b].e_par, L.xb[b].t, L.xb[b].hbit, 0u};
   L.xb_nb.assign((size_t)nhi * L.n_xb_p, 0u);
   for (uint32_t h = 0; h < nhi; ++h) {
     const uint32_t own = L.hi_base[h];
