@@ -240,7 +240,8 @@ def _translation_args(translations, n_sites):
 class Operator:
     """Matrix-free device Hamiltonian of one sector (alps_ed_operator).  With
     translations=(phases, site maps) it is the momentum sector of the
-    model's Sz sector, whose vectors are realified (length 2 * sector_dim)."""
+    model's Sz sector, whose complex vectors are stored as 2 * sector_dim
+    reals, (re, im) per element (the layout of a complex array)."""
 
     def __init__(self, model: SpinModel, translations=None):
         self.model = model

@@ -1,7 +1,8 @@
 // ed_bloch.hip -- H.v in a momentum (Bloch) sector.
 //
 // The sector's Hamiltonian is complex Hermitian, H = A + iB.  Vectors are
-// stored realified, (Re v, Im v), and the product is
+// stored as reals, the real and imaginary part of each element side by side
+// (the layout of a std::complex array), and the product is
 //   Re y = A Re x - B Im x,  Im y = B Re x + A Im x,
 // the real symmetric form of H.  For Hermitian H the Lanczos coefficients
 // are real, so the real recurrence on realified vectors is the complex
@@ -36,11 +37,15 @@ __global__ __launch_bounds__(kBlock) void bloch_hv_kernel(DevLayout L, DevBloch 
 // row; the lanes' sums meet by shuffles and the first lane adds the
 // diagonal and stores the row.  A wave's lanes read one contiguous run of
 // entries (the rows of a wave are consecutive in the list); the gathers of
-// x are the only scattered reads.
+// x, one (re, im) pair per entry, are the only scattered reads.
 template <class V, bool DOT>
 __global__ __launch_bounds__(kBlock) void bloch_list_kernel(DevBloch Bl, const V* __restrict__ x,
                                                             V* __restrict__ y, double* __restrict__ dot_out) {
+  typedef V V2 __attribute__((ext_vector_type(2)));
+  typedef double D2 __attribute__((ext_vector_type(2)));
   constexpr uint32_t P = kBlochLanes;
+  const V2* x2 = reinterpret_cast<const V2*>(x);
+  const D2* cf = reinterpret_cast<const D2*>(Bl.coef);
   const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
   const uint32_t i = g / P, sub = g % P;
   const uint32_t dk = Bl.dk;
@@ -51,30 +56,29 @@ __global__ __launch_bounds__(kBlock) void bloch_list_kernel(DevBloch Bl, const V
   if (i < dk) {
     inv = Bl.inv_norm[i];
     d = Bl.diag[i];
-    xr = (double)x[i];
-    xi = (double)x[dk + i];
+    const V2 xo = x2[i];
+    xr = (double)xo.x;
+    xi = (double)xo.y;
     const uint32_t k0 = Bl.row_ptr[i], k1 = Bl.row_ptr[i + 1];
     // two entries per lane and iteration, summed into two pairs
     uint32_t k = k0 + sub;
     for (; k + P < k1; k += 2 * P) {
       const uint32_t e0 = Bl.ent[k], e1 = Bl.ent[k + P];
-      const uint32_t a0 = e0 & Bl.mask_a, c0 = e0 >> Bl.sh_c;
-      const uint32_t a1 = e1 & Bl.mask_a, c1 = e1 >> Bl.sh_c;
-      const double cr0 = Bl.coef_re[c0], ci0 = Bl.coef_im[c0], cr1 = Bl.coef_re[c1], ci1 = Bl.coef_im[c1];
-      const double yr0 = (double)x[a0], yi0 = (double)x[dk + a0];
-      const double yr1 = (double)x[a1], yi1 = (double)x[dk + a1];
-      ar = fma(cr0, yr0, fma(-ci0, yi0, ar));
-      ai = fma(cr0, yi0, fma(ci0, yr0, ai));
-      br = fma(cr1, yr1, fma(-ci1, yi1, br));
-      bi = fma(cr1, yi1, fma(ci1, yr1, bi));
+      const D2 c0 = cf[e0 >> Bl.sh_c], c1 = cf[e1 >> Bl.sh_c];
+      const V2 v0 = x2[e0 & Bl.mask_a], v1 = x2[e1 & Bl.mask_a];
+      const double yr0 = (double)v0.x, yi0 = (double)v0.y, yr1 = (double)v1.x, yi1 = (double)v1.y;
+      ar = fma(c0.x, yr0, fma(-c0.y, yi0, ar));
+      ai = fma(c0.x, yi0, fma(c0.y, yr0, ai));
+      br = fma(c1.x, yr1, fma(-c1.y, yi1, br));
+      bi = fma(c1.x, yi1, fma(c1.y, yr1, bi));
     }
     if (k < k1) {
       const uint32_t e0 = Bl.ent[k];
-      const uint32_t a0 = e0 & Bl.mask_a, c0 = e0 >> Bl.sh_c;
-      const double cr0 = Bl.coef_re[c0], ci0 = Bl.coef_im[c0];
-      const double yr0 = (double)x[a0], yi0 = (double)x[dk + a0];
-      ar = fma(cr0, yr0, fma(-ci0, yi0, ar));
-      ai = fma(cr0, yi0, fma(ci0, yr0, ai));
+      const D2 c0 = cf[e0 >> Bl.sh_c];
+      const V2 v0 = x2[e0 & Bl.mask_a];
+      const double yr0 = (double)v0.x, yi0 = (double)v0.y;
+      ar = fma(c0.x, yr0, fma(-c0.y, yi0, ar));
+      ai = fma(c0.x, yi0, fma(c0.y, yr0, ai));
     }
     ar += br;
     ai += bi;
@@ -87,8 +91,10 @@ __global__ __launch_bounds__(kBlock) void bloch_list_kernel(DevBloch Bl, const V
   double part = 0.0;
   if (i < dk && sub == 0) {
     const double outr = fma(d, xr, inv * ar), outi = fma(d, xi, inv * ai);
-    y[i] = (V)outr;
-    y[dk + i] = (V)outi;
+    V2 o;
+    o.x = (V)outr;
+    o.y = (V)outi;
+    reinterpret_cast<V2*>(y)[i] = o;
     if (DOT) part = fma(xr, outr, xi * outi);
   }
   if (DOT) store_partial(dot_out, part);
@@ -109,7 +115,7 @@ __global__ __launch_bounds__(kBlock) void bloch_expand_kernel(DevBloch Bl, uint3
   if (a != 0xffffffffu) {
     const uint32_t t = (uint32_t)(pk >> 32) & 0xffu;
     const double pr = Bl.ph_re[t], pi = Bl.ph_im[t], nrm = Bl.norm[a];
-    const double wr = (double)w[a], wi = (double)w[Bl.dk + a];
+    const double wr = (double)w[2 * a], wi = (double)w[2 * a + 1];
     vr = nrm * (pr * wr - pi * wi);
     vi = nrm * (pr * wi + pi * wr);
   }

@@ -279,7 +279,7 @@ inline BlochSector bloch_sector(const BlochOrbits& o, const Translations& t, int
 struct BlochFlipRows {
   bool ok = false;
   std::vector<uint32_t> row_ptr, ent;
-  std::vector<double> diag, inv_norm, coef_re, coef_im;
+  std::vector<double> diag, inv_norm, coef;  // coef: (re, im) per code
   uint32_t sh_c = 0, mask_a = 0;
 };
 
@@ -308,15 +308,14 @@ inline BlochFlipRows bloch_flip_rows(const SectorLayout& L, const BlochSector& b
   if (ba + bc > 32 || n_code > (1u << 20)) return r;
   r.sh_c = (uint32_t)ba;
   r.mask_a = ba >= 32 ? 0xffffffffu : ((1u << ba) - 1u);
-  r.coef_re.resize(n_code);
-  r.coef_im.resize(n_code);
+  r.coef.resize(2 * n_code);
   for (uint64_t c = 0; c < cls.size(); ++c)
     for (uint64_t n = 0; n < b.norm_tab.size(); ++n)
       for (uint64_t t = 0; t < n_tr; ++t) {
         const uint64_t code = (c * n_ni + n) * n_tr + t;
         const double f = cls_t[c] * b.norm_tab[n];
-        r.coef_re[code] = f * b.ph_re[t];
-        r.coef_im[code] = f * b.ph_im[t];
+        r.coef[2 * code] = f * b.ph_re[t];
+        r.coef[2 * code + 1] = f * b.ph_im[t];
       }
   r.row_ptr.reserve(dk + 1);
   r.row_ptr.push_back(0);

@@ -325,8 +325,7 @@ void bloch_upload(alps_ed_operator op, const BlochSector& b) {
     pk.add(fr.ent, &d.ent);
     pk.add(fr.diag, &d.diag);
     pk.add(fr.inv_norm, &d.inv_norm);
-    pk.add(fr.coef_re, &d.coef_re);
-    pk.add(fr.coef_im, &d.coef_im);
+    pk.add(fr.coef, &d.coef);
     d.sh_c = fr.sh_c;
     d.mask_a = fr.mask_a;
   }

@@ -432,8 +432,9 @@ __device__ __forceinline__ void hv_segment(const DevLayout& L, uint32_t h, const
   }
 }
 
-// Row i of H.v in a momentum sector (realified vectors of 2 * Bl.dk
-// elements: real parts, then imaginary parts); rows i >= Bl.dk do nothing.
+// Row i of H.v in a momentum sector (vectors of 2 * Bl.dk reals, the real
+// and imaginary parts of element i at 2i and 2i + 1); rows i >= Bl.dk do
+// nothing.
 // The diagonal is the site terms plus each bond's parallel / antiparallel
 // value; each antiparallel bond flips to s' = s_i ^ mask, whose rank in the
 // fixed-Sz sector (split ranking) gives its representative j and
@@ -449,7 +450,7 @@ __device__ __forceinline__ double bloch_row(const DevLayout& L, const DevBloch& 
   const uint32_t s = Bl.rep_state[i];
   double d = 0.0;
   for (int site = 0; site < L.N; ++site) d += ((s >> (L.N - 1 - site)) & 1u) ? L.site_up[site] : L.site_dn[site];
-  const double xr = (double)x[i], xi = (double)x[dk + i];
+  const double xr = (double)x[2 * i], xi = (double)x[2 * i + 1];
   const double ni = Bl.norm[i];
   double ar = 0.0, ai = 0.0;
   for (int b = 0; b < L.n_bonds; ++b) {
@@ -469,13 +470,13 @@ __device__ __forceinline__ double bloch_row(const DevLayout& L, const DevBloch& 
     const uint32_t tr = (uint32_t)(pk >> 32) & 0xffu;
     const double f = tb * (Bl.norm_tab[(uint32_t)(pk >> 40)] / ni);
     const double cr = f * Bl.ph_re[tr], ci = f * Bl.ph_im[tr];
-    const double yr = (double)x[j], yi = (double)x[dk + j];
+    const double yr = (double)x[2 * j], yi = (double)x[2 * j + 1];
     ar += cr * yr - ci * yi;
     ai += cr * yi + ci * yr;
   }
   const double outr = d * xr + ar, outi = d * xi + ai;
-  y[i] = (V)outr;
-  y[dk + i] = (V)outi;
+  y[2 * i] = (V)outr;
+  y[2 * i + 1] = (V)outi;
   return DOT ? xr * outr + xi * outi : 0.0;
 }
 

@@ -45,7 +45,9 @@ struct DevLayout {
 };
 
 // A momentum sector (Bloch basis, ed_bloch.hpp) on the device.  Vectors of
-// the sector are complex and stored realified: (Re v[0..dk), Im v[0..dk)).
+// the sector are complex, stored as 2 * dk reals with the real and imaginary
+// parts of each element side by side (Re v_0, Im v_0, Re v_1, ...: the
+// layout of an array of std::complex).
 struct DevBloch {
   uint32_t dk;                // kept representatives
   const uint32_t* rep_state;  // [dk]
@@ -59,13 +61,12 @@ struct DevBloch {
   // (bloch_flip_rows, ed_bloch.hpp); null where not built (the batch, which
   // uses bloch_row).  Entry = target representative | coefficient code <<
   // sh_c; the code indexes the complex coefficient t_b * norm_a * phase_t
-  // (coef_re, coef_im), which the row scales by 1 / norm_i.
+  // (coef[2 code], coef[2 code + 1]), which the row scales by 1 / norm_i.
   const uint32_t* row_ptr;    // [dk + 1]
   const uint32_t* ent;
   const double* diag;         // [dk] the diagonal element of each row
   const double* inv_norm;     // [dk] 1 / norm
-  const double* coef_re;      // per coefficient code
-  const double* coef_im;
+  const double* coef;         // per coefficient code: re, im
   uint32_t sh_c, mask_a;
 };
 

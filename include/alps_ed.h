@@ -92,8 +92,10 @@ int alps_ed_operator_create(const alps_ed_model* model, alps_ed_operator* op);
  * translations as graph_helper::translations(k) lists them
  * (hypercubic.h:425-462): site maps perm[t*n_sites + i] (image of site i;
  * t = 0 is the identity) with phases (phase_re[t], phase_im[t]).  The
- * sector's vectors are complex and stored realified, (Re v[0..d),
- * Im v[0..d)): the operator's dimension is 2d and every vector call
+ * sector's vectors are complex and stored as 2d reals with the real and
+ * imaginary part of each element side by side (Re v_0, Im v_0, Re v_1, ...,
+ * the layout of a std::complex<double> array): the operator's dimension is
+ * 2d and every vector call
  * (matvec, Lanczos, Ritz vectors) works on that form.  Explicit states,
  * elements and spin sums are not available for it.  The model must be
  * invariant under the translations. */
@@ -156,7 +158,7 @@ int alps_ed_vector_nrm2(alps_ed_vector x, double* result);
 /* ietl::mult(m, x, y) -> y = prod(m, x) (interface/ublas.h:57-63). */
 int alps_ed_matvec(alps_ed_operator op, alps_ed_vector x, alps_ed_vector y);
 
-/* A momentum-sector vector w (realified, of an operator from
+/* A momentum-sector vector w ((re, im) pairs, of an operator from
  * alps_ed_operator_create_momentum) in the configuration basis of its
  * fixed-Sz sector: re + i im, FP64 vectors of C(N, n_up) elements.  State r
  * of a kept orbit gets phase_t * norm_a * w_a, with a its representative and
