@@ -35,9 +35,10 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE scaled by a copy kernel of known size
 # (profiles/hv_kernel_heisenberg_L24.md); valid for that kernel version and L
 # only.  The schedules that spill write their scratch too.
-PMC_TRAFFIC_BYTES = {24: [51.89e6, 71.02e6, 51.94e6, 51.80e6, 70.99e6, 52.29e6]}
-PMC_TRAFFIC_SOURCE = ("profiles/hv_kernel_heisenberg_L24.md, table 'HBM bytes per launch' "
-                      "(FETCH_SIZE + WRITE_SIZE, scaled by the copy kernel), row of the chosen schedule")
+PMC_TRAFFIC_BYTES = {24: [52.98e6, 53.06e6, 53.02e6, 52.91e6, 57.88e6, 53.44e6]}
+PMC_TRAFFIC_SOURCE = ("profiles/hv_kernel_heisenberg_L24.md, section 'HBM byte counters', column "
+                      "'traffic per launch' (FETCH_SIZE + WRITE_SIZE, scaled by the copy kernel), "
+                      "row of the chosen schedule")
 # H.v schedules (waves per SIMD, low-low / high-high / cross chunk sizes), in
 # the order of the engine's schedule table
 SCHEDULES = [(8, 4, 4, 2), (8, 4, 8, 2), (6, 4, 8, 2), (6, 4, 4, 2), (8, 8, 4, 2), (6, 8, 8, 4)]
