@@ -694,7 +694,7 @@ Group upload_group(alps_ed_batch b, const std::vector<Problem*>& run) {
 
 // Per-problem sizes that follow from its layout.
 void size_problem(Problem& p, alps_ed_dtype dtype) {
-  p.layout = build_layout(p.spec, -1, false);  // shared tables stay in L.low / L.flip
+  p.layout = build_layout(p.spec, -1, false, true);  // shared tables stay in L.low / L.flip; lists on the host
   if (p.trans) {
     // the Bloch basis at this momentum; the orbits are shared by the
     // momenta of the same Sz sector

@@ -69,6 +69,8 @@ struct alps_ed_operator_s {
   SectorLayout layout;
   DevLayout dev{};
   DevBuf tables;  // every layout table, packed (operator_upload)
+  DevBuf lists;   // per-high-word lists built on the device: hh_off and xb_nb,
+  DevBuf hh_ent;  // and the listed high-high flips
   uint64_t bytes = 0;
   uint64_t n = 0;  // vector length: D, or 2 * dk for a momentum sector (realified)
   int schedule = 0;  // H.v schedule for FP32 vectors (kHvSchedule)
@@ -289,10 +291,19 @@ void operator_upload(alps_ed_operator op) {
   ALPS_ED_UP(xn16)
   ALPS_ED_UP(xn_off)
   ALPS_ED_UP(ll_t)
-  ALPS_ED_UP(hh_off)
-  ALPS_ED_UP(hh_ent)
   ALPS_ED_UP(xb_c)
-  ALPS_ED_UP(xb_nb)
+  const bool dev_lists = L.hh_off.empty();  // host_lists = false: built below
+  const uint32_t *hh_mask = nullptr, *xb_hbit = nullptr;
+  const double* hh_t = nullptr;
+  if (dev_lists) {
+    pk.add(L.hh_mask, &hh_mask);
+    pk.add(L.hh_t, &hh_t);
+    pk.add(L.xb_hbit, &xb_hbit);
+  } else {
+    ALPS_ED_UP(hh_off)
+    ALPS_ED_UP(hh_ent)
+    ALPS_ED_UP(xb_nb)
+  }
   ALPS_ED_UP(site_dn)
   ALPS_ED_UP(site_up)
   ALPS_ED_UP(bond_p1)
@@ -304,6 +315,30 @@ void operator_upload(alps_ed_operator op) {
   pk.upload(op->tables);
   op->bytes = pk.bytes();
   op->n = L.D;
+  if (dev_lists && L.D > 0) {
+    // count, prefix sums and cross-bond neighbours, then (once the entry
+    // count is known) the entries; on the null stream, finished before return
+    const uint32_t nhi = 1u << L.NH;
+    const size_t off_bytes = ((size_t)(nhi + 1) * 4 + 255) & ~size_t(255);
+    const size_t xb_bytes = std::max<size_t>((size_t)nhi * L.n_xb_p * 4, 4);
+    op->lists.alloc(off_bytes + xb_bytes);
+    uint32_t* off = op->lists.as<uint32_t>();
+    uint32_t* xnb = reinterpret_cast<uint32_t*>(op->lists.as<char>() + off_bytes);
+    hip_check(launch_hh_count(nhi, d.hi_base, hh_mask, (int)L.hh_mask.size(), off, xb_hbit, L.n_xb_p, xnb, nullptr),
+              "high-high lists");
+    uint32_t total = 0;
+    hip_check(hipMemcpy(&total, off + nhi, sizeof(uint32_t), hipMemcpyDeviceToHost), "high-high lists");
+    if (total == 0xffffffffu) throw std::invalid_argument("high-high flip lists exceed 2^32-1 entries");
+    op->hh_ent.alloc(std::max<size_t>(total, 1) * sizeof(HHEntry));
+    hip_check(launch_hh_fill(nhi, d.hi_base, hh_mask, hh_t, (int)L.hh_mask.size(), off, op->hh_ent.as<HHEntry>(),
+                             nullptr),
+              "high-high lists");
+    hip_check(hipStreamSynchronize(nullptr), "high-high lists");
+    d.hh_off = off;
+    d.hh_ent = op->hh_ent.as<HHEntry>();
+    d.xb_nb = xnb;
+    op->bytes += op->lists.bytes + op->hh_ent.bytes;
+  }
 }
 
 // Turns an uploaded operator into its momentum sector.

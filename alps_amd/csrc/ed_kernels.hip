@@ -65,6 +65,88 @@ void launch_hv_schedule(const DevLayout& L, const V* x, V* y, double* dot_out, h
   launch_hv_with<V, DOT, W, 4, 4, 2>(L, x, y, dot_out, st, lds);
 }
 
+// ---------------------------------------------------------------------------
+// Per-high-word lists of a layout, built on the device (build_hv_lists)
+// ---------------------------------------------------------------------------
+// off[h + 1] = the high-high bonds that flip high word h (its two bits
+// differ), rounded up to kHHList; 0 where h has no segment; off[0] = 0.
+__global__ __launch_bounds__(kBlock) void hh_count_kernel(uint32_t nhi, const uint32_t* __restrict__ hi_base,
+                                                          const uint32_t* __restrict__ mask, int nb,
+                                                          uint32_t* __restrict__ off) {
+  const uint32_t h = blockIdx.x * kBlock + threadIdx.x;
+  if (h == 0) off[0] = 0u;
+  if (h >= nhi) return;
+  uint32_t n = 0;
+  if (hi_base[h] != 0xffffffffu)
+    for (int q = 0; q < nb; ++q) {
+      const uint32_t bits = h & mask[q];
+      n += (bits != 0u && bits != mask[q]) ? 1u : 0u;
+    }
+  off[h + 1] = (n + kHHList - 1) / kHHList * kHHList;
+}
+
+// In-place inclusive prefix sum of v[0..n) by one workgroup of kScanBlock
+// threads: each thread sums a contiguous chunk, the chunk sums are scanned
+// in LDS, then each thread rewrites its chunk.  Sums of 2^32 - 1 or more are
+// stored as 0xffffffff (the caller rejects them).
+constexpr int kScanBlock = 1024;
+__global__ __launch_bounds__(kScanBlock) void scan_inclusive_kernel(uint32_t* __restrict__ v, uint32_t n) {
+  __shared__ uint64_t part[kScanBlock];
+  const uint32_t chunk = (n + kScanBlock - 1) / kScanBlock;
+  const uint32_t b = threadIdx.x * chunk, e = min(n, b + chunk);
+  uint64_t s = 0;
+  for (uint32_t i = b; i < e; ++i) s += v[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int d = 1; d < kScanBlock; d <<= 1) {
+    const uint64_t add = threadIdx.x >= (unsigned)d ? part[threadIdx.x - d] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += add;
+    __syncthreads();
+  }
+  uint64_t run = threadIdx.x > 0 ? part[threadIdx.x - 1] : 0u;
+  for (uint32_t i = b; i < e; ++i) {
+    run += v[i];
+    v[i] = run >= 0xffffffffULL ? 0xffffffffu : (uint32_t)run;
+  }
+}
+
+// The entries of high word h at [off[h], off[h+1]): its flipping high-high
+// bonds in bond order (neighbour segment start, coefficient), then
+// coefficient-0 entries that read h's own segment.
+__global__ __launch_bounds__(kBlock) void hh_fill_kernel(uint32_t nhi, const uint32_t* __restrict__ hi_base,
+                                                         const uint32_t* __restrict__ mask,
+                                                         const double* __restrict__ t, int nb,
+                                                         const uint32_t* __restrict__ off, HHEntry* __restrict__ ent) {
+  const uint32_t h = blockIdx.x * kBlock + threadIdx.x;
+  if (h >= nhi) return;
+  const uint32_t own = hi_base[h];
+  if (own == 0xffffffffu) return;
+  uint32_t e = off[h];
+  const uint32_t e1 = off[h + 1];
+  for (int q = 0; q < nb; ++q) {
+    const uint32_t bits = h & mask[q];
+    if (bits != 0u && bits != mask[q]) ent[e++] = HHEntry{hi_base[h ^ mask[q]], 0u, t[q]};
+  }
+  for (; e < e1; ++e) ent[e] = HHEntry{own, 0u, 0.0};
+}
+
+// xb_nb[h * n_xb_p + c]: the first row of the segment of h ^ hbit[c], or of
+// h's own when that high word has none; 0 where h has no segment.
+__global__ __launch_bounds__(kBlock) void xb_nb_kernel(uint32_t nhi, int n_xb_p, const uint32_t* __restrict__ hi_base,
+                                                       const uint32_t* __restrict__ hbit, uint32_t* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= (uint64_t)nhi * n_xb_p) return;
+  const uint32_t h = (uint32_t)(i / n_xb_p), c = (uint32_t)(i % n_xb_p);
+  const uint32_t own = hi_base[h];
+  uint32_t v = 0u;
+  if (own != 0xffffffffu) {
+    const uint32_t nb = hi_base[h ^ hbit[c]];
+    v = nb == 0xffffffffu ? own : nb;
+  }
+  out[i] = v;
+}
+
 // One workgroup per item; 16-byte moves where source and destination allow.
 __global__ __launch_bounds__(kBlock) void scatter_copy_kernel(const unsigned char* __restrict__ src,
                                                               const ScatterItem* __restrict__ items) {
@@ -639,6 +721,25 @@ hipError_t launch_finish(Partials in, const double* scale, double* out, double* 
 template <class V>
 hipError_t launch_normalize(uint64_t n, V* v, Partials norm_in, double* b_out, hipStream_t st) {
   hipLaunchKernelGGL((normalize_kernel<V>), dim3(stream_blocks(n)), dim3(kBlock), 0, st, n, v, norm_in, b_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_hh_count(uint32_t nhi, const uint32_t* hi_base, const uint32_t* mask, int nb, uint32_t* off,
+                           const uint32_t* hbit, int n_xb_p, uint32_t* xb_nb, hipStream_t st) {
+  const unsigned g = (unsigned)((nhi + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(hh_count_kernel, dim3(g), dim3(kBlock), 0, st, nhi, hi_base, mask, nb, off);
+  hipLaunchKernelGGL(scan_inclusive_kernel, dim3(1), dim3(kScanBlock), 0, st, off + 1, nhi);
+  const uint64_t nx = (uint64_t)nhi * n_xb_p;
+  if (nx > 0)
+    hipLaunchKernelGGL(xb_nb_kernel, dim3((unsigned)((nx + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, nhi, n_xb_p,
+                       hi_base, hbit, xb_nb);
+  return hipGetLastError();
+}
+
+hipError_t launch_hh_fill(uint32_t nhi, const uint32_t* hi_base, const uint32_t* mask, const double* t, int nb,
+                          const uint32_t* off, HHEntry* ent, hipStream_t st) {
+  const unsigned g = (unsigned)((nhi + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(hh_fill_kernel, dim3(g), dim3(kBlock), 0, st, nhi, hi_base, mask, t, nb, off, ent);
   return hipGetLastError();
 }
 

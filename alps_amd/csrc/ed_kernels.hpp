@@ -191,6 +191,16 @@ template <class V>
 hipError_t launch_reorth(uint64_t n, int j, V* r, const double* scale, const ReorthDev& R, double* norm_out,
                          hipStream_t st);
 
+// The per-high-word lists of a layout on the device: launch_hh_count writes
+// off[0..nhi] (the prefix sums of each high word's flipping high-high bond
+// count, rounded up to kHHList; 0xffffffff once they reach 2^32 - 1) and
+// xb_nb[nhi * n_xb_p]; launch_hh_fill then writes the off[nhi] entries.  The
+// same lists build_layout makes on the host with host_lists.
+hipError_t launch_hh_count(uint32_t nhi, const uint32_t* hi_base, const uint32_t* mask, int nb, uint32_t* off,
+                           const uint32_t* hbit, int n_xb_p, uint32_t* xb_nb, hipStream_t st);
+hipError_t launch_hh_fill(uint32_t nhi, const uint32_t* hi_base, const uint32_t* mask, const double* t, int nb,
+                          const uint32_t* off, HHEntry* ent, hipStream_t st);
+
 // The update with its scalars (one launch instead of three): scale_j =
 // 1/sqrt(sum(norm_prev)), beta_{j-1} = sqrt(sum(norm_prev)) (when beta_jm1 is
 // set; it is null at j = 0), alpha_j = scale_j^2 * sum(hv_part); then

@@ -165,8 +165,10 @@ struct SectorLayout {
   // hh_ent, each the first row of the segment of h ^ mask and the bond's
   // coefficient.  Each list is padded to a multiple of kHHList with
   // coefficient-0 entries that read h's own segment.
-  std::vector<uint32_t> hh_off;           // [2^NH + 1]
+  std::vector<uint32_t> hh_off;           // [2^NH + 1] (empty when built on the device)
   std::vector<HHEntry> hh_ent;
+  std::vector<uint32_t> hh_mask;          // [n_hh] per high-high bond: both bits
+  std::vector<double> hh_t;               //   and its coefficient
   std::vector<uint32_t> xb_hbit;          // [n_xb_p], 0 = null
   std::vector<XBondConst> xb_c;           // [n_xb_p] per cross bond, null bonds zero
   // xb_nb[h * n_xb_p + c]: the first row of the segment of h ^ hbit_c, or of
@@ -304,8 +306,12 @@ inline std::shared_ptr<const FlipTables> flip_tables(int B, const std::vector<ui
 
 // copy_shared = false leaves lo_rank, lo_state, nbp16 and xn16 empty: the
 // batch engine reads them from the shared tables (L.low, L.flip), which it
-// uploads once per shape.
-inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1, bool copy_shared = true) {
+// uploads once per shape.  host_lists = false leaves the per-high-word lists
+// (hh_off, hh_ent, xb_nb) empty: a single operator builds them on the device
+// from hi_base and the bond masks (build_hv_lists), which avoids building
+// and uploading up to 2^NH * (high-high + cross bonds) entries on the host.
+inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1, bool copy_shared = true,
+                                 bool host_lists = false) {
   SectorLayout L;
   const int N = m.n_sites;
   if (N < 1 || N > kMaxSites) throw std::invalid_argument("number of sites must be in [1, 32]");
@@ -524,29 +530,59 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1, bool c
   }
   L.ll_t.assign(L.n_ll_p, 0.0);
   for (size_t b = 0; b < L.ll.size(); ++b) L.ll_t[b] = L.ll[b].t;
-  // flipping high-high bonds per high word, in bond order
-  L.hh_off.assign((size_t)nhi + 1, 0u);
-  L.hh_ent.clear();
-  for (uint32_t h = 0; h < nhi; ++h) {
-    const uint32_t own = L.hi_base[h];
-    if (own != 0xffffffffu) {
-      const size_t e0 = L.hh_ent.size();
-      for (const FlipBond& f : L.hh) {
-        const uint32_t bits = h & f.mask;
-        if (bits == 0u || bits == f.mask) continue;
-        L.hh_ent.push_back({L.hi_base[h ^ f.mask], 0u, f.t});
-      }
-      while ((L.hh_ent.size() - e0) % kHHList != 0) L.hh_ent.push_back({own, 0u, 0.0});
+  L.hh_mask.assign(L.hh.size(), 0u);
+  L.hh_t.assign(L.hh.size(), 0.0);
+  for (size_t q = 0; q < L.hh.size(); ++q) {
+    L.hh_mask[q] = L.hh[q].mask;
+    L.hh_t[q] = L.hh[q].t;
+  }
+  // flipping high-high bonds per high word, in bond order: counted, then
+  // written in place (branch-free: whether a bond flips is data-dependent,
+  // and a mispredicted branch per bond cost more than the rest of the loop)
+  if (host_lists) {
+    const size_t nb = L.hh.size();
+    std::vector<uint32_t> mk(nb);
+    std::vector<double> tq(nb);
+    for (size_t q = 0; q < nb; ++q) {
+      mk[q] = L.hh[q].mask;
+      tq[q] = L.hh[q].t;
     }
-    if (L.hh_ent.size() >= 0xffffffffULL) throw std::invalid_argument("high-high flip lists exceed 2^32-1 entries");
-    L.hh_off[h + 1] = (uint32_t)L.hh_ent.size();
+    auto flips = [&](uint32_t h, size_t q) -> uint32_t {
+      const uint32_t bits = h & mk[q];
+      return (uint32_t)(bits != 0u) & (uint32_t)(bits != mk[q]);
+    };
+    L.hh_off.assign((size_t)nhi + 1, 0u);
+    uint64_t total = 0;
+    for (uint32_t h = 0; h < nhi; ++h) {
+      if (L.hi_base[h] != 0xffffffffu) {
+        uint32_t n = 0;
+        for (size_t q = 0; q < nb; ++q) n += flips(h, q);
+        total += (uint64_t)round_up((int)n, kHHList);
+      }
+      L.hh_off[h + 1] = (uint32_t)std::min<uint64_t>(total, 0xffffffffULL);
+    }
+    if (total >= 0xffffffffULL) throw std::invalid_argument("high-high flip lists exceed 2^32-1 entries");
+    L.hh_ent.resize(total);
+    std::vector<HHEntry> tmp(nb + 1);
+    for (uint32_t h = 0; h < nhi; ++h) {
+      const uint32_t own = L.hi_base[h];
+      if (own == 0xffffffffu) continue;
+      size_t n = 0;
+      for (size_t q = 0; q < nb; ++q) {
+        tmp[n] = HHEntry{L.hi_base[h ^ mk[q]], 0u, tq[q]};
+        n += flips(h, q);
+      }
+      HHEntry* e = L.hh_ent.data() + L.hh_off[h];
+      std::copy(tmp.begin(), tmp.begin() + n, e);
+      std::fill(e + n, L.hh_ent.data() + L.hh_off[h + 1], HHEntry{own, 0u, 0.0});
+    }
   }
   L.xb_hbit.assign(L.n_xb_p, 0u);
   for (size_t b = 0; b < L.xb.size(); ++b) L.xb_hbit[b] = L.xb[b].hbit;
   L.xb_c.assign(L.n_xb_p, XBondConst{0.0, 0.0, 0u, 0u});
   for (size_t b = 0; b < L.xb.size(); ++b) L.xb_c[b] = {L.xb[b].e_anti - L.xb[b].e_par, L.xb[b].t, L.xb[b].hbit, 0u};
-  L.xb_nb.assign((size_t)nhi * L.n_xb_p, 0u);
-  for (uint32_t h = 0; h < nhi; ++h) {
+  if (host_lists) L.xb_nb.assign((size_t)nhi * L.n_xb_p, 0u);
+  for (uint32_t h = 0; h < (host_lists ? nhi : 0u); ++h) {
     const uint32_t own = L.hi_base[h];
     if (own == 0xffffffffu) continue;
     for (int c = 0; c < L.n_xb_p; ++c) {
