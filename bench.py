@@ -36,6 +36,10 @@ HBM_PEAK_GBS = 8000.0
 # (profiles/hv_kernel_heisenberg_L24.md); valid for that kernel version and L
 # only.  The schedules that spill write their scratch too.
 PMC_TRAFFIC_BYTES = {24: [52.98e6, 53.06e6, 53.02e6, 52.91e6, 57.88e6, 53.44e6]}
+# SHA-256 of the row kernel's source (alps_amd/csrc/ed_device.hpp) when those
+# counters were taken: if the source differs, the table no longer describes
+# the kernel that runs, and the bench reports traffic as unknown instead.
+PMC_TRAFFIC_KERNEL_SHA256 = "e058813be2b40c0d02372584370815868b6c04fc833bd4a0ce400adfec744c35"
 PMC_TRAFFIC_SOURCE = ("profiles/hv_kernel_heisenberg_L24.md, section 'HBM byte counters', column "
                       "'traffic per launch' (FETCH_SIZE + WRITE_SIZE, scaled by the copy kernel), "
                       "row of the chosen schedule")
@@ -479,6 +483,20 @@ def sweep_leg(L=20, sample=42):
             "speedup": round(one / batched, 1)}
 
 
+def traffic_table_valid(L):
+    """Whether PMC_TRAFFIC_BYTES describes the kernel that runs: a row for L
+    and the row kernel's source unchanged since the counters were taken."""
+    import hashlib
+
+    if L not in PMC_TRAFFIC_BYTES:
+        return False, f"no byte counters recorded for L={L}"
+    with open(os.path.join(ROOT, "alps_amd", "csrc", "ed_device.hpp"), "rb") as f:
+        sha = hashlib.sha256(f.read()).hexdigest()
+    if sha != PMC_TRAFFIC_KERNEL_SHA256:
+        return False, "row kernel source changed since the byte counters were taken; rerun them"
+    return True, None
+
+
 def eigen_leg(model, L, ev, op):
     """The L-ring's lowest levels with FP32 and FP64 vector storage, against
     the oracle's FP64 sparsediag of the same sector (stored for L=24 in
@@ -589,6 +607,7 @@ def main():
     wall = time.perf_counter() - t0
     ms_per_hv = ms / args.steps
     gbs = bytes_per_hv / (ms_per_hv * 1e-3) / 1e9
+    traffic_ok, traffic_note = traffic_table_valid(L)
 
     out = {
         "metric": "H·v GB/s (% HBM peak); batched solves/sec + p99 latency at 1e-10 rel err",
@@ -607,8 +626,8 @@ def main():
                    "L": L, "dim": D, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4),
-                     "traffic": PMC_TRAFFIC_BYTES[L][sched] if L in PMC_TRAFFIC_BYTES else None,
-                     "traffic_source": PMC_TRAFFIC_SOURCE if L in PMC_TRAFFIC_BYTES else None,
+                     "traffic": PMC_TRAFFIC_BYTES[L][sched] if traffic_ok else None,
+                     "traffic_source": PMC_TRAFFIC_SOURCE if traffic_ok else traffic_note,
                      "kernel": "hv_kernel<float,false,%d,%d,%d,%d>" % SCHEDULES[sched],
                      "bytes_per_launch": bytes_per_hv},
         "timed_wall_s": round(wall, 4),
