@@ -378,10 +378,14 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1, bool c
     for (size_t i = 0; i < cls.size();) {
       size_t j = i;
       while (j < cls.size() && binom(B, cls[j]) == binom(B, cls[i])) ++j;
-      std::vector<uint32_t> merged;
-      for (size_t q = i; q < j; ++q) merged.insert(merged.end(), by_len[cls[q]].begin(), by_len[cls[q]].end());
-      if (j - i > 1) std::sort(merged.begin(), merged.end());
-      L.proc_hi.insert(L.proc_hi.end(), merged.begin(), merged.end());
+      // each class's list is ascending: merge them (at most two classes,
+      // c and B - c, share a length)
+      const size_t start = L.proc_hi.size();
+      for (size_t q = i; q < j; ++q) {
+        const size_t mid = L.proc_hi.size();
+        L.proc_hi.insert(L.proc_hi.end(), by_len[cls[q]].begin(), by_len[cls[q]].end());
+        std::inplace_merge(L.proc_hi.begin() + start, L.proc_hi.begin() + mid, L.proc_hi.end());
+      }
       i = j;
     }
     if (!cls.empty()) L.max_seg = (uint32_t)binom(B, cls[0]);
@@ -427,7 +431,7 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1, bool c
     L.proc_hi.clear();
     for (int c : cls_order) {
       std::vector<uint32_t>& v = by_pc[c];
-      std::sort(v.begin(), v.end());  // address order: hi_base ascends with hi
+      if (!std::is_sorted(v.begin(), v.end())) std::sort(v.begin(), v.end());  // address order: hi_base ascends with hi
       // kClassSlices contiguous slices in turn, each dealt as eight runs
       const size_t slice = (v.size() + kClassSlices - 1) / kClassSlices;
       for (size_t s0 = 0; s0 < v.size(); s0 += slice) {
@@ -562,20 +566,23 @@ inline SectorLayout build_layout(const ModelSpec& m, int B_override = -1, bool c
       L.hh_off[h + 1] = (uint32_t)std::min<uint64_t>(total, 0xffffffffULL);
     }
     if (total >= 0xffffffffULL) throw std::invalid_argument("high-high flip lists exceed 2^32-1 entries");
-    L.hh_ent.resize(total);
-    std::vector<HHEntry> tmp(nb + 1);
+    // every bond's record is written at the list's next free entry, which
+    // advances only when the bond flips; the entry past a full list is the
+    // next high word's first (rewritten by it) or, after the last, one slack
+    // entry dropped at the end
+    L.hh_ent.resize(total + 1);
     for (uint32_t h = 0; h < nhi; ++h) {
       const uint32_t own = L.hi_base[h];
       if (own == 0xffffffffu) continue;
-      size_t n = 0;
-      for (size_t q = 0; q < nb; ++q) {
-        tmp[n] = HHEntry{L.hi_base[h ^ mk[q]], 0u, tq[q]};
-        n += flips(h, q);
-      }
       HHEntry* e = L.hh_ent.data() + L.hh_off[h];
-      std::copy(tmp.begin(), tmp.begin() + n, e);
-      std::fill(e + n, L.hh_ent.data() + L.hh_off[h + 1], HHEntry{own, 0u, 0.0});
+      HHEntry* const end = L.hh_ent.data() + L.hh_off[h + 1];
+      for (size_t q = 0; q < nb; ++q) {
+        *e = HHEntry{L.hi_base[h ^ mk[q]], 0u, tq[q]};
+        e += flips(h, q);
+      }
+      for (; e < end; ++e) *e = HHEntry{own, 0u, 0.0};
     }
+    L.hh_ent.pop_back();
   }
   L.xb_hbit.assign(L.n_xb_p, 0u);
   for (size_t b = 0; b < L.xb.size(); ++b) L.xb_hbit[b] = L.xb[b].hbit;
