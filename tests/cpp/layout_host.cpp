@@ -7,7 +7,8 @@
 //  - the flip tables read with the H.v kernel's addressing (row record
 //    (r/64)*64*n_pad + (r%64)*chunk, bond chunk k at k*64*chunk) against 4 *
 //    the flipped words' ranks, for every class, row and bond of random
-//    models, and every such read inside its class.
+//    models, and every such read inside its class;
+//  - the per-high-word flip lists against a recomputation from the states.
 // Prints "diag_err <max abs error>", "bloch_sectors <count>" and
 // "flip_bad <mismatches> flip_checked <entries>", or "bloch_mismatch ..."
 // and exits 1.
@@ -192,6 +193,69 @@ static long proc_check() {
   return bad;
 }
 
+// The per-high-word lists (host_lists): for every high word with a
+// segment, its entries are the high-high bonds whose two sites differ in a
+// state of that high word, in bond order, each with the first row of the
+// state with both sites swapped (found by ranking that state's word, not
+// from hi_base) and the bond's coefficient, then weight-0 entries of its own
+// segment up to a multiple of four; and every cross bond's neighbour
+// segment is the one of the state with the high site flipped, where such a
+// state exists.  Random models; returns the mismatches.
+static long list_check(long& checked) {
+  std::mt19937 rng(11);
+  std::uniform_real_distribution<double> U(-1, 1);
+  long bad = 0;
+  for (int trial = 0; trial < 24; ++trial) {
+    const int N = 13 + trial % 10;
+    ModelSpec s;
+    s.n_sites = N;
+    s.n_up = N / 2 - 1 + trial % 3;
+    for (int b = 0; b < 2 * N; ++b) {
+      const int a = (int)(rng() % N), c = (int)((a + 1 + rng() % (N - 1)) % N);
+      s.bonds.push_back(make_bond(a, c, U(rng), U(rng), U(rng)));
+    }
+    for (int i = 0; i < N; ++i) s.sites.push_back(make_site(U(rng), U(rng)));
+    const SectorLayout L = build_layout(s, -1, true, true);
+    const uint32_t nhi = 1u << L.NH;
+    // the first row of the segment of a high word: the rank of its smallest state
+    auto first_row = [&](uint32_t h) -> uint32_t {
+      const int kl = L.k - __builtin_popcount(h);
+      if (kl < 0 || kl > L.B) return 0xffffffffu;
+      const uint32_t lo = kl == 0 ? 0u : ((1u << kl) - 1u);
+      const uint32_t st = (h << L.B) | lo;
+      uint64_t r = 0;
+      int m = 0;
+      for (int p = 0; p < N; ++p)
+        if (st >> p & 1u) r += binom(p, ++m);
+      return (uint32_t)r;
+    };
+    for (uint32_t h = 0; h < nhi; ++h) {
+      const uint32_t own = first_row(h);
+      if (own != L.hi_base[h]) ++bad;
+      if (own == 0xffffffffu) continue;
+      std::vector<HHEntry> want;
+      for (const FlipBond& f : L.hh) {
+        const uint32_t bits = h & f.mask;
+        if (bits != 0u && bits != f.mask) want.push_back({first_row(h ^ f.mask), 0u, f.t});
+      }
+      while (want.size() % 4 != 0) want.push_back({own, 0u, 0.0});
+      const uint32_t e0 = L.hh_off[h], e1 = L.hh_off[h + 1];
+      if (e1 - e0 != want.size()) {
+        ++bad;
+        continue;
+      }
+      for (size_t i = 0; i < want.size(); ++i, ++checked)
+        if (L.hh_ent[e0 + i].nb != want[i].nb || L.hh_ent[e0 + i].t != want[i].t) ++bad;
+      for (int c = 0; c < L.n_xb_p; ++c, ++checked) {
+        const uint32_t hb = c < (int)L.xb.size() ? L.xb[c].hbit : 0u;
+        const uint32_t nb = first_row(h ^ hb);
+        if (L.xb_nb[(size_t)h * L.n_xb_p + c] != (nb == 0xffffffffu ? own : nb)) ++bad;
+      }
+    }
+  }
+  return bad;
+}
+
 // Parameter files and explicit graphs through the host parsers (the
 // sanitizer build runs them too): tasks, evaluated couplings, graph bonds,
 // and inputs that must be rejected with an error rather than misread.
@@ -235,6 +299,9 @@ int main() {
   std::printf("diag_err %.3e\n", diag_error());
   std::printf("parse_bad %ld\n", parse_check());
   std::printf("proc_bad %ld\n", proc_check());
+  long list_checked = 0;
+  const long list_bad = list_check(list_checked);
+  std::printf("list_bad %ld list_checked %ld\n", list_bad, list_checked);
   long bad = 0, checked = 0;
   flip_check(bad, checked);
   std::printf("flip_bad %ld flip_checked %ld\n", bad, checked);

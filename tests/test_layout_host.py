@@ -1,9 +1,10 @@
 """Host checks of the engine's sector layout and momentum-sector builders
 (tests/cpp/layout_host.cpp, no device): the diagonal tables against a direct
 sum over sites and bonds on random models, the lattice translations and
-Bloch bases against the oracle's restatement of blochbasisstates.h, and the
+Bloch bases against the oracle's restatement of blochbasisstates.h, the
 flip tables read with the H.v kernel's addressing against the flipped words'
-ranks."""
+ranks, and the per-high-word flip lists against a recomputation from the
+states."""
 import os
 import subprocess
 
@@ -21,4 +22,5 @@ def test_layout_diagonal_and_bloch_bases():
     assert int(vals["bloch_sectors"]) == 417
     assert int(vals["flip_bad"]) == 0 and int(vals["flip_checked"]) > 10**6
     assert int(vals["proc_bad"]) == 0
+    assert int(vals["list_bad"]) == 0 and int(vals["list_checked"]) > 10**5
     assert int(vals["parse_bad"]) == 0
