@@ -54,9 +54,12 @@ struct Reducer {
 };
 
 Reducer& thread_reducer() {
-  thread_local Reducer r;
-  r.ensure(2048);
-  return r;
+  // one per thread, kept until the process ends: a thread-exit destructor
+  // would free device memory during process exit, while the runtime's own
+  // teardown may already be under way
+  thread_local Reducer* r = new Reducer;
+  r->ensure(2048);
+  return *r;
 }
 
 }  // namespace
