@@ -69,6 +69,10 @@ Reducer& thread_reducer() {
 // ---------------------------------------------------------------------------
 
 struct alps_ed_operator_s {
+  // references: the caller's (released by alps_ed_operator_destroy) and one
+  // per Lanczos object created on it (released by alps_ed_lanczos_destroy);
+  // the operator is freed with the last
+  std::atomic<int> refs{1};
   SectorLayout layout;
   DevLayout dev{};
   DevBuf tables;  // every layout table, packed (operator_upload)
@@ -519,7 +523,7 @@ int alps_ed_operator_set_schedule(alps_ed_operator op, int schedule) {
 }
 
 int alps_ed_operator_destroy(alps_ed_operator op) {
-  delete op;
+  if (op && op->refs.fetch_sub(1) == 1) delete op;
   return ALPS_ED_OK;
 }
 
@@ -980,12 +984,16 @@ int alps_ed_lanczos_create(alps_ed_operator op, alps_ed_dtype dtype, int max_ste
     lz->scale.alloc((max_steps + 3) * sizeof(double));
     lz->red.ensure(std::max<size_t>(op_hv_blocks(op), stream_blocks(n)));
     lz->stream = current_stream();
+    op->refs.fetch_add(1);
     *out = lz.release();
   });
 }
 
 int alps_ed_lanczos_destroy(alps_ed_lanczos lz) {
+  if (!lz) return ALPS_ED_OK;
+  alps_ed_operator_s* op = lz->op;
   delete lz;
+  if (op && op->refs.fetch_sub(1) == 1) delete op;
   return ALPS_ED_OK;
 }
 

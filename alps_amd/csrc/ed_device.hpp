@@ -70,6 +70,22 @@ __device__ __forceinline__ double thread_partials(const Partials& P) {
 
 __device__ __forceinline__ double sum_partials(const Partials& P) { return block_sum_all(thread_partials(P)); }
 
+// The block's sum of v stored to *out by thread 0, for a block that makes
+// one such call (its LDS slots are written once): one barrier.
+__device__ __forceinline__ void store_block_sum_once(double* out, double v) {
+  __shared__ double once_smem[kBlock / 64];
+  v = wave_sum(v);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) once_smem[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) s += once_smem[w];
+    *out = s;
+  }
+}
+
 __device__ __forceinline__ void store_partial(double* out, double v) {
   const double s = block_sum_all(v);
   if (threadIdx.x == 0) out[blockIdx.x] = s;
@@ -426,10 +442,7 @@ __device__ __forceinline__ void hv_segment(const DevLayout& L, uint32_t h, const
     dotp = hv_rows<V, DOT, 3, CLL, CHH, CX>(L, h, x, y, xs, len, base, dh, nbp, xnp, dlp);
   else
     dotp = hv_rows<V, DOT, 4, CLL, CHH, CX>(L, h, x, y, xs, len, base, dh, nbp, xnp, dlp);
-  if (DOT) {
-    const double bs = block_sum_all(dotp);
-    if (tid == 0) *dot_slot = bs;
-  }
+  if (DOT) store_block_sum_once(dot_slot, dotp);
 }
 
 // Row i of H.v in a momentum sector (vectors of 2 * Bl.dk reals, the real

@@ -101,6 +101,9 @@ int alps_ed_operator_create(const alps_ed_model* model, alps_ed_operator* op);
  * invariant under the translations. */
 int alps_ed_operator_create_momentum(const alps_ed_model* model, int n_translations, const int* perm,
                                      const double* phase_re, const double* phase_im, alps_ed_operator* op);
+/* Releases the caller's reference to the operator: it is freed once no
+ * Lanczos object created on it remains (each holds a reference until
+ * alps_ed_lanczos_destroy). */
 int alps_ed_operator_destroy(alps_ed_operator op);
 /* No counterpart in the reference: the H.v schedule (waves per SIMD and
  * flip-loop chunk sizes) an operator uses with FP32 vectors.  Operators of
@@ -213,6 +216,7 @@ int alps_ed_lanczos_step_vectors_noalloc(alps_ed_operator op, alps_ed_vector v_p
  * allocated in chunks as the recurrence advances) for alps_ed_lanczos_ritz. */
 int alps_ed_lanczos_create(alps_ed_operator op, alps_ed_dtype dtype, int max_steps, int keep_basis,
                            alps_ed_lanczos* lz);
+/* Frees the Lanczos object and releases its reference to the operator. */
 int alps_ed_lanczos_destroy(alps_ed_lanczos lz);
 /* Reorthogonalization of the recurrence, set before alps_ed_lanczos_start.
  * mode 0: none, the reference's recurrence (make_step, lanczos.h:392-406,

@@ -81,3 +81,28 @@ def test_tiny_sector_fixed_iterations_past_its_dimension():
     assert r["dim"] == 6 and r["iterations"] == 50 and r["error_code"] == 1
     dense = np.unique(np.round(np.linalg.eigvalsh(O.dense(om, 2)), 10))
     np.testing.assert_allclose(r["ev"], dense[:2], atol=1e-10)
+
+
+def test_operator_destroyed_before_its_lanczos_object():
+    # the caller destroys the operator while a Lanczos object still uses it:
+    # the object keeps it alive until the object is destroyed, and the
+    # recurrence and Ritz vectors are those of an undisturbed run
+    def run(close_early):
+        op = A.Operator(chain(12))
+        lz = A.Lanczos(op, "f64", max_steps=40, keep_basis=True)
+        lz.start(seed=5)
+        if close_early:
+            op.close()
+            other = A.Operator(chain(14, jz=0.3))  # reuses freed memory if the operator were gone
+        lz.steps(30)
+        a, b = lz.coefficients()
+        (v,) = lz.ritz(np.full((1, 30), 1.0 / math.sqrt(30.0)))
+        out = (a.copy(), b.copy(), v.numpy().copy())
+        lz.close()
+        if not close_early:
+            op.close()
+        return out
+
+    ref, early = run(False), run(True)
+    for x, y in zip(ref, early):
+        assert np.array_equal(x, y)
