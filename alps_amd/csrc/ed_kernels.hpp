@@ -260,7 +260,8 @@ hipError_t launch_flip_pairs(const DevLayout& L, const V* v, const uint32_t* mas
                              double* out, hipStream_t st);
 
 // Lanes per row of the listed-flips Bloch kernel (4 measured fastest on the
-// L=24 ring's k sectors: 7.3 us against 7.9 us with 2 and 9.3 us with 8).
+// L=24 ring's k = 0 sector: 6.05 us against 6.19 us with 2 and 7.52 us with
+// 8, vectors as (re, im) pairs).
 constexpr int kBlochLanes = 4;
 // Blocks of a momentum-sector H.v launch, and of its dot partials.
 inline unsigned bloch_blocks(const DevBloch& B) {
