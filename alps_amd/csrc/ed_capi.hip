@@ -78,6 +78,7 @@ struct alps_ed_operator_s {
   DevBuf tables;  // every layout table, packed (operator_upload)
   DevBuf lists;   // per-high-word lists built on the device: hh_off and xb_nb,
   DevBuf hh_ent;  // and the listed high-high flips
+  DevBuf seg_rec; // per processing index: the segment's record (SegRec)
   uint64_t bytes = 0;
   uint64_t n = 0;  // vector length: D, or 2 * dk for a momentum sector (realified)
   int schedule = 0;  // H.v schedule for FP32 vectors (kHvSchedule)
@@ -345,6 +346,13 @@ void operator_upload(alps_ed_operator op) {
     d.hh_ent = op->hh_ent.as<HHEntry>();
     d.xb_nb = xnb;
     op->bytes += op->lists.bytes + op->hh_ent.bytes;
+  }
+  if (d.n_seg > 0) {
+    op->seg_rec.alloc((size_t)d.n_seg * sizeof(SegRec));
+    hip_check(launch_seg_rec(d, op->seg_rec.as<SegRec>(), nullptr), "segment records");
+    hip_check(hipStreamSynchronize(nullptr), "segment records");
+    d.seg_rec = op->seg_rec.as<SegRec>();
+    op->bytes += op->seg_rec.bytes;
   }
 }
 

@@ -208,7 +208,8 @@ __device__ __forceinline__ void st_at(void* base, uint32_t off, T v) {
 template <class V, bool DOT, int NQ, int CLL, int CHH, int CX>
 __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const V* __restrict__ x,
                                           V* __restrict__ y, double* xs, uint32_t len, uint32_t base, double dh,
-                                          const uint16_t* nbp, const uint16_t* xnp, const double* dlp) {
+                                          const uint16_t* nbp, const uint16_t* xnp, const double* dlp, uint32_t e0,
+                                          uint32_t e1) {
   constexpr uint32_t VS = (uint32_t)sizeof(V);
   constexpr int QL = NQ - 1;  // the last slot
   const uint32_t tid = threadIdx.x;
@@ -298,7 +299,6 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
   // same.  Bonds that do not flip would add exact zeros, so the sums are
   // those over all bonds.
   {
-    const uint32_t e0 = ldc(L.hh_off + h), e1 = ldc(L.hh_off + h + 1);
     for (uint32_t e = e0; e < e1; e += CHH) {
       // neighbour segment starts as element offsets (a select between
       // pointers would make the loads generic-address ones)
@@ -395,8 +395,8 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
   return dotp;
 }
 
-// H.v over the row segment of high word h (one workgroup of kBlock threads;
-// dynamic LDS of at least (L.max_seg + 1) doubles):
+// H.v over the row segment of record S (high word S.h; one workgroup of
+// kBlock threads; dynamic LDS of at least (L.max_seg + 1) doubles):
 //   y[r] = sum_c H[r][c] x[c] for the segment's rows r;
 // with DOT, *dot_slot receives the segment's sum of x_r (H x)_r (FP64, before
 // y is rounded to the storage type).
@@ -409,21 +409,17 @@ __device__ __forceinline__ double hv_rows(const DevLayout& L, uint32_t h, const 
 // after the first do nothing).  CLL, CHH,
 // CX: chunk sizes of the flip loops (the schedule, ed_kernels.hpp).
 template <class V, bool DOT, int CLL = 4, int CHH = 4, int CX = 2>
-__device__ __forceinline__ void hv_segment(const DevLayout& L, uint32_t h, const V* __restrict__ x,
-                                           V* __restrict__ y, double* __restrict__ dot_slot) {
+__device__ __forceinline__ void hv_segment_at(const DevLayout& L, const SegRec& S, const V* __restrict__ x,
+                                              V* __restrict__ y, double* __restrict__ dot_slot) {
   extern __shared__ __align__(16) unsigned char smem[];
   // the segment's slice of x in FP64 (converted once per row, not once per
   // low-low flip that reads it); entry len is the zero slot
   double* xs = reinterpret_cast<double*>(smem);
   const uint32_t tid = threadIdx.x;
-  const int kl = L.k - __builtin_popcount(h);
-  const uint32_t cls0 = ldc(L.lo_class_off + kl);
-  const uint32_t len = ldc(L.lo_class_off + kl + 1) - cls0;
-  const uint32_t base = ldc(L.hi_base + h);
-  const double dh = ldc(L.diag_hi + h);
-  const uint16_t* nbp = L.nbp16 + ldc(L.nbp_off + kl);
-  const uint16_t* xnp = L.xn16 + ldc(L.xn_off + kl);
-  const double* dlp = L.diag_lo_cls + cls0;
+  const uint32_t len = S.len;
+  const uint16_t* nbp = L.nbp16 + S.nbp;
+  const uint16_t* xnp = L.xn16 + S.xnp;
+  const double* dlp = L.diag_lo_cls + S.cls0;
   // row slots of this wave: slot q holds rows wave*64 + q*kBlock + lane
   const uint32_t w0 = tid & ~63u;
   const uint32_t nq = __builtin_amdgcn_readfirstlane(len > w0 ? (len - w0 + kBlock - 1) / kBlock : 0u);
@@ -435,14 +431,39 @@ __device__ __forceinline__ void hv_segment(const DevLayout& L, uint32_t h, const
   if (nq == 0)
     __syncthreads();
   else if (nq == 1)
-    dotp = hv_rows<V, DOT, 1, CLL, CHH, CX>(L, h, x, y, xs, len, base, dh, nbp, xnp, dlp);
+    dotp = hv_rows<V, DOT, 1, CLL, CHH, CX>(L, S.h, x, y, xs, len, S.base, S.dh, nbp, xnp, dlp, S.e0, S.e1);
   else if (nq == 2)
-    dotp = hv_rows<V, DOT, 2, CLL, CHH, CX>(L, h, x, y, xs, len, base, dh, nbp, xnp, dlp);
+    dotp = hv_rows<V, DOT, 2, CLL, CHH, CX>(L, S.h, x, y, xs, len, S.base, S.dh, nbp, xnp, dlp, S.e0, S.e1);
   else if (nq == 3)
-    dotp = hv_rows<V, DOT, 3, CLL, CHH, CX>(L, h, x, y, xs, len, base, dh, nbp, xnp, dlp);
+    dotp = hv_rows<V, DOT, 3, CLL, CHH, CX>(L, S.h, x, y, xs, len, S.base, S.dh, nbp, xnp, dlp, S.e0, S.e1);
   else
-    dotp = hv_rows<V, DOT, 4, CLL, CHH, CX>(L, h, x, y, xs, len, base, dh, nbp, xnp, dlp);
+    dotp = hv_rows<V, DOT, 4, CLL, CHH, CX>(L, S.h, x, y, xs, len, S.base, S.dh, nbp, xnp, dlp, S.e0, S.e1);
   if (DOT) store_block_sum_once(dot_slot, dotp);
+}
+
+// The record of the segment of high word h, looked up in the tables.
+__device__ __forceinline__ SegRec seg_record(const DevLayout& L, uint32_t h) {
+  const int kl = L.k - __builtin_popcount(h);
+  SegRec S;
+  S.h = h;
+  S.base = ldc(L.hi_base + h);
+  S.cls0 = ldc(L.lo_class_off + kl);
+  S.len = ldc(L.lo_class_off + kl + 1) - S.cls0;
+  S.nbp = ldc(L.nbp_off + kl);
+  S.xnp = ldc(L.xn_off + kl);
+  S.e0 = ldc(L.hh_off + h);
+  S.e1 = ldc(L.hh_off + h + 1);
+  S.dh = ldc(L.diag_hi + h);
+  S.pad = 0.0;
+  return S;
+}
+
+// H.v over the row segment of high word h (hv_segment_at with the record
+// looked up).
+template <class V, bool DOT, int CLL = 4, int CHH = 4, int CX = 2>
+__device__ __forceinline__ void hv_segment(const DevLayout& L, uint32_t h, const V* __restrict__ x,
+                                           V* __restrict__ y, double* __restrict__ dot_slot) {
+  hv_segment_at<V, DOT, CLL, CHH, CX>(L, seg_record(L, h), x, y, dot_slot);
 }
 
 // Row i of H.v in a momentum sector (vectors of 2 * Bl.dk reals, the real

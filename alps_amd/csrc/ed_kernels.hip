@@ -36,8 +36,15 @@ using namespace dev;
 template <class V, bool DOT, int W, int CLL, int CHH, int CX>
 __global__ __launch_bounds__(kBlock, W) void hv_kernel(DevLayout L, const V* __restrict__ x, V* __restrict__ y,
                                                        double* __restrict__ dot_out) {
-  const uint32_t h = ldc(L.proc_hi + blockIdx.x);
-  hv_segment<V, DOT, CLL, CHH, CX>(L, h, x, y, DOT ? dot_out + blockIdx.x : nullptr);
+  hv_segment_at<V, DOT, CLL, CHH, CX>(L, ldc_struct(L.seg_rec + blockIdx.x), x, y,
+                                      DOT ? dot_out + blockIdx.x : nullptr);
+}
+
+// seg_rec[i] for processing index i (the record hv_segment would look up).
+__global__ __launch_bounds__(kBlock) void seg_rec_kernel(DevLayout L, SegRec* __restrict__ out) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= L.n_seg) return;
+  out[i] = seg_record(L, L.proc_hi[i]);
 }
 
 template <class V, bool DOT, int W, int CLL, int CHH, int CX>
@@ -664,7 +671,7 @@ __global__ __launch_bounds__(64) void elements_kernel(DevLayout L, uint64_t row0
 template <class V>
 hipError_t launch_hv(const DevLayout& L, const V* x, V* y, double* dot_out, hipStream_t st, int schedule) {
   if (L.n_seg == 0) return hipSuccess;
-  if (schedule < 0 || schedule >= kHvSchedules) return hipErrorInvalidValue;
+  if (schedule < 0 || schedule >= kHvSchedules || L.seg_rec == nullptr) return hipErrorInvalidValue;
   const size_t lds = (size_t)(L.max_seg + 1) * sizeof(double);  // the FP64 slice of hv_segment
   if (dot_out)
     launch_hv_schedule<V, true>(L, x, y, dot_out, st, lds, schedule);
@@ -733,6 +740,12 @@ hipError_t launch_hh_count(uint32_t nhi, const uint32_t* hi_base, const uint32_t
   if (nx > 0)
     hipLaunchKernelGGL(xb_nb_kernel, dim3((unsigned)((nx + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, nhi, n_xb_p,
                        hi_base, hbit, xb_nb);
+  return hipGetLastError();
+}
+
+hipError_t launch_seg_rec(const DevLayout& L, SegRec* out, hipStream_t st) {
+  if (L.n_seg == 0) return hipSuccess;
+  hipLaunchKernelGGL(seg_rec_kernel, dim3((L.n_seg + kBlock - 1) / kBlock), dim3(kBlock), 0, st, L, out);
   return hipGetLastError();
 }
 

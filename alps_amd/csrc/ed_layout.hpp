@@ -128,6 +128,16 @@ struct XBondConst {
   uint32_t pad;
 };
 
+// What the H.v kernel needs of the segment it processes, one record per
+// processing index (built on the device for a single operator): its high
+// word, first row, class offset and length, flip-table offsets, the range
+// of its listed high-high flips and the high word's diagonal.  Read with
+// one pair of scalar loads instead of two dependent rounds of table reads.
+struct SegRec {
+  uint32_t h, base, cls0, len, nbp, xnp, e0, e1;
+  double dh, pad;
+};
+
 struct SectorLayout {
   int N = 0, k = 0, B = 0, NH = 0;
   uint64_t D = 0;

@@ -35,11 +35,11 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE scaled by a copy kernel of known size
 # (profiles/hv_kernel_heisenberg_L24.md); valid for that kernel version and L
 # only.  The schedules that spill write their scratch too.
-PMC_TRAFFIC_BYTES = {24: [52.98e6, 53.06e6, 53.02e6, 52.91e6, 57.88e6, 53.44e6]}
+PMC_TRAFFIC_BYTES = {24: [53.38e6, 53.47e6, 53.35e6, 53.30e6, 58.25e6, 54.01e6]}
 # SHA-256 of the row kernel's source (alps_amd/csrc/ed_device.hpp) when those
 # counters were taken: if the source differs, the table no longer describes
 # the kernel that runs, and the bench reports traffic as unknown instead.
-PMC_TRAFFIC_KERNEL_SHA256 = "cbbe040c3c58674adab76e87ecd1a6ca4818a9cbc905fdf0d794b7ff1376e92b"
+PMC_TRAFFIC_KERNEL_SHA256 = "8dd16111070282a21ca2d360f4733c5027e3ac0292ebee8e726dea2ee30cccc5"
 PMC_TRAFFIC_SOURCE = ("profiles/hv_kernel_heisenberg_L24.md, section 'HBM byte counters', column "
                       "'traffic per launch' (FETCH_SIZE + WRITE_SIZE, scaled by the copy kernel), "
                       "row of the chosen schedule")
