@@ -197,13 +197,14 @@ hipError_t launch_reorth(uint64_t n, int j, V* r, const double* scale, const Reo
 // count, rounded up to kHHList; 0xffffffff once they reach 2^32 - 1) and
 // xb_nb[nhi * n_xb_p]; launch_hh_fill then writes the off[nhi] entries.  The
 // same lists build_layout makes on the host with host_lists.
-// seg_rec[n_seg] of a layout whose tables and lists are on the device.
-hipError_t launch_seg_rec(const DevLayout& L, SegRec* out, hipStream_t st);
-
 hipError_t launch_hh_count(uint32_t nhi, const uint32_t* hi_base, const uint32_t* mask, int nb, uint32_t* off,
                            const uint32_t* hbit, int n_xb_p, uint32_t* xb_nb, hipStream_t st);
 hipError_t launch_hh_fill(uint32_t nhi, const uint32_t* hi_base, const uint32_t* mask, const double* t, int nb,
                           const uint32_t* off, HHEntry* ent, hipStream_t st);
+
+// seg_rec[n_seg] (SegRec, one per processing index) of a layout whose tables
+// and lists are on the device.
+hipError_t launch_seg_rec(const DevLayout& L, SegRec* out, hipStream_t st);
 
 // The update with its scalars (one launch instead of three): scale_j =
 // 1/sqrt(sum(norm_prev)), beta_{j-1} = sqrt(sum(norm_prev)) (when beta_jm1 is
